@@ -1,0 +1,149 @@
+/*
+ * semx.h -- C ABI of the MI355X-native spectral-element / Fourier
+ * Helmholtz core (libsemx.so).
+ *
+ * The ABI replaces the lower boundary of semtex's Helmholtz hot path: the
+ * work that semtex's C++ classes (Field, MatrixSys, ModalMatrixSys,
+ * AuxField) hand to BLAS/LAPACK, veclib gather/scatter and femlib.  Each
+ * entry point names the reference interface it stands in for.  Plain
+ * pointers and sizes only; device pointers are HIP device addresses,
+ * streams are hipStream_t passed as void* (NULL = default stream).
+ *
+ * Storage conventions are semtex's:
+ *   - element data row-major np x np, e*np*np + i*np + j (i = s row,
+ *     j = r column)                                  (src/element.h:23-24)
+ *   - a field is nz planes of planeSize doubles, planeSize = nel*np*np
+ *     rounded up to even                         (src/geometry.cpp:105-108)
+ *   - plane 2m = Re(mode m), 2m+1 = Im(mode m), plane 1 = Nyquist, which
+ *     Field::solve skips                  (src/auxfield.h:18-29, field.cpp:566)
+ *   - boundary-condition lines: nbound*np values per plane, one group of np
+ *     per boundary side in the mesh's boundary list, CCW along the side
+ *                                                    (src/field.cpp:77-150)
+ *
+ * Errors: every call returns a status (SEMX_OK = 0).  semtex prints and
+ * exits instead (veclib/util.c:23-45); semx_error_string() gives the text.
+ */
+#ifndef SEMX_H
+#define SEMX_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* --- status codes -------------------------------------------------------- */
+#define SEMX_OK           0
+#define SEMX_E_ARG        1   /* bad argument / size                       */
+#define SEMX_E_NOT_SPD    2   /* factorisation failed: matrix.cpp:261       */
+#define SEMX_E_MAXITER    3   /* PCG step limit: field.cpp:706 (WARNING)    */
+#define SEMX_E_HIP        4   /* HIP runtime error / no device              */
+#define SEMX_E_NOMEM      5   /* allocation failed                          */
+#define SEMX_E_JACOBIAN   6   /* nonpositive Jacobian: element.cpp:1358     */
+#define SEMX_E_MESH       7   /* inconsistent mesh connectivity             */
+#define SEMX_E_UNSUPPORTED 8  /* configuration not supported                */
+
+/* --- solver kinds (femlib/cfemdef.h:43-52 SolverKind) -------------------- */
+#define SEMX_DIRECT  0        /* static condensation + banded Cholesky      */
+#define SEMX_JACPCG  1        /* Jacobi-preconditioned CG                   */
+#define SEMX_MIXED   2        /* mode 0 DIRECT, other modes JACPCG          */
+
+/* --- transform direction (femlib/cfemdef.h TransformKind) --------------- */
+#define SEMX_FORWARD  1
+#define SEMX_INVERSE -1
+
+typedef struct semx_mesh_s* semx_mesh;
+typedef struct semx_helm_s* semx_helm;
+
+/* Text for a status code.                                                  */
+const char* semx_error_string (int status);
+
+/* Select the HIP device for subsequent calls of this thread and check that
+ * it is usable.                                                            */
+int semx_device_init (int device);
+
+/* --- mesh: Element construction + numbering ------------------------------
+ * Replaces Element::Element / Element::mapping (src/element.cpp:30-106,
+ * 1281-1436), Mesh::meshElmt (src/mesh.cpp:772-886) and the numbering that
+ * semtex obtains by running `enumerate` (src/bcmgr.cpp:586-824,
+ * utility/enumerate.cpp:120-943).
+ *   vx, vy : nel*4 CCW vertex coordinates of straight-sided quads
+ *   nbr    : nel*4, for side s of element e the index e2*4+s2 of the side
+ *            it is joined to (interior neighbour, or periodic partner: the
+ *            role of semtex's shared NODES and <P> surfaces), -1 on a
+ *            boundary.  Joined sides are traversed in opposite directions.
+ *   skind  : nel*4 side kinds: 0 joined, 1 essential (<D>), 2 natural
+ *            (<N>).  The boundary list is every side with skind != 0 in
+ *            (element, side) order.
+ *   svv_mn, svv_eps, kinvis : SVV tokens SVV_MN, SVV_EPSN, KINVIS
+ *            (src/svv.cpp:41-79); svv_mn < 0 switches SVV off.
+ * Host-only; device copies are made on first use.                         */
+int semx_mesh_create (int np, int nel, const double* vx, const double* vy,
+                      const int* nbr, const int* skind, int svv_mn,
+                      double svv_eps, double kinvis, semx_mesh* mesh);
+int semx_mesh_destroy (semx_mesh mesh);
+/* info[10] = np nel next nint nglobal nsolve nband nbound nPlane planeSize,
+ * nband including the diagonal (utility/enumerate.cpp:903-943).           */
+int semx_mesh_info (semx_mesh mesh, int* info);
+/* Node coordinates, nel*np*np each (host arrays).                          */
+int semx_mesh_nodes (semx_mesh mesh, double* x, double* y);
+/* Boundary list: element and side of each boundary (host arrays, nbound). */
+int semx_mesh_bsides (semx_mesh mesh, int* belmt, int* bside);
+/* NumberSys btog (nel*next), bmask (nel*next), emask (nel)
+ * (src/numbersys.h:4-75).                                                  */
+int semx_mesh_numbering (semx_mesh mesh, int* btog, int* bmask, int* emask);
+/* Number of distinct element geometric-factor sets (semtex Family
+ * semantics, femlib/family.c:171-192: equal within 6e-14).                */
+int semx_mesh_nsets (semx_mesh mesh, int* nsets);
+
+/* --- Helmholtz systems: ModalMatrixSys --------------------------------------
+ * Replaces ModalMatrixSys::ModalMatrixSys (src/matrix.cpp:33-110) and
+ * MatrixSys::MatrixSys (src/matrix.cpp:139-323): one system per Fourier
+ * mode base_mode..base_mode+nmodes-1 with betak2 = (beta*mode)^2, for a
+ * field of nz planes.  method: SEMX_DIRECT, SEMX_JACPCG or SEMX_MIXED.
+ * Setup (elemental static condensation, band assembly and Cholesky
+ * factorisation) runs on the host; factors are uploaded to the device.   */
+int semx_helm_create (semx_mesh mesh, double lambda2, double beta,
+                      int base_mode, int nmodes, int nz, int method,
+                      semx_helm* helm);
+int semx_helm_destroy (semx_helm helm);
+/* info[8] = nmodes nz method nsolve(mode0) nband nblk(block size)
+ *           bytes_per_solve_factor(MB*1000, int) singular(mode0)           */
+int semx_helm_info (semx_helm helm, int* info);
+/* Host copies of one element's condensed matrices for mode index m
+ * (HelmholtzSC, src/element.cpp:131-229): hbb next^2, hbi next*nint,
+ * hii nint^2, all row-major in emap order.                                 */
+int semx_helm_element (semx_helm helm, int m, int elmt, double* hbb,
+                       double* hbi, double* hii);
+
+/* --- the solve: Field::solve ------------------------------------------------
+ * Replaces Field::solve (src/field.cpp:509-725) for a field of nz planes:
+ * f (forcing, device, nz*planeSize) is destroyed, u (device, nz*planeSize)
+ * holds the initial guess (used by JACPCG) and receives the solution;
+ * bc (device, nz*nbound*np) holds the boundary values in Fourier space.
+ * tol_rel, step_max: TOL_REL, STEP_MAX for JACPCG (field.cpp:649-706).
+ * iters (host, may be NULL) receives the largest PCG iteration count.
+ * Returns SEMX_E_MAXITER when a PCG plane hit step_max (semtex: WARNING). */
+int semx_helm_solve (semx_helm helm, double* f, const double* bc, double* u,
+                     double tol_rel, int step_max, int* iters, void* stream);
+
+/* --- kernel timing -------------------------------------------------------
+ * No counterpart in semtex (dclock(), veclib/util.c:65-85, is never called
+ * on the solve path).  When enabled, each kernel launch of the solve path is
+ * bracketed by HIP events on its own stream; query returns the summed
+ * device time (ms) and launch count per kernel name (e.g. "k_band_solve").*/
+int semx_profile_enable (int on);
+int semx_profile_reset (void);
+int semx_profile_query (const char* kernel, double* total_ms, long long* count);
+
+/* --- device memory helpers (plumbing for callers without their own) ---- */
+int semx_malloc (void** ptr, size_t bytes);
+int semx_free (void* ptr);
+int semx_memcpy_htod (void* dst, const void* src, size_t bytes);
+int semx_memcpy_dtoh (void* dst, const void* src, size_t bytes);
+int semx_synchronize (void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,146 @@
+"""semtex_amd -- MI355X-native spectral-element / Fourier Helmholtz core.
+
+Python mirror of the semtex classes on the hot path, over the C ABI of
+libsemx.so (include/semx.h):
+
+  Mesh            Element construction + global numbering
+                  (src/element.cpp:30-106, src/bcmgr.cpp:586-824)
+  ModalMatrixSys  one Helmholtz system per Fourier mode (src/matrix.cpp:33-110)
+  solve()         Field::solve (src/field.cpp:509-725)
+
+Device buffers are torch tensors on the GPU (float64, contiguous); torch is
+used only for memory and streams.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _capi
+from ._capi import SemxError, check, load
+from .mesh import bc_lines, rect_mesh, side_nodes
+
+DIRECT, JACPCG, MIXED = 0, 1, 2
+FORWARD, INVERSE = 1, -1
+
+__all__ = ["Mesh", "ModalMatrixSys", "solve", "DIRECT", "JACPCG", "MIXED",
+           "rect_mesh", "bc_lines", "side_nodes", "SemxError", "load"]
+
+
+def _dp(a):
+    return a.ctypes.data_as(_capi.D)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_capi.I)
+
+
+class Mesh:
+    """Elements + numbering of a quad mesh (see semtex_amd.mesh)."""
+
+    def __init__(self, np_, m, svv_mn=-1, svv_eps=0.0, kinvis=1.0):
+        L = load()
+        vx = np.ascontiguousarray(m["vx"], dtype=np.float64)
+        vy = np.ascontiguousarray(m["vy"], dtype=np.float64)
+        nbr = np.ascontiguousarray(m["nbr"], dtype=np.int32)
+        sk = np.ascontiguousarray(m["skind"], dtype=np.int32)
+        h = C.c_void_p()
+        check(L.semx_mesh_create(np_, int(m["nel"]), _dp(vx), _dp(vy), _ip(nbr), _ip(sk),
+                                 svv_mn, svv_eps, kinvis, C.byref(h)), "semx_mesh_create")
+        self.h = h
+        info = np.zeros(10, dtype=np.int32)
+        check(L.semx_mesh_info(h, _ip(info)))
+        (self.np, self.nel, self.next, self.nint, self.nglobal, self.nsolve,
+         self.nband, self.nbound, self.nPlane, self.planeSize) = [int(v) for v in info]
+        self.npnp = self.np * self.np
+        self.x = np.zeros(self.nPlane)
+        self.y = np.zeros(self.nPlane)
+        check(L.semx_mesh_nodes(h, _dp(self.x), _dp(self.y)))
+        self.belmt = np.zeros(max(self.nbound, 1), dtype=np.int32)
+        self.bside = np.zeros(max(self.nbound, 1), dtype=np.int32)
+        check(L.semx_mesh_bsides(h, _ip(self.belmt), _ip(self.bside)))
+        self.belmt = self.belmt[:self.nbound]
+        self.bside = self.bside[:self.nbound]
+        self.btog = np.zeros(self.nel * self.next, dtype=np.int32)
+        self.bmask = np.zeros_like(self.btog)
+        self.emask = np.zeros(self.nel, dtype=np.int32)
+        check(L.semx_mesh_numbering(h, _ip(self.btog), _ip(self.bmask), _ip(self.emask)))
+        ns = C.c_int()
+        check(L.semx_mesh_nsets(h, C.byref(ns)))
+        self.nsets = ns.value
+        self.nline = self.nbound * self.np
+
+    def bc_lines(self, field):
+        return bc_lines(field, self.belmt, self.bside, self.np)
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                load().semx_mesh_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class ModalMatrixSys:
+    """Helmholtz systems for modes base..base+nmodes-1 of a field of nz
+    planes (src/matrix.cpp:33-110)."""
+
+    def __init__(self, mesh, lambda2, beta, base_mode, nmodes, nz, method=DIRECT):
+        L = load()
+        h = C.c_void_p()
+        check(L.semx_helm_create(mesh.h, float(lambda2), float(beta), int(base_mode),
+                                 int(nmodes), int(nz), int(method), C.byref(h)),
+              "semx_helm_create")
+        self.h = h
+        self.mesh = mesh
+        self.nz = nz
+        self.nmodes = nmodes
+        info = np.zeros(8, dtype=np.int32)
+        check(L.semx_helm_info(h, _ip(info)))
+        self.info = dict(nmodes=int(info[0]), nz=int(info[1]), method=int(info[2]),
+                         nsolve0=int(info[3]), ldab=int(info[4]), nb=int(info[5]),
+                         factor_kB=int(info[6]), singular0=int(info[7]))
+
+    def element(self, m, e):
+        M = self.mesh
+        hbb = np.zeros((M.next, M.next))
+        hbi = np.zeros((M.next, max(M.nint, 1)))
+        hii = np.zeros((max(M.nint, 1), max(M.nint, 1)))
+        check(load().semx_helm_element(self.h, m, e, _dp(hbb), _dp(hbi), _dp(hii)))
+        return hbb, hbi[:, :M.nint], hii[:M.nint, :M.nint]
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                load().semx_helm_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    import torch
+    assert t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()
+    return C.c_void_p(t.data_ptr())
+
+
+def solve(msys, f, bc, u, tol_rel=1e-12, step_max=10000, stream=None):
+    """Field::solve over all planes: f (destroyed), bc, u are device tensors
+    of nz*planeSize, nz*nbound*np and nz*planeSize doubles.  Returns the
+    largest PCG iteration count (0 for DIRECT)."""
+    import torch
+    M = msys.mesh
+    assert f.numel() == msys.nz * M.planeSize and u.numel() == msys.nz * M.planeSize
+    if M.nbound:
+        assert bc is not None and bc.numel() == msys.nz * M.nline
+    if stream is None:
+        stream = torch.cuda.current_stream().cuda_stream
+    it = C.c_int(0)
+    check(load().semx_helm_solve(msys.h, _ptr(f), _ptr(bc), _ptr(u), float(tol_rel),
+                                 int(step_max), C.byref(it), C.c_void_p(stream)),
+          "semx_helm_solve")
+    return it.value

@@ -1,0 +1,52 @@
+"""Build libsemx.so (HIP kernels for gfx950 + host setup) in-tree.
+
+Usage: python -m semtex_amd.build   (or semtex_amd.build.build())
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "libsemx.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+SOURCES = ["semx_setup.cpp", "semx_solve.hip"]
+
+# Packed FP32 instructions are switched off for all device code.
+DEVICE_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
+                "-packed-fp32-ops"]
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
+          "-I", os.path.join(ROOT, "include"), "-I", CSRC]
+
+
+def _run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+
+
+def build(verbose: bool = True) -> str:
+    objs = []
+    bdir = os.path.join(HERE, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    for src in SOURCES:
+        path = os.path.join(CSRC, src)
+        obj = os.path.join(bdir, src + ".o")
+        if src.endswith(".hip"):
+            cmd = [HIPCC, "-c", path, "-o", obj] + COMMON + DEVICE_FLAGS
+        else:
+            cmd = ["g++", "-c", path, "-o", obj, "-pthread"] + COMMON
+        _run(cmd)
+        objs.append(obj)
+    tmp = OUT + ".tmp"
+    _run([HIPCC, "-shared", "-o", tmp] + objs + DEVICE_FLAGS + ["-pthread"])
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

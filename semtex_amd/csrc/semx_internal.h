@@ -1,0 +1,129 @@
+// semx_internal.h -- host-side data structures shared by the setup code
+// (semx_setup.cpp) and the HIP solve code (semx_solve.hip).
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "semx.h"
+
+namespace semx {
+
+constexpr double EPSDP = 6.0e-14;  // roundoff threshold of veclib/cveclib.h:58
+constexpr double EPSSP = 6.0e-7;   // singular-system threshold, cveclib.h:54
+
+// Geometric factor slots of one element set (Element::mapping outputs).
+enum { G_Q1 = 0, G_Q2, G_Q3, G_Q4, G_DRDX, G_DSDX, G_DRDY, G_DSDY, G_N };
+
+struct Ops {
+  int np = 0;
+  std::vector<double> z, w, DV, DT, SDV, SDT;  // np, np, np*np x4
+  std::vector<int> emap, pmap;                 // np*np
+};
+
+// Device-resident copy of a mesh.
+struct MeshDev {
+  double* G = nullptr;       // nset*G_N*npnp
+  int* eset = nullptr;       // nel
+  int* emap = nullptr;       // npnp
+  int* btog = nullptr;       // nel*next
+  int* emask = nullptr;      // nel
+  double* ops = nullptr;     // DV DT SDV SDT, 4*npnp
+  int* asm_ptr = nullptr;    // nglobal+1
+  int* asm_idx = nullptr;    // nel*next
+  int* ess_src = nullptr;    // nglobal-nsolve
+  int* nat_ptr = nullptr;    // nel+1
+  int* nat_b = nullptr;      // natural boundary indices
+  int* bside = nullptr;      // nbound
+  double* barea = nullptr;   // nbound*np
+  double* imass = nullptr;   // nglobal
+};
+
+struct Mesh {
+  int np = 0, nel = 0, next = 0, nint = 0, npnp = 0, nPlane = 0, planeSize = 0;
+  Ops ops;
+  std::vector<double> x, y;          // nel*npnp node coordinates
+  int nset = 0;
+  std::vector<int> eset;             // element -> geometric-factor set
+  std::vector<double> G;             // nset*G_N*npnp
+  std::vector<int> setQ3;            // set has nonzero Q3
+  int nglobal = 0, nsolve = 0, nband = 0;
+  std::vector<int> btog, bmask, emask;
+  int nbound = 0;
+  std::vector<int> belmt, bside, bkind;
+  std::vector<double> barea;         // nbound*np side quadrature weights
+  std::vector<double> imass;         // nglobal inverse global mass
+  std::vector<int> ess_src;          // essential node -> bc-line index
+  std::vector<int> asm_ptr, asm_idx; // global node -> element-boundary slots
+  std::vector<int> nat_ptr, nat_b;   // element -> natural boundaries
+  bool on_device = false;
+  int device = -1;
+  MeshDev d;
+};
+
+// One block-triangular band factor on the device, per mode.
+struct ModeSys {
+  int kind = SEMX_DIRECT;     // DIRECT or JACPCG
+  double betak2 = 0.0;
+  int n = 0;                  // solved unknowns (nsolve, or nsolve-1 pinned)
+  int singular = 0;
+  size_t off_H = 0;           // offset (doubles) into band arrays
+  size_t off_L = 0;           // offset (doubles) into block-inverse arrays
+  int nblk = 0;
+};
+
+struct HelmDev {
+  double* hbi = nullptr;      // nmodes*nset*next*nint
+  double* hii = nullptr;      // nmodes*nset*nint*nint
+  double* Hr = nullptr;       // row band storage, all modes
+  double* Hc = nullptr;       // column band storage, all modes
+  double* Linv = nullptr;     // (U_JJ^T)^-1 blocks, row-major
+  double* LinvT = nullptr;    // transposes
+  int* mode_n = nullptr;      // per mode n
+  long long* mode_offH = nullptr;
+  long long* mode_offL = nullptr;
+  double* pc = nullptr;       // JACPCG inverse diagonals, nmodes*npts
+  // scratch
+  double* Fb = nullptr;       // nplanes*nel*next
+  double* Fi = nullptr;       // nplanes*nel*nint
+  double* g = nullptr;        // nz*nzero essential values
+  double* sol = nullptr;      // nplanes*solstride
+  int* plane_of = nullptr;    // nplanes -> field plane
+};
+
+struct Helm {
+  Mesh* M = nullptr;
+  double lambda2 = 0.0, beta = 0.0;
+  int base = 0, nmodes = 0, nz = 0, method = SEMX_DIRECT;
+  int nb = 32;                // band-solve block size
+  int kd = 0, ldab = 0;
+  std::vector<ModeSys> modes;
+  std::vector<double> hbb, hbi, hii;  // host, nmodes*nset*...
+  std::vector<double> Hr, Hc, Linv, LinvT;  // host, released after upload
+  std::vector<double> pc;                   // nmodes*npts
+  size_t band_doubles = 0, inv_doubles = 0;
+  int nplanes = 0, solstride = 0, npts = 0;
+  std::vector<int> plane_of;
+  bool on_device = false;
+  HelmDev d;
+};
+
+// setup (semx_setup.cpp)
+int mesh_build(Mesh& M, int np, int nel, const double* vx, const double* vy,
+               const int* nbr, const int* skind, int svv_mn, double svv_eps,
+               double kinvis);
+int helm_build(Helm& H);
+void gll(int np, std::vector<double>& z, std::vector<double>& w);
+void gll_deriv(int np, const std::vector<double>& z, std::vector<double>& D);
+void svv_derivative(int np, int svv_mn, double svv_eps, double kinvis,
+                    std::vector<double>& SDV);
+
+// device (semx_solve.hip)
+int mesh_upload(Mesh& M);
+void mesh_release(Mesh& M);
+int helm_upload(Helm& H);
+void helm_release(Helm& H);
+
+}  // namespace semx

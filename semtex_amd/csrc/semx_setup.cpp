@@ -1,0 +1,824 @@
+// semx_setup.cpp -- host-side setup of the spectral-element Helmholtz core:
+// GLL operators, element geometry, global numbering, elemental static
+// condensation and banded Cholesky factorisation.  This is the work of
+// semtex's Element / Mesh / BCmgr / MatrixSys constructors; it runs once
+// per system and is excluded from solve timings.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <thread>
+#include <utility>
+
+#include "semx_internal.h"
+
+namespace semx {
+
+// ---------------------------------------------------------------------------
+// Quadrature: Gauss-Lobatto-Legendre points/weights and the derivative
+// matrix of the Lagrange interpolants (femlib/polylib.c:312-340, 513-556).
+// ---------------------------------------------------------------------------
+
+static void legendre_pd(int n, long double x, long double& p, long double& d) {
+  long double pm = 1.0L, pc = x, dm = 0.0L, dc = 1.0L;
+  if (n == 0) { p = 1.0L; d = 0.0L; return; }
+  for (int k = 2; k <= n; ++k) {
+    long double pn = ((2 * k - 1) * x * pc - (k - 1) * pm) / k;
+    long double dn = dm + (2 * k - 1) * pc;
+    pm = pc; pc = pn; dm = dc; dc = dn;
+  }
+  p = pc; d = dc;
+}
+
+void gll(int np, std::vector<double>& z, std::vector<double>& w) {
+  const int N = np - 1;
+  z.assign(np, 0.0); w.assign(np, 0.0);
+  if (np == 1) { w[0] = 2.0; return; }
+  z[0] = -1.0; z[N] = 1.0;
+  // Interior nodes are the roots of L'_N; Newton from Chebyshev-Lobatto
+  // guesses, negative half only, mirrored for exact antisymmetry.
+  for (int j = 1; 2 * j <= N; ++j) {
+    long double x = -std::cos(3.14159265358979323846264338327950288L * j / N);
+    for (int it = 0; it < 100; ++it) {
+      long double p, d;
+      legendre_pd(N, x, p, d);
+      long double d2 = (2.0L * x * d - N * (N + 1.0L) * p) / (1.0L - x * x);
+      long double dx = d / d2;
+      x -= dx;
+      if (std::fabs(dx) < 1e-19L) break;
+    }
+    z[j] = static_cast<double>(x);
+    z[N - j] = -z[j];
+  }
+  if (N % 2 == 0) z[N / 2] = 0.0;
+  for (int j = 0; j <= N; ++j) {
+    long double p, d;
+    legendre_pd(N, z[j], p, d);
+    w[j] = static_cast<double>(2.0L / (N * (N + 1.0L) * p * p));
+  }
+}
+
+void gll_deriv(int np, const std::vector<double>& z, std::vector<double>& D) {
+  const int N = np - 1;
+  D.assign(np * np, 0.0);
+  std::vector<long double> L(np);
+  for (int i = 0; i < np; ++i) { long double d; legendre_pd(N, z[i], L[i], d); }
+  for (int i = 0; i < np; ++i)
+    for (int j = 0; j < np; ++j) {
+      if (i != j)
+        D[i * np + j] = static_cast<double>(L[i] / (L[j] * ((long double)z[i] - z[j])));
+    }
+  if (np > 1) {
+    D[0] = -0.25 * N * (N + 1);
+    D[N * np + N] = 0.25 * N * (N + 1);
+  }
+}
+
+// SVV-stabilised derivative SDV = BW diag(sqrt S) FW DV with the Legendre
+// transform pair of femlib/operators.c:317-480 (src/svv.cpp:126-185).
+void svv_derivative(int np, int svv_mn, double svv_eps, double kinvis,
+                    std::vector<double>& SDV) {
+  const int N = np - 1;
+  std::vector<double> z, w, DV;
+  gll(np, z, w);
+  gll_deriv(np, z, DV);
+  std::vector<double> Lt(np * np), S(np, 1.0);  // Lt[k*np+j] = L_k(z_j)
+  for (int k = 0; k < np; ++k)
+    for (int j = 0; j < np; ++j) {
+      long double p, d;
+      legendre_pd(k, z[j], p, d);
+      Lt[k * np + j] = static_cast<double>(p);
+    }
+  if (svv_mn >= 0 && svv_mn < N && svv_eps > EPSDP) {
+    for (int i = 0; i < np; ++i) S[i] = 0.0;
+    for (int i = svv_mn + 1; i < np; ++i) {
+      const double q = (N - i) / (1.0 * svv_mn - i);
+      S[i] = std::exp(-q * q);
+    }
+    for (int i = 0; i < np; ++i) S[i] = 1.0 + svv_eps / kinvis * S[i];
+  }
+  // A = diag(sqrt S) FW, FW[k][j] = C_k w_j L_k(z_j)
+  std::vector<double> A(np * np), B(np * np);
+  for (int k = 0; k < np; ++k) {
+    const double ck = (k < N) ? 0.5 * (2 * k + 1) : 0.5 * N;
+    for (int j = 0; j < np; ++j) A[k * np + j] = std::sqrt(S[k]) * ck * w[j] * Lt[k * np + j];
+  }
+  // B = BW A, BW[i][k] = L_k(z_i)
+  for (int i = 0; i < np; ++i)
+    for (int j = 0; j < np; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < np; ++k) s += Lt[k * np + i] * A[k * np + j];
+      B[i * np + j] = s;
+    }
+  SDV.assign(np * np, 0.0);
+  for (int i = 0; i < np; ++i)
+    for (int j = 0; j < np; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < np; ++k) s += B[i * np + k] * DV[k * np + j];
+      SDV[i * np + j] = s;
+    }
+}
+
+static void build_ops(Ops& o, int np, int svv_mn, double svv_eps, double kinvis) {
+  o.np = np;
+  gll(np, o.z, o.w);
+  gll_deriv(np, o.z, o.DV);
+  o.DT.assign(np * np, 0.0);
+  for (int i = 0; i < np; ++i)
+    for (int j = 0; j < np; ++j) o.DT[j * np + i] = o.DV[i * np + j];
+  svv_derivative(np, svv_mn, svv_eps, kinvis, o.SDV);
+  o.SDT.assign(np * np, 0.0);
+  for (int i = 0; i < np; ++i)
+    for (int j = 0; j < np; ++j) o.SDT[j * np + i] = o.SDV[i * np + j];
+  // Edge-major permutation (femlib/mapping.c:46-132): CCW boundary from
+  // node 0, then interior row-major.
+  o.emap.clear();
+  const int nm = np - 1;
+  for (int j = 0; j < nm; ++j) o.emap.push_back(j);                 // bottom
+  for (int i = 0; i < nm; ++i) o.emap.push_back(i * np + nm);       // right
+  for (int j = nm; j > 0; --j) o.emap.push_back(nm * np + j);       // top
+  for (int i = nm; i > 0; --i) o.emap.push_back(i * np);            // left
+  for (int i = 1; i < nm; ++i)
+    for (int j = 1; j < nm; ++j) o.emap.push_back(i * np + j);
+  o.pmap.assign(np * np, 0);
+  for (int k = 0; k < np * np; ++k) o.pmap[o.emap[k]] = k;
+}
+
+// ---------------------------------------------------------------------------
+// Element geometry (src/mesh.cpp:772-886, src/element.cpp:1281-1436).
+// ---------------------------------------------------------------------------
+
+static void element_nodes(const Ops& o, const double* vx, const double* vy,
+                          double* x, double* y) {
+  const int np = o.np;
+  // Straight sides: the Coons patch of straight sides is the bilinear map
+  // of the four vertices evaluated at the GLL node pairs.
+  for (int i = 0; i < np; ++i)
+    for (int j = 0; j < np; ++j) {
+      const double r = o.z[j], s = o.z[i];
+      const double n0 = 0.25 * (1 - r) * (1 - s), n1 = 0.25 * (1 + r) * (1 - s);
+      const double n2 = 0.25 * (1 + r) * (1 + s), n3 = 0.25 * (1 - r) * (1 + s);
+      x[i * np + j] = n0 * vx[0] + n1 * vx[1] + n2 * vx[2] + n3 * vx[3];
+      y[i * np + j] = n0 * vy[0] + n1 * vy[1] + n2 * vy[2] + n3 * vy[3];
+    }
+}
+
+static int element_factors(const Ops& o, const double* x, const double* y, double* g) {
+  const int np = o.np, npnp = np * np;
+  for (int i = 0; i < np; ++i)
+    for (int j = 0; j < np; ++j) {
+      double xr = 0, xs = 0, yr = 0, ys = 0;
+      for (int k = 0; k < np; ++k) {
+        xr += o.DV[j * np + k] * x[i * np + k];
+        yr += o.DV[j * np + k] * y[i * np + k];
+        xs += o.DV[i * np + k] * x[k * np + j];
+        ys += o.DV[i * np + k] * y[k * np + j];
+      }
+      const double jac = xr * ys - xs * yr;
+      if (jac < 4 * EPSDP) return SEMX_E_JACOBIAN;
+      const double W = o.w[i] * o.w[j];
+      const int q = i * np + j;
+      g[G_Q1 * npnp + q] = (xs * xs + ys * ys) / jac * W;
+      g[G_Q2 * npnp + q] = (xr * xr + yr * yr) / jac * W;
+      g[G_Q3 * npnp + q] = -(yr * ys + xr * xs) / jac * W;
+      g[G_Q4 * npnp + q] = jac * W;
+      g[G_DRDX * npnp + q] = ys / jac;
+      g[G_DRDY * npnp + q] = -xs / jac;
+      g[G_DSDX * npnp + q] = -yr / jac;
+      g[G_DSDY * npnp + q] = xr / jac;
+    }
+  // Null-mapping: factors that vanish to roundoff are exact zeros
+  // (src/element.cpp:1431-1435).
+  const int slots[5] = {G_DRDX, G_DRDY, G_DSDX, G_DSDY, G_Q3};
+  for (int s : slots) {
+    double n2 = 0;
+    for (int q = 0; q < npnp; ++q) n2 += g[s * npnp + q] * g[s * npnp + q];
+    if (std::sqrt(n2) < 4 * EPSDP) std::fill(g + s * npnp, g + (s + 1) * npnp, 0.0);
+  }
+  return SEMX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Numbering: connectivity of element sides, essential nodes last, RCM.
+// ---------------------------------------------------------------------------
+
+static int band_of(const std::vector<int>& btog, const std::vector<int>& bmask,
+                   int nel, int next) {
+  int nb = 0;
+  for (int e = 0; e < nel; ++e) {
+    int lo = INT32_MAX, hi = -1;
+    for (int i = 0; i < next; ++i)
+      if (!bmask[e * next + i]) {
+        lo = std::min(lo, btog[e * next + i]);
+        hi = std::max(hi, btog[e * next + i]);
+      }
+    if (hi >= 0) nb = std::max(nb, hi - lo);
+  }
+  return nb + 1;
+}
+
+struct Graph {
+  std::vector<int> ptr, adj;
+  int n = 0;
+  int deg(int v) const { return ptr[v + 1] - ptr[v]; }
+};
+
+static std::vector<int> bfs_levels(const Graph& G, int root, std::vector<int>& level) {
+  std::vector<int> order;
+  order.reserve(G.n);
+  level.assign(G.n, -1);
+  level[root] = 0;
+  order.push_back(root);
+  for (size_t h = 0; h < order.size(); ++h) {
+    const int v = order[h];
+    for (int k = G.ptr[v]; k < G.ptr[v + 1]; ++k) {
+      const int u = G.adj[k];
+      if (level[u] < 0) { level[u] = level[v] + 1; order.push_back(u); }
+    }
+  }
+  return order;
+}
+
+// George-Liu pseudo-peripheral node search followed by Cuthill-McKee from
+// that root, per connected component; returned as the reversed order.
+static std::vector<int> rcm(const Graph& G, int start) {
+  std::vector<int> perm;
+  perm.reserve(G.n);
+  std::vector<char> done(G.n, 0);
+  std::vector<int> level;
+  int seed = start;
+  while ((int)perm.size() < G.n) {
+    while (seed < G.n && done[seed]) ++seed;
+    if (seed >= G.n) seed = 0;
+    while (done[seed]) ++seed;
+    int root = seed;
+    // pseudo-peripheral refinement (restricted to this component)
+    int ecc = -1;
+    for (int it = 0; it < 8; ++it) {
+      std::vector<int> ord = bfs_levels(G, root, level);
+      const int e = level[ord.back()];
+      if (e <= ecc) break;
+      ecc = e;
+      int best = ord.back();
+      for (int v : ord)
+        if (level[v] == e && G.deg(v) < G.deg(best)) best = v;
+      root = best;
+    }
+    // Cuthill-McKee from root
+    std::vector<int> comp;
+    comp.push_back(root);
+    done[root] = 1;
+    for (size_t h = 0; h < comp.size(); ++h) {
+      const int v = comp[h];
+      std::vector<int> nb;
+      for (int k = G.ptr[v]; k < G.ptr[v + 1]; ++k)
+        if (!done[G.adj[k]]) { done[G.adj[k]] = 1; nb.push_back(G.adj[k]); }
+      std::stable_sort(nb.begin(), nb.end(),
+                       [&](int a, int b) { return G.deg(a) < G.deg(b); });
+      comp.insert(comp.end(), nb.begin(), nb.end());
+    }
+    perm.insert(perm.end(), comp.begin(), comp.end());
+  }
+  std::reverse(perm.begin(), perm.end());
+  return perm;  // perm[newid] = oldid
+}
+
+static int find_root(std::vector<int>& par, int a) {
+  while (par[a] != a) { par[a] = par[par[a]]; a = par[a]; }
+  return a;
+}
+
+static int number_mesh(Mesh& M, const int* nbr, const int* skind) {
+  const int np = M.np, nel = M.nel, next = M.next, nm = np - 1, ni = np - 2;
+  const int nside = 4 * nel;
+  // -- validate the side connectivity: joined sides pair up symmetrically,
+  //    every unjoined side carries a boundary kind
+  for (int i = 0; i < nside; ++i) {
+    const int j = nbr[i];
+    if (j >= 0) {
+      if (j >= nside || j == i || nbr[j] != i || skind[i] != 0) return SEMX_E_MESH;
+    } else if (skind[i] != 1 && skind[i] != 2) {
+      return SEMX_E_MESH;
+    }
+  }
+  // -- corners of joined sides coincide: side (e,s) runs from corner s to
+  //    corner s+1, its partner runs the other way (src/mesh.cpp:889-972)
+  std::vector<int> par(nside);
+  std::iota(par.begin(), par.end(), 0);
+  for (int i = 0; i < nside; ++i) {
+    const int j = nbr[i];
+    if (j <= i) continue;
+    const int e = i / 4, s = i % 4, e2 = j / 4, s2 = j % 4;
+    const int a0 = find_root(par, 4 * e + s), b0 = find_root(par, 4 * e2 + (s2 + 1) % 4);
+    par[a0] = b0;
+    const int a1 = find_root(par, 4 * e + (s + 1) % 4), b1 = find_root(par, 4 * e2 + s2);
+    par[a1] = b1;
+  }
+  // -- provisional ids: corners by class, side interiors per joined pair
+  std::vector<int> cid(nside, -1), sfirst(nside, -1);
+  std::vector<int> prov(nel * next);
+  std::vector<double> gx, gy;
+  int ng = 0;
+  auto newnode = [&](double px, double py) { gx.push_back(px); gy.push_back(py); return ng++; };
+  const std::vector<int>& em = M.ops.emap;
+  for (int e = 0; e < nel; ++e)
+    for (int s = 0; s < 4; ++s) {
+      const int i = 4 * e + s;
+      const int root = find_root(par, i);
+      if (cid[root] < 0) {
+        const int q0 = e * M.npnp + em[s * nm];
+        cid[root] = newnode(M.x[q0], M.y[q0]);
+      }
+      prov[e * next + s * nm] = cid[root];
+      const int j = nbr[i];
+      if (j >= 0 && j < i) {
+        for (int k = 0; k < ni; ++k) prov[e * next + s * nm + 1 + k] = sfirst[j] + (ni - 1 - k);
+        sfirst[i] = sfirst[j];
+      } else {
+        sfirst[i] = ng;
+        for (int k = 0; k < ni; ++k) {
+          const int q = e * M.npnp + em[s * nm + 1 + k];
+          newnode(M.x[q], M.y[q]);
+          prov[e * next + s * nm + 1 + k] = sfirst[i] + k;
+        }
+      }
+    }
+  // -- essential nodes and the boundary list
+  std::vector<char> ess(ng, 0);
+  M.belmt.clear(); M.bside.clear(); M.bkind.clear();
+  for (int e = 0; e < nel; ++e)
+    for (int s = 0; s < 4; ++s) {
+      const int k = skind[4 * e + s];
+      if (!k) continue;
+      if (k != 1 && k != 2) return SEMX_E_ARG;
+      M.belmt.push_back(e); M.bside.push_back(s); M.bkind.push_back(k);
+      if (k == 1)
+        for (int i = 0; i <= nm; ++i) ess[prov[e * next + (s * nm + i) % next]] = 1;
+    }
+  M.nbound = static_cast<int>(M.belmt.size());
+  int nunk = 0;
+  for (int v = 0; v < ng; ++v) nunk += !ess[v];
+  // -- unknown-node graph (nodes sharing an element)
+  std::vector<int> uid(ng, -1), uold;
+  for (int v = 0; v < ng; ++v) if (!ess[v]) { uid[v] = (int)uold.size(); uold.push_back(v); }
+  Graph G;
+  G.n = nunk;
+  {
+    std::vector<std::vector<int>> nbr(nunk);
+    for (int e = 0; e < nel; ++e)
+      for (int i = 0; i < next; ++i) {
+        const int a = uid[prov[e * next + i]];
+        if (a < 0) continue;
+        for (int k = 0; k < next; ++k) {
+          const int b = uid[prov[e * next + k]];
+          if (b >= 0 && b != a) nbr[a].push_back(b);
+        }
+      }
+    G.ptr.assign(nunk + 1, 0);
+    for (int a = 0; a < nunk; ++a) {
+      std::sort(nbr[a].begin(), nbr[a].end());
+      nbr[a].erase(std::unique(nbr[a].begin(), nbr[a].end()), nbr[a].end());
+      G.ptr[a + 1] = G.ptr[a] + (int)nbr[a].size();
+    }
+    G.adj.reserve(G.ptr[nunk]);
+    for (int a = 0; a < nunk; ++a) G.adj.insert(G.adj.end(), nbr[a].begin(), nbr[a].end());
+  }
+  // -- candidate orderings; keep the narrowest band (the role of enumerate's
+  //    -O2/-O3 root searches, utility/enumerate.cpp:680-809).
+  std::vector<int> bmask(nel * next);
+  for (int i = 0; i < nel * next; ++i) bmask[i] = ess[prov[i]];
+  auto apply = [&](const std::vector<int>& perm, std::vector<int>& btog) {
+    std::vector<int> newid(ng, -1);
+    for (int k = 0; k < nunk; ++k) newid[uold[perm[k]]] = k;
+    int ne = 0;
+    for (int v = 0; v < ng; ++v) if (ess[v]) newid[v] = nunk + ne++;
+    btog.resize(nel * next);
+    for (int i = 0; i < nel * next; ++i) btog[i] = newid[prov[i]];
+  };
+  std::vector<std::vector<int>> cands;
+  if (nunk > 0) {
+    std::vector<int> natural(nunk);
+    std::iota(natural.begin(), natural.end(), 0);
+    cands.push_back(natural);
+    for (int start : {0, nunk / 3, (2 * nunk) / 3, nunk - 1}) cands.push_back(rcm(G, start));
+    for (int axis = 0; axis < 2; ++axis) {
+      std::vector<int> p = natural;
+      const double eps = 1e-9;
+      std::stable_sort(p.begin(), p.end(), [&](int a, int b) {
+        const double* c1 = axis ? gx.data() : gy.data();
+        const double* c2 = axis ? gy.data() : gx.data();
+        const int va = uold[a], vb = uold[b];
+        if (std::fabs(c1[va] - c1[vb]) > eps) return c1[va] < c1[vb];
+        return c2[va] < c2[vb];
+      });
+      cands.push_back(p);
+    }
+  } else
+    cands.push_back({});
+  int best = -1, bestband = INT32_MAX;
+  std::vector<int> bt;
+  for (size_t c = 0; c < cands.size(); ++c) {
+    apply(cands[c], bt);
+    const int b = band_of(bt, bmask, nel, next);
+    if (b < bestband) { bestband = b; best = (int)c; }
+  }
+  apply(cands[best], M.btog);
+  M.bmask = bmask;
+  M.nglobal = ng;
+  M.nsolve = nunk;
+  M.nband = bestband;
+  M.emask.assign(nel, 0);
+  for (int e = 0; e < nel; ++e)
+    for (int i = 0; i < next; ++i) if (M.bmask[e * next + i]) M.emask[e] = 1;
+  return SEMX_OK;
+}
+
+// ---------------------------------------------------------------------------
+
+int mesh_build(Mesh& M, int np, int nel, const double* vx, const double* vy,
+               const int* nbr, const int* skind, int svv_mn, double svv_eps,
+               double kinvis) {
+  if (np < 3 || np > 16 || nel < 1) return SEMX_E_ARG;
+  M.np = np; M.nel = nel; M.npnp = np * np;
+  M.next = 4 * (np - 1); M.nint = M.npnp - M.next;
+  M.nPlane = nel * M.npnp;
+  M.planeSize = M.nPlane + (M.nPlane % 2);
+  build_ops(M.ops, np, svv_mn, svv_eps, kinvis);
+  const int npnp = M.npnp;
+  M.x.assign((size_t)nel * npnp, 0.0);
+  M.y.assign((size_t)nel * npnp, 0.0);
+  std::vector<double> g((size_t)nel * G_N * npnp);
+  std::vector<double> xr(npnp), yr(npnp);
+  for (int e = 0; e < nel; ++e) {
+    element_nodes(M.ops, vx + 4 * e, vy + 4 * e, &M.x[e * npnp], &M.y[e * npnp]);
+    // Factors from coordinates relative to vertex 0: translated copies of
+    // one element shape then give equal factors and share one set.
+    const double rx[4] = {0.0, vx[4 * e + 1] - vx[4 * e], vx[4 * e + 2] - vx[4 * e],
+                          vx[4 * e + 3] - vx[4 * e]};
+    const double ry[4] = {0.0, vy[4 * e + 1] - vy[4 * e], vy[4 * e + 2] - vy[4 * e],
+                          vy[4 * e + 3] - vy[4 * e]};
+    element_nodes(M.ops, rx, ry, xr.data(), yr.data());
+    const int st = element_factors(M.ops, xr.data(), yr.data(), &g[(size_t)e * G_N * npnp]);
+    if (st) return st;
+  }
+  // -- deduplicate geometric factor sets (femlib/family.c:171-192 semantics)
+  M.eset.assign(nel, -1);
+  M.G.clear(); M.setQ3.clear();
+  M.nset = 0;
+  std::vector<int> reps;
+  for (int e = 0; e < nel; ++e) {
+    const double* ge = &g[(size_t)e * G_N * npnp];
+    int found = -1;
+    for (int s = 0; s < M.nset && found < 0; ++s) {
+      const double* gs = &g[(size_t)reps[s] * G_N * npnp];
+      bool same = true;
+      for (int q = 0; q < G_N * npnp && same; ++q) same = std::fabs(ge[q] - gs[q]) <= EPSDP;
+      if (same) found = s;
+    }
+    if (found < 0) {
+      found = M.nset++;
+      reps.push_back(e);
+      M.G.insert(M.G.end(), ge, ge + G_N * npnp);
+      bool q3 = false;
+      for (int q = 0; q < npnp; ++q) q3 |= ge[G_Q3 * npnp + q] != 0.0;
+      M.setQ3.push_back(q3);
+    }
+    M.eset[e] = found;
+  }
+  int st = number_mesh(M, nbr, skind);
+  if (st) return st;
+  const int next = M.next, nm = np - 1;
+  // -- inverse global mass (src/bcmgr.cpp:806-822)
+  M.imass.assign(M.nglobal, 0.0);
+  for (int e = 0; e < nel; ++e) {
+    const double* Q4 = &M.G[(size_t)M.eset[e] * G_N * npnp + G_Q4 * npnp];
+    for (int i = 0; i < next; ++i) M.imass[M.btog[e * next + i]] += Q4[M.ops.emap[i]];
+  }
+  for (double& v : M.imass) v = 1.0 / v;
+  // -- side quadrature weights |dx/dxi| w (src/element.cpp:1653-1744)
+  M.barea.assign((size_t)M.nbound * np, 0.0);
+  for (int b = 0; b < M.nbound; ++b) {
+    const int e = M.belmt[b], s = M.bside[b];
+    for (int i = 0; i < np; ++i) {
+      double tx = 0, ty = 0;
+      for (int k = 0; k < np; ++k) {
+        int q;
+        switch (s) {
+          case 0: q = k; break;
+          case 1: q = k * np + nm; break;
+          case 2: q = nm * np + k; break;
+          default: q = k * np; break;
+        }
+        tx += M.ops.DV[i * np + k] * M.x[e * npnp + q];
+        ty += M.ops.DV[i * np + k] * M.y[e * npnp + q];
+      }
+      M.barea[b * np + i] = std::sqrt(tx * tx + ty * ty) * M.ops.w[i];
+    }
+  }
+  // -- essential values: last writer in boundary order (Field::getEssential,
+  //    src/field.cpp:994-1019 with Essential::set, condition.cpp:80-101)
+  const int nzero = M.nglobal - M.nsolve;
+  M.ess_src.assign(nzero, -1);
+  for (int b = 0; b < M.nbound; ++b) {
+    if (M.bkind[b] != 1) continue;
+    const int* bmap = &M.btog[M.belmt[b] * next];
+    const int s = M.bside[b];
+    for (int i = 0; i <= nm; ++i) {
+      const int gid = bmap[(s * nm + i) % next];
+      M.ess_src[gid - M.nsolve] = b * np + i;
+    }
+  }
+  for (int v : M.ess_src) if (v < 0) return SEMX_E_MESH;
+  // -- assembly lists: global node -> element-boundary slots (owner gather,
+  //    replaces the scatter-sum of Veclib::scatr_sum)
+  M.asm_ptr.assign(M.nglobal + 1, 0);
+  for (int i = 0; i < nel * next; ++i) M.asm_ptr[M.btog[i] + 1]++;
+  for (int gI = 0; gI < M.nglobal; ++gI) M.asm_ptr[gI + 1] += M.asm_ptr[gI];
+  M.asm_idx.assign(nel * next, 0);
+  {
+    std::vector<int> fill(M.asm_ptr.begin(), M.asm_ptr.end() - 1);
+    for (int i = 0; i < nel * next; ++i) M.asm_idx[fill[M.btog[i]]++] = i;
+  }
+  // -- natural boundaries per element
+  M.nat_ptr.assign(nel + 1, 0);
+  for (int b = 0; b < M.nbound; ++b) if (M.bkind[b] == 2) M.nat_ptr[M.belmt[b] + 1]++;
+  for (int e = 0; e < nel; ++e) M.nat_ptr[e + 1] += M.nat_ptr[e];
+  M.nat_b.assign(M.nat_ptr[nel], 0);
+  {
+    std::vector<int> fill(M.nat_ptr.begin(), M.nat_ptr.end() - 1);
+    for (int b = 0; b < M.nbound; ++b)
+      if (M.bkind[b] == 2) M.nat_b[fill[M.belmt[b]]++] = b;
+  }
+  return SEMX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Elemental Helmholtz operator and static condensation.
+// ---------------------------------------------------------------------------
+
+// y = (K + hcon M) x on one element by sum factorisation, the operator of
+// src/element.cpp:1564-1650; dv/dt the (SVV) derivative and transpose.
+static void elmt_apply(int np, const double* g, const double* dv, const double* dt,
+                       double hcon, const double* x, double* y) {
+  const int npnp = np * np;
+  std::vector<double> R(npnp), S(npnp);
+  const double *Q1 = g + G_Q1 * npnp, *Q2 = g + G_Q2 * npnp, *Q3 = g + G_Q3 * npnp,
+               *Q4 = g + G_Q4 * npnp;
+  for (int i = 0; i < np; ++i)
+    for (int j = 0; j < np; ++j) {
+      double r = 0, s = 0;
+      for (int q = 0; q < np; ++q) {
+        r += x[i * np + q] * dt[q * np + j];
+        s += dv[i * np + q] * x[q * np + j];
+      }
+      const int k = i * np + j;
+      R[k] = Q1[k] * r + Q3[k] * s;
+      S[k] = Q2[k] * s + Q3[k] * r;
+    }
+  for (int i = 0; i < np; ++i)
+    for (int j = 0; j < np; ++j) {
+      double t = Q4[i * np + j] * x[i * np + j] * hcon;
+      for (int q = 0; q < np; ++q) t += dt[i * np + q] * S[q * np + j] + R[i * np + q] * dv[q * np + j];
+      y[i * np + j] = t;
+    }
+}
+
+// Cholesky of a small dense SPD matrix in place (lower), returns 0 on success.
+static int dense_chol(int n, double* A) {
+  for (int j = 0; j < n; ++j) {
+    double d = A[j * n + j];
+    for (int k = 0; k < j; ++k) d -= A[j * n + k] * A[j * n + k];
+    if (!(d > 0.0)) return 1;
+    d = std::sqrt(d);
+    A[j * n + j] = d;
+    for (int i = j + 1; i < n; ++i) {
+      double s = A[i * n + j];
+      for (int k = 0; k < j; ++k) s -= A[i * n + k] * A[j * n + k];
+      A[i * n + j] = s / d;
+    }
+  }
+  return 0;
+}
+
+// Inverse of an SPD matrix from its Cholesky factor (full symmetric result).
+static int spd_inverse(int n, double* A) {
+  if (dense_chol(n, A)) return 1;
+  std::vector<double> Linv(n * n, 0.0);
+  for (int c = 0; c < n; ++c) {
+    for (int i = 0; i < n; ++i) {
+      double s = (i == c) ? 1.0 : 0.0;
+      for (int k = c; k < i; ++k) s -= A[i * n + k] * Linv[k * n + c];
+      Linv[i * n + c] = (i < c) ? 0.0 : s / A[i * n + i];
+    }
+  }
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      double s = 0;
+      for (int k = std::max(i, j); k < n; ++k) s += Linv[k * n + i] * Linv[k * n + j];
+      A[i * n + j] = s;
+    }
+  return 0;
+}
+
+// Elemental matrix in emap order, condensed: hbb <- hbb - hbi hii^-1 hib,
+// hbi <- hbi hii^-1, hii <- hii^-1 (src/element.cpp:131-229).
+static int condense(const Mesh& M, const double* g, double lambda2, double betak2,
+                    double* hbb, double* hbi, double* hii) {
+  const int np = M.np, npnp = M.npnp, next = M.next, nint = M.nint;
+  const double* dv = (lambda2 > EPSDP) ? M.ops.SDV.data() : M.ops.DV.data();
+  const double* dt = (lambda2 > EPSDP) ? M.ops.SDT.data() : M.ops.DT.data();
+  std::vector<double> H((size_t)npnp * npnp), e(npnp), col(npnp);
+  for (int c = 0; c < npnp; ++c) {         // column c (emap order) = H e_c
+    std::fill(e.begin(), e.end(), 0.0);
+    e[M.ops.emap[c]] = 1.0;
+    elmt_apply(np, g, dv, dt, lambda2 + betak2, e.data(), col.data());
+    for (int r = 0; r < npnp; ++r) H[(size_t)r * npnp + c] = col[M.ops.emap[r]];
+  }
+  // symmetrise (the operator is symmetric; removes roundoff asymmetry)
+  for (int r = 0; r < npnp; ++r)
+    for (int c = r + 1; c < npnp; ++c) {
+      const double a = 0.5 * (H[(size_t)r * npnp + c] + H[(size_t)c * npnp + r]);
+      H[(size_t)r * npnp + c] = H[(size_t)c * npnp + r] = a;
+    }
+  for (int r = 0; r < next; ++r)
+    for (int c = 0; c < next; ++c) hbb[r * next + c] = H[(size_t)r * npnp + c];
+  if (nint == 0) return SEMX_OK;
+  for (int r = 0; r < nint; ++r)
+    for (int c = 0; c < nint; ++c) hii[r * nint + c] = H[(size_t)(next + r) * npnp + next + c];
+  if (spd_inverse(nint, hii)) return SEMX_E_NOT_SPD;
+  std::vector<double> hb(next * nint);
+  for (int r = 0; r < next; ++r)
+    for (int c = 0; c < nint; ++c) hb[r * nint + c] = H[(size_t)r * npnp + next + c];
+  for (int r = 0; r < next; ++r)
+    for (int c = 0; c < nint; ++c) {
+      double s = 0;
+      for (int k = 0; k < nint; ++k) s += hb[r * nint + k] * hii[k * nint + c];
+      hbi[r * nint + c] = s;
+    }
+  for (int r = 0; r < next; ++r)
+    for (int c = 0; c < next; ++c) {
+      double s = 0;
+      for (int k = 0; k < nint; ++k) s += hbi[r * nint + k] * hb[c * nint + k];
+      hbb[r * next + c] -= s;
+    }
+  return SEMX_OK;
+}
+
+// Banded Cholesky on row band storage A[i*ldab + d] = A(i, i+d), d = 0..kd.
+static int band_chol_rows(int n, int kd, int ldab, double* A) {
+  for (int j = 0; j < n; ++j) {
+    double* rj = A + (size_t)j * ldab;
+    if (!(rj[0] > 0.0)) return SEMX_E_NOT_SPD;
+    const double ujj = std::sqrt(rj[0]);
+    rj[0] = ujj;
+    const int km = std::min(kd, n - 1 - j);
+    const double inv = 1.0 / ujj;
+    for (int d = 1; d <= km; ++d) rj[d] *= inv;
+    for (int i = 1; i <= km; ++i) {
+      const double u = rj[i];
+      if (u == 0.0) continue;
+      double* ri = A + (size_t)(j + i) * ldab;
+      const int len = km - i;
+      for (int l = 0; l <= len; ++l) ri[l] -= u * rj[i + l];
+    }
+  }
+  return SEMX_OK;
+}
+
+int helm_build(Helm& H) {
+  Mesh& M = *H.M;
+  const int next = M.next, nint = M.nint, nset = M.nset, npnp = M.npnp;
+  const int nmodes = H.nmodes;
+  H.npts = M.nglobal + M.nel * nint;
+  H.kd = M.nband - 1;
+  H.ldab = M.nband;
+  H.modes.assign(nmodes, ModeSys());
+  H.hbb.assign((size_t)nmodes * nset * next * next, 0.0);
+  H.hbi.assign((size_t)nmodes * nset * next * std::max(nint, 1), 0.0);
+  H.hii.assign((size_t)nmodes * nset * std::max(nint * nint, 1), 0.0);
+  size_t offH = 0, offL = 0;
+  const int nb = H.nb;
+  for (int m = 0; m < nmodes; ++m) {
+    ModeSys& S = H.modes[m];
+    const double bk = H.beta * (H.base + m);
+    S.betak2 = bk * bk;
+    S.kind = (H.method == SEMX_MIXED) ? ((H.base + m == 0) ? SEMX_DIRECT : SEMX_JACPCG)
+                                      : H.method;
+    // Singular: no essential BCs and lambda2 + betak2 ~ 0 (matrix.cpp:167-169).
+    S.singular = (H.lambda2 + S.betak2) < EPSSP && M.nglobal == M.nsolve;
+    S.n = S.singular ? M.nsolve - 1 : M.nsolve;
+    if (S.kind == SEMX_DIRECT) {
+      S.nblk = (S.n + nb - 1) / nb;
+      S.off_H = offH;
+      S.off_L = offL;
+      offH += (size_t)(S.nblk * nb + nb) * H.ldab;
+      offL += (size_t)S.nblk * nb * nb;
+    }
+  }
+  H.band_doubles = offH;
+  H.inv_doubles = offL;
+  H.Hr.assign(offH, 0.0);
+  H.Hc.assign(offH, 0.0);
+  H.Linv.assign(offL, 0.0);
+  H.LinvT.assign(offL, 0.0);
+  H.pc.assign((size_t)nmodes * H.npts, 0.0);
+
+  std::vector<int> status(nmodes, SEMX_OK);
+  auto work = [&](int m) {
+    ModeSys& S = H.modes[m];
+    // elemental matrices per geometric set
+    for (int s = 0; s < nset; ++s) {
+      const double* g = &M.G[(size_t)s * G_N * npnp];
+      const size_t ms = (size_t)m * nset + s;
+      const int st = condense(M, g, H.lambda2, S.betak2, &H.hbb[ms * next * next],
+                              &H.hbi[ms * next * std::max(nint, 1)],
+                              &H.hii[ms * std::max(nint * nint, 1)]);
+      if (st) { status[m] = st; return; }
+    }
+    if (S.kind == SEMX_DIRECT) {
+      const int n = S.n, kd = H.kd, ldab = H.ldab;
+      double* A = &H.Hr[S.off_H];
+      // assemble condensed boundary matrices into row band storage
+      for (int e = 0; e < M.nel; ++e) {
+        const double* hbb = &H.hbb[((size_t)m * nset + M.eset[e]) * next * next];
+        const int* bm = &M.btog[e * next];
+        for (int i = 0; i < next; ++i) {
+          const int r = bm[i];
+          if (r >= n) continue;
+          for (int k = 0; k < next; ++k) {
+            const int c = bm[k];
+            if (c >= n || c < r) continue;
+            A[(size_t)r * ldab + (c - r)] += hbb[i * next + k];
+          }
+        }
+      }
+      const int st = band_chol_rows(n, kd, ldab, A);
+      if (st) { status[m] = st; return; }
+      // column storage Hc[j*ldab + kd - (j - i)] = U(i, j)
+      double* C = &H.Hc[S.off_H];
+      for (int i = 0; i < n; ++i)
+        for (int d = 0; d <= kd && i + d < n; ++d)
+          C[(size_t)(i + d) * ldab + kd - d] = A[(size_t)i * ldab + d];
+      // diagonal-block inverses: Linv_J = (U_JJ^T)^-1, lower triangular
+      for (int J = 0; J < S.nblk; ++J) {
+        const int j0 = J * nb;
+        double* L = &H.Linv[S.off_L + (size_t)J * nb * nb];
+        double* LT = &H.LinvT[S.off_L + (size_t)J * nb * nb];
+        std::vector<double> T(nb * nb, 0.0);  // T = U_JJ^T (lower), identity pad
+        for (int a = 0; a < nb; ++a)
+          for (int b = 0; b <= a; ++b) {
+            const int r = j0 + b, c = j0 + a;
+            if (c < n) T[a * nb + b] = (c - r <= kd) ? A[(size_t)r * ldab + (c - r)] : 0.0;
+            else T[a * nb + b] = (a == b) ? 1.0 : 0.0;
+          }
+        for (int col = 0; col < nb; ++col)
+          for (int a = 0; a < nb; ++a) {
+            double s = (a == col) ? 1.0 : 0.0;
+            for (int b = col; b < a; ++b) s -= T[a * nb + b] * L[b * nb + col];
+            L[a * nb + col] = (a < col) ? 0.0 : s / T[a * nb + a];
+          }
+        for (int a = 0; a < nb; ++a)
+          for (int b = 0; b < nb; ++b) LT[b * nb + a] = L[a * nb + b];
+      }
+    } else {
+      // Jacobi preconditioner: inverse assembled diagonal (matrix.cpp:276-317)
+      double* PC = &H.pc[(size_t)m * H.npts];
+      // element diagonal (emap order) per geometric set: diagonal entries of
+      // the operator applied to unit vectors (src/element.cpp:1512-1561)
+      const double* dv = (H.lambda2 > EPSDP) ? M.ops.SDV.data() : M.ops.DV.data();
+      const double* dt = (H.lambda2 > EPSDP) ? M.ops.SDT.data() : M.ops.DT.data();
+      std::vector<double> dg((size_t)nset * npnp), ev(npnp), col(npnp);
+      for (int s = 0; s < nset; ++s) {
+        const double* g = &M.G[(size_t)s * G_N * npnp];
+        for (int c = 0; c < npnp; ++c) {
+          std::fill(ev.begin(), ev.end(), 0.0);
+          ev[M.ops.emap[c]] = 1.0;
+          elmt_apply(M.np, g, dv, dt, H.lambda2 + S.betak2, ev.data(), col.data());
+          dg[(size_t)s * npnp + c] = col[M.ops.emap[c]];
+        }
+      }
+      for (int e = 0; e < M.nel; ++e) {
+        const double* d = &dg[(size_t)M.eset[e] * npnp];
+        for (int c = 0; c < next; ++c) PC[M.btog[e * next + c]] += d[c];
+        for (int c = next; c < npnp; ++c) PC[M.nglobal + e * nint + (c - next)] = d[c];
+      }
+      for (int i = 0; i < H.npts; ++i) PC[i] = 1.0 / PC[i];
+    }
+  };
+  const int nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nth; ++t)
+    pool.emplace_back([&, t]() { for (int m = t; m < nmodes; m += nth) work(m); });
+  for (auto& th : pool) th.join();
+  for (int st : status) if (st) return st;
+  // plane list: every plane except the Nyquist plane 1 (field.cpp:566)
+  H.plane_of.clear();
+  for (int k = 0; k < H.nz; ++k)
+    if (!(H.nz > 1 && k == 1)) H.plane_of.push_back(k);
+  H.nplanes = (int)H.plane_of.size();
+  H.solstride = M.nsolve + 2;
+  return SEMX_OK;
+}
+
+}  // namespace semx
