@@ -77,11 +77,13 @@ struct ModeSys {
 struct HelmDev {
   double* hbi = nullptr;      // nmodes*nset*next*nint
   double* hii = nullptr;      // nmodes*nset*nint*nint
-  double* Hr = nullptr;       // row band storage, all modes
-  double* Hc = nullptr;       // column band storage, all modes
+  double* Pf = nullptr;       // forward-sweep panels, all modes
+  double* Pb = nullptr;       // backward-sweep panels, all modes
   double* Linv = nullptr;     // (U_JJ^T)^-1 blocks, row-major
   double* LinvT = nullptr;    // transposes
   int* mode_n = nullptr;      // per mode n
+  int* dmode = nullptr;       // direct modes, one band-solve workgroup each
+  int* mode_plane = nullptr;  // per mode: up to 2 plane-list indices (-1 none)
   long long* mode_offH = nullptr;
   long long* mode_offL = nullptr;
   double* pc = nullptr;       // JACPCG inverse diagonals, nmodes*npts
@@ -97,15 +99,18 @@ struct Helm {
   Mesh* M = nullptr;
   double lambda2 = 0.0, beta = 0.0;
   int base = 0, nmodes = 0, nz = 0, method = SEMX_DIRECT;
-  int nb = 32;                // band-solve block size
-  int kd = 0, ldab = 0;
+  int nb = 32;                // band-solve block size (columns per step)
+  int npass = 1, bt = 1024;   // band-solve row passes, threads per block
+  int parts = 1;              // lanes sharing one update row
+  int kd = 0, ldab = 0, kdp = 0;
   std::vector<ModeSys> modes;
   std::vector<double> hbb, hbi, hii;  // host, nmodes*nset*...
-  std::vector<double> Hr, Hc, Linv, LinvT;  // host, released after upload
+  std::vector<double> Pf, Pb, Linv, LinvT;  // host, released after upload
   std::vector<double> pc;                   // nmodes*npts
   size_t band_doubles = 0, inv_doubles = 0;
   int nplanes = 0, solstride = 0, npts = 0;
   std::vector<int> plane_of;
+  std::vector<int> dmode, mode_plane;
   bool on_device = false;
   HelmDev d;
 };

@@ -164,17 +164,24 @@ static void element_nodes(const Ops& o, const double* vx, const double* vy,
     }
 }
 
-static int element_factors(const Ops& o, const double* x, const double* y, double* g) {
+// Geometric factors of a straight-sided quad.  x(r,s) is the bilinear map
+// ax + bx r + cx s + dx r s of the vertices, so its partials are evaluated in
+// closed form (the quantities src/element.cpp:1281-1436 forms with the GLL
+// derivative matrix); axis-aligned rectangles then give exactly zero cross
+// terms.
+static int element_factors(const Ops& o, const double* vx, const double* vy, double* g) {
   const int np = o.np, npnp = np * np;
+  const double bx = 0.25 * ((vx[1] - vx[0]) + (vx[2] - vx[3]));
+  const double cx = 0.25 * ((vx[3] - vx[0]) + (vx[2] - vx[1]));
+  const double dx = 0.25 * ((vx[0] - vx[1]) + (vx[2] - vx[3]));
+  const double by = 0.25 * ((vy[1] - vy[0]) + (vy[2] - vy[3]));
+  const double cy = 0.25 * ((vy[3] - vy[0]) + (vy[2] - vy[1]));
+  const double dy = 0.25 * ((vy[0] - vy[1]) + (vy[2] - vy[3]));
   for (int i = 0; i < np; ++i)
     for (int j = 0; j < np; ++j) {
-      double xr = 0, xs = 0, yr = 0, ys = 0;
-      for (int k = 0; k < np; ++k) {
-        xr += o.DV[j * np + k] * x[i * np + k];
-        yr += o.DV[j * np + k] * y[i * np + k];
-        xs += o.DV[i * np + k] * x[k * np + j];
-        ys += o.DV[i * np + k] * y[k * np + j];
-      }
+      const double r = o.z[j], s = o.z[i];
+      const double xr = bx + dx * s, xs = cx + dx * r;
+      const double yr = by + dy * s, ys = cy + dy * r;
       const double jac = xr * ys - xs * yr;
       if (jac < 4 * EPSDP) return SEMX_E_JACOBIAN;
       const double W = o.w[i] * o.w[j];
@@ -402,18 +409,32 @@ static int number_mesh(Mesh& M, const int* nbr, const int* skind) {
     std::iota(natural.begin(), natural.end(), 0);
     cands.push_back(natural);
     for (int start : {0, nunk / 3, (2 * nunk) / 3, nunk - 1}) cands.push_back(rcm(G, start));
-    for (int axis = 0; axis < 2; ++axis) {
-      std::vector<int> p = natural;
-      const double eps = 1e-9;
-      std::stable_sort(p.begin(), p.end(), [&](int a, int b) {
+    // coordinate sorts (row by row), plain and folded (rows taken from both
+    // ends alternately, which keeps periodic wrap-around neighbours close)
+    for (int axis = 0; axis < 2; ++axis)
+      for (int fold = 0; fold < 2; ++fold) {
         const double* c1 = axis ? gx.data() : gy.data();
         const double* c2 = axis ? gy.data() : gx.data();
-        const int va = uold[a], vb = uold[b];
-        if (std::fabs(c1[va] - c1[vb]) > eps) return c1[va] < c1[vb];
-        return c2[va] < c2[vb];
-      });
-      cands.push_back(p);
-    }
+        double lo = 1e300, hi = -1e300;
+        for (int v : uold) { lo = std::min(lo, c1[v]); hi = std::max(hi, c1[v]); }
+        const double span = std::max(hi - lo, 1e-300);
+        const double eps = 1e-9 * std::max(1.0, span);
+        auto key = [&](int v, int& side) {
+          const double t = (c1[v] - lo) / span;
+          side = (t > 0.5 + 1e-12);
+          return fold ? std::min(t, 1.0 - t) * span : c1[v] - lo;
+        };
+        std::vector<int> p = natural;
+        std::stable_sort(p.begin(), p.end(), [&](int a, int b) {
+          const int va = uold[a], vb = uold[b];
+          int sa, sb;
+          const double ka = key(va, sa), kb = key(vb, sb);
+          if (std::fabs(ka - kb) > eps) return ka < kb;
+          if (fold && sa != sb) return sa < sb;
+          return c2[va] < c2[vb];
+        });
+        cands.push_back(p);
+      }
   } else
     cands.push_back({});
   int best = -1, bestband = INT32_MAX;
@@ -449,17 +470,11 @@ int mesh_build(Mesh& M, int np, int nel, const double* vx, const double* vy,
   M.x.assign((size_t)nel * npnp, 0.0);
   M.y.assign((size_t)nel * npnp, 0.0);
   std::vector<double> g((size_t)nel * G_N * npnp);
-  std::vector<double> xr(npnp), yr(npnp);
   for (int e = 0; e < nel; ++e) {
     element_nodes(M.ops, vx + 4 * e, vy + 4 * e, &M.x[e * npnp], &M.y[e * npnp]);
-    // Factors from coordinates relative to vertex 0: translated copies of
-    // one element shape then give equal factors and share one set.
-    const double rx[4] = {0.0, vx[4 * e + 1] - vx[4 * e], vx[4 * e + 2] - vx[4 * e],
-                          vx[4 * e + 3] - vx[4 * e]};
-    const double ry[4] = {0.0, vy[4 * e + 1] - vy[4 * e], vy[4 * e + 2] - vy[4 * e],
-                          vy[4 * e + 3] - vy[4 * e]};
-    element_nodes(M.ops, rx, ry, xr.data(), yr.data());
-    const int st = element_factors(M.ops, xr.data(), yr.data(), &g[(size_t)e * G_N * npnp]);
+    // Factors depend on vertex differences only: translated copies of one
+    // element shape give equal factors and share one set.
+    const int st = element_factors(M.ops, vx + 4 * e, vy + 4 * e, &g[(size_t)e * G_N * npnp]);
     if (st) return st;
   }
   // -- deduplicate geometric factor sets (femlib/family.c:171-192 semantics)
@@ -467,13 +482,23 @@ int mesh_build(Mesh& M, int np, int nel, const double* vx, const double* vy,
   M.G.clear(); M.setQ3.clear();
   M.nset = 0;
   std::vector<int> reps;
+  // Tolerance per factor array: 6e-14 relative to the array's magnitude
+  // (semtex compares within 6e-14 absolute, femlib/family.c:171-192, which
+  // separates translated copies whose inverse partials are O(1/h)).
+  std::vector<double> scale(G_N, 1.0);
+  for (int k = 0; k < G_N; ++k)
+    for (size_t q = 0; q < (size_t)nel * npnp; ++q) {
+      const size_t e = q / npnp, i = q % npnp;
+      scale[k] = std::max(scale[k], std::fabs(g[e * G_N * npnp + k * npnp + i]));
+    }
   for (int e = 0; e < nel; ++e) {
     const double* ge = &g[(size_t)e * G_N * npnp];
     int found = -1;
     for (int s = 0; s < M.nset && found < 0; ++s) {
       const double* gs = &g[(size_t)reps[s] * G_N * npnp];
       bool same = true;
-      for (int q = 0; q < G_N * npnp && same; ++q) same = std::fabs(ge[q] - gs[q]) <= EPSDP;
+      for (int q = 0; q < G_N * npnp && same; ++q)
+        same = std::fabs(ge[q] - gs[q]) <= EPSDP * scale[q / npnp];
       if (same) found = s;
     }
     if (found < 0) {
@@ -697,6 +722,18 @@ int helm_build(Helm& H) {
   H.hbb.assign((size_t)nmodes * nset * next * next, 0.0);
   H.hbi.assign((size_t)nmodes * nset * next * std::max(nint, 1), 0.0);
   H.hii.assign((size_t)nmodes * nset * std::max(nint * nint, 1), 0.0);
+  // Band-solve blocking: NB columns per step, the off-diagonal entries of a
+  // block stored as a zero-padded [NB][kdp] panel, kdp = kd rounded to 64.
+  {
+    const int kd = std::max(H.kd, 1);
+    // thread r owns update row r; bands wider than 512 use narrower blocks
+    // and several rows per thread
+    if (kd <= 512) { H.nb = 32; H.npass = 1; H.bt = 64; while (H.bt < kd) H.bt *= 2; }
+    else if (kd <= 1024) { H.nb = 16; H.npass = 2; H.bt = 512; }
+    else if (kd <= 2048) { H.nb = 8; H.npass = 4; H.bt = 512; }
+    else { H.nb = 8; H.npass = 0; H.bt = 0; }   // no direct solve for this band
+    H.kdp = (kd + 63) / 64 * 64;
+  }
   size_t offH = 0, offL = 0;
   const int nb = H.nb;
   for (int m = 0; m < nmodes; ++m) {
@@ -709,17 +746,18 @@ int helm_build(Helm& H) {
     S.singular = (H.lambda2 + S.betak2) < EPSSP && M.nglobal == M.nsolve;
     S.n = S.singular ? M.nsolve - 1 : M.nsolve;
     if (S.kind == SEMX_DIRECT) {
+      if (H.npass == 0) return SEMX_E_UNSUPPORTED;
       S.nblk = (S.n + nb - 1) / nb;
       S.off_H = offH;
       S.off_L = offL;
-      offH += (size_t)(S.nblk * nb + nb) * H.ldab;
+      offH += (size_t)S.nblk * nb * H.kdp;
       offL += (size_t)S.nblk * nb * nb;
     }
   }
   H.band_doubles = offH;
   H.inv_doubles = offL;
-  H.Hr.assign(offH, 0.0);
-  H.Hc.assign(offH, 0.0);
+  H.Pf.assign(offH, 0.0);
+  H.Pb.assign(offH, 0.0);
   H.Linv.assign(offL, 0.0);
   H.LinvT.assign(offL, 0.0);
   H.pc.assign((size_t)nmodes * H.npts, 0.0);
@@ -737,9 +775,11 @@ int helm_build(Helm& H) {
       if (st) { status[m] = st; return; }
     }
     if (S.kind == SEMX_DIRECT) {
-      const int n = S.n, kd = H.kd, ldab = H.ldab;
-      double* A = &H.Hr[S.off_H];
+      const int n = S.n, kd = H.kd, ldab = H.ldab, kdp = H.kdp;
+      std::vector<double> band((size_t)std::max(n, 1) * ldab, 0.0);
+      double* A = band.data();
       // assemble condensed boundary matrices into row band storage
+      // A[r*ldab + (c-r)] = A(r, c), c >= r (src/matrix.cpp:236-241)
       for (int e = 0; e < M.nel; ++e) {
         const double* hbb = &H.hbb[((size_t)m * nset + M.eset[e]) * next * next];
         const int* bm = &M.btog[e * next];
@@ -755,11 +795,23 @@ int helm_build(Helm& H) {
       }
       const int st = band_chol_rows(n, kd, ldab, A);
       if (st) { status[m] = st; return; }
-      // column storage Hc[j*ldab + kd - (j - i)] = U(i, j)
-      double* C = &H.Hc[S.off_H];
-      for (int i = 0; i < n; ++i)
-        for (int d = 0; d <= kd && i + d < n; ++d)
-          C[(size_t)(i + d) * ldab + kd - d] = A[(size_t)i * ldab + d];
+      // forward panels: Pf[J][jj][r] = U(j0+jj, j0+nb+r); backward panels:
+      // Pb[J][jj][r] = U(j0-1-r, j0+jj); zero outside the band
+      for (int J = 0; J < S.nblk; ++J) {
+        const int j0 = J * nb;
+        double* pf = &H.Pf[S.off_H + (size_t)J * nb * kdp];
+        double* pb = &H.Pb[S.off_H + (size_t)J * nb * kdp];
+        for (int jj = 0; jj < nb; ++jj) {
+          const int j = j0 + jj;
+          if (j >= n) continue;
+          for (int r = 0; r < kdp; ++r) {
+            const int df = nb + r - jj, i_f = j0 + nb + r;
+            if (df <= kd && i_f < n) pf[(size_t)jj * kdp + r] = A[(size_t)j * ldab + df];
+            const int db = jj + 1 + r, i_b = j0 - 1 - r;
+            if (db <= kd && i_b >= 0) pb[(size_t)jj * kdp + r] = A[(size_t)i_b * ldab + db];
+          }
+        }
+      }
       // diagonal-block inverses: Linv_J = (U_JJ^T)^-1, lower triangular
       for (int J = 0; J < S.nblk; ++J) {
         const int j0 = J * nb;
@@ -817,7 +869,17 @@ int helm_build(Helm& H) {
   for (int k = 0; k < H.nz; ++k)
     if (!(H.nz > 1 && k == 1)) H.plane_of.push_back(k);
   H.nplanes = (int)H.plane_of.size();
-  H.solstride = M.nsolve + 2;
+  H.solstride = M.nglobal + (M.nglobal % 2);
+  // band-solve work list: one workgroup per direct mode, solving every
+  // plane of that mode (Re and Im share the factor)
+  H.mode_plane.assign(2 * nmodes, -1);
+  for (int p = 0; p < H.nplanes; ++p) {
+    const int k = H.plane_of[p];
+    H.mode_plane[2 * (k >> 1) + (k & 1)] = p;
+  }
+  H.dmode.clear();
+  for (int m = 0; m < nmodes; ++m)
+    if (H.modes[m].kind == SEMX_DIRECT) H.dmode.push_back(m);
   return SEMX_OK;
 }
 

@@ -38,9 +38,7 @@ using namespace semx;
 
 namespace {
 
-constexpr int NB = 32;         // band-solve block size (columns per step)
-constexpr int BAND_THREADS = 1024;
-constexpr int ELT_THREADS = 256;  // element kernels: np*np <= 256
+constexpr int ELT_THREADS = 1024;  // element kernels: several elements per block
 
 struct ElemArgs {
   int np, npnp, next, nint, nel, planeSize, nline, nsolve, nzero, nset;
@@ -80,199 +78,350 @@ __global__ void k_ess_gather(const double* __restrict__ bc, const int* __restric
   g[(size_t)p * nzero + m] = bc[(size_t)k * nline + ess_src[m]];
 }
 
+// Elements of one plane per block: CE = blockDim / (np*np) elements, thread
+// t works on element t / npnp, node t % npnp.
 __global__ __launch_bounds__(ELT_THREADS) void k_condense(ElemArgs a) {
-  const int e = blockIdx.x, p = blockIdx.y, t = threadIdx.x;
+  extern __shared__ double esh[];
+  const int t = threadIdx.x, p = blockIdx.y;
   const int k = a.plane_of[p];
   const int np = a.np, npnp = a.npnp, next = a.next, nint = a.nint;
+  const int ce = ELT_THREADS / npnp;
+  const int el = t / npnp, pt = t - el * npnp;
+  const int e = blockIdx.x * ce + el;
+  const bool act = el < ce && e < a.nel;
   const int m = k >> 1;
   const double bk = a.beta * (a.base + m);
   const double hcon = a.lambda2 + bk * bk;
   const bool svv = a.lambda2 > EPSDP;
-  __shared__ double sD[ELT_THREADS], sT[ELT_THREADS], sU[ELT_THREADS];
-  __shared__ double sR[ELT_THREADS], sS[ELT_THREADS], sF[ELT_THREADS];
-  const int set = a.eset[e];
-  const double* g = a.G + (size_t)set * G_N * npnp;
+  double* sD = esh;                       // npnp
+  double* sT = sD + npnp;                 // npnp
+  double* sU = sT + npnp;                 // ce*npnp
+  double* sR = sU + ce * npnp;
+  double* sS = sR + ce * npnp;
+  double* sF = sS + ce * npnp;
   if (t < npnp) {
     sD[t] = a.ops[(svv ? 2 : 0) * npnp + t];
     sT[t] = a.ops[(svv ? 3 : 1) * npnp + t];
   }
+  const int set = act ? a.eset[e] : 0;
+  const double* g = a.G + (size_t)set * G_N * npnp;
   double F = 0.0;
-  if (t < npnp) F = g[G_Q4 * npnp + t] * a.f[(size_t)k * a.planeSize + (size_t)e * npnp + t];
-  if (a.emask[e]) {
-    if (t < npnp) sU[t] = 0.0;
+  if (act) F = g[G_Q4 * npnp + pt] * a.f[(size_t)k * a.planeSize + (size_t)e * npnp + pt];
+  const int em = act ? a.emask[e] : 0;
+  // -(M f + H g): the lifting H g only on elements with essential nodes
+  if (__syncthreads_or(em)) {
+    const int base = el * npnp;
+    if (act) sU[base + pt] = 0.0;
     __syncthreads();
-    if (t < next) {
-      const int gid = a.btog[e * next + t];
-      sU[a.emap[t]] = (gid >= a.nsolve) ? a.g[(size_t)p * a.nzero + (gid - a.nsolve)] : 0.0;
+    if (act && em && pt < next) {
+      const int gid = a.btog[e * next + pt];
+      sU[base + a.emap[pt]] =
+          (gid >= a.nsolve) ? a.g[(size_t)p * a.nzero + (gid - a.nsolve)] : 0.0;
     }
     __syncthreads();
-    const int i = t / np, j = t - (t / np) * np;
-    if (t < npnp) {
-      double r = 0.0, s = 0.0;
+    const int i = pt / np, j = pt - (pt / np) * np;
+    if (act) {
+      double r = 0.0, s2 = 0.0;
       for (int q = 0; q < np; ++q) {
-        r += sU[i * np + q] * sT[q * np + j];
-        s += sD[i * np + q] * sU[q * np + j];
+        r += sU[base + i * np + q] * sT[q * np + j];
+        s2 += sD[i * np + q] * sU[base + q * np + j];
       }
-      const double q1 = g[G_Q1 * npnp + t], q2 = g[G_Q2 * npnp + t], q3 = g[G_Q3 * npnp + t];
-      sR[t] = q1 * r + q3 * s;
-      sS[t] = q2 * s + q3 * r;
+      const double q1 = g[G_Q1 * npnp + pt], q2 = g[G_Q2 * npnp + pt], q3 = g[G_Q3 * npnp + pt];
+      sR[base + pt] = q1 * r + q3 * s2;
+      sS[base + pt] = q2 * s2 + q3 * r;
     }
     __syncthreads();
-    if (t < npnp) {
-      double T = g[G_Q4 * npnp + t] * sU[t] * hcon;
-      double s1 = 0.0, s2 = 0.0;
+    if (act && em) {
+      double T = g[G_Q4 * npnp + pt] * sU[base + pt] * hcon;
+      double s1 = 0.0, s3 = 0.0;
       for (int q = 0; q < np; ++q) {
-        s1 += sT[i * np + q] * sS[q * np + j];
-        s2 += sR[i * np + q] * sD[q * np + j];
+        s1 += sT[i * np + q] * sS[base + q * np + j];
+        s3 += sR[base + i * np + q] * sD[q * np + j];
       }
       T += s1;
-      T += s2;
+      T += s3;
       F += T;
     }
   }
-  if (t < npnp) sF[t] = -F;
+  if (act) sF[el * npnp + pt] = -F;
   __syncthreads();
-  const size_t pe = (size_t)p * a.nel + e;
-  if (t < nint) a.Fi[pe * nint + t] = sF[a.emap[next + t]];
-  if (t < next) {
-    const double* hbi = a.hbi + ((size_t)m * a.nset + set) * next * nint;
-    double s = 0.0;
-    for (int q = 0; q < nint; ++q) s += hbi[t * nint + q] * sF[a.emap[next + q]];
-    double fb = sF[a.emap[t]] - s;
-    const int nm = np - 1;
-    for (int c = a.nat_ptr[e]; c < a.nat_ptr[e + 1]; ++c) {
-      const int b = a.nat_b[c];
-      const int side = a.bside[b];
-      const int i = (t - side * nm + next) % next;
-      if (i <= nm) fb += a.bc[(size_t)k * a.nline + b * np + i] * a.barea[b * np + i];
+  // interior part out; boundary part condensed: F_b - hbi F_i, plus <h,w>
+  for (int idx = t; idx < ce * npnp; idx += ELT_THREADS) {
+    const int le = idx / npnp, q = idx - le * npnp;
+    const int ee = blockIdx.x * ce + le;
+    if (ee >= a.nel) continue;
+    const size_t pe = (size_t)p * a.nel + ee;
+    const double* Fe = sF + le * npnp;
+    if (q >= next) {
+      a.Fi[pe * nint + (q - next)] = Fe[a.emap[q]];
+    } else {
+      const int b = q;
+      const double* hbi = a.hbi + ((size_t)m * a.nset + a.eset[ee]) * next * nint;
+      double s0 = 0.0, s1 = 0.0;
+      int c = 0;
+      for (; c + 1 < nint; c += 2) {
+        s0 += hbi[b * nint + c] * Fe[a.emap[next + c]];
+        s1 += hbi[b * nint + c + 1] * Fe[a.emap[next + c + 1]];
+      }
+      if (c < nint) s0 += hbi[b * nint + c] * Fe[a.emap[next + c]];
+      double fb = Fe[a.emap[b]] - (s0 + s1);
+      const int nm = np - 1;
+      for (int cc = a.nat_ptr[ee]; cc < a.nat_ptr[ee + 1]; ++cc) {
+        const int bb = a.nat_b[cc];
+        const int side = a.bside[bb];
+        const int ii = (b - side * nm + next) % next;
+        if (ii <= nm) fb += a.bc[(size_t)k * a.nline + bb * np + ii] * a.barea[bb * np + ii];
+      }
+      a.Fb[pe * next + b] = fb;
     }
-    a.Fb[pe * next + t] = fb;
   }
 }
 
 struct BandArgs {
-  const double* Hr;
-  const double* Hc;
+  const double* Pf;         // forward panels
+  const double* Pb;         // backward panels
   const double* Linv;
   const double* LinvT;
   const int* mode_n;
-  const long long* offH;
+  const long long* offP;
   const long long* offL;
-  const int* plane_of;
+  const int* dmode;
+  const int* mode_plane;
   const int* asm_ptr;
   const int* asm_idx;
   const double* Fb;
   double* sol;
-  int solstride, kd, ldab, nel, next, ylen;
+  int solstride, kd, kdp, nel, next, ring;
 };
 
-__device__ inline double half_wave_sum(double v) {
-  // reduction over aligned groups of 32 lanes of a 64-wide wavefront
-  v += __shfl_xor(v, 16);
-  v += __shfl_xor(v, 8);
-  v += __shfl_xor(v, 4);
-  v += __shfl_xor(v, 2);
-  v += __shfl_xor(v, 1);
+template <int G>
+__device__ inline double group_sum(double v) {
+  // sum over aligned groups of G lanes (G a power of two <= 32)
+#pragma unroll
+  for (int off = G / 2; off >= 1; off /= 2) v += __shfl_xor(v, off);
   return v;
 }
 
-__global__ __launch_bounds__(BAND_THREADS) void k_band_solve(BandArgs a) {
-  extern __shared__ double ylds[];
-  const int p = blockIdx.x, tid = threadIdx.x;
-  const int k = a.plane_of[p];
-  const int m = k >> 1;
+// Condensed right-hand side of every plane: owner gather of the element-
+// boundary contributions (the direct-stiffness sum of
+// Element::local2globalSumSC), zero beyond the solved unknowns.
+__global__ void k_assemble(const double* __restrict__ Fb, const int* __restrict__ asm_ptr,
+                           const int* __restrict__ asm_idx, const int* __restrict__ plane_of,
+                           const int* __restrict__ mode_n, double* __restrict__ sol,
+                           int solstride, int nglobal, int nelnext) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = blockIdx.y;
+  if (i >= nglobal) return;
+  const int n = mode_n[plane_of[p] >> 1];
+  double v = 0.0;
+  if (i < n) {
+    const double* fb = Fb + (size_t)p * nelnext;
+    for (int k = asm_ptr[i]; k < asm_ptr[i + 1]; ++k) v += fb[asm_idx[k]];
+  }
+  sol[(size_t)p * solstride + i] = v;
+}
+
+// One sweep of the block-triangular band solve for NR right-hand sides.
+// Only a window of the right-hand side lives in LDS: a ring of W (power of
+// two >= kd + 2 NB) slots, index i in slot i & (W-1).  Entries are admitted
+// from global memory (the assembled RHS, then the forward result) as the
+// sweep reaches them, and each block's solved values go to global memory
+// and to a small LDS staging buffer.
+//   forward : y_J <- Linv_J y_J;  y[j0+NB+r] -= sum_jj Pf_J[jj][r] y[j0+jj]
+//   backward: x_J <- LinvT_J y_J; y[j0-1-r]  -= sum_jj Pb_J[jj][r] x[j0+jj]
+// Thread r owns update row r (NPASS rows per thread); the panel slice,
+// inverse slice and admitted entries of block s+D are loaded into
+// registers while block s is processed; two barriers per block.
+template <int NB, int NPASS, int BT, int NR, int D, bool FWD>
+__device__ inline void band_sweep(const double* __restrict__ P, const double* __restrict__ L,
+                                  double* const (&ring)[NR], double* const (&gy)[NR],
+                                  double* __restrict__ ybuf, int n, int nblk, int kd, int kdp,
+                                  int wm) {
+  constexpr int QPT = (NB * NB / BT > 0) ? NB * NB / BT : 1;  // diag products per thread
+  constexpr int G = NB / QPT;                                 // lanes per diag row
+  const int tid = threadIdx.x;
+  const int npad = nblk * NB;
+  const bool dact = tid < NB * G;
+  const int da = dact ? tid / G : 0, dg = dact ? tid % G : 0;
+  double pbuf[D + 1][NPASS][NB], lbuf[D + 1][QPT], abuf[D + 1][NR];
+
+  // first window: forward [0, min(npad, NB+kd)), backward [max(0,j0-kd), npad)
+  {
+    const int lo = FWD ? 0 : max(0, (nblk - 1) * NB - kd);
+    const int hi = FWD ? min(npad, NB + kd) : npad;
+    for (int i = lo + tid; i < hi; i += BT) {
+#pragma unroll
+      for (int c = 0; c < NR; ++c) ring[c][i & wm] = (i < n) ? gy[c][i] : 0.0;
+    }
+  }
+  // entries admitted at step s >= 1: forward [j0+kd, j0+kd+NB) below npad,
+  // backward [j0-kd, j0-kd+NB) from 0 up; thread t < NB owns one
+  auto admit_index = [&](int s) {
+    const int J = FWD ? s : nblk - 1 - s;
+    const int i = FWD ? J * NB + kd + tid : J * NB - kd + tid;
+    const bool ok = tid < NB && (FWD ? (i >= (NB + kd) && i < npad) : (i >= 0));
+    return ok ? i : -1;
+  };
+
+  auto load = [&](int s, double (&pp)[NPASS][NB], double (&ll)[QPT], double (&ad)[NR]) {
+    const int J = FWD ? s : nblk - 1 - s;
+    const double* pj = P + (size_t)J * NB * kdp;
+#pragma unroll
+    for (int q = 0; q < NPASS; ++q) {
+      const int r = min(tid + q * BT, kdp - 1);
+#pragma unroll
+      for (int jj = 0; jj < NB; ++jj) pp[q][jj] = pj[(size_t)jj * kdp + r];
+    }
+    const double* lj = L + (size_t)J * NB * NB + da * NB + dg * QPT;
+#pragma unroll
+    for (int q = 0; q < QPT; ++q) ll[q] = lj[q];
+    // unconditional load from a clamped index (the validity select is made
+    // where the value is used, so no branch splits the load stream)
+    const int ic = min(max(admit_index(s), 0), n - 1);
+#pragma unroll
+    for (int c = 0; c < NR; ++c) ad[c] = gy[c][ic];
+  };
+
+  auto step = [&](int s, double (&pp)[NPASS][NB], double (&ll)[QPT], double (&ad)[NR]) {
+    const int J = FWD ? s : nblk - 1 - s;
+    const int j0 = J * NB;
+    const int r0 = j0 & wm;  // blocks never wrap: W is a multiple of NB
+    const int ia = admit_index(s);
+    if (s > 0 && ia >= 0) {
+#pragma unroll
+      for (int c = 0; c < NR; ++c) ring[c][ia & wm] = (ia < n) ? ad[c] : 0.0;
+    }
+    __syncthreads();
+    double v[NR];
+#pragma unroll
+    for (int c = 0; c < NR; ++c) {
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < QPT; ++q) acc += ll[q] * ring[c][r0 + dg * QPT + q];
+      v[c] = group_sum<G>(acc);
+    }
+    if (dact && dg == 0) {
+#pragma unroll
+      for (int c = 0; c < NR; ++c) {
+        ybuf[c * NB + da] = v[c];
+        if (j0 + da < n) gy[c][j0 + da] = v[c];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NPASS; ++q) {
+      const int r = tid + q * BT;
+      const int i = FWD ? j0 + NB + r : j0 - 1 - r;
+      if (r < kd && (FWD ? i < n : i >= 0)) {
+        const int ri = i & wm;
+#pragma unroll
+        for (int c = 0; c < NR; ++c) {
+          double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll
+          for (int jj = 0; jj < NB; jj += 4) {
+            a0 += pp[q][jj] * ybuf[c * NB + jj];
+            a1 += pp[q][jj + 1] * ybuf[c * NB + jj + 1];
+            a2 += pp[q][jj + 2] * ybuf[c * NB + jj + 2];
+            a3 += pp[q][jj + 3] * ybuf[c * NB + jj + 3];
+          }
+          ring[c][ri] -= (a0 + a1) + (a2 + a3);
+        }
+      }
+    }
+  };
+
+  if (nblk <= 0) return;
+  // Prefetches are unconditional (the step index is clamped, so the last
+  // ones reload the final block): every path into a step then has the same
+  // number of loads in flight and waits only for its own block's data.
+#pragma unroll
+  for (int d = 0; d < D; ++d) load(min(d, nblk - 1), pbuf[d], lbuf[d], abuf[d]);
+  for (int s = 0; s < nblk; s += D + 1) {
+#pragma unroll
+    for (int u = 0; u <= D; ++u) {
+      if (s + u >= nblk) break;
+      const int nx = (u + D) % (D + 1);
+      load(min(s + u + D, nblk - 1), pbuf[nx], lbuf[nx], abuf[nx]);
+      step(s + u, pbuf[u], lbuf[u], abuf[u]);
+    }
+  }
+  // the backward sweep re-reads, from global memory, values that other
+  // threads of this workgroup wrote during the forward sweep
+  if (FWD) __threadfence();
+  __syncthreads();
+}
+
+template <int NB, int NPASS, int BT, int NR, int D>
+__global__ __launch_bounds__(BT) void k_band_solve(BandArgs a) {
+  extern __shared__ double lds[];
+  const int m = a.dmode[blockIdx.x];
   const int n = a.mode_n[m];
-  const int kd = a.kd, ldab = a.ldab;
   const int nblk = (n + NB - 1) / NB;
-  double* y = ylds + kd;  // y[-kd .. nblk*NB + kd + NB)
-  for (int i = tid; i < a.ylen; i += BAND_THREADS) ylds[i] = 0.0;
-  __syncthreads();
-  // condensed RHS: owner gather of the element-boundary contributions
-  const double* Fb = a.Fb + (size_t)p * a.nel * a.next;
-  for (int gI = tid; gI < n; gI += BAND_THREADS) {
-    double s = 0.0;
-    for (int c = a.asm_ptr[gI]; c < a.asm_ptr[gI + 1]; ++c) s += Fb[a.asm_idx[c]];
-    y[gI] = s;
-  }
-  const double* Hr = a.Hr + a.offH[m];
-  const double* Hc = a.Hc + a.offH[m];
-  const double* L = a.Linv + a.offL[m];
-  const double* LT = a.LinvT + a.offL[m];
-  const int ra = tid >> 5, rq = tid & 31;  // diagonal-block thread layout
-  // -- forward: U^T y = b
-  for (int J = 0; J < nblk; ++J) {
-    const int j0 = J * NB;
-    __syncthreads();
-    double v = L[(size_t)J * NB * NB + ra * NB + rq] * y[j0 + rq];
-    v = half_wave_sum(v);
-    __syncthreads();
-    if (rq == 0) y[j0 + ra] = v;
-    __syncthreads();
-    for (int r = tid; r < kd; r += BAND_THREADS) {
-      const int i = j0 + NB + r;
-      if (i >= n) break;
-      double acc = 0.0;
-      const double* h = Hr + (size_t)j0 * ldab + NB + r;
+  double* ring[NR];
+  double* gy[NR];
 #pragma unroll
-      for (int jj = 0; jj < NB; ++jj) {
-        const int d = NB + r - jj;
-        if (d <= kd) acc += h[(size_t)jj * ldab - jj] * y[j0 + jj];
-      }
-      y[i] -= acc;
-    }
+  for (int c = 0; c < NR; ++c) {
+    int p = a.mode_plane[2 * m + c];
+    // a mode with fewer planes than NR repeats its first plane; the
+    // duplicate sweeps identical data and writes identical values
+    if (p < 0) p = a.mode_plane[2 * m];
+    ring[c] = lds + (size_t)c * a.ring;
+    gy[c] = a.sol + (size_t)p * a.solstride;
   }
-  // -- backward: U x = y
-  for (int J = nblk - 1; J >= 0; --J) {
-    const int j0 = J * NB;
-    __syncthreads();
-    double v = LT[(size_t)J * NB * NB + ra * NB + rq] * y[j0 + rq];
-    v = half_wave_sum(v);
-    __syncthreads();
-    if (rq == 0) y[j0 + ra] = v;
-    __syncthreads();
-    for (int r = tid; r < kd; r += BAND_THREADS) {
-      const int i = j0 - 1 - r;
-      if (i < 0) break;
-      double acc = 0.0;
-      const double* h = Hc + (size_t)j0 * ldab + kd - 1 - r;
-#pragma unroll
-      for (int jj = 0; jj < NB; ++jj) {
-        if (jj + 1 + r <= kd) acc += h[(size_t)jj * ldab - jj] * y[j0 + jj];
-      }
-      y[i] -= acc;
-    }
-  }
-  __syncthreads();
-  double* sol = a.sol + (size_t)p * a.solstride;
-  for (int i = tid; i < n; i += BAND_THREADS) sol[i] = y[i];
+  double* ybuf = lds + (size_t)NR * a.ring;
+  band_sweep<NB, NPASS, BT, NR, D, true>(a.Pf + a.offP[m], a.Linv + a.offL[m], ring, gy,
+                                          ybuf, n, nblk, a.kd, a.kdp, a.ring - 1);
+  band_sweep<NB, NPASS, BT, NR, D, false>(a.Pb + a.offP[m], a.LinvT + a.offL[m], ring, gy,
+                                           ybuf, n, nblk, a.kd, a.kdp, a.ring - 1);
 }
 
 __global__ __launch_bounds__(ELT_THREADS) void k_backsub(ElemArgs a) {
-  const int e = blockIdx.x, p = blockIdx.y, t = threadIdx.x;
+  extern __shared__ double esh[];
+  const int t = threadIdx.x, p = blockIdx.y;
   const int k = a.plane_of[p];
   const int next = a.next, nint = a.nint, npnp = a.npnp;
+  const int ce = ELT_THREADS / npnp;
   const int m = k >> 1;
   const int n = a.mode_n[m];
-  __shared__ double sub[ELT_THREADS], sFi[ELT_THREADS];
-  const size_t pe = (size_t)p * a.nel + e;
-  double* out = a.u + (size_t)k * a.planeSize + (size_t)e * npnp;
-  if (t < next) {
-    const int gid = a.btog[e * next + t];
-    const double ub = (gid < n) ? a.sol[(size_t)p * a.solstride + gid] : 0.0;
-    sub[t] = ub;
-    out[a.emap[t]] = (gid < n) ? ub
-                   : (gid >= a.nsolve ? a.g[(size_t)p * a.nzero + (gid - a.nsolve)] : 0.0);
+  double* sub = esh;                 // ce*next
+  double* sFi = sub + ce * next;     // ce*nint
+  for (int idx = t; idx < ce * npnp; idx += ELT_THREADS) {
+    const int le = idx / npnp, q = idx - le * npnp;
+    const int ee = blockIdx.x * ce + le;
+    if (ee >= a.nel) continue;
+    const size_t pe = (size_t)p * a.nel + ee;
+    if (q < next) {
+      const int gid = a.btog[ee * next + q];
+      const double ub = (gid < n) ? a.sol[(size_t)p * a.solstride + gid] : 0.0;
+      sub[le * next + q] = ub;
+      a.u[(size_t)k * a.planeSize + (size_t)ee * npnp + a.emap[q]] =
+          (gid < n) ? ub : (gid >= a.nsolve ? a.g[(size_t)p * a.nzero + (gid - a.nsolve)] : 0.0);
+    } else {
+      sFi[le * nint + (q - next)] = a.Fi[pe * nint + (q - next)];
+    }
   }
-  if (t < nint) sFi[t] = a.Fi[pe * nint + t];
   __syncthreads();
-  if (t < nint) {
-    const int set = a.eset[e];
+  for (int idx = t; idx < ce * nint; idx += ELT_THREADS) {
+    const int le = idx / nint, q = idx - le * nint;
+    const int ee = blockIdx.x * ce + le;
+    if (ee >= a.nel) continue;
+    const int set = a.eset[ee];
     const double* hii = a.hii + ((size_t)m * a.nset + set) * nint * nint;
     const double* hbi = a.hbi + ((size_t)m * a.nset + set) * next * nint;
-    double s1 = 0.0, s2 = 0.0;
-    for (int c = 0; c < nint; ++c) s1 += hii[t * nint + c] * sFi[c];
-    for (int b = 0; b < next; ++b) s2 += hbi[b * nint + t] * sub[b];
-    out[a.emap[next + t]] = s1 - s2;
+    const double* fi = sFi + le * nint;
+    const double* ub = sub + le * next;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int c = 0;
+    for (; c + 1 < nint; c += 2) {
+      s0 += hii[q * nint + c] * fi[c];
+      s1 += hii[q * nint + c + 1] * fi[c + 1];
+    }
+    if (c < nint) s0 += hii[q * nint + c] * fi[c];
+    for (int b = 0; b + 1 < next; b += 2) {
+      s2 += hbi[b * nint + q] * ub[b];
+      s3 += hbi[(b + 1) * nint + q] * ub[b + 1];
+    }
+    a.u[(size_t)k * a.planeSize + (size_t)ee * npnp + a.emap[next + q]] = (s0 + s1) - (s2 + s3);
   }
 }
 
@@ -382,8 +531,10 @@ int helm_upload(Helm& H) {
   }
   if ((st = dev_upload(&H.d.hbi, H.hbi.data(), H.hbi.size()))) return st;
   if ((st = dev_upload(&H.d.hii, H.hii.data(), H.hii.size()))) return st;
-  if ((st = dev_upload(&H.d.Hr, H.Hr.data(), H.Hr.size()))) return st;
-  if ((st = dev_upload(&H.d.Hc, H.Hc.data(), H.Hc.size()))) return st;
+  if ((st = dev_upload(&H.d.Pf, H.Pf.data(), H.Pf.size()))) return st;
+  if ((st = dev_upload(&H.d.Pb, H.Pb.data(), H.Pb.size()))) return st;
+  if ((st = dev_upload(&H.d.dmode, H.dmode.data(), H.dmode.size()))) return st;
+  if ((st = dev_upload(&H.d.mode_plane, H.mode_plane.data(), H.mode_plane.size()))) return st;
   if ((st = dev_upload(&H.d.Linv, H.Linv.data(), H.Linv.size()))) return st;
   if ((st = dev_upload(&H.d.LinvT, H.LinvT.data(), H.LinvT.size()))) return st;
   if ((st = dev_upload(&H.d.mode_n, mn.data(), mn.size()))) return st;
@@ -398,8 +549,8 @@ int helm_upload(Helm& H) {
   if ((st = dev_upload<double>(&H.d.g, nullptr, np_ * (nzero ? nzero : 1)))) return st;
   if ((st = dev_upload<double>(&H.d.sol, nullptr, np_ * H.solstride))) return st;
   // host band copies are no longer needed
-  std::vector<double>().swap(H.Hr);
-  std::vector<double>().swap(H.Hc);
+  std::vector<double>().swap(H.Pf);
+  std::vector<double>().swap(H.Pb);
   std::vector<double>().swap(H.Linv);
   std::vector<double>().swap(H.LinvT);
   H.on_device = true;
@@ -407,7 +558,8 @@ int helm_upload(Helm& H) {
 }
 
 void helm_release(Helm& H) {
-  dev_free(H.d.hbi); dev_free(H.d.hii); dev_free(H.d.Hr); dev_free(H.d.Hc);
+  dev_free(H.d.hbi); dev_free(H.d.hii); dev_free(H.d.Pf); dev_free(H.d.Pb);
+  dev_free(H.d.dmode); dev_free(H.d.mode_plane);
   dev_free(H.d.Linv); dev_free(H.d.LinvT); dev_free(H.d.mode_n); dev_free(H.d.mode_offH);
   dev_free(H.d.mode_offL); dev_free(H.d.pc); dev_free(H.d.Fb); dev_free(H.d.Fi);
   dev_free(H.d.g); dev_free(H.d.sol); dev_free(H.d.plane_of);
@@ -520,7 +672,7 @@ int semx_helm_create(semx_mesh mesh, double lambda2, double beta, int base_mode,
   Helm& H = h->H;
   H.M = &mesh->M;
   H.lambda2 = lambda2; H.beta = beta; H.base = base_mode; H.nmodes = nmodes;
-  H.nz = nz; H.method = method; H.nb = NB;
+  H.nz = nz; H.method = method;
   const int st = helm_build(H);
   if (st) { delete h; return st; }
   *helm = h;
@@ -593,29 +745,60 @@ int semx_helm_solve(semx_helm helm, double* f, const double* bc, double* u, doub
   ea.bside = M.d.bside; ea.barea = M.d.barea; ea.hbi = H.d.hbi; ea.hii = H.d.hii;
   ea.plane_of = H.d.plane_of; ea.mode_n = H.d.mode_n; ea.Fb = H.d.Fb; ea.Fi = H.d.Fi;
   ea.g = H.d.g; ea.sol = H.d.sol; ea.solstride = H.solstride; ea.f = f; ea.bc = bc; ea.u = u;
-  dim3 eg(M.nel, H.nplanes);
+  const int ce = ELT_THREADS / M.npnp;
+  dim3 eg((M.nel + ce - 1) / ce, H.nplanes);
+  const size_t lds_c = (2 * (size_t)M.npnp + 4 * (size_t)ce * M.npnp) * sizeof(double);
+  const size_t lds_b = (size_t)ce * (M.next + M.nint) * sizeof(double);
+  if (lds_c > 64 * 1024 || lds_b > 64 * 1024) {
+    HIPCHK(hipFuncSetAttribute((const void*)k_condense,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_c));
+    HIPCHK(hipFuncSetAttribute((const void*)k_backsub,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b));
+  }
   hipEvent_t ev;
   prof_begin(s, &ev);
-  hipLaunchKernelGGL(k_condense, eg, dim3(ELT_THREADS), 0, s, ea);
+  hipLaunchKernelGGL(k_condense, eg, dim3(ELT_THREADS), lds_c, s, ea);
   prof_end(s, "k_condense", ev);
-  BandArgs ba;
-  ba.Hr = H.d.Hr; ba.Hc = H.d.Hc; ba.Linv = H.d.Linv; ba.LinvT = H.d.LinvT;
-  ba.mode_n = H.d.mode_n; ba.offH = H.d.mode_offH; ba.offL = H.d.mode_offL;
-  ba.plane_of = H.d.plane_of; ba.asm_ptr = M.d.asm_ptr; ba.asm_idx = M.d.asm_idx;
-  ba.Fb = H.d.Fb; ba.sol = H.d.sol; ba.solstride = H.solstride; ba.kd = H.kd;
-  ba.ldab = H.ldab; ba.nel = M.nel; ba.next = M.next;
-  int nblk_max = 0;
-  for (const ModeSys& S : H.modes) nblk_max = S.nblk > nblk_max ? S.nblk : nblk_max;
-  ba.ylen = H.kd + nblk_max * NB + H.kd + NB;
-  const size_t lds = (size_t)ba.ylen * sizeof(double);
-  if (lds > 160 * 1024) return SEMX_E_UNSUPPORTED;
-  HIPCHK(hipFuncSetAttribute((const void*)k_band_solve,
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  if (!H.dmode.empty()) {
+    prof_begin(s, &ev);
+    hipLaunchKernelGGL(k_assemble, dim3((M.nglobal + 255) / 256, H.nplanes), dim3(256), 0, s,
+                       H.d.Fb, M.d.asm_ptr, M.d.asm_idx, H.d.plane_of, H.d.mode_n, H.d.sol,
+                       H.solstride, M.nglobal, M.nel * M.next);
+    prof_end(s, "k_assemble", ev);
+    BandArgs ba;
+    ba.Pf = H.d.Pf; ba.Pb = H.d.Pb; ba.Linv = H.d.Linv; ba.LinvT = H.d.LinvT;
+    ba.mode_n = H.d.mode_n; ba.offP = H.d.mode_offH; ba.offL = H.d.mode_offL;
+    ba.dmode = H.d.dmode; ba.mode_plane = H.d.mode_plane;
+    ba.asm_ptr = M.d.asm_ptr; ba.asm_idx = M.d.asm_idx;
+    ba.Fb = H.d.Fb; ba.sol = H.d.sol; ba.solstride = H.solstride; ba.kd = H.kd;
+    ba.kdp = H.kdp; ba.nel = M.nel; ba.next = M.next;
+    ba.ring = 64;
+    while (ba.ring < H.kd + 2 * H.nb) ba.ring *= 2;
+    int nr = 1;
+    for (int m : H.dmode) if (H.mode_plane[2 * m + 1] >= 0) nr = 2;
+    const size_t lds = (size_t)nr * (ba.ring + H.nb) * sizeof(double);
+    if (lds > 160 * 1024) return SEMX_E_UNSUPPORTED;
+    const void* kfn = nullptr;
+#define SEMX_PICK(NB_, NP_, BT_, D_)                                               \
+    if (H.nb == NB_ && H.npass == NP_ && H.bt == BT_)                              \
+      kfn = (nr == 2) ? (const void*)k_band_solve<NB_, NP_, BT_, 2, D_>            \
+                      : (const void*)k_band_solve<NB_, NP_, BT_, 1, D_>;
+    SEMX_PICK(32, 1, 64, 1)
+    SEMX_PICK(32, 1, 128, 1)
+    SEMX_PICK(32, 1, 256, 1)
+    SEMX_PICK(32, 1, 512, 1)
+    SEMX_PICK(16, 2, 512, 1)
+    SEMX_PICK(8, 4, 512, 1)
+#undef SEMX_PICK
+    if (!kfn) return SEMX_E_UNSUPPORTED;
+    HIPCHK(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    void* kargs[] = {&ba};
+    prof_begin(s, &ev);
+    HIPCHK(hipLaunchKernel(kfn, dim3((unsigned)H.dmode.size()), dim3(H.bt), kargs, lds, s));
+    prof_end(s, "k_band_solve", ev);
+  }
   prof_begin(s, &ev);
-  hipLaunchKernelGGL(k_band_solve, dim3(H.nplanes), dim3(BAND_THREADS), lds, s, ba);
-  prof_end(s, "k_band_solve", ev);
-  prof_begin(s, &ev);
-  hipLaunchKernelGGL(k_backsub, eg, dim3(ELT_THREADS), 0, s, ea);
+  hipLaunchKernelGGL(k_backsub, eg, dim3(ELT_THREADS), lds_b, s, ea);
   prof_end(s, "k_backsub", ev);
   HIPCHK(hipGetLastError());
   return SEMX_OK;

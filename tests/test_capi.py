@@ -1,0 +1,156 @@
+"""C ABI and host-side setup of libsemx.so, checked on the CPU (no GPU calls).
+
+- the library loads and exports every entry point include/semx.h declares;
+- the mesh numbering satisfies semtex's invariants (src/bcmgr.cpp:586-824,
+  utility/enumerate.cpp:621-943);
+- geometry and elemental static condensation agree with the oracle;
+- error paths return status codes where semtex would exit.
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import semtex_amd as sx
+from semtex_amd import _capi
+from semtex_amd.mesh import rect_mesh
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", fn)).read()
+            names |= set(re.findall(r"^\w[\w\s\*]*?\b(semx_\w+)\s*\(", text, re.M))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = sx.load()
+    names = declared_symbols()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    # and the Python prototypes cover the same set
+    assert names <= set(_capi.PROTOTYPES), names - set(_capi.PROTOTYPES)
+
+
+def test_error_strings():
+    L = sx.load()
+    assert L.semx_error_string(0) == b"ok"
+    assert b"SPD" in L.semx_error_string(2)
+
+
+MESHES = [
+    ("dirichlet", 9, dict(nx=16, ny=16)),
+    ("mixed-distorted", 6, dict(nx=4, ny=3, bc=("D", "N", "D", "N"), distort=0.3)),
+    ("periodic", 7, dict(nx=6, ny=4, periodic_x=True, periodic_y=True)),
+    ("periodic-x", 5, dict(nx=5, ny=3, periodic_x=True, bc=("D", "D", "N", "D"))),
+]
+
+
+@pytest.mark.parametrize("name,np_,kw", MESHES, ids=[m[0] for m in MESHES])
+def test_numbering_invariants(name, np_, kw):
+    m = rect_mesh(**kw)
+    P = sx.Mesh(np_, m)
+    Q = O.Mesh(np_, m)
+    # same counts as the oracle's independent numbering
+    assert (P.nglobal, P.nsolve, P.nbound) == (Q.nglobal, Q.nsolve, Q.nbound)
+    assert np.allclose(P.x, Q.x, atol=1e-14) and np.allclose(P.y, Q.y, atol=1e-14)
+    btog = P.btog.reshape(P.nel, P.next)
+    bmask = P.bmask.reshape(P.nel, P.next)
+    # every global node used; essential nodes exactly the last ones
+    assert set(btog.ravel()) == set(range(P.nglobal))
+    assert np.all((btog >= P.nsolve) == (bmask == 1))
+    assert np.all(P.emask == bmask.any(1))
+    # band width (incl. diagonal) as utility/enumerate.cpp:903-943 defines it
+    nb = 0
+    for e in range(P.nel):
+        s = btog[e][bmask[e] == 0]
+        if s.size:
+            nb = max(nb, s.max() - s.min())
+    assert P.nband == nb + 1
+    # nodes sharing a global number coincide (modulo periodic shifts)
+    emap, _ = O.edgemaps(np_)
+    x = P.x.reshape(P.nel, -1)[:, emap[:P.next]]
+    y = P.y.reshape(P.nel, -1)[:, emap[:P.next]]
+    Lx = 1.0 if kw.get("periodic_x") else None
+    Ly = 1.0 if kw.get("periodic_y") else None
+    for g in range(P.nglobal):
+        pts = np.argwhere(btog == g)
+        xs = np.array([x[e, b] for e, b in pts])
+        ys = np.array([y[e, b] for e, b in pts])
+        dx = xs - xs[0]
+        dy = ys - ys[0]
+        if Lx: dx = dx - np.round(dx / Lx) * Lx
+        if Ly: dy = dy - np.round(dy / Ly) * Ly
+        assert np.abs(dx).max() < 1e-12 and np.abs(dy).max() < 1e-12
+
+
+def test_uniform_mesh_shares_one_geometric_set():
+    for np_, nx in ((9, 16), (11, 32), (8, 8)):
+        P = sx.Mesh(np_, rect_mesh(nx, nx, 0, 2 * np.pi, 0, 2 * np.pi))
+        assert P.nsets == 1
+    assert sx.Mesh(6, rect_mesh(3, 3, distort=0.3)).nsets > 1
+
+
+@pytest.mark.parametrize("lam,beta", [(3.0, 1.0), (0.0, 2.0), (20000.0, 1.0)])
+def test_element_condensation_matches_oracle(lam, beta):
+    np_ = 7
+    m = rect_mesh(3, 3, distort=0.25)
+    P = sx.Mesh(np_, m)
+    Q = O.Mesh(np_, m)
+    S = sx.ModalMatrixSys(P, lam, beta, 0, 3, 6, sx.DIRECT)
+    for mode in (0, 2):
+        bk2 = (beta * mode) ** 2
+        for e in (0, 4, 8):
+            hbb, hbi, hii = S.element(mode, e)
+            ob = np.zeros((P.next, P.next))
+            oi = np.zeros((P.next, P.nint))
+            oii = np.zeros((P.nint, P.nint))
+            O.lib().orc_helm_sc(C.c_void_p(Q.h), e, C.c_double(lam), C.c_double(bk2),
+                                O.dp(ob), O.dp(oi), O.dp(oii))
+            for a, b in ((hbb, ob), (hbi, oi), (hii, oii)):
+                assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
+
+
+def test_singular_system_is_pinned():
+    P = sx.Mesh(6, rect_mesh(4, 4, periodic_x=True, periodic_y=True))
+    S = sx.ModalMatrixSys(P, 0.0, 1.0, 0, 2, 4, sx.DIRECT)
+    assert S.info["singular0"] == 1 and S.info["nsolve0"] == P.nsolve - 1
+    S2 = sx.ModalMatrixSys(P, 1.0, 1.0, 0, 2, 4, sx.DIRECT)
+    assert S2.info["singular0"] == 0
+
+
+def test_error_paths():
+    L = sx.load()
+    m = rect_mesh(2, 2)
+    h = C.c_void_p()
+    vx = np.ascontiguousarray(m["vx"]); vy = np.ascontiguousarray(m["vy"])
+    nbr = np.ascontiguousarray(m["nbr"], dtype=np.int32)
+    sk = np.ascontiguousarray(m["skind"], dtype=np.int32)
+    D, I = _capi.D, _capi.I
+    # np out of range
+    assert L.semx_mesh_create(2, 4, vx.ctypes.data_as(D), vy.ctypes.data_as(D),
+                              nbr.ctypes.data_as(I), sk.ctypes.data_as(I), -1, 0.0, 1.0,
+                              C.byref(h)) == 1
+    # inverted element: nonpositive Jacobian (src/element.cpp:1358-1361)
+    bad = vx.copy(); bad[0, [1, 2]] = bad[0, [0, 3]] - 0.25
+    assert L.semx_mesh_create(5, 4, bad.ctypes.data_as(D), vy.ctypes.data_as(D),
+                              nbr.ctypes.data_as(I), sk.ctypes.data_as(I), -1, 0.0, 1.0,
+                              C.byref(h)) == 6
+    # inconsistent connectivity
+    nb2 = nbr.copy(); nb2[0, 1] = 0
+    assert L.semx_mesh_create(5, 4, vx.ctypes.data_as(D), vy.ctypes.data_as(D),
+                              nb2.ctypes.data_as(I), sk.ctypes.data_as(I), -1, 0.0, 1.0,
+                              C.byref(h)) == 7
+    P = sx.Mesh(5, m)
+    # nz inconsistent with the mode count
+    assert L.semx_helm_create(P.h, 1.0, 1.0, 0, 3, 4, 0, C.byref(h)) == 1
+    with pytest.raises(sx.SemxError):
+        sx.ModalMatrixSys(P, 1.0, 1.0, 0, 2, 4, 7)
