@@ -48,6 +48,10 @@ extern "C" {
 #define SEMX_JACPCG  1        /* Jacobi-preconditioned CG                   */
 #define SEMX_MIXED   2        /* mode 0 DIRECT, other modes JACPCG          */
 
+/* --- JACPCG iteration precision (semx_helm_set_pcg) ------------------- */
+#define SEMX_PCG_FP64          0   /* FP64 vectors, semtex's loop exactly    */
+#define SEMX_PCG_FP32_REFINED  1   /* FP32 inner PCG, FP64 refinement        */
+
 /* --- transform direction (femlib/cfemdef.h TransformKind) --------------- */
 #define SEMX_FORWARD  1
 #define SEMX_INVERSE -1
@@ -122,10 +126,22 @@ int semx_helm_element (semx_helm helm, int m, int elmt, double* hbb,
  * holds the initial guess (used by JACPCG) and receives the solution;
  * bc (device, nz*nbound*np) holds the boundary values in Fourier space.
  * tol_rel, step_max: TOL_REL, STEP_MAX for JACPCG (field.cpp:649-706).
- * iters (host, may be NULL) receives the largest PCG iteration count.
- * Returns SEMX_E_MAXITER when a PCG plane hit step_max (semtex: WARNING). */
+ * The call is asynchronous on stream unless iters (host) is non-NULL: then
+ * it waits for the stream and stores the largest PCG iteration count
+ * (summed over inner solves with SEMX_PCG_FP32_REFINED), and returns
+ * SEMX_E_MAXITER when a PCG plane hit its limit (semtex prints a WARNING,
+ * field.cpp:713, and keeps the last iterate; so does this call).          */
 int semx_helm_solve (semx_helm helm, double* f, const double* bc, double* u,
                      double tol_rel, int step_max, int* iters, void* stream);
+
+/* Precision of the JACPCG planes of a system.  SEMX_PCG_FP64 (the default)
+ * is the FP64 loop of Field::solve (src/field.cpp:627-720).
+ * SEMX_PCG_FP32_REFINED has no counterpart in semtex: the residual
+ * r = b - A x is formed in FP64, A d = r is solved by FP32 Jacobi-PCG to
+ * relative residual inner_tol, x += d, repeated (at most max_outer times)
+ * until |r| <= tol_rel |b| in FP64 -- the same stopping test as FP64.     */
+int semx_helm_set_pcg (semx_helm helm, int precision, double inner_tol,
+                       int max_outer);
 
 /* --- kernel timing -------------------------------------------------------
  * No counterpart in semtex (dclock(), veclib/util.c:65-85, is never called

@@ -1042,7 +1042,7 @@ void orc_helm_global (const orc_sys* S, const double* x, double* y)
 /* src/field.cpp:771-856 (Field::HelmholtzOperator), scalar branch.      */
 {
   const orc_mesh* M = S->M;
-  const int nel = M->nel, npnp = M->npnp, next = M->next, nint = M->nint;
+  const int nel = M->nel, next = M->next, nint = M->nint;
   double P[32*32];
   int e, i;
   memset (y, 0, S->npts * sizeof (double));
@@ -1190,12 +1190,13 @@ int orc_solve_plane (const orc_sys* S, double* forcing, const double* bc,
   }
 }
 
-int orc_field_solve (const orc_mesh* M, int nz, double lambda2, double beta,
-                     int method, double* f, const double* bc, double* u,
-                     double tol_rel, int step_max, int nthreads, int nrep,
-                     double* t_solve)
+static int field_solve_planes (const orc_mesh* M, int nz, double lambda2, double beta,
+                        int method, double* f, const double* bc, double* u,
+                        double tol_rel, int step_max, int nthreads, int nrep,
+                        double* t_solve, int* iters)
 /* Field::solve over planes (field.cpp:564-723) with one MatrixSys per
- * mode (matrix.cpp:33-110; SVV z-weights are 1 with SVV off).            */
+ * mode (matrix.cpp:33-110; SVV z-weights are 1 with SVV off).  iters (may
+ * be NULL) receives each plane's PCG iteration count, -1 at the limit.   */
 {
   const int nmodes = (nz > 1) ? nz / 2 : 1;
   const int nline = M->nbound * M->np;
@@ -1206,8 +1207,10 @@ int orc_field_solve (const orc_mesh* M, int nz, double lambda2, double beta,
   if (nthreads > 0) omp_set_num_threads (nthreads);
 #pragma omp parallel for schedule(dynamic, 1)
   for (m = 0; m < nmodes; m++) {
+    /* MIXED: DIRECT for mode 0, JACPCG otherwise (matrix.cpp:93-96) */
+    const int kind = (method == 2) ? (m == 0 ? ORC_DIRECT : ORC_JACPCG) : method;
     const double bk = beta * m;
-    sys[m] = orc_sys_create (M, lambda2, bk * bk, method);
+    sys[m] = orc_sys_create (M, lambda2, bk * bk, kind);
   }
   {
     /* repeated solves restore the (destroyed) forcing from a copy, outside
@@ -1223,10 +1226,13 @@ int orc_field_solve (const orc_mesh* M, int nz, double lambda2, double beta,
       t0 = omp_get_wtime ();
 #pragma omp parallel for schedule(dynamic, 1) reduction(+:warn)
       for (k = 0; k < nz; k++) {
+        int it;
         if (nz > 1 && k == 1) continue;
-        if (orc_solve_plane (sys[k >> 1], f + (size_t) k * M->planeSize,
-                             bc + (size_t) k * nline, u + (size_t) k * M->planeSize,
-                             tol_rel, step_max) < 0) warn++;
+        it = orc_solve_plane (sys[k >> 1], f + (size_t) k * M->planeSize,
+                              bc + (size_t) k * nline, u + (size_t) k * M->planeSize,
+                              tol_rel, step_max);
+        if (it < 0) warn++;
+        if (iters) iters[k] = it;
       }
       tsum += omp_get_wtime () - t0;
     }
@@ -1236,6 +1242,23 @@ int orc_field_solve (const orc_mesh* M, int nz, double lambda2, double beta,
   for (m = 0; m < nmodes; m++) orc_sys_destroy (sys[m]);
   free (sys);
   return warn;
+}
+
+int orc_field_solve (const orc_mesh* M, int nz, double lambda2, double beta,
+                     int method, double* f, const double* bc, double* u,
+                     double tol_rel, int step_max, int nthreads, int nrep,
+                     double* t_solve)
+{
+  return field_solve_planes (M, nz, lambda2, beta, method, f, bc, u, tol_rel, step_max,
+                      nthreads, nrep, t_solve, NULL);
+}
+
+int orc_field_solve_iters (const orc_mesh* M, int nz, double lambda2, double beta,
+                           int method, double* f, const double* bc, double* u,
+                           double tol_rel, int step_max, int* iters)
+{
+  return field_solve_planes (M, nz, lambda2, beta, method, f, bc, u, tol_rel, step_max,
+                      0, 1, NULL, iters);
 }
 
 /* ======================================================================== *

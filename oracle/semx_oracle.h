@@ -103,6 +103,11 @@ int orc_field_solve (const orc_mesh* M, int nz, double lambda2, double beta,
                      int method, double* f, const double* bc, double* u,
                      double tol_rel, int step_max, int nthreads, int nrep,
                      double* t_solve);
+/* The same, one repetition, with each plane's PCG iteration count in
+ * iters[nz] (-1: step limit; plane 1 untouched when nz > 1).              */
+int orc_field_solve_iters (const orc_mesh* M, int nz, double lambda2, double beta,
+                           int method, double* f, const double* bc, double* u,
+                           double tol_rel, int step_max, int* iters);
 
 /* --- Fourier transform & field operators -------------------------------- */
 /* dDFTr (fourier.c:47-171) semantics: ntrn lines of tlen reals strided by

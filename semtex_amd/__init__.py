@@ -22,9 +22,12 @@ from ._capi import SemxError, check, load
 from .mesh import bc_lines, rect_mesh, side_nodes
 
 DIRECT, JACPCG, MIXED = 0, 1, 2
+PCG_FP64, PCG_FP32_REFINED = 0, 1
+E_MAXITER = 3
 FORWARD, INVERSE = 1, -1
 
 __all__ = ["Mesh", "ModalMatrixSys", "solve", "DIRECT", "JACPCG", "MIXED",
+           "PCG_FP64", "PCG_FP32_REFINED",
            "rect_mesh", "bc_lines", "side_nodes", "SemxError", "load"]
 
 
@@ -103,6 +106,12 @@ class ModalMatrixSys:
                          nsolve0=int(info[3]), ldab=int(info[4]), nb=int(info[5]),
                          factor_kB=int(info[6]), singular0=int(info[7]))
 
+    def set_pcg(self, precision=PCG_FP64, inner_tol=1e-5, max_outer=12):
+        """Precision of the JACPCG planes (FP64, or FP32 inner PCG under FP64
+        iterative refinement)."""
+        check(load().semx_helm_set_pcg(self.h, int(precision), float(inner_tol),
+                                       int(max_outer)), "semx_helm_set_pcg")
+
     def element(self, m, e):
         M = self.mesh
         hbb = np.zeros((M.next, M.next))
@@ -128,10 +137,12 @@ def _ptr(t):
     return C.c_void_p(t.data_ptr())
 
 
-def solve(msys, f, bc, u, tol_rel=1e-12, step_max=10000, stream=None):
+def solve(msys, f, bc, u, tol_rel=1e-12, step_max=10000, stream=None, iters=True):
     """Field::solve over all planes: f (destroyed), bc, u are device tensors
-    of nz*planeSize, nz*nbound*np and nz*planeSize doubles.  Returns the
-    largest PCG iteration count (0 for DIRECT)."""
+    of nz*planeSize, nz*nbound*np and nz*planeSize doubles.  With iters the
+    call waits for the stream and returns the largest PCG iteration count
+    (0 for DIRECT); a plane at its step limit gives a RuntimeWarning, as
+    semtex's "step limit exceeded" WARNING (src/field.cpp:713)."""
     import torch
     M = msys.mesh
     assert f.numel() == msys.nz * M.planeSize and u.numel() == msys.nz * M.planeSize
@@ -140,7 +151,12 @@ def solve(msys, f, bc, u, tol_rel=1e-12, step_max=10000, stream=None):
     if stream is None:
         stream = torch.cuda.current_stream().cuda_stream
     it = C.c_int(0)
-    check(load().semx_helm_solve(msys.h, _ptr(f), _ptr(bc), _ptr(u), float(tol_rel),
-                                 int(step_max), C.byref(it), C.c_void_p(stream)),
-          "semx_helm_solve")
+    st = load().semx_helm_solve(msys.h, _ptr(f), _ptr(bc), _ptr(u), float(tol_rel),
+                                int(step_max), C.byref(it) if iters else None,
+                                C.c_void_p(stream))
+    if st == E_MAXITER:
+        import warnings
+        warnings.warn("Field::solve: step limit exceeded", RuntimeWarning)
+    else:
+        check(st, "semx_helm_solve")
     return it.value

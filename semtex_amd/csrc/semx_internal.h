@@ -28,6 +28,7 @@ struct MeshDev {
   double* G = nullptr;       // nset*G_N*npnp
   int* eset = nullptr;       // nel
   int* emap = nullptr;       // npnp
+  int* pmap = nullptr;       // npnp, inverse of emap
   int* btog = nullptr;       // nel*next
   int* emask = nullptr;      // nel
   double* ops = nullptr;     // DV DT SDV SDT, 4*npnp
@@ -87,6 +88,11 @@ struct HelmDev {
   long long* mode_offH = nullptr;
   long long* mode_offL = nullptr;
   double* pc = nullptr;       // JACPCG inverse diagonals, nmodes*npts
+  float* pc32 = nullptr;      // the same in FP32
+  int* mode_kind = nullptr;   // per mode: DIRECT or JACPCG
+  int* pcg_plane = nullptr;   // plane-list indices solved by PCG
+  double* work = nullptr;     // PCG vectors, one slice per PCG plane
+  int* iters = nullptr;       // PCG iterations per PCG plane
   // scratch
   double* Fb = nullptr;       // nplanes*nel*next
   double* Fi = nullptr;       // nplanes*nel*nint
@@ -111,6 +117,11 @@ struct Helm {
   int nplanes = 0, solstride = 0, npts = 0;
   std::vector<int> plane_of;
   std::vector<int> dmode, mode_plane;
+  std::vector<int> pcg_plane;         // plane-list indices of JACPCG modes
+  int pcg_prec = 0;                   // SEMX_PCG_FP64 or SEMX_PCG_FP32_REFINED
+  double inner_tol = 1e-5;            // inner relative tolerance (refined)
+  int max_outer = 12;                 // refinement sweeps (refined)
+  size_t workstride = 0;              // doubles per PCG plane
   bool on_device = false;
   HelmDev d;
 };

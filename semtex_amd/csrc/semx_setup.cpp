@@ -880,6 +880,12 @@ int helm_build(Helm& H) {
   H.dmode.clear();
   for (int m = 0; m < nmodes; ++m)
     if (H.modes[m].kind == SEMX_DIRECT) H.dmode.push_back(m);
+  // PCG work list: one workgroup per plane of a JACPCG mode
+  H.pcg_plane.clear();
+  for (int p = 0; p < H.nplanes; ++p)
+    if (H.modes[H.plane_of[p] >> 1].kind == SEMX_JACPCG) H.pcg_plane.push_back(p);
+  H.workstride = 5 * (size_t)H.npts + (size_t)M.nel * next;
+  H.workstride += H.workstride & 1;
   return SEMX_OK;
 }
 

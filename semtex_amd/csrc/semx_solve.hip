@@ -58,6 +58,7 @@ struct ElemArgs {
   const double* hii;
   const int* plane_of;
   const int* mode_n;
+  const int* mode_kind;   // per mode: SEMX_DIRECT or SEMX_JACPCG
   double* Fb;
   double* Fi;
   double* g;
@@ -156,14 +157,16 @@ __global__ __launch_bounds__(ELT_THREADS) void k_condense(ElemArgs a) {
       a.Fi[pe * nint + (q - next)] = Fe[a.emap[q]];
     } else {
       const int b = q;
-      const double* hbi = a.hbi + ((size_t)m * a.nset + a.eset[ee]) * next * nint;
       double s0 = 0.0, s1 = 0.0;
-      int c = 0;
-      for (; c + 1 < nint; c += 2) {
-        s0 += hbi[b * nint + c] * Fe[a.emap[next + c]];
-        s1 += hbi[b * nint + c + 1] * Fe[a.emap[next + c + 1]];
+      if (a.mode_kind[m] == SEMX_DIRECT) {  // PCG modes keep the raw F_b
+        const double* hbi = a.hbi + ((size_t)m * a.nset + a.eset[ee]) * next * nint;
+        int c = 0;
+        for (; c + 1 < nint; c += 2) {
+          s0 += hbi[b * nint + c] * Fe[a.emap[next + c]];
+          s1 += hbi[b * nint + c + 1] * Fe[a.emap[next + c + 1]];
+        }
+        if (c < nint) s0 += hbi[b * nint + c] * Fe[a.emap[next + c]];
       }
-      if (c < nint) s0 += hbi[b * nint + c] * Fe[a.emap[next + c]];
       double fb = Fe[a.emap[b]] - (s0 + s1);
       const int nm = np - 1;
       for (int cc = a.nat_ptr[ee]; cc < a.nat_ptr[ee + 1]; ++cc) {
@@ -207,11 +210,12 @@ __device__ inline double group_sum(double v) {
 // Element::local2globalSumSC), zero beyond the solved unknowns.
 __global__ void k_assemble(const double* __restrict__ Fb, const int* __restrict__ asm_ptr,
                            const int* __restrict__ asm_idx, const int* __restrict__ plane_of,
-                           const int* __restrict__ mode_n, double* __restrict__ sol,
-                           int solstride, int nglobal, int nelnext) {
+                           const int* __restrict__ mode_n, const int* __restrict__ mode_kind,
+                           double* __restrict__ sol, int solstride, int nglobal, int nelnext) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int p = blockIdx.y;
   if (i >= nglobal) return;
+  if (mode_kind[plane_of[p] >> 1] != SEMX_DIRECT) return;
   const int n = mode_n[plane_of[p] >> 1];
   double v = 0.0;
   if (i < n) {
@@ -382,6 +386,7 @@ __global__ __launch_bounds__(ELT_THREADS) void k_backsub(ElemArgs a) {
   const int next = a.next, nint = a.nint, npnp = a.npnp;
   const int ce = ELT_THREADS / npnp;
   const int m = k >> 1;
+  if (a.mode_kind[m] != SEMX_DIRECT) return;
   const int n = a.mode_n[m];
   double* sub = esh;                 // ce*next
   double* sFi = sub + ce * next;     // ce*nint
@@ -423,6 +428,289 @@ __global__ __launch_bounds__(ELT_THREADS) void k_backsub(ElemArgs a) {
     }
     a.u[(size_t)k * a.planeSize + (size_t)ee * npnp + a.emap[next + q]] = (s0 + s1) - (s2 + s3);
   }
+}
+
+// ===========================================================================
+// Jacobi-preconditioned CG, one persistent workgroup per plane: semtex's
+// JACPCG branch of Field::solve (src/field.cpp:627-720) with the global
+// operator of Field::HelmholtzOperator (src/field.cpp:771-856).
+// Vectors are globally numbered: nglobal element-boundary nodes, then the
+// element interiors in emap order.  Rows of essential (and pinned) nodes are
+// held at zero.  T is the precision of the iteration vectors; with
+// refinement the FP64 residual b - A x is recomputed after each inner T
+// solve and the correction added to the FP64 solution.
+// ===========================================================================
+
+struct PcgArgs {
+  int np, npnp, next, nint, nel, nglobal, nsolve, nzero, npts, planeSize, nset;
+  double lambda2, beta;
+  int base;
+  const double* G;
+  const int* eset;
+  const int* emap;
+  const int* pmap;
+  const int* btog;
+  const double* ops;
+  const int* asm_ptr;
+  const int* asm_idx;
+  const int* plane_of;
+  const int* pcg_plane;   // PCG plane-list indices
+  const int* mode_n;
+  const double* pc;       // nmodes*npts
+  const float* pc32;      // nmodes*npts
+  const double* Fb;
+  const double* Fi;
+  const double* g;
+  double* u;
+  double* work;           // per PCG plane, workstride doubles
+  size_t workstride;
+  double tol;             // TOL_REL
+  int step_max;           // STEP_MAX (per inner solve with refinement)
+  double inner_tol;       // inner relative tolerance with refinement
+  int max_outer;
+  int* iters;             // per PCG plane: iterations done (negative: limit hit)
+};
+
+constexpr int PCG_THREADS = 1024;
+
+__device__ inline double block_sum(double v, double* red) {
+#pragma unroll
+  for (int off = 32; off >= 1; off /= 2) v += __shfl_xor(v, off);
+  const int wave = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[wave] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int w = 0; w < PCG_THREADS / 64; ++w) t += red[w];
+  return t;
+}
+
+// y = A x on the globally numbered vector, rows [n, nglobal) zero; returns
+// the dot product w . y (w may be x itself).
+template <typename T>
+__device__ double pcg_matvec(const PcgArgs& a, int m, int n, const T* __restrict__ x,
+                             T* __restrict__ y, T* __restrict__ lb, const T* __restrict__ w,
+                             T* sh, double* red) {
+  const int tid = threadIdx.x, np = a.np, npnp = a.npnp, next = a.next, nint = a.nint;
+  const int ce = PCG_THREADS / npnp;
+  const double bk = a.beta * (a.base + m);
+  const T hcon = (T)(a.lambda2 + bk * bk);
+  const bool svv = a.lambda2 > EPSDP;
+  T* sD = sh;
+  T* sT = sD + npnp;
+  T* sU = sT + npnp;
+  T* sR = sU + ce * npnp;
+  T* sS = sR + ce * npnp;
+  if (tid < npnp) {
+    sD[tid] = (T)a.ops[(svv ? 2 : 0) * npnp + tid];
+    sT[tid] = (T)a.ops[(svv ? 3 : 1) * npnp + tid];
+  }
+  const int el = tid / npnp, pt = tid - (tid / npnp) * npnp;
+  const int i = pt / np, j = pt - (pt / np) * np;
+  const int pm = a.pmap[min(pt, npnp - 1)];
+  double wy = 0.0;
+  for (int e0 = 0; e0 < a.nel; e0 += ce) {
+    const int e = e0 + el;
+    const bool act = el < ce && e < a.nel;
+    const int base = el * npnp;
+    __syncthreads();
+    if (act) {
+      T v;
+      if (pm < next) {
+        const int gid = a.btog[e * next + pm];
+        v = (gid < n) ? x[gid] : (T)0;
+      } else {
+        v = x[a.nglobal + e * nint + (pm - next)];
+      }
+      sU[base + pt] = v;
+    }
+    __syncthreads();
+    const double* g = a.G + (size_t)(act ? a.eset[e] : 0) * G_N * npnp;
+    if (act) {
+      T r = 0, s2 = 0;
+      for (int q = 0; q < np; ++q) {
+        r += sU[base + i * np + q] * sT[q * np + j];
+        s2 += sD[i * np + q] * sU[base + q * np + j];
+      }
+      const T q1 = (T)g[G_Q1 * npnp + pt], q2 = (T)g[G_Q2 * npnp + pt];
+      const T q3 = (T)g[G_Q3 * npnp + pt];
+      sR[base + pt] = q1 * r + q3 * s2;
+      sS[base + pt] = q2 * s2 + q3 * r;
+    }
+    __syncthreads();
+    if (act) {
+      T out = (T)g[G_Q4 * npnp + pt] * sU[base + pt] * hcon;
+      T s1 = 0, s3 = 0;
+      for (int q = 0; q < np; ++q) {
+        s1 += sT[i * np + q] * sS[base + q * np + j];
+        s3 += sR[base + i * np + q] * sD[q * np + j];
+      }
+      out += s1;
+      out += s3;
+      if (pm < next) {
+        lb[e * next + pm] = out;
+      } else {
+        const int idx = a.nglobal + e * nint + (pm - next);
+        y[idx] = out;
+        wy += (double)w[idx] * (double)out;
+      }
+    }
+  }
+  __syncthreads();
+  for (int gI = tid; gI < a.nglobal; gI += PCG_THREADS) {
+    T v = 0;
+    if (gI < n)
+      for (int c = a.asm_ptr[gI]; c < a.asm_ptr[gI + 1]; ++c) v += lb[a.asm_idx[c]];
+    y[gI] = v;
+    wy += (double)w[gI] * (double)v;
+  }
+  return block_sum(wy, red);
+}
+
+// PCG iterations on A x = r (r holds the residual of the initial x and is
+// updated in place) until r.r <= epsb2 or the step limit; semtex's loop.
+template <typename T>
+__device__ int pcg_core(const PcgArgs& a, int m, int n, T* x, T* r, T* p, T* q, T* lb,
+                        const T* pc, double epsb2, int step_max, T* sh, double* red) {
+  const int tid = threadIdx.x, npts = a.npts;
+  double r2 = 0.0, rho1 = 0.0, rho2 = 1.0;
+  for (int i = tid; i < npts; i += PCG_THREADS) {
+    const double ri = (double)r[i];
+    r2 += ri * ri;
+    rho1 += ri * (double)pc[i] * ri;
+  }
+  r2 = block_sum(r2, red);
+  rho1 = block_sum(rho1, red);
+  int it = 0;
+  while (r2 > epsb2 && ++it < step_max) {
+    const T beta = (it == 1) ? (T)0 : (T)(rho1 / rho2);
+    for (int i = tid; i < npts; i += PCG_THREADS) {
+      const T z = pc[i] * r[i];
+      p[i] = (it == 1) ? z : z + beta * p[i];
+    }
+    const double pq = pcg_matvec<T>(a, m, n, p, q, lb, p, sh, red);
+    const T alpha = (T)(rho1 / pq);
+    double nr2 = 0.0, nrho = 0.0;
+    for (int i = tid; i < npts; i += PCG_THREADS) {
+      x[i] += alpha * p[i];
+      const T ri = r[i] - alpha * q[i];
+      r[i] = ri;
+      nr2 += (double)ri * (double)ri;
+      nrho += (double)ri * (double)pc[i] * (double)ri;
+    }
+    rho2 = rho1;
+    r2 = block_sum(nr2, red);
+    rho1 = block_sum(nrho, red);
+  }
+  __syncthreads();
+  return (it == step_max) ? -it : it;
+}
+
+template <typename T, bool REFINE>
+__global__ __launch_bounds__(PCG_THREADS) void k_pcg(PcgArgs a) {
+  extern __shared__ double pcg_lds[];
+  double* red = pcg_lds;               // 32 doubles
+  T* sh = (T*)(pcg_lds + 32);
+  const int tid = threadIdx.x;
+  const int pidx = a.pcg_plane[blockIdx.x];
+  const int k = a.plane_of[pidx];
+  const int m = k >> 1;
+  const int n = a.mode_n[m];
+  const int npts = a.npts, nglobal = a.nglobal, npnp = a.npnp, next = a.next;
+  double* wk = a.work + (size_t)blockIdx.x * a.workstride;
+  double* X = wk;                 // FP64 solution
+  double* R = X + npts;           // FP64 residual
+  double* B = R + npts;           // FP64 right-hand side
+  double* P = B + npts;
+  double* Q = P + npts;
+  double* LB = Q + npts;          // nel*next element-boundary scratch
+  const double* fb = a.Fb + (size_t)pidx * a.nel * next;
+  const double* fi = a.Fi + (size_t)pidx * a.nel * a.nint;
+  const double* pcm = a.pc + (size_t)m * npts;
+  double* uk = a.u + (size_t)k * a.planeSize;
+  // b: assembled -(M f + H g) + <h,w>, essential and pinned rows zero
+  // (Field::buildRHS, src/field.cpp:859-925)
+  double bb = 0.0;
+  for (int gI = tid; gI < nglobal; gI += PCG_THREADS) {
+    double v = 0.0;
+    if (gI < n)
+      for (int c = a.asm_ptr[gI]; c < a.asm_ptr[gI + 1]; ++c) v += fb[a.asm_idx[c]];
+    B[gI] = v;
+    bb += v * v;
+  }
+  for (int i = tid; i < a.nel * a.nint; i += PCG_THREADS) {
+    B[nglobal + i] = fi[i];
+    bb += fi[i] * fi[i];
+  }
+  const double epsb = a.tol * sqrt(block_sum(bb, red));
+  const double epsb2 = epsb * epsb;
+  // x0: the current field, essential and pinned entries zeroed
+  // (Field::local2global, src/field.cpp:928-947)
+  for (int gI = tid; gI < nglobal; gI += PCG_THREADS) {
+    const int c = a.asm_idx[a.asm_ptr[gI]];
+    const int e = c / next, b = c - (c / next) * next;
+    X[gI] = (gI < n) ? uk[(size_t)e * npnp + a.emap[b]] : 0.0;
+  }
+  for (int i = tid; i < a.nel * a.nint; i += PCG_THREADS) {
+    const int e = i / a.nint, q = i - (i / a.nint) * a.nint;
+    X[nglobal + i] = uk[(size_t)e * npnp + a.emap[next + q]];
+  }
+  __syncthreads();
+  // r = b - A x0
+  pcg_matvec<double>(a, m, n, X, Q, LB, X, (double*)sh, red);
+  for (int i = tid; i < npts; i += PCG_THREADS) R[i] = B[i] - Q[i];
+  __syncthreads();
+  int its;
+  if constexpr (!REFINE) {
+    its = pcg_core<double>(a, m, n, X, R, P, Q, LB, pcm, epsb2, a.step_max, (double*)sh, red);
+  } else {
+    // FP64 iterative refinement around a T-precision PCG on A d = r
+    T* d = (T*)P;                    // inner vectors share the P/Q/LB space
+    T* rr = d + npts;
+    T* pp = rr + npts;
+    T* qq = pp + npts;
+    T* lb = qq + npts;
+    const float* pc32 = a.pc32 + (size_t)m * npts;
+    its = 0;
+    for (int outer = 0;; ++outer) {
+      double rn = 0.0;
+      for (int i = tid; i < npts; i += PCG_THREADS) rn += R[i] * R[i];
+      rn = block_sum(rn, red);
+      if (rn <= epsb2) break;
+      if (outer == a.max_outer) { its = -its; break; }
+      for (int i = tid; i < npts; i += PCG_THREADS) {
+        d[i] = (T)0;
+        rr[i] = (T)R[i];
+      }
+      __syncthreads();
+      const double ein = a.inner_tol * a.inner_tol * rn;
+      const int it = pcg_core<T>(a, m, n, d, rr, pp, qq, lb, (const T*)pc32, ein, a.step_max,
+                                 sh, red);
+      its += (it < 0) ? -it : it;
+      for (int i = tid; i < npts; i += PCG_THREADS) X[i] += (double)d[i];
+      __syncthreads();
+      pcg_matvec<double>(a, m, n, X, Q, LB, X, (double*)sh, red);
+      for (int i = tid; i < npts; i += PCG_THREADS) R[i] = B[i] - Q[i];
+      __syncthreads();
+    }
+  }
+  // unknowns back to the plane with the essential values re-imposed
+  // (Field::global2local + setEssential, src/field.cpp:950-969, 1022-1040)
+  for (int idx = tid; idx < a.nel * npnp; idx += PCG_THREADS) {
+    const int e = idx / npnp, q = idx - (idx / npnp) * npnp;
+    const int pm = a.pmap[q];
+    double v;
+    if (pm < next) {
+      const int gid = a.btog[e * next + pm];
+      v = (gid < n) ? X[gid]
+        : (gid >= a.nsolve ? a.g[(size_t)pidx * a.nzero + (gid - a.nsolve)] : 0.0);
+    } else {
+      v = X[nglobal + e * a.nint + (pm - next)];
+    }
+    uk[idx] = v;
+  }
+  if (tid == 0) a.iters[blockIdx.x] = its;
 }
 
 // --- kernel timing: HIP events recorded around launches on their stream ---
@@ -493,6 +781,7 @@ int mesh_upload(Mesh& M) {
   if ((st = dev_upload(&M.d.G, M.G.data(), M.G.size()))) return st;
   if ((st = dev_upload(&M.d.eset, M.eset.data(), M.eset.size()))) return st;
   if ((st = dev_upload(&M.d.emap, M.ops.emap.data(), M.ops.emap.size()))) return st;
+  if ((st = dev_upload(&M.d.pmap, M.ops.pmap.data(), M.ops.pmap.size()))) return st;
   if ((st = dev_upload(&M.d.btog, M.btog.data(), M.btog.size()))) return st;
   if ((st = dev_upload(&M.d.emask, M.emask.data(), M.emask.size()))) return st;
   if ((st = dev_upload(&M.d.ops, ops.data(), ops.size()))) return st;
@@ -509,7 +798,8 @@ int mesh_upload(Mesh& M) {
 }
 
 void mesh_release(Mesh& M) {
-  dev_free(M.d.G); dev_free(M.d.eset); dev_free(M.d.emap); dev_free(M.d.btog);
+  dev_free(M.d.G); dev_free(M.d.eset); dev_free(M.d.emap); dev_free(M.d.pmap);
+  dev_free(M.d.btog);
   dev_free(M.d.emask); dev_free(M.d.ops); dev_free(M.d.asm_ptr); dev_free(M.d.asm_idx);
   dev_free(M.d.ess_src); dev_free(M.d.nat_ptr); dev_free(M.d.nat_b); dev_free(M.d.bside);
   dev_free(M.d.barea); dev_free(M.d.imass);
@@ -540,7 +830,18 @@ int helm_upload(Helm& H) {
   if ((st = dev_upload(&H.d.mode_n, mn.data(), mn.size()))) return st;
   if ((st = dev_upload(&H.d.mode_offH, oh.data(), oh.size()))) return st;
   if ((st = dev_upload(&H.d.mode_offL, ol.data(), ol.size()))) return st;
-  if ((st = dev_upload(&H.d.pc, H.pc.data(), H.pc.size()))) return st;
+  std::vector<int> mk(nm);
+  for (int m = 0; m < nm; ++m) mk[m] = H.modes[m].kind;
+  if ((st = dev_upload(&H.d.mode_kind, mk.data(), mk.size()))) return st;
+  if (!H.pcg_plane.empty()) {
+    std::vector<float> pc32(H.pc.begin(), H.pc.end());
+    if ((st = dev_upload(&H.d.pc, H.pc.data(), H.pc.size()))) return st;
+    if ((st = dev_upload(&H.d.pc32, pc32.data(), pc32.size()))) return st;
+    if ((st = dev_upload(&H.d.pcg_plane, H.pcg_plane.data(), H.pcg_plane.size()))) return st;
+    if ((st = dev_upload<double>(&H.d.work, nullptr, H.pcg_plane.size() * H.workstride)))
+      return st;
+    if ((st = dev_upload<int>(&H.d.iters, nullptr, H.pcg_plane.size()))) return st;
+  }
   if ((st = dev_upload(&H.d.plane_of, H.plane_of.data(), H.plane_of.size()))) return st;
   const size_t np_ = (size_t)H.nplanes;
   const int nzero = M.nglobal - M.nsolve;
@@ -563,6 +864,8 @@ void helm_release(Helm& H) {
   dev_free(H.d.Linv); dev_free(H.d.LinvT); dev_free(H.d.mode_n); dev_free(H.d.mode_offH);
   dev_free(H.d.mode_offL); dev_free(H.d.pc); dev_free(H.d.Fb); dev_free(H.d.Fi);
   dev_free(H.d.g); dev_free(H.d.sol); dev_free(H.d.plane_of);
+  dev_free(H.d.pc32); dev_free(H.d.mode_kind); dev_free(H.d.pcg_plane); dev_free(H.d.work);
+  dev_free(H.d.iters);
   H.on_device = false;
 }
 
@@ -718,11 +1021,9 @@ int semx_helm_solve(semx_helm helm, double* f, const double* bc, double* u, doub
   if (!helm || !f || !u) return SEMX_E_ARG;
   Helm& H = helm->H;
   Mesh& M = *H.M;
-  (void)tol_rel; (void)step_max;
   if (iters) *iters = 0;
-  for (const ModeSys& S : H.modes)
-    if (S.kind != SEMX_DIRECT) return SEMX_E_UNSUPPORTED;
   if (M.nbound && !bc) return SEMX_E_ARG;
+  if (!H.pcg_plane.empty() && (!(tol_rel >= 0.0) || step_max < 1)) return SEMX_E_ARG;
   int st = helm_upload(H);
   if (st) return st;
   hipStream_t s = (hipStream_t)stream;
@@ -743,7 +1044,8 @@ int semx_helm_solve(semx_helm helm, double* f, const double* bc, double* u, doub
   ea.G = M.d.G; ea.eset = M.d.eset; ea.emap = M.d.emap; ea.btog = M.d.btog;
   ea.emask = M.d.emask; ea.ops = M.d.ops; ea.nat_ptr = M.d.nat_ptr; ea.nat_b = M.d.nat_b;
   ea.bside = M.d.bside; ea.barea = M.d.barea; ea.hbi = H.d.hbi; ea.hii = H.d.hii;
-  ea.plane_of = H.d.plane_of; ea.mode_n = H.d.mode_n; ea.Fb = H.d.Fb; ea.Fi = H.d.Fi;
+  ea.plane_of = H.d.plane_of; ea.mode_n = H.d.mode_n; ea.mode_kind = H.d.mode_kind;
+  ea.Fb = H.d.Fb; ea.Fi = H.d.Fi;
   ea.g = H.d.g; ea.sol = H.d.sol; ea.solstride = H.solstride; ea.f = f; ea.bc = bc; ea.u = u;
   const int ce = ELT_THREADS / M.npnp;
   dim3 eg((M.nel + ce - 1) / ce, H.nplanes);
@@ -762,8 +1064,8 @@ int semx_helm_solve(semx_helm helm, double* f, const double* bc, double* u, doub
   if (!H.dmode.empty()) {
     prof_begin(s, &ev);
     hipLaunchKernelGGL(k_assemble, dim3((M.nglobal + 255) / 256, H.nplanes), dim3(256), 0, s,
-                       H.d.Fb, M.d.asm_ptr, M.d.asm_idx, H.d.plane_of, H.d.mode_n, H.d.sol,
-                       H.solstride, M.nglobal, M.nel * M.next);
+                       H.d.Fb, M.d.asm_ptr, M.d.asm_idx, H.d.plane_of, H.d.mode_n,
+                       H.d.mode_kind, H.d.sol, H.solstride, M.nglobal, M.nel * M.next);
     prof_end(s, "k_assemble", ev);
     BandArgs ba;
     ba.Pf = H.d.Pf; ba.Pb = H.d.Pb; ba.Linv = H.d.Linv; ba.LinvT = H.d.LinvT;
@@ -797,10 +1099,63 @@ int semx_helm_solve(semx_helm helm, double* f, const double* bc, double* u, doub
     HIPCHK(hipLaunchKernel(kfn, dim3((unsigned)H.dmode.size()), dim3(H.bt), kargs, lds, s));
     prof_end(s, "k_band_solve", ev);
   }
-  prof_begin(s, &ev);
-  hipLaunchKernelGGL(k_backsub, eg, dim3(ELT_THREADS), lds_b, s, ea);
-  prof_end(s, "k_backsub", ev);
+  if (!H.dmode.empty()) {
+    prof_begin(s, &ev);
+    hipLaunchKernelGGL(k_backsub, eg, dim3(ELT_THREADS), lds_b, s, ea);
+    prof_end(s, "k_backsub", ev);
+  }
+  if (!H.pcg_plane.empty()) {
+    PcgArgs pa;
+    pa.np = M.np; pa.npnp = M.npnp; pa.next = M.next; pa.nint = M.nint; pa.nel = M.nel;
+    pa.nglobal = M.nglobal; pa.nsolve = M.nsolve; pa.nzero = nzero; pa.npts = H.npts;
+    pa.planeSize = M.planeSize; pa.nset = M.nset;
+    pa.lambda2 = H.lambda2; pa.beta = H.beta; pa.base = H.base;
+    pa.G = M.d.G; pa.eset = M.d.eset; pa.emap = M.d.emap; pa.pmap = M.d.pmap;
+    pa.btog = M.d.btog; pa.ops = M.d.ops; pa.asm_ptr = M.d.asm_ptr; pa.asm_idx = M.d.asm_idx;
+    pa.plane_of = H.d.plane_of; pa.pcg_plane = H.d.pcg_plane; pa.mode_n = H.d.mode_n;
+    pa.pc = H.d.pc; pa.pc32 = H.d.pc32; pa.Fb = H.d.Fb; pa.Fi = H.d.Fi; pa.g = H.d.g;
+    pa.u = u; pa.work = H.d.work; pa.workstride = H.workstride;
+    pa.tol = tol_rel; pa.step_max = step_max; pa.inner_tol = H.inner_tol;
+    pa.max_outer = H.max_outer; pa.iters = H.d.iters;
+    const int cep = PCG_THREADS / M.npnp;
+    const bool refine = H.pcg_prec == SEMX_PCG_FP32_REFINED;
+    const size_t lds = 32 * sizeof(double) +
+        (2 * (size_t)M.npnp + 3 * (size_t)cep * M.npnp) * sizeof(double);
+    const void* kfn = refine ? (const void*)k_pcg<float, true> : (const void*)k_pcg<double, false>;
+    if (lds > 64 * 1024)
+      HIPCHK(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    void* kargs[] = {&pa};
+    prof_begin(s, &ev);
+    HIPCHK(hipLaunchKernel(kfn, dim3((unsigned)H.pcg_plane.size()), dim3(PCG_THREADS), kargs,
+                           lds, s));
+    prof_end(s, "k_pcg", ev);
+    HIPCHK(hipGetLastError());
+    if (iters) {
+      // the step-limit WARNING of src/field.cpp:713 becomes a status
+      std::vector<int> it(H.pcg_plane.size());
+      HIPCHK(hipMemcpyAsync(it.data(), H.d.iters, it.size() * sizeof(int),
+                            hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      int mx = 0;
+      bool limit = false;
+      for (int v : it) { limit |= v < 0; mx = std::max(mx, v < 0 ? -v : v); }
+      *iters = mx;
+      if (limit) return SEMX_E_MAXITER;
+    }
+  }
   HIPCHK(hipGetLastError());
+  return SEMX_OK;
+}
+
+int semx_helm_set_pcg(semx_helm helm, int precision, double inner_tol, int max_outer) {
+  if (!helm) return SEMX_E_ARG;
+  if (precision != SEMX_PCG_FP64 && precision != SEMX_PCG_FP32_REFINED) return SEMX_E_ARG;
+  if (precision == SEMX_PCG_FP32_REFINED &&
+      (!(inner_tol > 0.0 && inner_tol < 1.0) || max_outer < 1))
+    return SEMX_E_ARG;
+  Helm& H = helm->H;
+  H.pcg_prec = precision;
+  if (precision == SEMX_PCG_FP32_REFINED) { H.inner_tol = inner_tol; H.max_outer = max_outer; }
   return SEMX_OK;
 }
 

@@ -38,6 +38,8 @@ def lib():
         L.orc_field_solve.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double,
                                       C.c_int, _D, _D, _D, C.c_double, C.c_int,
                                       C.c_int, C.c_int, _D]
+        L.orc_field_solve_iters.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double,
+                                            C.c_int, _D, _D, _D, C.c_double, C.c_int, _I]
         L.orc_dftr.argtypes = [_D, C.c_int, C.c_int, C.c_int]
         L.orc_gradient.argtypes = [C.c_void_p, C.c_int, C.c_double, _D, C.c_int]
         L.orc_smooth.argtypes = [C.c_void_p, C.c_int, _D]
@@ -151,6 +153,17 @@ def field_solve(mesh, nz, lambda2, beta, method, f, bc, u, tol=1e-12,
                               dp(f), dp(bc), dp(u), tol, step_max, nthreads,
                               nrep, C.byref(t))
     return w, t.value
+
+
+def field_solve_iters(mesh, nz, lambda2, beta, method, f, bc, u, tol=1e-12,
+                      step_max=10000):
+    """One Field::solve; returns each plane's PCG iteration count (-1 at
+    the step limit, 0 for DIRECT planes and the skipped plane 1)."""
+    it = np.zeros(nz, dtype=np.int32)
+    lib().orc_field_solve_iters(C.c_void_p(mesh.h), nz, lambda2, beta, method, dp(f),
+                                dp(bc), dp(u), tol, step_max,
+                                it.ctypes.data_as(C.POINTER(C.c_int)))
+    return it
 
 
 def dftr(data, tlen, ntrn, sign):

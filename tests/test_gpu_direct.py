@@ -4,46 +4,23 @@ CPU FP64 oracle, through the C ABI.  Bar: relative L2 <= 1e-12 vs the oracle
 import numpy as np
 import pytest
 
-import oracle_lib as O
 import semtex_amd as sx
 from semtex_amd.mesh import rect_mesh
-from semtex_amd.workloads import helmholtz_batch
+from solve_helpers import PERIODIC, drop_mode0_constant, gpu_solve, make_case, oracle_solve, rel
 
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-12
 
 
-def gpu_solve(torch, P, S, f, bc, u0=None):
-    dev = torch.device("cuda:0")
-    tf = torch.from_numpy(np.ascontiguousarray(f).ravel().copy()).to(dev)
-    tb = torch.from_numpy(np.ascontiguousarray(bc).ravel().copy()).to(dev)
-    tu = torch.zeros_like(tf) if u0 is None else torch.from_numpy(u0.ravel().copy()).to(dev)
-    sx.solve(S, tf, tb, tu)
-    torch.cuda.synchronize()
-    return tu.cpu().numpy().reshape(f.shape)
-
-
-def oracle_solve(Q, nz, lambda2, beta, f, bc):
-    ff = np.ascontiguousarray(f.copy())
-    u = np.zeros_like(ff)
-    O.field_solve(Q, nz, lambda2, beta, 0, ff, np.ascontiguousarray(bc), u)
-    return u
-
-
-def rel(a, b):
-    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
-
-
-P_ = "P"
 CASES = [
     # name, nx, np, nz, lambda2, beta, distort, bc  (bc "P": periodic both ways)
     ("helmholtz1-2d", 2, 11, 1, 16.0, 1.0, 0.0, ("D", "D", "D", "D")),
     ("natural-2d", 4, 8, 1, 2.5, 1.0, 0.0, ("D", "N", "D", "N")),
     ("distorted-3d", 4, 6, 8, 1.0, 1.3, 0.3, ("D", "N", "D", "D")),
     ("cfg1-shape", 16, 9, 64, 20000.0, 1.0, 0.0, ("D", "D", "D", "D")),
-    ("periodic-kd607-singular", 16, 9, 4, 0.0, 1.0, 0.0, P_),
-    ("periodic-kd1059", 20, 12, 2, 3.0, 1.0, 0.0, P_),
+    ("periodic-kd607-singular", 16, 9, 4, 0.0, 1.0, 0.0, PERIODIC),
+    ("periodic-kd1059", 20, 12, 2, 3.0, 1.0, 0.0, PERIODIC),
     ("cfg2-mesh-k0", 32, 11, 1, 0.0, 1.0, 0.0, ("D", "D", "D", "D")),
 ]
 
@@ -53,51 +30,24 @@ def test_direct_parity(cuda, case):
     import torch
     name, nx, np_, nz, lam2, beta, dist, bc = case
     print(f"[direct parity] {name}", flush=True)
-    x1 = 2 * np.pi if nx >= 16 else 1.0
-    per = bc == P_
-    m = rect_mesh(nx, nx, 0, x1, 0, x1, bc=("D",) * 4 if per else bc, distort=dist,
-                  periodic_x=per, periodic_y=per)
-    P = sx.Mesh(np_, m)
-    Q = O.Mesh(np_, m)
-    nmodes = max(nz // 2, 1)
-    S = sx.ModalMatrixSys(P, lam2, beta, 0, nmodes, nz, sx.DIRECT)
-    f, bcl, ue = helmholtz_batch(P, nz, lam2, beta, seed=11)
-    # natural sides carry the outward normal derivative of u*: use the
-    # oracle's own evaluation so both see identical inputs
-    if "N" in bc:
-        bcl = natural_lines(P, nz, lam2, beta, bcl, ue)
-    ug = gpu_solve(torch, P, S, f, bcl)
-    uo = oracle_solve(Q, nz, lam2, beta, f, bcl)
+    P, Q, f, bcl, ue = make_case(nx, np_, nz, lam2, beta, dist, bc)
+    S = sx.ModalMatrixSys(P, lam2, beta, 0, max(nz // 2, 1), nz, sx.DIRECT)
+    ug, _ = gpu_solve(torch, P, S, f, bcl)
+    uo, _ = oracle_solve(Q, nz, lam2, beta, f, bcl)
     if S.info["singular0"]:
         # Mode 0 is pinned at the last-numbered node of each solver's own
         # numbering (src/matrix.cpp:167-169), so the two agree up to a
         # constant; after removing it the bar is the same relative L2.
-        d = ug[0, :P.nPlane] - uo[0, :P.nPlane]
-        ug[0, :P.nPlane] -= d.mean()
+        c = drop_mode0_constant(P, ug, uo)
         r0 = rel(ug[0, :P.nPlane], uo[0, :P.nPlane])
-        print(f"  singular mode 0: offset {d.mean():.6f}, rel L2 {r0:.3e}", flush=True)
+        print(f"  singular mode 0: offset {c:.6f}, rel L2 {r0:.3e}", flush=True)
         assert r0 <= TOL
-        ug[0, :P.nPlane] = uo[0, :P.nPlane]
     r = rel(ug[:, :P.nPlane], uo[:, :P.nPlane])
     print(f"  rel L2 gpu vs oracle = {r:.3e}", flush=True)
     assert r <= TOL
-    if "N" not in bc and not per:
+    if "N" not in bc and bc != PERIODIC:
         err = np.abs(ug[:, :P.nPlane] - ue[:, :P.nPlane]).max()
         assert err < 1e-5
-
-
-def natural_lines(P, nz, lam2, beta, bcl, ue):
-    """Replace natural-side BC values by a smooth synthetic flux (the same
-    numbers go to both solvers)."""
-    out = bcl.copy()
-    np_ = P.np
-    rng = np.random.default_rng(5)
-    for b in range(P.nbound):
-        e, s = P.belmt[b], P.bside[b]
-        if s in (1, 3):   # natural sides of these meshes are right/left
-            out[:, b * np_:(b + 1) * np_] = rng.standard_normal((nz, 1)) * np.cos(
-                P.y[e * np_ * np_ + np.array(sx.side_nodes(np_, s))])[None, :]
-    return np.ascontiguousarray(out)
 
 
 def test_regress_helmholtz1(cuda):
@@ -111,7 +61,7 @@ def test_regress_helmholtz1(cuda):
     f = np.zeros((1, P.planeSize)); f[0, :P.nPlane] = np.sin(np.pi * x) * np.sin(np.pi * y)
     ex = -np.sin(np.pi * x) * np.sin(np.pi * y) / (2 * np.pi ** 2 + 16.0)
     bc = P.bc_lines(ex[None, :])
-    u = gpu_solve(torch, P, S, f, bc)
+    u, _ = gpu_solve(torch, P, S, f, bc)
     assert np.abs(u[0, :P.nPlane] - ex).max() < 1e-12
 
 
@@ -124,5 +74,5 @@ def test_regress_laplace1(cuda):
     x, y = P.x, P.y
     ex = np.sin(2 * np.pi * x) * np.sinh(2 * np.pi * y) / np.sinh(2 * np.pi)
     f = np.zeros((1, P.planeSize))
-    u = gpu_solve(torch, P, S, f, P.bc_lines(ex[None, :]))
+    u, _ = gpu_solve(torch, P, S, f, P.bc_lines(ex[None, :]))
     assert f"{np.abs(u[0, :P.nPlane] - ex).max():.3e}" == "1.052e-10"
