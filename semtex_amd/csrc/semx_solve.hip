@@ -646,9 +646,12 @@ __global__ __launch_bounds__(PCG_THREADS) void k_pcg(PcgArgs a) {
   const double epsb = a.tol * sqrt(block_sum(bb, red));
   const double epsb2 = epsb * epsb;
   // x0: the current field, essential and pinned entries zeroed
-  // (Field::local2global, src/field.cpp:928-947)
+  // (Field::local2global, src/field.cpp:928-947).  Where element copies of
+  // a shared node differ, the copy written last wins (Veclib::gathr_scatr
+  // over elements in order, src/element.cpp:323): the last entry of the
+  // node's assembly list, which is in increasing element-storage order.
   for (int gI = tid; gI < nglobal; gI += PCG_THREADS) {
-    const int c = a.asm_idx[a.asm_ptr[gI]];
+    const int c = a.asm_idx[a.asm_ptr[gI + 1] - 1];
     const int e = c / next, b = c - (c / next) * next;
     X[gI] = (gI < n) ? uk[(size_t)e * npnp + a.emap[b]] : 0.0;
   }

@@ -39,8 +39,10 @@ def natural_lines(P, nz, bcl):
 def make_case(nx, np_, nz, lam2, beta, dist, bc, seed=11):
     """Mesh pair (product, oracle) and seeded forcing / BC lines / exact
     solution for lap u - (lambda2 + (beta k)^2) u = f on every plane."""
-    x1 = 2 * np.pi if nx >= 16 else 1.0
     per = bc == PERIODIC
+    # periodic meshes span [0, 2 pi]^2 so that the manufactured forcing is
+    # periodic and the singular k = 0 system consistent (zero-mean f)
+    x1 = 2 * np.pi if (nx >= 16 or per) else 1.0
     m = rect_mesh(nx, nx, 0, x1, 0, x1, bc=("D",) * 4 if per else bc, distort=dist,
                   periodic_x=per, periodic_y=per)
     P = sx.Mesh(np_, m)
