@@ -37,6 +37,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="target CPU-baseline sample length")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip timing the JACPCG / MIXED variants")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain device copy of 256 MiB (scale for HBM "
+                         "byte counters) and print its timing")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (all Fourier modes of a "
@@ -45,12 +50,30 @@ def main():
 
     import torch
     import semtex_amd as sx
-    from semtex_amd.workloads import config_mesh, helmholtz_batch
+    from semtex_amd.workloads import helmholtz_batch, workload_mesh
 
     dev = torch.device("cuda:0")
+    if args.calibrate_copy:
+        n = 1 << 25   # 32 Mi doubles = 256 MiB each way
+        x = torch.ones(n, dtype=torch.float64, device=dev)
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            y.copy_(x)
+        e1.record()
+        torch.cuda.synchronize()
+        cms = e0.elapsed_time(e1) / args.steps
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8,
+                          "ms": cms, "GBps": 2 * n * 8 / (cms * 1e-3) / 1e9}), flush=True)
+        return
     nu, dt, beta = 0.01, 0.005, 1.0
     lam2 = 1.0 / (nu * dt)
-    np_, m, nz = config_mesh("cfg1")
+    np_, m, nz = workload_mesh("square16_np9_nz64")
     t0 = time.time()
     P = sx.Mesh(np_, m)
     S = sx.ModalMatrixSys(P, lam2, beta, 0, nz // 2, nz, sx.DIRECT)
@@ -62,37 +85,53 @@ def main():
     nplanes = nz - 1
     npts = P.nglobal + P.nel * P.nint
     dofs = npts * nplanes
-
     stream = torch.cuda.current_stream()
-    for _ in range(args.warmup):
-        sx.solve(S, tf, tb, tu)
-    torch.cuda.synchronize()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        sx.solve(S, tf, tb, tu)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    ms = ev0.elapsed_time(ev1) / args.steps
-
-    # live per-kernel timing on the launch stream (separate pass)
     L = sx.load()
     import ctypes as C
-    L.semx_profile_reset()
-    L.semx_profile_enable(1)
-    kprof = {}
-    for _ in range(args.steps):
-        sx.solve(S, tf, tb, tu)
-    torch.cuda.synchronize()
-    for name in ("k_ess_gather", "k_condense", "k_band_solve", "k_backsub"):
-        tot = C.c_double(); cnt = C.c_longlong()
-        L.semx_profile_query(name.encode(), C.byref(tot), C.byref(cnt))
-        kprof[name] = tot.value / max(cnt.value, 1)
-    L.semx_profile_enable(0)
 
-    u = tu.cpu().numpy().reshape(nz, P.planeSize)
+    def timed(msys, steps, warmup, tol=1e-12):
+        """Mean ms per all-plane solve and mean per-kernel ms (second pass
+        with HIP events on the launch stream).  Every solve starts from
+        u = 0 (the PCG initial guess; the zero fill is inside the timed
+        region); the forcing is only read by the solve path."""
+        for _ in range(warmup):
+            tu.zero_()
+            sx.solve(msys, tf, tb, tu, tol_rel=tol, iters=False)
+        torch.cuda.synchronize()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        for _ in range(steps):
+            tu.zero_()
+            sx.solve(msys, tf, tb, tu, tol_rel=tol, iters=False)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        ms_ = ev0.elapsed_time(ev1) / steps
+        L.semx_profile_reset()
+        L.semx_profile_enable(1)
+        for _ in range(steps):
+            tu.zero_()
+            sx.solve(msys, tf, tb, tu, tol_rel=tol, iters=False)
+        torch.cuda.synchronize()
+        prof = {}
+        for name in ("k_ess_gather", "k_condense", "k_assemble", "k_band_solve",
+                     "k_backsub", "k_pcg"):
+            tot = C.c_double(); cnt = C.c_longlong()
+            L.semx_profile_query(name.encode(), C.byref(tot), C.byref(cnt))
+            if cnt.value:
+                prof[name] = tot.value / cnt.value
+        L.semx_profile_enable(0)
+        return ms_, prof
+
+    def fresh_solve(msys, tol=1e-12):
+        tf.copy_(torch.from_numpy(f.ravel().copy()).to(dev))
+        tu.zero_()
+        it = sx.solve(msys, tf, tb, tu, tol_rel=tol)
+        torch.cuda.synchronize()
+        return tu.cpu().numpy().reshape(nz, P.planeSize), it
+
+    ms, kprof = timed(S, args.steps, args.warmup)
+    u, _ = fresh_solve(S)
     err_exact = float(np.abs(u[:, :P.nPlane] - ue[:, :P.nPlane]).max())
 
     # algorithmic bytes of k_band_solve per launch: every mode's factor read
@@ -105,6 +144,13 @@ def main():
     band_bytes = bytes_factor + bytes_rhs
     band_ms = kprof["k_band_solve"]
     achieved = band_bytes / (band_ms * 1e-3) / 1e9
+
+    # HBM bytes per k_band_solve launch from the committed rocprofv3 counter
+    # summary (FETCH_SIZE / WRITE_SIZE scaled by a plain copy kernel)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_band_solve_traffic.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("traffic_bytes")
 
     result = {
         "metric": METRIC,
@@ -120,7 +166,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": "cfg1 all-mode direct FP64 Helmholtz: 16x16 quads np=9 on [0,2pi]^2, "
+            "workload": "all-mode direct FP64 Helmholtz: 16x16 quads np=9 on [0,2pi]^2, "
                         "Nz=64 (63 planes), lambda2_k=k^2+1/(nu*dt), nu=0.01, dt=0.005, "
                         "Dirichlet, seeded smooth solutions",
             "dof_per_plane": npts, "planes": nplanes, "nsolve": n0, "nband": ldab,
@@ -134,13 +180,26 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
             "bytes_per_launch": band_bytes,
             "avg_launch_ms": band_ms,
         },
         "kernel_ms": kprof,
         "max_abs_err_vs_exact": err_exact,
     }
+
+    # the iterative methods on the same workload (not the headline)
+    variants = {}
+    if not args.no_variants:
+        for label, method, prec in (("jacpcg_fp64", sx.JACPCG, sx.PCG_FP64),
+                                    ("mixed_fp32_refined", sx.MIXED, sx.PCG_FP32_REFINED)):
+            V = sx.ModalMatrixSys(P, lam2, beta, 0, nz // 2, nz, method)
+            V.set_pcg(prec, inner_tol=1e-4, max_outer=30)
+            uv, it = fresh_solve(V, tol=1e-12)
+            vms, vprof = timed(V, max(1, args.steps // 4), max(1, args.warmup // 4))
+            variants[label] = {"ms_per_step": vms, "value": dofs / (vms * 1e-3),
+                               "iterations_max": it, "kernel_ms": vprof, "u": uv}
+            del V
 
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -161,10 +220,19 @@ def main():
             "unit": "DOF-solves/s",
             "cores": cores,
             "kind": "port",
-            "sample": f"the same cfg1 all-mode solve repeated {nrep}x (setup excluded), "
+            "sample": f"the same all-mode solve repeated {nrep}x (setup excluded), "
                       f"FP64 oracle, OpenMP over planes; {tcpu * 1e3:.2f} ms per solve",
         }
         result["rel_l2_vs_fp64_oracle"] = rel
+        for v in variants.values():
+            uv = v["u"]
+            v["rel_l2_vs_fp64_oracle"] = float(
+                np.linalg.norm(uv[:, :P.nPlane] - uo[:, :P.nPlane]) /
+                np.linalg.norm(uo[:, :P.nPlane]))
+    for v in variants.values():
+        v.pop("u")
+    if variants:
+        result["iterative_variants"] = variants
     print(json.dumps(result), flush=True)
 
 

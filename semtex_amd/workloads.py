@@ -56,16 +56,19 @@ def helmholtz_batch(mesh, nz, lambda2, beta, seed):
     return f, np.ascontiguousarray(bc), ue
 
 
-def config_mesh(name):
-    """Meshes of the benchmark configurations: (np, mesh dict, nz)."""
+WORKLOADS = {
+    # name: (np, K, nz, periodic) -- K x K uniform quads on [0, 2 pi]^2
+    "square8_np8": (8, 8, 1, False),
+    "square16_np9_nz64": (9, 16, 64, False),
+    "square32_np11_nz128": (11, 32, 128, False),
+    "periodic24_np9_nz96": (9, 24, 96, True),
+}
+
+
+def workload_mesh(name):
+    """Mesh of a named benchmark workload: (np, mesh dict, nz).  All are
+    K x K uniform quads on [0, 2 pi]^2, Dirichlet on every edge unless
+    periodic both ways."""
+    np_, K, nz, per = WORKLOADS[name]
     two_pi = 2 * np.pi
-    if name == "cfg0":
-        return 8, rect_mesh(8, 8, 0, two_pi, 0, two_pi), 1
-    if name == "cfg1":
-        return 9, rect_mesh(16, 16, 0, two_pi, 0, two_pi), 64
-    if name == "cfg2":
-        return 11, rect_mesh(32, 32, 0, two_pi, 0, two_pi), 128
-    if name == "cfg3":
-        return 9, rect_mesh(24, 24, 0, two_pi, 0, two_pi, periodic_x=True,
-                            periodic_y=True), 96
-    raise KeyError(name)
+    return np_, rect_mesh(K, K, 0, two_pi, 0, two_pi, periodic_x=per, periodic_y=per), nz

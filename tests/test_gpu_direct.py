@@ -18,10 +18,10 @@ CASES = [
     ("helmholtz1-2d", 2, 11, 1, 16.0, 1.0, 0.0, ("D", "D", "D", "D")),
     ("natural-2d", 4, 8, 1, 2.5, 1.0, 0.0, ("D", "N", "D", "N")),
     ("distorted-3d", 4, 6, 8, 1.0, 1.3, 0.3, ("D", "N", "D", "D")),
-    ("cfg1-shape", 16, 9, 64, 20000.0, 1.0, 0.0, ("D", "D", "D", "D")),
+    ("uniform16-np9-nz64", 16, 9, 64, 20000.0, 1.0, 0.0, ("D", "D", "D", "D")),
     ("periodic-kd607-singular", 16, 9, 4, 0.0, 1.0, 0.0, PERIODIC),
     ("periodic-kd1059", 20, 12, 2, 3.0, 1.0, 0.0, PERIODIC),
-    ("cfg2-mesh-k0", 32, 11, 1, 0.0, 1.0, 0.0, ("D", "D", "D", "D")),
+    ("uniform32-np11-k0", 32, 11, 1, 0.0, 1.0, 0.0, ("D", "D", "D", "D")),
 ]
 
 
