@@ -58,6 +58,7 @@ extern "C" {
 
 typedef struct semx_mesh_s* semx_mesh;
 typedef struct semx_helm_s* semx_helm;
+typedef struct semx_dns_s*  semx_dns;
 
 /* Text for a status code.                                                  */
 const char* semx_error_string (int status);
@@ -142,6 +143,48 @@ int semx_helm_solve (semx_helm helm, double* f, const double* bc, double* u,
  * until |r| <= tol_rel |b| in FP64 -- the same stopping test as FP64.     */
 int semx_helm_set_pcg (semx_helm helm, int precision, double inner_tol,
                        int max_outer);
+
+/* --- field operators (device data, caller's stream) ------------------------
+ * dDFTr (femlib/fourier.c:47-171): ntrn real transforms of length tlen,
+ * successive points ntrn apart (a field of tlen planes of ntrn values),
+ * in place.  sign SEMX_FORWARD: X_j = (1/tlen) sum x_k exp(-2 pi i jk/tlen)
+ * packed per column as [Re0, Re_tlen/2, Re1, Im1, Re2, Im2, ...];
+ * SEMX_INVERSE: the unscaled inverse of that packing.  tlen must be
+ * 2^p 3^q 5^r and ntrn even (SEMX_E_ARG otherwise, as dDFTr's error).
+ * Uses a library scratch buffer: concurrent calls from several host
+ * threads are not supported.                                              */
+int semx_dft_r (double* data, int tlen, int ntrn, int sign, void* stream);
+/* AuxField::gradient (src/auxfield.cpp:514-665) on nz planes of data (in
+ * place): dir 0 = d/dx, 1 = d/dy (Element::grad, src/element.cpp:552-597,
+ * per plane), 2 = d/dz of Fourier planes (mode k: Re <- -beta k Im,
+ * Im <- beta k Re; planes 0 and 1 zeroed).                                 */
+int semx_gradient (semx_mesh mesh, int nz, double beta, double* data, int dir,
+                   void* stream);
+/* Field::smooth (src/field.cpp:190-236): mass-weighted average of the
+ * element-boundary values of every plane.                                  */
+int semx_smooth (semx_mesh mesh, int nz, double* data, void* stream);
+
+/* --- DNS time stepping: integrate -------------------------------------------
+ * Replaces the time loop of integrate (dns/integrate.cpp:79-221) with the
+ * skew-symmetric (advection 0, nonlinear.cpp:36-211) or alternating
+ * (advection 1, :214-411) nonlinear terms, waveProp, setPForce, project and
+ * the pressure / viscous Field::solve calls (DIRECT; JACPCG start-up
+ * systems while step < norder = N_TIME), Cartesian, on a mesh without
+ * boundaries (periodic in x and y; SEMX_E_UNSUPPORTED otherwise).  nz = 1
+ * is 2D (u, v); nz > 1 is 3D3C with Fourier z (nz = 2 or 2^p 3^q 5^r).
+ * create builds the velocity (lambda2 = alpha0/(dt kinvis)) and pressure
+ * (lambda2 = 0) systems and the device history (Us, Uf of integrate).    */
+int semx_dns_create (semx_mesh mesh, int nz, double beta, double dt,
+                     double kinvis, int norder, int advection, double tol_rel,
+                     int step_max, semx_dns* dns);
+int semx_dns_destroy (semx_dns dns);
+/* nstep steps; u, v, w (NULL in 2D), p: device fields of nz*planeSize in
+ * Fourier space, read at entry and written at exit.  The history carries
+ * over between calls, as integrate's static storage does.                */
+int semx_dns_run (semx_dns dns, int nstep, double* u, double* v, double* w,
+                  double* p, void* stream);
+/* info[4] = steps taken, nz, velocity components, norder                  */
+int semx_dns_info (semx_dns dns, int* info);
 
 /* --- kernel timing -------------------------------------------------------
  * No counterpart in semtex (dclock(), veclib/util.c:65-85, is never called

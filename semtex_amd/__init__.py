@@ -7,6 +7,10 @@ libsemx.so (include/semx.h):
                   (src/element.cpp:30-106, src/bcmgr.cpp:586-824)
   ModalMatrixSys  one Helmholtz system per Fourier mode (src/matrix.cpp:33-110)
   solve()         Field::solve (src/field.cpp:509-725)
+  dft_r()         dDFTr along z (femlib/fourier.c:47-171)
+  gradient()      AuxField::gradient (src/auxfield.cpp:514-665)
+  smooth()        Field::smooth (src/field.cpp:190-236)
+  DNS             the time loop of integrate (dns/integrate.cpp:79-221)
 
 Device buffers are torch tensors on the GPU (float64, contiguous); torch is
 used only for memory and streams.
@@ -26,7 +30,8 @@ PCG_FP64, PCG_FP32_REFINED = 0, 1
 E_MAXITER = 3
 FORWARD, INVERSE = 1, -1
 
-__all__ = ["Mesh", "ModalMatrixSys", "solve", "DIRECT", "JACPCG", "MIXED",
+__all__ = ["Mesh", "ModalMatrixSys", "solve", "dft_r", "gradient", "smooth", "DNS",
+           "DIRECT", "JACPCG", "MIXED",
            "PCG_FP64", "PCG_FP32_REFINED",
            "rect_mesh", "bc_lines", "side_nodes", "SemxError", "load"]
 
@@ -160,3 +165,67 @@ def solve(msys, f, bc, u, tol_rel=1e-12, step_max=10000, stream=None, iters=True
     else:
         check(st, "semx_helm_solve")
     return it.value
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream().cuda_stream
+    return C.c_void_p(stream)
+
+
+def dft_r(data, tlen, ntrn, sign, stream=None):
+    """dDFTr in place on a device tensor of tlen*ntrn doubles."""
+    assert data.numel() >= tlen * ntrn
+    check(load().semx_dft_r(_ptr(data), int(tlen), int(ntrn), int(sign), _stream(stream)),
+          "semx_dft_r")
+
+
+def gradient(mesh, nz, beta, data, direction, stream=None):
+    """AuxField::gradient in place (direction 0: x, 1: y, 2: z in Fourier space)."""
+    assert data.numel() == nz * mesh.planeSize
+    check(load().semx_gradient(mesh.h, int(nz), float(beta), _ptr(data), int(direction),
+                               _stream(stream)), "semx_gradient")
+
+
+def smooth(mesh, nz, data, stream=None):
+    """Field::smooth in place: mass-weighted average along element edges."""
+    assert data.numel() == nz * mesh.planeSize
+    check(load().semx_smooth(mesh.h, int(nz), _ptr(data), _stream(stream)), "semx_smooth")
+
+
+class DNS:
+    """Device time stepping of semtex's dns solver on a periodic mesh
+    (integrate, dns/integrate.cpp:79-221); fields in Fourier space."""
+
+    def __init__(self, mesh, nz, beta, dt, kinvis, norder=2, advection=0, tol_rel=1e-12,
+                 step_max=500):
+        h = C.c_void_p()
+        check(load().semx_dns_create(mesh.h, int(nz), float(beta), float(dt), float(kinvis),
+                                     int(norder), int(advection), float(tol_rel),
+                                     int(step_max), C.byref(h)), "semx_dns_create")
+        self.h = h
+        self.mesh = mesh
+        self.nz = nz
+
+    def run(self, nstep, u, v, w, p, stream=None):
+        n = self.nz * self.mesh.planeSize
+        for t in (u, v, p) + ((w,) if self.nz > 1 else ()):
+            assert t.numel() == n
+        check(load().semx_dns_run(self.h, int(nstep), _ptr(u), _ptr(v),
+                                  _ptr(w) if self.nz > 1 else None, _ptr(p),
+                                  _stream(stream)), "semx_dns_run")
+
+    @property
+    def steps(self):
+        info = np.zeros(4, dtype=np.int32)
+        check(load().semx_dns_info(self.h, _ip(info)))
+        return int(info[0])
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                load().semx_dns_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass

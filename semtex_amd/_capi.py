@@ -34,6 +34,14 @@ PROTOTYPES = {
     "semx_helm_element": (C.c_int, [VP, C.c_int, C.c_int, D, D, D]),
     "semx_helm_solve": (C.c_int, [VP, VP, VP, VP, C.c_double, C.c_int, I, VP]),
     "semx_helm_set_pcg": (C.c_int, [VP, C.c_int, C.c_double, C.c_int]),
+    "semx_dft_r": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, VP]),
+    "semx_gradient": (C.c_int, [VP, C.c_int, C.c_double, VP, C.c_int, VP]),
+    "semx_smooth": (C.c_int, [VP, C.c_int, VP, VP]),
+    "semx_dns_create": (C.c_int, [VP, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                                  C.c_int, C.c_double, C.c_int, C.POINTER(VP)]),
+    "semx_dns_destroy": (C.c_int, [VP]),
+    "semx_dns_run": (C.c_int, [VP, C.c_int, VP, VP, VP, VP, VP]),
+    "semx_dns_info": (C.c_int, [VP, I]),
     "semx_malloc": (C.c_int, [C.POINTER(VP), C.c_size_t]),
     "semx_free": (C.c_int, [VP]),
     "semx_memcpy_htod": (C.c_int, [VP, VP, C.c_size_t]),

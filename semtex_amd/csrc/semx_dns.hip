@@ -1,0 +1,639 @@
+// semx_dns.hip -- field operators and the 3D3C Navier-Stokes time step of
+// semtex's dns solver on CDNA4 (gfx950):
+//
+//   k_dft_z      dDFTr along z, femlib/fourier.c:47-171: every column of a
+//                field (ntrn columns, z-stride ntrn) transformed as one
+//                dense real N x N product with semtex's packing
+//                [Re0, Re_N/2, Re1, Im1, ...] and 1/N forward scaling
+//   k_grad_xy    AuxField::gradient x/y, src/auxfield.cpp:514-665 with
+//                Element::grad, src/element.cpp:552-597
+//   k_grad_z     AuxField::gradient z in Fourier space, auxfield.cpp:636-657
+//   k_smooth     Field::smooth, src/field.cpp:190-236 (mass-weighted DSS)
+//   elementwise  AuxField times / timesMinus / axpy / scaling passes
+//
+// and the host driver of integrate (dns/integrate.cpp:79-221) with
+// waveProp, setPForce, project and the velocity/pressure solves of
+// Field::solve (semx_solve.hip).  Periodic (boundary-free) meshes only.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <utility>
+#include <vector>
+
+#include "semx_hip.h"
+
+using namespace semx;
+
+namespace {
+
+constexpr int DFT_THREADS = 256;
+constexpr int DFT_OC = 32;          // output rows per thread
+constexpr int OP_THREADS = 1024;    // element-operator blocks
+
+// out[r][col] = sum_k W[r][k] in[k][col], r < N; W padded with zero rows to
+// a multiple of DFT_OC.  W is indexed uniformly across the wavefront, so its
+// entries arrive through scalar loads; the columns are coalesced.
+__global__ __launch_bounds__(DFT_THREADS) void k_dft_z(const double* __restrict__ in,
+                                                       double* __restrict__ out,
+                                                       const double* __restrict__ W,
+                                                       int N, int ncol, long long stride) {
+  const int col = blockIdx.x * DFT_THREADS + threadIdx.x;
+  const int r0 = blockIdx.y * DFT_OC;
+  if (col >= ncol) return;
+  double acc[DFT_OC];
+#pragma unroll
+  for (int o = 0; o < DFT_OC; ++o) acc[o] = 0.0;
+  const double* w = W + (size_t)r0 * N;
+  for (int k = 0; k < N; ++k) {
+    const double x = in[(size_t)k * stride + col];
+#pragma unroll
+    for (int o = 0; o < DFT_OC; ++o) acc[o] += w[o * N + k] * x;
+  }
+#pragma unroll
+  for (int o = 0; o < DFT_OC; ++o)
+    if (r0 + o < N) out[(size_t)(r0 + o) * stride + col] = acc[o];
+}
+
+// d/dx (dir 0) or d/dy (dir 1) on every element of every plane:
+// tgt = (D_s u) * ds/dx_i + (u D_r^T) * dr/dx_i, plain (non-SVV) D.
+__global__ __launch_bounds__(OP_THREADS) void k_grad_xy(double* __restrict__ data,
+                                                        const double* __restrict__ G,
+                                                        const int* __restrict__ eset,
+                                                        const double* __restrict__ ops, int np,
+                                                        int nel, int planeSize, int dir) {
+  extern __shared__ double gsh[];
+  const int npnp = np * np;
+  const int ce = OP_THREADS / npnp;
+  const int t = threadIdx.x;
+  const int el = t / npnp, pt = t - el * npnp;
+  const int e = blockIdx.x * ce + el;
+  const bool act = el < ce && e < nel;
+  double* sD = gsh;
+  double* sT = sD + npnp;
+  double* sU = sT + npnp;
+  if (t < npnp) {
+    sD[t] = ops[t];
+    sT[t] = ops[npnp + t];
+  }
+  double* u = data + (size_t)blockIdx.y * planeSize + (size_t)e * npnp;
+  if (act) sU[el * npnp + pt] = u[pt];
+  __syncthreads();
+  if (!act) return;
+  const int i = pt / np, j = pt - (pt / np) * np;
+  const double* su = sU + el * npnp;
+  double A = 0.0, B = 0.0;
+  for (int q = 0; q < np; ++q) {
+    A += su[i * np + q] * sT[q * np + j];
+    B += sD[i * np + q] * su[q * np + j];
+  }
+  const double* g = G + (size_t)eset[e] * G_N * npnp;
+  const double rx = g[(dir ? G_DRDY : G_DRDX) * npnp + pt];
+  const double sx = g[(dir ? G_DSDY : G_DSDX) * npnp + pt];
+  u[pt] = B * sx + A * rx;
+}
+
+// z-derivative of Fourier planes: mode k Re <- -beta k Im, Im <- beta k Re;
+// mode 0 and the Nyquist plane become zero.  src may equal dst.
+__global__ void k_grad_z(const double* src, double* dst, int nmodes, int planeSize, double beta) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  if (idx >= (size_t)planeSize || k >= nmodes) return;
+  const size_t re = (size_t)(2 * k) * planeSize + idx, im = re + planeSize;
+  if (k == 0) {
+    dst[re] = 0.0;
+    dst[im] = 0.0;
+  } else {
+    const double bk = beta * k;
+    const double r = src[re], m = src[im];
+    dst[re] = -bk * m;
+    dst[im] = bk * r;
+  }
+}
+
+// Mass-weighted average of element-boundary values over the copies of each
+// global node, per plane; copies are summed in increasing storage order.
+__global__ void k_smooth(double* __restrict__ data, const double* __restrict__ G,
+                         const int* __restrict__ eset, const int* __restrict__ emap,
+                         const int* __restrict__ asm_ptr, const int* __restrict__ asm_idx,
+                         const double* __restrict__ imass, int next, int npnp, int nglobal,
+                         int planeSize) {
+  const int gI = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gI >= nglobal) return;
+  double* src = data + (size_t)blockIdx.y * planeSize;
+  double s = 0.0;
+  for (int c = asm_ptr[gI]; c < asm_ptr[gI + 1]; ++c) {
+    const int e = asm_idx[c] / next, b = asm_idx[c] - (asm_idx[c] / next) * next;
+    const int q = emap[b];
+    s += G[((size_t)eset[e] * G_N + G_Q4) * npnp + q] * src[(size_t)e * npnp + q];
+  }
+  s *= imass[gI];
+  for (int c = asm_ptr[gI]; c < asm_ptr[gI + 1]; ++c) {
+    const int e = asm_idx[c] / next, b = asm_idx[c] - (asm_idx[c] / next) * next;
+    src[(size_t)e * npnp + emap[b]] = s;
+  }
+}
+
+// --- elementwise passes over whole fields -------------------------------
+__global__ void k_times(size_t n, double* __restrict__ t, const double* __restrict__ a,
+                        const double* __restrict__ b) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) t[i] = a[i] * b[i];
+}
+
+__global__ void k_times_minus(size_t n, double* __restrict__ t, const double* __restrict__ a,
+                              const double* __restrict__ b) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) t[i] -= a[i] * b[i];
+}
+
+__global__ void k_sub(size_t n, double* __restrict__ t, const double* __restrict__ a) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) t[i] -= a[i];
+}
+
+__global__ void k_fill(size_t n, double* __restrict__ t, double a) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) t[i] = a;
+}
+
+__global__ void k_copy(size_t n, double* __restrict__ t, const double* __restrict__ a) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) t[i] = a[i];
+}
+
+__global__ void k_scale(size_t n, double* __restrict__ t, double a) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) t[i] *= a;
+}
+
+__global__ void k_axpy(size_t n, double* __restrict__ t, double a, const double* __restrict__ x) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) t[i] += a * x[i];
+}
+
+struct WaveArgs {
+  const double* us[3];
+  const double* uf[3];
+  double a[3], b[3];
+  int nlev;
+};
+
+// waveProp (dns/integrate.cpp:224-259): u = sum_q (-alpha_{q+1} Us_q) +
+// (beta_q dt Uf_q), accumulated in semtex's order.
+__global__ void k_wave(size_t n, double* __restrict__ u, WaveArgs w) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int q = 0; q < w.nlev; ++q) {
+    s += w.a[q] * w.us[q][i];
+    s += w.b[q] * w.uf[q][i];
+  }
+  u[i] = s;
+}
+
+// setPForce (dns/integrate.cpp:262-277): f0 = (f0 + f1 [+ f2]) / dt
+__global__ void k_divergence_sum(size_t n, double* __restrict__ f0, const double* __restrict__ f1,
+                                 const double* __restrict__ f2, double dt) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = f0[i] + f1[i];
+  if (f2) s += f2[i];
+  f0[i] = s / dt;
+}
+
+inline dim3 grid1(size_t n, int bs) { return dim3((unsigned)((n + bs - 1) / bs)); }
+
+// --- dense DFT matrices ----------------------------------------------------
+bool fft_length_ok(int n) {
+  if (n < 1) return false;
+  while (n % 2 == 0) n /= 2;
+  while (n % 3 == 0) n /= 3;
+  while (n % 5 == 0) n /= 5;
+  return n == 1;
+}
+
+// Row-major (rows padded to a multiple of DFT_OC) matrix of the real
+// transform of length N.  sign +1: semtex's forward packing with 1/N;
+// sign -1: the unscaled inverse.  Angles are reduced exactly (jk mod N)
+// and evaluated in long double.
+std::vector<double> dft_matrix(int N, int sign) {
+  const int rows = (N + DFT_OC - 1) / DFT_OC * DFT_OC;
+  std::vector<double> W((size_t)rows * N, 0.0);
+  const long double two_pi = 6.283185307179586476925286766559L;
+  auto cs = [&](long long jk, long double* c, long double* s) {
+    const long double a = two_pi * (long double)(jk % N) / (long double)N;
+    *c = cosl(a);
+    *s = sinl(a);
+  };
+  const int nh = N / 2;
+  if (sign == 1) {
+    for (int k = 0; k < N; ++k) {
+      W[0 * N + k] = (double)(1.0L / N);
+      if (N > 1) W[1 * N + k] = (double)(((k & 1) ? -1.0L : 1.0L) / N);
+      for (int j = 1; j < nh; ++j) {
+        long double c, s;
+        cs((long long)j * k, &c, &s);
+        W[(size_t)(2 * j) * N + k] = (double)(c / N);
+        W[(size_t)(2 * j + 1) * N + k] = (double)(-s / N);
+      }
+    }
+  } else {
+    for (int k = 0; k < N; ++k) {
+      double* r = &W[(size_t)k * N];
+      r[0] = 1.0;
+      if (N > 1) r[1] = (k & 1) ? -1.0 : 1.0;
+      for (int j = 1; j < nh; ++j) {
+        long double c, s;
+        cs((long long)j * k, &c, &s);
+        r[2 * j] = (double)(2.0L * c);
+        r[2 * j + 1] = (double)(-2.0L * s);
+      }
+    }
+  }
+  return W;
+}
+
+struct DftPlan { int N, sign; double* W; };
+std::vector<DftPlan> g_plans;   // device matrices, kept for the process
+
+int dft_plan(int N, int sign, const double** W) {
+  for (const DftPlan& p : g_plans)
+    if (p.N == N && p.sign == sign) { *W = p.W; return SEMX_OK; }
+  std::vector<double> h = dft_matrix(N, sign);
+  double* d = nullptr;
+  int st = dev_upload(&d, h.data(), h.size());
+  if (st) return st;
+  g_plans.push_back({N, sign, d});
+  *W = d;
+  return SEMX_OK;
+}
+
+// out-of-place transform of ntrn columns of length N (z-stride ntrn)
+int dft_launch(const double* in, double* out, int N, int ntrn, int sign, hipStream_t s) {
+  const double* W = nullptr;
+  int st = dft_plan(N, sign, &W);
+  if (st) return st;
+  dim3 g((ntrn + DFT_THREADS - 1) / DFT_THREADS, (N + DFT_OC - 1) / DFT_OC);
+  hipEvent_t ev;
+  prof_begin(s, &ev);
+  hipLaunchKernelGGL(k_dft_z, g, dim3(DFT_THREADS), 0, s, in, out, W, N, ntrn,
+                     (long long)ntrn);
+  prof_end(s, "k_dft_z", ev);
+  HIPCHK(hipGetLastError());
+  return SEMX_OK;
+}
+
+int grad_launch(Mesh& M, int nz, double beta, double* data, int dir, hipStream_t s) {
+  hipEvent_t ev;
+  if (dir < 2) {
+    const int ce = OP_THREADS / M.npnp;
+    dim3 g((M.nel + ce - 1) / ce, nz);
+    const size_t lds = (2 + (size_t)ce) * M.npnp * sizeof(double);
+    prof_begin(s, &ev);
+    hipLaunchKernelGGL(k_grad_xy, g, dim3(OP_THREADS), lds, s, data, M.d.G, M.d.eset, M.d.ops,
+                       M.np, M.nel, M.planeSize, dir);
+    prof_end(s, "k_grad_xy", ev);
+  } else {
+    dim3 g((M.planeSize + 255) / 256, nz / 2);
+    prof_begin(s, &ev);
+    hipLaunchKernelGGL(k_grad_z, g, dim3(256), 0, s, data, data, nz / 2, M.planeSize, beta);
+    prof_end(s, "k_grad_z", ev);
+  }
+  HIPCHK(hipGetLastError());
+  return SEMX_OK;
+}
+
+int smooth_launch(Mesh& M, int nz, double* data, hipStream_t s) {
+  hipEvent_t ev;
+  dim3 g((M.nglobal + 255) / 256, nz);
+  prof_begin(s, &ev);
+  hipLaunchKernelGGL(k_smooth, g, dim3(256), 0, s, data, M.d.G, M.d.eset, M.d.emap, M.d.asm_ptr,
+                     M.d.asm_idx, M.d.imass, M.next, M.npnp, M.nglobal, M.planeSize);
+  prof_end(s, "k_smooth", ev);
+  HIPCHK(hipGetLastError());
+  return SEMX_OK;
+}
+
+// Integration coefficients, src/integration.cpp:99-164.
+void stiffly_stable(int n, double* c) {
+  if (n == 1) { c[0] = 1.0; c[1] = -1.0; }
+  else if (n == 2) { c[0] = 1.5; c[1] = -2.0; c[2] = 0.5; }
+  else { c[0] = 11.0 / 6.0; c[1] = -3.0; c[2] = 1.5; c[3] = -1.0 / 3.0; }
+}
+
+void extrapolation(int n, double* c) {
+  if (n == 1) { c[0] = 1.0; }
+  else if (n == 2) { c[0] = 2.0; c[1] = -1.0; }
+  else { c[0] = 3.0; c[1] = -3.0; c[2] = 1.0; }
+}
+
+}  // namespace
+
+// ===========================================================================
+// DNS state: the fields of integrate() held on the device.
+// ===========================================================================
+
+struct semx_dns_s {
+  Mesh* M = nullptr;
+  int nz = 1, ncom = 2, ndim = 2, nord = 2, advection = 0, step = 0, step_max = 500, toggle = 1;
+  double beta = 1.0, dt = 0.0, kinvis = 0.0, tol_rel = 1e-12;
+  size_t ntot = 0;
+  double* store = nullptr;              // all fields in one allocation
+  double* U[3] = {nullptr, nullptr, nullptr};
+  double* P = nullptr;
+  double* Us[3][3] = {};                // [level][component]
+  double* Uf[3][3] = {};
+  double* tmp = nullptr;
+  double* spare = nullptr;              // transform output, swapped in
+  Helm* vel = nullptr;                  // DIRECT, lambda2 = alpha0/(dt nu)
+  Helm* pre = nullptr;                  // DIRECT, lambda2 = 0
+  Helm* start[2] = {nullptr, nullptr};  // JACPCG start-up velocity systems
+};
+
+namespace {
+
+int dns_transform(semx_dns_s& D, double*& a, int sign, hipStream_t s) {
+  Mesh& M = *D.M;
+  if (D.nz == 1) return SEMX_OK;
+  if (D.nz == 2) {   // AuxField::transform with two planes (auxfield.cpp:1133-1170)
+    const size_t n = M.planeSize;
+    if (sign == 1) hipLaunchKernelGGL(k_fill, grid1(n, 256), dim3(256), 0, s, n, a + n, 0.0);
+    else hipLaunchKernelGGL(k_copy, grid1(n, 256), dim3(256), 0, s, n, a + n, a);
+    HIPCHK(hipGetLastError());
+    return SEMX_OK;
+  }
+  int st = dft_launch(a, D.spare, D.nz, M.planeSize, sign, s);
+  if (st) return st;
+  std::swap(a, D.spare);
+  return SEMX_OK;
+}
+
+int ew_launch_check() {
+  HIPCHK(hipGetLastError());
+  return SEMX_OK;
+}
+
+#define EW(kernel, ...)                                                          \
+  do {                                                                           \
+    hipLaunchKernelGGL(kernel, grid1(n, 256), dim3(256), 0, s, n, __VA_ARGS__); \
+    if (int st_ = ew_launch_check()) return st_;                                 \
+  } while (0)
+
+#define CHK(x) do { if (int st_ = (x)) return st_; } while (0)
+
+// skewSymmetric / altSkewSymmetric (dns/nonlinear.cpp:36-411, Cartesian):
+// on entry U holds the Fourier velocity; on exit U holds the physical-space
+// velocity, Us[0] the Fourier copy and Uf[0] the Fourier N(u), smoothed.
+int dns_nonlinear(semx_dns_s& D, hipStream_t s) {
+  Mesh& M = *D.M;
+  const size_t n = D.ntot;
+  hipEvent_t ev;
+  prof_begin(s, &ev);
+  for (int i = 0; i < D.ncom; ++i) {
+    EW(k_fill, D.Uf[0][i], 0.0);
+    EW(k_copy, D.Us[0][i], D.U[i]);
+    CHK(dns_transform(D, D.U[i], -1, s));
+  }
+  const bool conv = D.advection == 0 || (D.advection == 1 && D.toggle);
+  const bool cons = D.advection == 0 || (D.advection == 1 && !D.toggle);
+  for (int i = 0; i < D.ncom; ++i) {
+    double* N = D.Uf[0][i];
+    for (int j = 0; j < D.ndim; ++j) {
+      if (conv) {   // n_i -= u_j d(u_i)/dx_j
+        if (j == 2) {
+          dim3 g((M.planeSize + 255) / 256, D.nz / 2);
+          hipLaunchKernelGGL(k_grad_z, g, dim3(256), 0, s, D.Us[0][i], D.tmp, D.nz / 2,
+                             M.planeSize, D.beta);
+          CHK(ew_launch_check());
+          CHK(dns_transform(D, D.tmp, -1, s));
+        } else {
+          EW(k_copy, D.tmp, D.U[i]);
+          CHK(grad_launch(M, D.nz, D.beta, D.tmp, j, s));
+        }
+        EW(k_times_minus, N, D.U[j], D.tmp);
+      }
+      if (cons) {   // n_i -= d(u_i u_j)/dx_j
+        EW(k_times, D.tmp, D.U[i], D.U[j]);
+        if (j == 2) CHK(dns_transform(D, D.tmp, +1, s));
+        CHK(grad_launch(M, D.nz, D.beta, D.tmp, j, s));
+        if (j == 2) CHK(dns_transform(D, D.tmp, -1, s));
+        EW(k_sub, N, D.tmp);
+      }
+    }
+    if (D.advection == 0) EW(k_scale, N, 0.5);
+    CHK(dns_transform(D, D.Uf[0][i], +1, s));
+    CHK(smooth_launch(M, D.nz, D.Uf[0][i], s));
+  }
+  if (D.advection == 1) D.toggle = 1 - D.toggle;
+  prof_end(s, "dns_nonlinear", ev);
+  return SEMX_OK;
+}
+
+// rollm (src/sem.h:106-120): level q takes level q-1, level 0 the last.
+void roll(double* lev[3][3], int nord, int ncom) {
+  if (nord < 2) return;
+  for (int j = 0; j < ncom; ++j) {
+    double* t = lev[nord - 1][j];
+    for (int i = nord - 1; i; --i) lev[i][j] = lev[i - 1][j];
+    lev[0][j] = t;
+  }
+}
+
+// One time step of integrate (dns/integrate.cpp:79-221).
+int dns_step(semx_dns_s& D, hipStream_t s) {
+  Mesh& M = *D.M;
+  const size_t n = D.ntot;
+  const int ncom = D.ncom, nord = D.nord;
+  CHK(dns_nonlinear(D, s));
+  D.step += 1;
+  const int Je = std::min(D.step, nord);
+  double alpha[4], beta[3];
+  stiffly_stable(Je, alpha);
+  extrapolation(Je, beta);
+  for (int q = 0; q < Je; ++q) beta[q] *= D.dt;
+  // waveProp
+  for (int i = 0; i < ncom; ++i) {
+    WaveArgs w{};
+    w.nlev = Je;
+    for (int q = 0; q < Je; ++q) {
+      w.us[q] = D.Us[q][i];
+      w.uf[q] = D.Uf[q][i];
+      w.a[q] = -alpha[q + 1];
+      w.b[q] = beta[q];
+    }
+    EW(k_wave, D.U[i], w);
+  }
+  // swapData (U[i], Us[0][i]) and roll of the forcing levels
+  for (int i = 0; i < ncom; ++i) std::swap(D.U[i], D.Us[0][i]);
+  roll(D.Uf, nord, ncom);
+  // setPForce
+  for (int i = 0; i < D.ndim; ++i) {
+    EW(k_copy, D.Uf[0][i], D.Us[0][i]);
+    CHK(grad_launch(M, D.nz, D.beta, D.Uf[0][i], i, s));
+  }
+  EW(k_divergence_sum, D.Uf[0][0], D.Uf[0][1], D.ndim > 2 ? D.Uf[0][2] : nullptr, D.dt);
+  // pressure solve
+  CHK(helm_solve(*D.pre, D.Uf[0][0], nullptr, D.P, D.tol_rel, D.step_max, nullptr, s));
+  // project
+  {
+    const double a = -1.0 / (D.dt * D.kinvis), b = 1.0 / D.kinvis;
+    for (int i = 0; i < ncom; ++i) {
+      std::swap(D.Us[0][i], D.Uf[0][i]);
+      EW(k_scale, D.Uf[0][i], a);
+    }
+    for (int i = 0; i < D.ndim; ++i) {
+      EW(k_copy, D.Us[0][0], D.P);
+      CHK(grad_launch(M, D.nz, D.beta, D.Us[0][0], i, s));
+      EW(k_axpy, D.Uf[0][i], b, D.Us[0][0]);
+    }
+  }
+  for (int i = 0; i < ncom; ++i) EW(k_copy, D.Us[0][i], D.U[i]);
+  roll(D.Us, nord, ncom);
+  // viscous solves: start-up JACPCG systems while step < N_TIME
+  Helm* V = (D.step < nord) ? D.start[D.step - 1] : D.vel;
+  for (int i = 0; i < ncom; ++i)
+    CHK(helm_solve(*V, D.Uf[0][i], nullptr, D.U[i], D.tol_rel, D.step_max, nullptr, s));
+  return SEMX_OK;
+}
+
+void dns_free(semx_dns_s* D) {
+  if (!D) return;
+  for (Helm* h : {D->vel, D->pre, D->start[0], D->start[1]}) {
+    if (!h) continue;
+    helm_release(*h);
+    delete h;
+  }
+  dev_free(D->store);
+  delete D;
+}
+
+int make_system(Mesh& M, double lambda2, double beta, int nz, int method, Helm** out) {
+  Helm* H = new (std::nothrow) Helm();
+  if (!H) return SEMX_E_NOMEM;
+  H->M = &M;
+  H->lambda2 = lambda2; H->beta = beta; H->base = 0;
+  H->nmodes = nz > 1 ? nz / 2 : 1; H->nz = nz; H->method = method;
+  int st = helm_build(*H);
+  if (!st) st = helm_upload(*H);
+  if (st) { helm_release(*H); delete H; return st; }
+  *out = H;
+  return SEMX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int semx_dft_r(double* data, int tlen, int ntrn, int sign, void* stream) {
+  if (!data || tlen < 1 || ntrn < 0 || (sign != 1 && sign != -1)) return SEMX_E_ARG;
+  if (tlen < 2 || ntrn == 0) return SEMX_OK;
+  if (!fft_length_ok(tlen) || (ntrn & 1)) return SEMX_E_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  static double* scratch = nullptr;
+  static size_t cap = 0;
+  const size_t need = (size_t)tlen * ntrn;
+  if (need > cap) {
+    HIPCHK(hipStreamSynchronize(s));
+    dev_free(scratch);
+    int st = dev_upload<double>(&scratch, nullptr, need);
+    if (st) { cap = 0; return st; }
+    cap = need;
+  }
+  int st = dft_launch(data, scratch, tlen, ntrn, sign, s);
+  if (st) return st;
+  HIPCHK(hipMemcpyAsync(data, scratch, need * sizeof(double), hipMemcpyDeviceToDevice, s));
+  return SEMX_OK;
+}
+
+int semx_gradient(semx_mesh mesh, int nz, double beta, double* data, int dir, void* stream) {
+  if (!mesh || !data || nz < 1 || dir < 0 || dir > 2) return SEMX_E_ARG;
+  if (dir == 2 && (nz < 2 || (nz & 1))) return SEMX_E_ARG;
+  Mesh& M = mesh->M;
+  int st = mesh_upload(M);
+  if (st) return st;
+  return grad_launch(M, nz, beta, data, dir, (hipStream_t)stream);
+}
+
+int semx_smooth(semx_mesh mesh, int nz, double* data, void* stream) {
+  if (!mesh || !data || nz < 1) return SEMX_E_ARG;
+  Mesh& M = mesh->M;
+  int st = mesh_upload(M);
+  if (st) return st;
+  return smooth_launch(M, nz, data, (hipStream_t)stream);
+}
+
+int semx_dns_create(semx_mesh mesh, int nz, double beta, double dt, double kinvis, int norder,
+                    int advection, double tol_rel, int step_max, semx_dns* dns) {
+  if (!mesh || !dns || nz < 1 || (nz > 1 && (nz & 1)) || norder < 1 || norder > 3 ||
+      advection < 0 || advection > 1 || !(dt > 0.0) || !(kinvis > 0.0) || step_max < 1)
+    return SEMX_E_ARG;
+  if (nz > 2 && !fft_length_ok(nz)) return SEMX_E_ARG;
+  *dns = nullptr;
+  Mesh& M = mesh->M;
+  if (M.nbound) return SEMX_E_UNSUPPORTED;   // boundary conditions are not part of the DNS path
+  semx_dns_s* D = new (std::nothrow) semx_dns_s();
+  if (!D) return SEMX_E_NOMEM;
+  D->M = &M; D->nz = nz; D->nord = norder; D->advection = advection; D->beta = beta;
+  D->dt = dt; D->kinvis = kinvis; D->tol_rel = tol_rel; D->step_max = step_max;
+  D->ncom = nz > 1 ? 3 : 2; D->ndim = D->ncom;
+  D->ntot = (size_t)nz * M.planeSize;
+  int st = mesh_upload(M);
+  const size_t nfield = 3 + 1 + 2 * 3 * 3 + 2;
+  if (!st) st = dev_upload<double>(&D->store, nullptr, nfield * D->ntot);
+  if (st) { dns_free(D); return st; }
+  double* f = D->store;
+  for (int i = 0; i < 3; ++i) D->U[i] = f + (size_t)i * D->ntot;
+  f += 3 * D->ntot;
+  D->P = f; f += D->ntot;
+  for (int q = 0; q < 3; ++q)
+    for (int i = 0; i < 3; ++i) {
+      D->Us[q][i] = f; f += D->ntot;
+      D->Uf[q][i] = f; f += D->ntot;
+    }
+  D->tmp = f; f += D->ntot;
+  D->spare = f;
+  double alpha[4];
+  stiffly_stable(norder, alpha);
+  st = make_system(M, alpha[0] / (dt * kinvis), beta, nz, SEMX_DIRECT, &D->vel);
+  if (!st) st = make_system(M, 0.0, beta, nz, SEMX_DIRECT, &D->pre);
+  for (int q = 1; q < norder && !st; ++q) {
+    stiffly_stable(q, alpha);
+    st = make_system(M, alpha[0] / (dt * kinvis), beta, nz, SEMX_JACPCG, &D->start[q - 1]);
+  }
+  if (st) { dns_free(D); return st; }
+  *dns = D;
+  return SEMX_OK;
+}
+
+int semx_dns_destroy(semx_dns dns) {
+  if (!dns) return SEMX_E_ARG;
+  dns_free(dns);
+  return SEMX_OK;
+}
+
+int semx_dns_run(semx_dns dns, int nstep, double* u, double* v, double* w, double* p,
+                 void* stream) {
+  if (!dns || nstep < 0 || !u || !v || !p || (dns->ncom == 3 && !w)) return SEMX_E_ARG;
+  semx_dns_s& D = *dns;
+  hipStream_t s = (hipStream_t)stream;
+  double* ext[3] = {u, v, w};
+  const size_t bytes = D.ntot * sizeof(double);
+  for (int i = 0; i < D.ncom; ++i)
+    HIPCHK(hipMemcpyAsync(D.U[i], ext[i], bytes, hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync(D.P, p, bytes, hipMemcpyDeviceToDevice, s));
+  for (int k = 0; k < nstep; ++k) CHK(dns_step(D, s));
+  for (int i = 0; i < D.ncom; ++i)
+    HIPCHK(hipMemcpyAsync(ext[i], D.U[i], bytes, hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync(p, D.P, bytes, hipMemcpyDeviceToDevice, s));
+  return SEMX_OK;
+}
+
+int semx_dns_info(semx_dns dns, int* info) {
+  if (!dns || !info) return SEMX_E_ARG;
+  info[0] = dns->step; info[1] = dns->nz; info[2] = dns->ncom; info[3] = dns->nord;
+  return SEMX_OK;
+}
+
+}  // extern "C"

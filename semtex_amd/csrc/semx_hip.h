@@ -1,0 +1,51 @@
+// semx_hip.h -- HIP-side helpers shared by the kernel sources of libsemx:
+// status-checked HIP calls, device uploads, and the launch timing registry
+// behind semx_profile_*.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+
+#include "semx_internal.h"
+
+#define HIPCHK(x)                                                        \
+  do {                                                                   \
+    hipError_t e_ = (x);                                                 \
+    if (e_ != hipSuccess) {                                              \
+      fprintf(stderr, "semx: %s failed: %s\n", #x, hipGetErrorString(e_)); \
+      return SEMX_E_HIP;                                                 \
+    }                                                                    \
+  } while (0)
+
+namespace semx {
+
+// HIP events around a launch on its stream, accumulated per kernel name
+// while profiling is enabled (no-ops otherwise).
+void prof_begin(hipStream_t s, hipEvent_t* a);
+void prof_end(hipStream_t s, const char* name, hipEvent_t a);
+void prof_flush();
+
+template <class T>
+inline int dev_upload(T** dptr, const T* src, size_t count) {
+  *dptr = nullptr;
+  if (count == 0) count = 1;
+  HIPCHK(hipMalloc((void**)dptr, count * sizeof(T)));
+  if (src) HIPCHK(hipMemcpy(*dptr, src, count * sizeof(T), hipMemcpyHostToDevice));
+  else HIPCHK(hipMemset(*dptr, 0, count * sizeof(T)));
+  return SEMX_OK;
+}
+
+template <class T>
+inline void dev_free(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+// Field::solve over all planes of one system (the body of semx_helm_solve).
+int helm_solve(Helm& H, double* f, const double* bc, double* u, double tol_rel, int step_max,
+               int* iters, hipStream_t s);
+
+}  // namespace semx
+
+struct semx_mesh_s { semx::Mesh M; };
+struct semx_helm_s { semx::Helm H; };

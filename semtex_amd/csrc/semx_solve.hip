@@ -23,18 +23,9 @@
 #include <cstring>
 #include <new>
 
-#include "semx_internal.h"
+#include "semx_hip.h"
 
 using namespace semx;
-
-#define HIPCHK(x)                                                        \
-  do {                                                                   \
-    hipError_t e_ = (x);                                                 \
-    if (e_ != hipSuccess) {                                              \
-      fprintf(stderr, "semx: %s failed: %s\n", #x, hipGetErrorString(e_)); \
-      return SEMX_E_HIP;                                                 \
-    }                                                                    \
-  } while (0)
 
 namespace {
 
@@ -716,12 +707,16 @@ __global__ __launch_bounds__(PCG_THREADS) void k_pcg(PcgArgs a) {
   if (tid == 0) a.iters[blockIdx.x] = its;
 }
 
+}  // namespace
+
+namespace semx {
+
 // --- kernel timing: HIP events recorded around launches on their stream ---
 struct ProfRec { const char* name; hipEvent_t a, b; };
 struct ProfAcc { const char* name; double ms; long long count; };
-bool g_prof = false;
-std::vector<ProfRec> g_pending;
-std::vector<ProfAcc> g_acc;
+static bool g_prof = false;
+static std::vector<ProfRec> g_pending;
+static std::vector<ProfAcc> g_acc;
 
 void prof_begin(hipStream_t s, hipEvent_t* a) {
   *a = nullptr;
@@ -751,26 +746,6 @@ void prof_flush() {
   }
   g_pending.clear();
 }
-
-template <class T>
-int dev_upload(T** dptr, const T* src, size_t count) {
-  *dptr = nullptr;
-  if (count == 0) count = 1;
-  HIPCHK(hipMalloc((void**)dptr, count * sizeof(T)));
-  if (src) HIPCHK(hipMemcpy(*dptr, src, count * sizeof(T), hipMemcpyHostToDevice));
-  else HIPCHK(hipMemset(*dptr, 0, count * sizeof(T)));
-  return SEMX_OK;
-}
-
-template <class T>
-void dev_free(T*& p) {
-  if (p) (void)hipFree((void*)p);
-  p = nullptr;
-}
-
-}  // namespace
-
-namespace semx {
 
 int mesh_upload(Mesh& M) {
   if (M.on_device) return SEMX_OK;
@@ -872,14 +847,140 @@ void helm_release(Helm& H) {
   H.on_device = false;
 }
 
+int helm_solve(Helm& H, double* f, const double* bc, double* u, double tol_rel, int step_max,
+               int* iters, hipStream_t s) {
+  Mesh& M = *H.M;
+  if (iters) *iters = 0;
+  if (M.nbound && !bc) return SEMX_E_ARG;
+  if (!H.pcg_plane.empty() && (!(tol_rel >= 0.0) || step_max < 1)) return SEMX_E_ARG;
+  int st = helm_upload(H);
+  if (st) return st;
+  const int nzero = M.nglobal - M.nsolve;
+  const int nline = M.nbound * M.np;
+  if (nzero > 0) {
+    dim3 grid((nzero + 255) / 256, H.nplanes);
+    hipEvent_t ev;
+    prof_begin(s, &ev);
+    hipLaunchKernelGGL(k_ess_gather, grid, dim3(256), 0, s, bc, M.d.ess_src, H.d.plane_of,
+                       H.d.g, nzero, nline);
+    prof_end(s, "k_ess_gather", ev);
+  }
+  ElemArgs ea;
+  ea.np = M.np; ea.npnp = M.npnp; ea.next = M.next; ea.nint = M.nint; ea.nel = M.nel;
+  ea.planeSize = M.planeSize; ea.nline = nline; ea.nsolve = M.nsolve; ea.nzero = nzero;
+  ea.nset = M.nset; ea.lambda2 = H.lambda2; ea.beta = H.beta; ea.base = H.base;
+  ea.G = M.d.G; ea.eset = M.d.eset; ea.emap = M.d.emap; ea.btog = M.d.btog;
+  ea.emask = M.d.emask; ea.ops = M.d.ops; ea.nat_ptr = M.d.nat_ptr; ea.nat_b = M.d.nat_b;
+  ea.bside = M.d.bside; ea.barea = M.d.barea; ea.hbi = H.d.hbi; ea.hii = H.d.hii;
+  ea.plane_of = H.d.plane_of; ea.mode_n = H.d.mode_n; ea.mode_kind = H.d.mode_kind;
+  ea.Fb = H.d.Fb; ea.Fi = H.d.Fi;
+  ea.g = H.d.g; ea.sol = H.d.sol; ea.solstride = H.solstride; ea.f = f; ea.bc = bc; ea.u = u;
+  const int ce = ELT_THREADS / M.npnp;
+  dim3 eg((M.nel + ce - 1) / ce, H.nplanes);
+  const size_t lds_c = (2 * (size_t)M.npnp + 4 * (size_t)ce * M.npnp) * sizeof(double);
+  const size_t lds_b = (size_t)ce * (M.next + M.nint) * sizeof(double);
+  if (lds_c > 64 * 1024 || lds_b > 64 * 1024) {
+    HIPCHK(hipFuncSetAttribute((const void*)k_condense,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_c));
+    HIPCHK(hipFuncSetAttribute((const void*)k_backsub,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b));
+  }
+  hipEvent_t ev;
+  prof_begin(s, &ev);
+  hipLaunchKernelGGL(k_condense, eg, dim3(ELT_THREADS), lds_c, s, ea);
+  prof_end(s, "k_condense", ev);
+  if (!H.dmode.empty()) {
+    prof_begin(s, &ev);
+    hipLaunchKernelGGL(k_assemble, dim3((M.nglobal + 255) / 256, H.nplanes), dim3(256), 0, s,
+                       H.d.Fb, M.d.asm_ptr, M.d.asm_idx, H.d.plane_of, H.d.mode_n,
+                       H.d.mode_kind, H.d.sol, H.solstride, M.nglobal, M.nel * M.next);
+    prof_end(s, "k_assemble", ev);
+    BandArgs ba;
+    ba.Pf = H.d.Pf; ba.Pb = H.d.Pb; ba.Linv = H.d.Linv; ba.LinvT = H.d.LinvT;
+    ba.mode_n = H.d.mode_n; ba.offP = H.d.mode_offH; ba.offL = H.d.mode_offL;
+    ba.dmode = H.d.dmode; ba.mode_plane = H.d.mode_plane;
+    ba.asm_ptr = M.d.asm_ptr; ba.asm_idx = M.d.asm_idx;
+    ba.Fb = H.d.Fb; ba.sol = H.d.sol; ba.solstride = H.solstride; ba.kd = H.kd;
+    ba.kdp = H.kdp; ba.nel = M.nel; ba.next = M.next;
+    ba.ring = 64;
+    while (ba.ring < H.kd + 2 * H.nb) ba.ring *= 2;
+    int nr = 1;
+    for (int m : H.dmode) if (H.mode_plane[2 * m + 1] >= 0) nr = 2;
+    const size_t lds = (size_t)nr * (ba.ring + H.nb) * sizeof(double);
+    if (lds > 160 * 1024) return SEMX_E_UNSUPPORTED;
+    const void* kfn = nullptr;
+#define SEMX_PICK(NB_, NP_, BT_, D_)                                               \
+    if (H.nb == NB_ && H.npass == NP_ && H.bt == BT_)                              \
+      kfn = (nr == 2) ? (const void*)k_band_solve<NB_, NP_, BT_, 2, D_>            \
+                      : (const void*)k_band_solve<NB_, NP_, BT_, 1, D_>;
+    SEMX_PICK(32, 1, 64, 1)
+    SEMX_PICK(32, 1, 128, 1)
+    SEMX_PICK(32, 1, 256, 1)
+    SEMX_PICK(32, 1, 512, 1)
+    SEMX_PICK(16, 2, 512, 1)
+    SEMX_PICK(8, 4, 512, 1)
+#undef SEMX_PICK
+    if (!kfn) return SEMX_E_UNSUPPORTED;
+    HIPCHK(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    void* kargs[] = {&ba};
+    prof_begin(s, &ev);
+    HIPCHK(hipLaunchKernel(kfn, dim3((unsigned)H.dmode.size()), dim3(H.bt), kargs, lds, s));
+    prof_end(s, "k_band_solve", ev);
+  }
+  if (!H.dmode.empty()) {
+    prof_begin(s, &ev);
+    hipLaunchKernelGGL(k_backsub, eg, dim3(ELT_THREADS), lds_b, s, ea);
+    prof_end(s, "k_backsub", ev);
+  }
+  if (!H.pcg_plane.empty()) {
+    PcgArgs pa;
+    pa.np = M.np; pa.npnp = M.npnp; pa.next = M.next; pa.nint = M.nint; pa.nel = M.nel;
+    pa.nglobal = M.nglobal; pa.nsolve = M.nsolve; pa.nzero = nzero; pa.npts = H.npts;
+    pa.planeSize = M.planeSize; pa.nset = M.nset;
+    pa.lambda2 = H.lambda2; pa.beta = H.beta; pa.base = H.base;
+    pa.G = M.d.G; pa.eset = M.d.eset; pa.emap = M.d.emap; pa.pmap = M.d.pmap;
+    pa.btog = M.d.btog; pa.ops = M.d.ops; pa.asm_ptr = M.d.asm_ptr; pa.asm_idx = M.d.asm_idx;
+    pa.plane_of = H.d.plane_of; pa.pcg_plane = H.d.pcg_plane; pa.mode_n = H.d.mode_n;
+    pa.pc = H.d.pc; pa.pc32 = H.d.pc32; pa.Fb = H.d.Fb; pa.Fi = H.d.Fi; pa.g = H.d.g;
+    pa.u = u; pa.work = H.d.work; pa.workstride = H.workstride;
+    pa.tol = tol_rel; pa.step_max = step_max; pa.inner_tol = H.inner_tol;
+    pa.max_outer = H.max_outer; pa.iters = H.d.iters;
+    const int cep = PCG_THREADS / M.npnp;
+    const bool refine = H.pcg_prec == SEMX_PCG_FP32_REFINED;
+    const size_t lds = 32 * sizeof(double) +
+        (2 * (size_t)M.npnp + 3 * (size_t)cep * M.npnp) * sizeof(double);
+    const void* kfn = refine ? (const void*)k_pcg<float, true> : (const void*)k_pcg<double, false>;
+    if (lds > 64 * 1024)
+      HIPCHK(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    void* kargs[] = {&pa};
+    prof_begin(s, &ev);
+    HIPCHK(hipLaunchKernel(kfn, dim3((unsigned)H.pcg_plane.size()), dim3(PCG_THREADS), kargs,
+                           lds, s));
+    prof_end(s, "k_pcg", ev);
+    HIPCHK(hipGetLastError());
+    if (iters) {
+      // the step-limit WARNING of src/field.cpp:713 becomes a status
+      std::vector<int> it(H.pcg_plane.size());
+      HIPCHK(hipMemcpyAsync(it.data(), H.d.iters, it.size() * sizeof(int),
+                            hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      int mx = 0;
+      bool limit = false;
+      for (int v : it) { limit |= v < 0; mx = std::max(mx, v < 0 ? -v : v); }
+      *iters = mx;
+      if (limit) return SEMX_E_MAXITER;
+    }
+  }
+  HIPCHK(hipGetLastError());
+  return SEMX_OK;
+}
+
 }  // namespace semx
 
 // ===========================================================================
 // C ABI
 // ===========================================================================
 
-struct semx_mesh_s { Mesh M; };
-struct semx_helm_s { Helm H; };
 
 extern "C" {
 
@@ -1022,132 +1123,7 @@ int semx_helm_element(semx_helm helm, int m, int elmt, double* hbb, double* hbi,
 int semx_helm_solve(semx_helm helm, double* f, const double* bc, double* u, double tol_rel,
                     int step_max, int* iters, void* stream) {
   if (!helm || !f || !u) return SEMX_E_ARG;
-  Helm& H = helm->H;
-  Mesh& M = *H.M;
-  if (iters) *iters = 0;
-  if (M.nbound && !bc) return SEMX_E_ARG;
-  if (!H.pcg_plane.empty() && (!(tol_rel >= 0.0) || step_max < 1)) return SEMX_E_ARG;
-  int st = helm_upload(H);
-  if (st) return st;
-  hipStream_t s = (hipStream_t)stream;
-  const int nzero = M.nglobal - M.nsolve;
-  const int nline = M.nbound * M.np;
-  if (nzero > 0) {
-    dim3 grid((nzero + 255) / 256, H.nplanes);
-    hipEvent_t ev;
-    prof_begin(s, &ev);
-    hipLaunchKernelGGL(k_ess_gather, grid, dim3(256), 0, s, bc, M.d.ess_src, H.d.plane_of,
-                       H.d.g, nzero, nline);
-    prof_end(s, "k_ess_gather", ev);
-  }
-  ElemArgs ea;
-  ea.np = M.np; ea.npnp = M.npnp; ea.next = M.next; ea.nint = M.nint; ea.nel = M.nel;
-  ea.planeSize = M.planeSize; ea.nline = nline; ea.nsolve = M.nsolve; ea.nzero = nzero;
-  ea.nset = M.nset; ea.lambda2 = H.lambda2; ea.beta = H.beta; ea.base = H.base;
-  ea.G = M.d.G; ea.eset = M.d.eset; ea.emap = M.d.emap; ea.btog = M.d.btog;
-  ea.emask = M.d.emask; ea.ops = M.d.ops; ea.nat_ptr = M.d.nat_ptr; ea.nat_b = M.d.nat_b;
-  ea.bside = M.d.bside; ea.barea = M.d.barea; ea.hbi = H.d.hbi; ea.hii = H.d.hii;
-  ea.plane_of = H.d.plane_of; ea.mode_n = H.d.mode_n; ea.mode_kind = H.d.mode_kind;
-  ea.Fb = H.d.Fb; ea.Fi = H.d.Fi;
-  ea.g = H.d.g; ea.sol = H.d.sol; ea.solstride = H.solstride; ea.f = f; ea.bc = bc; ea.u = u;
-  const int ce = ELT_THREADS / M.npnp;
-  dim3 eg((M.nel + ce - 1) / ce, H.nplanes);
-  const size_t lds_c = (2 * (size_t)M.npnp + 4 * (size_t)ce * M.npnp) * sizeof(double);
-  const size_t lds_b = (size_t)ce * (M.next + M.nint) * sizeof(double);
-  if (lds_c > 64 * 1024 || lds_b > 64 * 1024) {
-    HIPCHK(hipFuncSetAttribute((const void*)k_condense,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_c));
-    HIPCHK(hipFuncSetAttribute((const void*)k_backsub,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b));
-  }
-  hipEvent_t ev;
-  prof_begin(s, &ev);
-  hipLaunchKernelGGL(k_condense, eg, dim3(ELT_THREADS), lds_c, s, ea);
-  prof_end(s, "k_condense", ev);
-  if (!H.dmode.empty()) {
-    prof_begin(s, &ev);
-    hipLaunchKernelGGL(k_assemble, dim3((M.nglobal + 255) / 256, H.nplanes), dim3(256), 0, s,
-                       H.d.Fb, M.d.asm_ptr, M.d.asm_idx, H.d.plane_of, H.d.mode_n,
-                       H.d.mode_kind, H.d.sol, H.solstride, M.nglobal, M.nel * M.next);
-    prof_end(s, "k_assemble", ev);
-    BandArgs ba;
-    ba.Pf = H.d.Pf; ba.Pb = H.d.Pb; ba.Linv = H.d.Linv; ba.LinvT = H.d.LinvT;
-    ba.mode_n = H.d.mode_n; ba.offP = H.d.mode_offH; ba.offL = H.d.mode_offL;
-    ba.dmode = H.d.dmode; ba.mode_plane = H.d.mode_plane;
-    ba.asm_ptr = M.d.asm_ptr; ba.asm_idx = M.d.asm_idx;
-    ba.Fb = H.d.Fb; ba.sol = H.d.sol; ba.solstride = H.solstride; ba.kd = H.kd;
-    ba.kdp = H.kdp; ba.nel = M.nel; ba.next = M.next;
-    ba.ring = 64;
-    while (ba.ring < H.kd + 2 * H.nb) ba.ring *= 2;
-    int nr = 1;
-    for (int m : H.dmode) if (H.mode_plane[2 * m + 1] >= 0) nr = 2;
-    const size_t lds = (size_t)nr * (ba.ring + H.nb) * sizeof(double);
-    if (lds > 160 * 1024) return SEMX_E_UNSUPPORTED;
-    const void* kfn = nullptr;
-#define SEMX_PICK(NB_, NP_, BT_, D_)                                               \
-    if (H.nb == NB_ && H.npass == NP_ && H.bt == BT_)                              \
-      kfn = (nr == 2) ? (const void*)k_band_solve<NB_, NP_, BT_, 2, D_>            \
-                      : (const void*)k_band_solve<NB_, NP_, BT_, 1, D_>;
-    SEMX_PICK(32, 1, 64, 1)
-    SEMX_PICK(32, 1, 128, 1)
-    SEMX_PICK(32, 1, 256, 1)
-    SEMX_PICK(32, 1, 512, 1)
-    SEMX_PICK(16, 2, 512, 1)
-    SEMX_PICK(8, 4, 512, 1)
-#undef SEMX_PICK
-    if (!kfn) return SEMX_E_UNSUPPORTED;
-    HIPCHK(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    void* kargs[] = {&ba};
-    prof_begin(s, &ev);
-    HIPCHK(hipLaunchKernel(kfn, dim3((unsigned)H.dmode.size()), dim3(H.bt), kargs, lds, s));
-    prof_end(s, "k_band_solve", ev);
-  }
-  if (!H.dmode.empty()) {
-    prof_begin(s, &ev);
-    hipLaunchKernelGGL(k_backsub, eg, dim3(ELT_THREADS), lds_b, s, ea);
-    prof_end(s, "k_backsub", ev);
-  }
-  if (!H.pcg_plane.empty()) {
-    PcgArgs pa;
-    pa.np = M.np; pa.npnp = M.npnp; pa.next = M.next; pa.nint = M.nint; pa.nel = M.nel;
-    pa.nglobal = M.nglobal; pa.nsolve = M.nsolve; pa.nzero = nzero; pa.npts = H.npts;
-    pa.planeSize = M.planeSize; pa.nset = M.nset;
-    pa.lambda2 = H.lambda2; pa.beta = H.beta; pa.base = H.base;
-    pa.G = M.d.G; pa.eset = M.d.eset; pa.emap = M.d.emap; pa.pmap = M.d.pmap;
-    pa.btog = M.d.btog; pa.ops = M.d.ops; pa.asm_ptr = M.d.asm_ptr; pa.asm_idx = M.d.asm_idx;
-    pa.plane_of = H.d.plane_of; pa.pcg_plane = H.d.pcg_plane; pa.mode_n = H.d.mode_n;
-    pa.pc = H.d.pc; pa.pc32 = H.d.pc32; pa.Fb = H.d.Fb; pa.Fi = H.d.Fi; pa.g = H.d.g;
-    pa.u = u; pa.work = H.d.work; pa.workstride = H.workstride;
-    pa.tol = tol_rel; pa.step_max = step_max; pa.inner_tol = H.inner_tol;
-    pa.max_outer = H.max_outer; pa.iters = H.d.iters;
-    const int cep = PCG_THREADS / M.npnp;
-    const bool refine = H.pcg_prec == SEMX_PCG_FP32_REFINED;
-    const size_t lds = 32 * sizeof(double) +
-        (2 * (size_t)M.npnp + 3 * (size_t)cep * M.npnp) * sizeof(double);
-    const void* kfn = refine ? (const void*)k_pcg<float, true> : (const void*)k_pcg<double, false>;
-    if (lds > 64 * 1024)
-      HIPCHK(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    void* kargs[] = {&pa};
-    prof_begin(s, &ev);
-    HIPCHK(hipLaunchKernel(kfn, dim3((unsigned)H.pcg_plane.size()), dim3(PCG_THREADS), kargs,
-                           lds, s));
-    prof_end(s, "k_pcg", ev);
-    HIPCHK(hipGetLastError());
-    if (iters) {
-      // the step-limit WARNING of src/field.cpp:713 becomes a status
-      std::vector<int> it(H.pcg_plane.size());
-      HIPCHK(hipMemcpyAsync(it.data(), H.d.iters, it.size() * sizeof(int),
-                            hipMemcpyDeviceToHost, s));
-      HIPCHK(hipStreamSynchronize(s));
-      int mx = 0;
-      bool limit = false;
-      for (int v : it) { limit |= v < 0; mx = std::max(mx, v < 0 ? -v : v); }
-      *iters = mx;
-      if (limit) return SEMX_E_MAXITER;
-    }
-  }
-  HIPCHK(hipGetLastError());
-  return SEMX_OK;
+  return helm_solve(helm->H, f, bc, u, tol_rel, step_max, iters, (hipStream_t)stream);
 }
 
 int semx_helm_set_pcg(semx_helm helm, int precision, double inner_tol, int max_outer) {
