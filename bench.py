@@ -29,6 +29,11 @@ METRIC = "Helmholtz DOF-solves/s (all modes) & DNS steps/s at 1e-10 rel err vs F
 HBM_PEAK_GBS = 8000.0
 
 
+def progress(msg):
+    """Progress on stderr (stdout carries only the JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -39,6 +44,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip timing the JACPCG / MIXED variants")
+    ap.add_argument("--dns-steps", type=int, default=20,
+                    help="timed DNS steps (after N_TIME start-up steps); 0 skips the DNS")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 256 MiB (scale for HBM "
                          "byte counters) and print its timing")
@@ -130,7 +137,9 @@ def main():
         torch.cuda.synchronize()
         return tu.cpu().numpy().reshape(nz, P.planeSize), it
 
+    progress(f"helmholtz setup {t_setup:.2f} s; timing {args.steps} solves")
     ms, kprof = timed(S, args.steps, args.warmup)
+    progress(f"direct solve {ms:.4f} ms")
     u, _ = fresh_solve(S)
     err_exact = float(np.abs(u[:, :P.nPlane] - ue[:, :P.nPlane]).max())
 
@@ -195,6 +204,7 @@ def main():
                                     ("mixed_fp32_refined", sx.MIXED, sx.PCG_FP32_REFINED)):
             V = sx.ModalMatrixSys(P, lam2, beta, 0, nz // 2, nz, method)
             V.set_pcg(prec, inner_tol=1e-4, max_outer=30)
+            progress(f"variant {label}")
             uv, it = fresh_solve(V, tol=1e-12)
             vms, vprof = timed(V, max(1, args.steps // 4), max(1, args.warmup // 4))
             variants[label] = {"ms_per_step": vms, "value": dofs / (vms * 1e-3),
@@ -202,6 +212,7 @@ def main():
             del V
 
     if not args.no_cpu:
+        progress("cpu baseline (oracle)")
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as O
         cores = min(16, os.cpu_count() or 1)
@@ -233,7 +244,67 @@ def main():
         v.pop("u")
     if variants:
         result["iterative_variants"] = variants
+    if args.dns_steps > 0:
+        result["dns"] = dns_bench(args, torch, sx, L)
     print(json.dumps(result), flush=True)
+
+
+def dns_bench(args, torch, sx, L):
+    """DNS steps/s: Taylor-Green vortex on [0,2pi]^3 periodic, 24x24 quads
+    np=9, Nz=96, Re=1600 (nu=1/1600), dt=1e-3, N_TIME=3, full
+    skew-symmetric nonlinear terms, DIRECT velocity and pressure systems;
+    timed after the start-up steps, fields resident on the GPU."""
+    import ctypes as C
+    from semtex_amd.workloads import taylor_green_3d, workload_mesh
+    dev = torch.device("cuda:0")
+    np_, m, nz = workload_mesh("periodic24_np9_nz96")
+    nu, dt, beta, norder = 1.0 / 1600, 1e-3, 1.0, 3
+    progress("dns setup (host factorisation of the velocity and pressure systems)")
+    t0 = time.time()
+    P = sx.Mesh(np_, m)
+    D = sx.DNS(P, nz, beta, dt, nu, norder=norder, advection=0, tol_rel=1e-12, step_max=500)
+    t_setup = time.time() - t0
+    u, v, w = (torch.from_numpy(a.ravel().copy()).to(dev)
+               for a in taylor_green_3d(P, nz, beta, seed=3))
+    p = torch.zeros_like(u)
+    progress(f"dns setup {t_setup:.1f} s; start-up steps")
+    D.run(norder + 2, u, v, w, p)          # start-up (JACPCG) steps and warm-up
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    D.run(args.dns_steps, u, v, w, p)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.dns_steps
+    L.semx_profile_reset()
+    L.semx_profile_enable(1)
+    D.run(args.dns_steps, u, v, w, p)
+    torch.cuda.synchronize()
+    prof = {}
+    for name in ("k_band_solve", "k_condense", "k_assemble", "k_backsub", "k_dft_z",
+                 "k_grad_xy", "k_grad_z", "k_smooth", "dns_nonlinear"):
+        tot = C.c_double(); cnt = C.c_longlong()
+        L.semx_profile_query(name.encode(), C.byref(tot), C.byref(cnt))
+        if cnt.value:
+            prof[name] = {"ms_per_step": tot.value / args.dns_steps,
+                          "launches_per_step": cnt.value / args.dns_steps}
+    L.semx_profile_enable(0)
+    umax = float(u.abs().max().item())
+    return {
+        "metric": "DNS steps/s",
+        "value": 1e3 / ms,
+        "ms_per_step": ms,
+        "steps": args.dns_steps,
+        "dtype": "f64",
+        "workload": "Taylor-Green + 1e-3 curl perturbation, [0,2pi]^3 periodic, 24x24 quads "
+                    "np=9, Nz=96, nu=1/1600, dt=1e-3, N_TIME=3, skew-symmetric, DIRECT",
+        "setup_s": round(t_setup, 2),
+        "steps_total": D.steps,
+        "max_abs_u_fourier": umax,
+        "kernel_ms": prof,
+    }
 
 
 if __name__ == "__main__":

@@ -72,3 +72,57 @@ def workload_mesh(name):
     np_, K, nz, per = WORKLOADS[name]
     two_pi = 2 * np.pi
     return np_, rect_mesh(K, K, 0, two_pi, 0, two_pi, periodic_x=per, periodic_y=per), nz
+
+
+def fourier_pack(phys):
+    """z-planes of physical values (nz, n) -> semtex's Fourier planes
+    (dDFTr forward packing [Re0, Re_nz/2, Re1, Im1, ...], scaled 1/nz)."""
+    nz = phys.shape[0]
+    if nz == 1:
+        return phys.copy()
+    X = np.fft.rfft(phys, axis=0) / nz
+    out = np.zeros_like(phys)
+    out[0] = X[0].real
+    out[1] = X[nz // 2].real
+    for j in range(1, nz // 2):
+        out[2 * j] = X[j].real
+        out[2 * j + 1] = X[j].imag
+    return out
+
+
+def taylor_green_3d(mesh, nz, beta=1.0, seed=3, amp=1e-3):
+    """Fourier-space (u, v, w) of the Taylor-Green vortex
+    u = sin x cos y cos z, v = -cos x sin y cos z, w = 0 on [0, 2 pi]^3
+    (z = 2 pi k / (beta nz)), plus a divergence-free perturbation
+    curl(A) with A a seeded random periodic potential of wavenumbers
+    |k| <= 3, scaled to max |curl A| = amp.  Arrays (nz, planeSize)."""
+    rng = np.random.default_rng(seed)
+    x, y = mesh.x, mesh.y
+    z = 2 * np.pi * np.arange(nz) / (beta * nz)
+    X, Z = x[None, :], z[:, None]
+    Y = y[None, :]
+    u = np.sin(X) * np.cos(Y) * np.cos(beta * Z)
+    v = -np.cos(X) * np.sin(Y) * np.cos(beta * Z)
+    w = np.zeros_like(u)
+    # A_c = sum a cos(kx x + ky y + kz z + phi); curl A analytically
+    du = np.zeros_like(u); dv = np.zeros_like(u); dw = np.zeros_like(u)
+    for _ in range(12):
+        k = rng.integers(-3, 4, size=3)
+        if not k.any():
+            continue
+        a = rng.standard_normal(3)
+        phi = rng.uniform(0, 2 * np.pi)
+        arg = k[0] * X + k[1] * Y + k[2] * beta * Z + phi
+        s = -np.sin(arg)     # d/d(arg) of cos
+        kz = k[2] * beta
+        # curl A = (dAz/dy - dAy/dz, dAx/dz - dAz/dx, dAy/dx - dAx/dy)
+        du += (a[2] * k[1] - a[1] * kz) * s
+        dv += (a[0] * kz - a[2] * k[0]) * s
+        dw += (a[1] * k[0] - a[0] * k[1]) * s
+    scale = amp / max(np.abs(du).max(), np.abs(dv).max(), np.abs(dw).max(), 1e-300)
+    fields = []
+    for f, d in ((u, du), (v, dv), (w, dw)):
+        full = np.zeros((nz, mesh.planeSize))
+        full[:, :mesh.nPlane] = fourier_pack(f + scale * d)
+        fields.append(full)
+    return fields
