@@ -135,6 +135,17 @@ int semx_helm_element (semx_helm helm, int m, int elmt, double* hbb,
 int semx_helm_solve (semx_helm helm, double* f, const double* bc, double* u,
                      double tol_rel, int step_max, int* iters, void* stream);
 
+/* Field::solve for nfield (1..3) fields that share one ModalMatrixSys, as
+ * the velocity components u, v, w do in semtex (the MatrixSys cache,
+ * src/matrix.cpp:85-91, and Solve per component, dns/integrate.cpp:
+ * 365-395).  f, bc, u: host arrays of nfield device pointers, each as in
+ * semx_helm_solve (bc may be NULL on a mesh without boundaries).  The band
+ * factor of each mode is streamed once for all fields' planes.            */
+int semx_helm_solve_fields (semx_helm helm, int nfield, double* const* f,
+                            const double* const* bc, double* const* u,
+                            double tol_rel, int step_max, int* iters,
+                            void* stream);
+
 /* Precision of the JACPCG planes of a system.  SEMX_PCG_FP64 (the default)
  * is the FP64 loop of Field::solve (src/field.cpp:627-720).
  * SEMX_PCG_FP32_REFINED has no counterpart in semtex: the residual

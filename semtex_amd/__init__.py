@@ -30,7 +30,7 @@ PCG_FP64, PCG_FP32_REFINED = 0, 1
 E_MAXITER = 3
 FORWARD, INVERSE = 1, -1
 
-__all__ = ["Mesh", "ModalMatrixSys", "solve", "dft_r", "gradient", "smooth", "DNS",
+__all__ = ["Mesh", "ModalMatrixSys", "solve", "solve_fields", "dft_r", "gradient", "smooth", "DNS",
            "DIRECT", "JACPCG", "MIXED",
            "PCG_FP64", "PCG_FP32_REFINED",
            "rect_mesh", "bc_lines", "side_nodes", "SemxError", "load"]
@@ -164,6 +164,33 @@ def solve(msys, f, bc, u, tol_rel=1e-12, step_max=10000, stream=None, iters=True
         warnings.warn("Field::solve: step limit exceeded", RuntimeWarning)
     else:
         check(st, "semx_helm_solve")
+    return it.value
+
+
+def solve_fields(msys, fs, bcs, us, tol_rel=1e-12, step_max=10000, stream=None, iters=True):
+    """Field::solve for up to 3 fields sharing msys (as u, v, w do): one
+    sweep of each mode's factor for all of them.  Lists of device tensors;
+    bcs may be None on a mesh without boundaries."""
+    import torch
+    M = msys.mesh
+    nf = len(fs)
+    assert len(us) == nf and 1 <= nf <= 3
+    for t in list(fs) + list(us):
+        assert t.numel() == msys.nz * M.planeSize
+    arr = C.c_void_p * nf
+    fa = arr(*[_ptr(t) for t in fs])
+    ua = arr(*[_ptr(t) for t in us])
+    ba = arr(*[_ptr(t) for t in bcs]) if bcs is not None else None
+    if stream is None:
+        stream = torch.cuda.current_stream().cuda_stream
+    it = C.c_int(0)
+    st = load().semx_helm_solve_fields(msys.h, nf, fa, ba, ua, float(tol_rel), int(step_max),
+                                       C.byref(it) if iters else None, C.c_void_p(stream))
+    if st == E_MAXITER:
+        import warnings
+        warnings.warn("Field::solve: step limit exceeded", RuntimeWarning)
+    else:
+        check(st, "semx_helm_solve_fields")
     return it.value
 
 

@@ -34,6 +34,8 @@ PROTOTYPES = {
     "semx_helm_element": (C.c_int, [VP, C.c_int, C.c_int, D, D, D]),
     "semx_helm_solve": (C.c_int, [VP, VP, VP, VP, C.c_double, C.c_int, I, VP]),
     "semx_helm_set_pcg": (C.c_int, [VP, C.c_int, C.c_double, C.c_int]),
+    "semx_helm_solve_fields": (C.c_int, [VP, C.c_int, C.POINTER(VP), C.POINTER(VP),
+                                         C.POINTER(VP), C.c_double, C.c_int, I, VP]),
     "semx_dft_r": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, VP]),
     "semx_gradient": (C.c_int, [VP, C.c_int, C.c_double, VP, C.c_int, VP]),
     "semx_smooth": (C.c_int, [VP, C.c_int, VP, VP]),

@@ -492,9 +492,11 @@ int dns_step(semx_dns_s& D, hipStream_t s) {
   for (int i = 0; i < ncom; ++i) EW(k_copy, D.Us[0][i], D.U[i]);
   roll(D.Us, nord, ncom);
   // viscous solves: start-up JACPCG systems while step < N_TIME
+  // (the velocity components share one system: one sweep of its factor)
   Helm* V = (D.step < nord) ? D.start[D.step - 1] : D.vel;
-  for (int i = 0; i < ncom; ++i)
-    CHK(helm_solve(*V, D.Uf[0][i], nullptr, D.U[i], D.tol_rel, D.step_max, nullptr, s));
+  double* fs[3] = {D.Uf[0][0], D.Uf[0][1], D.Uf[0][2]};
+  double* us[3] = {D.U[0], D.U[1], D.U[2]};
+  CHK(helm_solve_fields(*V, ncom, fs, nullptr, us, D.tol_rel, D.step_max, nullptr, s));
   return SEMX_OK;
 }
 

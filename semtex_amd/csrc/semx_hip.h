@@ -44,6 +44,11 @@ inline void dev_free(T*& p) {
 // Field::solve over all planes of one system (the body of semx_helm_solve).
 int helm_solve(Helm& H, double* f, const double* bc, double* u, double tol_rel, int step_max,
                int* iters, hipStream_t s);
+// The same for nf <= SEMX_MAX_FIELDS fields sharing the system: the element
+// kernels run per field, the band factor is streamed once for all of them.
+int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* bcs,
+                      double* const* us, double tol_rel, int step_max, int* iters,
+                      hipStream_t s);
 
 }  // namespace semx
 

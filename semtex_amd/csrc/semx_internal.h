@@ -11,6 +11,7 @@
 
 namespace semx {
 
+constexpr int SEMX_MAX_FIELDS = 3;   // fields per batched solve (u, v, w)
 constexpr double EPSDP = 6.0e-14;  // roundoff threshold of veclib/cveclib.h:58
 constexpr double EPSSP = 6.0e-7;   // singular-system threshold, cveclib.h:54
 
@@ -122,6 +123,7 @@ struct Helm {
   double inner_tol = 1e-5;            // inner relative tolerance (refined)
   int max_outer = 12;                 // refinement sweeps (refined)
   size_t workstride = 0;              // doubles per PCG plane
+  int nf_cap = 0;                     // fields the device scratch holds
   bool on_device = false;
   HelmDev d;
 };

@@ -186,6 +186,8 @@ struct BandArgs {
   const double* Fb;
   double* sol;
   int solstride, kd, kdp, nel, next, ring;
+  int npl;                  // right-hand sides per field and mode (1 or 2 planes)
+  long long fstride;        // sol offset between fields
 };
 
 template <int G>
@@ -356,18 +358,246 @@ __global__ __launch_bounds__(BT) void k_band_solve(BandArgs a) {
   double* gy[NR];
 #pragma unroll
   for (int c = 0; c < NR; ++c) {
-    int p = a.mode_plane[2 * m + c];
-    // a mode with fewer planes than NR repeats its first plane; the
-    // duplicate sweeps identical data and writes identical values
+    // right-hand side c: plane c % npl of the mode in field c / npl; a mode
+    // with fewer planes than npl repeats its first plane (the duplicate
+    // sweeps identical data and writes identical values)
+    const int fld = c / a.npl, kk = c - fld * a.npl;
+    int p = a.mode_plane[2 * m + kk];
     if (p < 0) p = a.mode_plane[2 * m];
     ring[c] = lds + (size_t)c * a.ring;
-    gy[c] = a.sol + (size_t)p * a.solstride;
+    gy[c] = a.sol + fld * a.fstride + (size_t)p * a.solstride;
   }
   double* ybuf = lds + (size_t)NR * a.ring;
   band_sweep<NB, NPASS, BT, NR, D, true>(a.Pf + a.offP[m], a.Linv + a.offL[m], ring, gy,
                                           ybuf, n, nblk, a.kd, a.kdp, a.ring - 1);
   band_sweep<NB, NPASS, BT, NR, D, false>(a.Pb + a.offP[m], a.LinvT + a.offL[m], ring, gy,
                                            ybuf, n, nblk, a.kd, a.kdp, a.ring - 1);
+}
+
+// ===========================================================================
+// Cooperative band solve: W workgroups per mode in a lagged pipeline.
+//
+// The off-diagonal panel of block J (NB columns x kdp rows) is cut into
+// NB-row chunks; chunk c of block J feeds block J+1+c (forward; block
+// J-1-c backward).  Lane 0 of a mode (the lead) owns chunks [0, C1): it
+// solves each diagonal block, publishes y_J, and applies the chunks that
+// feed the next C1 blocks.  Lanes w >= 1 (followers) own further chunks;
+// each waits for y_J, applies its chunks into an LDS accumulator per
+// target block, and publishes the finished partial sum of target T once
+// it has applied its last contribution to T (from block T-1-c_lo(w)).  The
+// lead reads those sums when it reaches T, which followers finished at
+// least c_lo(w) >= C1 steps earlier, so in the steady state nobody waits.
+// Hand-offs are flags in global memory set with release and read with
+// acquire semantics at agent scope; flag values grow with every launch
+// (2 epoch forward, 2 epoch + 1 backward) so nothing is cleared between
+// solves.  Every wait is bounded: a timeout sets the error word and the
+// workgroup leaves, so the grid always drains.
+// ===========================================================================
+
+struct CoopArgs {
+  const double* Pf;
+  const double* Pb;
+  const double* Linv;
+  const double* LinvT;
+  const int* mode_n;
+  const long long* offP;
+  const long long* offL;
+  const int* dmode;
+  const int* mode_plane;
+  double* sol;
+  double* part;             // [md][lane][nblkmax][NR][NB]
+  unsigned* yflag;          // [md][nblkmax]
+  unsigned* pflag;          // [md][lane][nblkmax]
+  const int* clo;           // [W+1] chunk ranges per lane
+  int* err;
+  int solstride, kd, kdp, ndirect, W, nblkmax;
+  unsigned epoch;
+};
+
+constexpr int COOP_BT = 256;
+constexpr int COOP_TT = COOP_BT - 64;   // task threads; the last wave signals
+constexpr int COOP_RING = 16;           // accumulator blocks per lane (>= chunks + 2)
+constexpr long COOP_SPIN = 1L << 24;    // polls before a wait gives up
+
+__device__ inline bool coop_wait(const unsigned* f, unsigned target, int* err) {
+  for (long it = 0; it < COOP_SPIN; ++it) {
+    if (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= target) return true;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return false;
+}
+
+__device__ inline void coop_signal(unsigned* f, unsigned v) {
+  __hip_atomic_store(f, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One sweep; returns false if a wait timed out (the whole workgroup agrees).
+// The first COOP_TT threads own panel rows ("tasks"); the last wave does the
+// flag traffic, so its release/acquire waits never stall on panel loads.
+template <int NB, int NR, int RPT, bool FWD>
+__device__ bool coop_sweep(const CoopArgs& a, int md, int lane, const double* __restrict__ P,
+                           const double* __restrict__ L, double* const (&gy)[NR], int n,
+                           int nblk, double* acc, double* rhs, double* ybuf, int* sflag) {
+  constexpr int QPT = (NB * NB / COOP_BT > 0) ? NB * NB / COOP_BT : 1;
+  constexpr int G = NB / QPT;
+  const int tid = threadIdx.x;
+  const bool syncer = tid == COOP_TT;
+  const unsigned target = 2u * a.epoch + (FWD ? 0u : 1u);
+  const int c_lo = a.clo[lane], c_hi = a.clo[lane + 1];
+  const int ntask = (c_hi - c_lo) * NB;
+  const int W = a.W;
+  unsigned* yflag = a.yflag + (size_t)md * a.nblkmax;
+  unsigned* pflag = a.pflag + (size_t)md * W * a.nblkmax;
+  double* part = a.part + (size_t)md * W * a.nblkmax * NR * NB;
+  const bool lead = lane == 0;
+  const int last = lead ? nblk - 1 : nblk - 2 - c_lo;   // last step with work
+  for (int i = tid; i < COOP_RING * NR * NB; i += COOP_BT) acc[i] = 0.0;
+
+  double pp[RPT][NB];
+  auto load_panel = [&](int s) {
+    const int J = FWD ? s : nblk - 1 - s;
+    const double* pj = P + (size_t)J * NB * a.kdp;
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int t = min(tid + q * COOP_TT, max(ntask - 1, 0));
+      const int r = min(c_lo * NB + t, a.kdp - 1);
+#pragma unroll
+      for (int jj = 0; jj < NB; ++jj) pp[q][jj] = pj[(size_t)jj * a.kdp + r];
+    }
+  };
+  auto apply = [&](int s) {
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int t = tid + q * COOP_TT;
+      if (tid < COOP_TT && t < ntask) {
+        const int c = c_lo + t / NB, r = t - (t / NB) * NB;
+        const int tslot = (s + 1 + c) & (COOP_RING - 1);
+        const int tr = FWD ? r : NB - 1 - r;
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+          double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll
+          for (int jj = 0; jj < NB; jj += 4) {
+            a0 += pp[q][jj] * ybuf[k * NB + jj];
+            a1 += pp[q][jj + 1] * ybuf[k * NB + jj + 1];
+            a2 += pp[q][jj + 2] * ybuf[k * NB + jj + 2];
+            a3 += pp[q][jj + 3] * ybuf[k * NB + jj + 3];
+          }
+          acc[(tslot * NR + k) * NB + tr] += (a0 + a1) + (a2 + a3);
+        }
+      }
+    }
+  };
+  if (last >= 0 && tid < COOP_TT) load_panel(0);
+  __syncthreads();
+
+  for (int s = 0; s <= last; ++s) {
+    const int J = FWD ? s : nblk - 1 - s;
+    const int j0 = J * NB;
+    const int slot = s & (COOP_RING - 1);
+    if (lead) {
+      // 1. the followers' finished partial sums for this block
+      if (syncer) {
+        int ok = 1;
+        for (int w = 1; w < W && ok; ++w)
+          if (s >= 1 + a.clo[w])
+            ok = coop_wait(pflag + (size_t)w * a.nblkmax + s, target, a.err);
+        *sflag = ok;
+      }
+      __syncthreads();
+      if (!*sflag) return false;
+      // 2. right-hand side of the block
+      if (tid < NR * NB) {
+        const int c = tid / NB, i = tid - c * NB;
+        const int row = j0 + i;
+        double v = (row < n) ? gy[c][row] : 0.0;
+        v -= acc[slot * NR * NB + tid];
+        for (int w = 1; w < W; ++w)
+          if (s >= 1 + a.clo[w]) v -= part[(((size_t)w * a.nblkmax + s) * NR + c) * NB + i];
+        rhs[tid] = v;
+        acc[slot * NR * NB + tid] = 0.0;
+      }
+      __syncthreads();
+      // 3. diagonal block: y_J = L_J rhs, to LDS and to the solution vector
+      {
+        const bool dact = tid < NB * G;
+        const int da = dact ? tid / G : 0, dg = dact ? tid % G : 0;
+        const double* lj = L + (size_t)J * NB * NB + da * NB + dg * QPT;
+#pragma unroll
+        for (int c = 0; c < NR; ++c) {
+          double accd = 0.0;
+#pragma unroll
+          for (int q = 0; q < QPT; ++q) accd += lj[q] * rhs[c * NB + dg * QPT + q];
+          const double v = group_sum<G>(accd);
+          if (dact && dg == 0) {
+            ybuf[c * NB + da] = v;
+            if (j0 + da < n) gy[c][j0 + da] = v;
+          }
+        }
+      }
+      __syncthreads();
+      // 4. publish y_J; 5. apply the lead's chunks; 6. prefetch
+      if (syncer) coop_signal(yflag + s, target);
+      apply(s);
+      if (s < last && tid < COOP_TT) load_panel(s + 1);
+    } else {
+      if (syncer) *sflag = coop_wait(yflag + s, target, a.err);
+      __syncthreads();
+      if (!*sflag) return false;
+      if (tid < NR * NB) {
+        const int c = tid / NB, i = tid - c * NB;
+        const int row = j0 + i;
+        ybuf[tid] = (row < n) ? gy[c][row] : 0.0;
+      }
+      __syncthreads();
+      apply(s);
+      __syncthreads();
+      // target s+1+c_lo received its last contribution from this lane
+      const int T = s + 1 + c_lo;
+      const int tslot = T & (COOP_RING - 1);
+      if (T < nblk && tid < NR * NB) {
+        part[((size_t)lane * a.nblkmax + T) * NR * NB + tid] = acc[tslot * NR * NB + tid];
+        acc[tslot * NR * NB + tid] = 0.0;
+      }
+      __syncthreads();
+      if (syncer && T < nblk) coop_signal(pflag + (size_t)lane * a.nblkmax + T, target);
+      if (s < last && tid < COOP_TT) load_panel(s + 1);
+    }
+  }
+  return true;
+}
+
+template <int NB, int NR, int RPT>
+__global__ __launch_bounds__(COOP_BT) void k_band_coop(CoopArgs a) {
+  __shared__ double acc[COOP_RING * NR * NB];
+  __shared__ double rhs[NR * NB];
+  __shared__ double ybuf[NR * NB];
+  __shared__ int sflag;
+  // blocks of one mode share an XCD: dispatch is round-robin over 8 XCDs
+  const int b = blockIdx.x;
+  const int md = (b / (8 * a.W)) * 8 + (b & 7);
+  const int lane = (b >> 3) % a.W;
+  if (md >= a.ndirect) return;
+  const int m = a.dmode[md];
+  const int n = a.mode_n[m];
+  const int nblk = (n + NB - 1) / NB;
+  double* gy[NR];
+#pragma unroll
+  for (int c = 0; c < NR; ++c) {
+    int p = a.mode_plane[2 * m + c];
+    if (p < 0) p = a.mode_plane[2 * m];
+    gy[c] = a.sol + (size_t)p * a.solstride;
+  }
+  if (!coop_sweep<NB, NR, RPT, true>(a, md, lane, a.Pf + a.offP[m], a.Linv + a.offL[m], gy, n,
+                                     nblk, acc, rhs, ybuf, &sflag))
+    return;
+  // the backward sweep of the lead reads, from global memory, values that
+  // other threads of this workgroup wrote during the forward sweep
+  __threadfence();
+  __syncthreads();
+  coop_sweep<NB, NR, RPT, false>(a, md, lane, a.Pb + a.offP[m], a.LinvT + a.offL[m], gy, n,
+                                 nblk, acc, rhs, ybuf, &sflag);
 }
 
 __global__ __launch_bounds__(ELT_THREADS) void k_backsub(ElemArgs a) {
@@ -816,17 +1046,8 @@ int helm_upload(Helm& H) {
     if ((st = dev_upload(&H.d.pc, H.pc.data(), H.pc.size()))) return st;
     if ((st = dev_upload(&H.d.pc32, pc32.data(), pc32.size()))) return st;
     if ((st = dev_upload(&H.d.pcg_plane, H.pcg_plane.data(), H.pcg_plane.size()))) return st;
-    if ((st = dev_upload<double>(&H.d.work, nullptr, H.pcg_plane.size() * H.workstride)))
-      return st;
-    if ((st = dev_upload<int>(&H.d.iters, nullptr, H.pcg_plane.size()))) return st;
   }
   if ((st = dev_upload(&H.d.plane_of, H.plane_of.data(), H.plane_of.size()))) return st;
-  const size_t np_ = (size_t)H.nplanes;
-  const int nzero = M.nglobal - M.nsolve;
-  if ((st = dev_upload<double>(&H.d.Fb, nullptr, np_ * M.nel * M.next))) return st;
-  if ((st = dev_upload<double>(&H.d.Fi, nullptr, np_ * M.nel * (M.nint ? M.nint : 1)))) return st;
-  if ((st = dev_upload<double>(&H.d.g, nullptr, np_ * (nzero ? nzero : 1)))) return st;
-  if ((st = dev_upload<double>(&H.d.sol, nullptr, np_ * H.solstride))) return st;
   // host band copies are no longer needed
   std::vector<double>().swap(H.Pf);
   std::vector<double>().swap(H.Pb);
@@ -844,26 +1065,62 @@ void helm_release(Helm& H) {
   dev_free(H.d.g); dev_free(H.d.sol); dev_free(H.d.plane_of);
   dev_free(H.d.pc32); dev_free(H.d.mode_kind); dev_free(H.d.pcg_plane); dev_free(H.d.work);
   dev_free(H.d.iters);
+  H.nf_cap = 0;
   H.on_device = false;
 }
 
-int helm_solve(Helm& H, double* f, const double* bc, double* u, double tol_rel, int step_max,
-               int* iters, hipStream_t s) {
+int helm_scratch(Helm& H, int nf, hipStream_t s) {
+  if (nf <= H.nf_cap) return SEMX_OK;
+  Mesh& M = *H.M;
+  HIPCHK(hipStreamSynchronize(s));
+  dev_free(H.d.Fb); dev_free(H.d.Fi); dev_free(H.d.g); dev_free(H.d.sol);
+  dev_free(H.d.work); dev_free(H.d.iters);
+  H.nf_cap = 0;
+  const size_t np_ = (size_t)H.nplanes * nf;
+  const int nzero = M.nglobal - M.nsolve;
+  int st;
+  if ((st = dev_upload<double>(&H.d.Fb, nullptr, np_ * M.nel * M.next))) return st;
+  if ((st = dev_upload<double>(&H.d.Fi, nullptr, np_ * M.nel * (M.nint ? M.nint : 1)))) return st;
+  if ((st = dev_upload<double>(&H.d.g, nullptr, np_ * (nzero ? nzero : 1)))) return st;
+  if ((st = dev_upload<double>(&H.d.sol, nullptr, np_ * H.solstride))) return st;
+  if (!H.pcg_plane.empty()) {
+    const size_t npcg = H.pcg_plane.size() * nf;
+    if ((st = dev_upload<double>(&H.d.work, nullptr, npcg * H.workstride))) return st;
+    if ((st = dev_upload<int>(&H.d.iters, nullptr, npcg))) return st;
+  }
+  H.nf_cap = nf;
+  return SEMX_OK;
+}
+
+int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* bcs,
+                      double* const* us, double tol_rel, int step_max, int* iters,
+                      hipStream_t s) {
   Mesh& M = *H.M;
   if (iters) *iters = 0;
-  if (M.nbound && !bc) return SEMX_E_ARG;
+  if (nf < 1 || nf > SEMX_MAX_FIELDS || !fs || !us) return SEMX_E_ARG;
+  for (int i = 0; i < nf; ++i) {
+    if (!fs[i] || !us[i]) return SEMX_E_ARG;
+    if (M.nbound && (!bcs || !bcs[i])) return SEMX_E_ARG;
+  }
   if (!H.pcg_plane.empty() && (!(tol_rel >= 0.0) || step_max < 1)) return SEMX_E_ARG;
   int st = helm_upload(H);
+  if (!st) st = helm_scratch(H, nf, s);
   if (st) return st;
   const int nzero = M.nglobal - M.nsolve;
   const int nline = M.nbound * M.np;
+  const size_t pFb = (size_t)H.nplanes * M.nel * M.next;
+  const size_t pFi = (size_t)H.nplanes * M.nel * (M.nint ? M.nint : 1);
+  const size_t pg = (size_t)H.nplanes * (nzero ? nzero : 1);
+  const size_t psol = (size_t)H.nplanes * H.solstride;
+  hipEvent_t ev;
   if (nzero > 0) {
     dim3 grid((nzero + 255) / 256, H.nplanes);
-    hipEvent_t ev;
-    prof_begin(s, &ev);
-    hipLaunchKernelGGL(k_ess_gather, grid, dim3(256), 0, s, bc, M.d.ess_src, H.d.plane_of,
-                       H.d.g, nzero, nline);
-    prof_end(s, "k_ess_gather", ev);
+    for (int i = 0; i < nf; ++i) {
+      prof_begin(s, &ev);
+      hipLaunchKernelGGL(k_ess_gather, grid, dim3(256), 0, s, bcs[i], M.d.ess_src,
+                         H.d.plane_of, H.d.g + i * pg, nzero, nline);
+      prof_end(s, "k_ess_gather", ev);
+    }
   }
   ElemArgs ea;
   ea.np = M.np; ea.npnp = M.npnp; ea.next = M.next; ea.nint = M.nint; ea.nel = M.nel;
@@ -874,7 +1131,16 @@ int helm_solve(Helm& H, double* f, const double* bc, double* u, double tol_rel, 
   ea.bside = M.d.bside; ea.barea = M.d.barea; ea.hbi = H.d.hbi; ea.hii = H.d.hii;
   ea.plane_of = H.d.plane_of; ea.mode_n = H.d.mode_n; ea.mode_kind = H.d.mode_kind;
   ea.Fb = H.d.Fb; ea.Fi = H.d.Fi;
-  ea.g = H.d.g; ea.sol = H.d.sol; ea.solstride = H.solstride; ea.f = f; ea.bc = bc; ea.u = u;
+  ea.g = H.d.g; ea.sol = H.d.sol; ea.solstride = H.solstride;
+  ea.f = nullptr; ea.bc = nullptr; ea.u = nullptr;
+  // per-field arguments of the element kernels
+  auto field_args = [&](int i) {
+    ElemArgs e = ea;
+    e.f = fs[i]; e.bc = bcs ? bcs[i] : nullptr; e.u = us[i];
+    e.Fb = H.d.Fb + i * pFb; e.Fi = H.d.Fi + i * pFi; e.g = H.d.g + i * pg;
+    e.sol = H.d.sol + i * psol;
+    return e;
+  };
   const int ce = ELT_THREADS / M.npnp;
   dim3 eg((M.nel + ce - 1) / ce, H.nplanes);
   const size_t lds_c = (2 * (size_t)M.npnp + 4 * (size_t)ce * M.npnp) * sizeof(double);
@@ -885,16 +1151,21 @@ int helm_solve(Helm& H, double* f, const double* bc, double* u, double tol_rel, 
     HIPCHK(hipFuncSetAttribute((const void*)k_backsub,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b));
   }
-  hipEvent_t ev;
-  prof_begin(s, &ev);
-  hipLaunchKernelGGL(k_condense, eg, dim3(ELT_THREADS), lds_c, s, ea);
-  prof_end(s, "k_condense", ev);
-  if (!H.dmode.empty()) {
+  for (int i = 0; i < nf; ++i) {
+    ElemArgs e = field_args(i);
     prof_begin(s, &ev);
-    hipLaunchKernelGGL(k_assemble, dim3((M.nglobal + 255) / 256, H.nplanes), dim3(256), 0, s,
-                       H.d.Fb, M.d.asm_ptr, M.d.asm_idx, H.d.plane_of, H.d.mode_n,
-                       H.d.mode_kind, H.d.sol, H.solstride, M.nglobal, M.nel * M.next);
-    prof_end(s, "k_assemble", ev);
+    hipLaunchKernelGGL(k_condense, eg, dim3(ELT_THREADS), lds_c, s, e);
+    prof_end(s, "k_condense", ev);
+  }
+  if (!H.dmode.empty()) {
+    for (int i = 0; i < nf; ++i) {
+      prof_begin(s, &ev);
+      hipLaunchKernelGGL(k_assemble, dim3((M.nglobal + 255) / 256, H.nplanes), dim3(256), 0, s,
+                         H.d.Fb + i * pFb, M.d.asm_ptr, M.d.asm_idx, H.d.plane_of, H.d.mode_n,
+                         H.d.mode_kind, H.d.sol + i * psol, H.solstride, M.nglobal,
+                         M.nel * M.next);
+      prof_end(s, "k_assemble", ev);
+    }
     BandArgs ba;
     ba.Pf = H.d.Pf; ba.Pb = H.d.Pb; ba.Linv = H.d.Linv; ba.LinvT = H.d.LinvT;
     ba.mode_n = H.d.mode_n; ba.offP = H.d.mode_offH; ba.offL = H.d.mode_offL;
@@ -904,15 +1175,27 @@ int helm_solve(Helm& H, double* f, const double* bc, double* u, double tol_rel, 
     ba.kdp = H.kdp; ba.nel = M.nel; ba.next = M.next;
     ba.ring = 64;
     while (ba.ring < H.kd + 2 * H.nb) ba.ring *= 2;
-    int nr = 1;
-    for (int m : H.dmode) if (H.mode_plane[2 * m + 1] >= 0) nr = 2;
+    ba.fstride = (long long)psol;
+    int npl = 1;
+    for (int m : H.dmode) if (H.mode_plane[2 * m + 1] >= 0) npl = 2;
+    ba.npl = npl;
+    // all fields in one sweep of the factor when the windows fit in LDS,
+    // else one launch per field
+    int nr = npl * nf, launches = 1;
+    if ((size_t)nr * (ba.ring + H.nb) * sizeof(double) > 160 * 1024 ||
+        (nr != 1 && nr != 2 && nr != 4 && nr != 6)) {
+      nr = npl;
+      launches = nf;
+    }
     const size_t lds = (size_t)nr * (ba.ring + H.nb) * sizeof(double);
     if (lds > 160 * 1024) return SEMX_E_UNSUPPORTED;
     const void* kfn = nullptr;
-#define SEMX_PICK(NB_, NP_, BT_, D_)                                               \
-    if (H.nb == NB_ && H.npass == NP_ && H.bt == BT_)                              \
-      kfn = (nr == 2) ? (const void*)k_band_solve<NB_, NP_, BT_, 2, D_>            \
-                      : (const void*)k_band_solve<NB_, NP_, BT_, 1, D_>;
+#define SEMX_PICK(NB_, NP_, BT_, D_)                                        \
+    if (H.nb == NB_ && H.npass == NP_ && H.bt == BT_)                       \
+      kfn = (nr == 6)   ? (const void*)k_band_solve<NB_, NP_, BT_, 6, D_>   \
+            : (nr == 4) ? (const void*)k_band_solve<NB_, NP_, BT_, 4, D_>   \
+            : (nr == 2) ? (const void*)k_band_solve<NB_, NP_, BT_, 2, D_>   \
+                        : (const void*)k_band_solve<NB_, NP_, BT_, 1, D_>;
     SEMX_PICK(32, 1, 64, 1)
     SEMX_PICK(32, 1, 128, 1)
     SEMX_PICK(32, 1, 256, 1)
@@ -922,15 +1205,22 @@ int helm_solve(Helm& H, double* f, const double* bc, double* u, double tol_rel, 
 #undef SEMX_PICK
     if (!kfn) return SEMX_E_UNSUPPORTED;
     HIPCHK(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    void* kargs[] = {&ba};
-    prof_begin(s, &ev);
-    HIPCHK(hipLaunchKernel(kfn, dim3((unsigned)H.dmode.size()), dim3(H.bt), kargs, lds, s));
-    prof_end(s, "k_band_solve", ev);
+    for (int l = 0; l < launches; ++l) {
+      BandArgs bl = ba;
+      bl.sol = H.d.sol + (launches > 1 ? l * psol : 0);
+      void* kargs[] = {&bl};
+      prof_begin(s, &ev);
+      HIPCHK(hipLaunchKernel(kfn, dim3((unsigned)H.dmode.size()), dim3(H.bt), kargs, lds, s));
+      prof_end(s, "k_band_solve", ev);
+    }
   }
   if (!H.dmode.empty()) {
-    prof_begin(s, &ev);
-    hipLaunchKernelGGL(k_backsub, eg, dim3(ELT_THREADS), lds_b, s, ea);
-    prof_end(s, "k_backsub", ev);
+    for (int i = 0; i < nf; ++i) {
+      ElemArgs e = field_args(i);
+      prof_begin(s, &ev);
+      hipLaunchKernelGGL(k_backsub, eg, dim3(ELT_THREADS), lds_b, s, e);
+      prof_end(s, "k_backsub", ev);
+    }
   }
   if (!H.pcg_plane.empty()) {
     PcgArgs pa;
@@ -942,7 +1232,7 @@ int helm_solve(Helm& H, double* f, const double* bc, double* u, double tol_rel, 
     pa.btog = M.d.btog; pa.ops = M.d.ops; pa.asm_ptr = M.d.asm_ptr; pa.asm_idx = M.d.asm_idx;
     pa.plane_of = H.d.plane_of; pa.pcg_plane = H.d.pcg_plane; pa.mode_n = H.d.mode_n;
     pa.pc = H.d.pc; pa.pc32 = H.d.pc32; pa.Fb = H.d.Fb; pa.Fi = H.d.Fi; pa.g = H.d.g;
-    pa.u = u; pa.work = H.d.work; pa.workstride = H.workstride;
+    pa.u = nullptr; pa.work = H.d.work; pa.workstride = H.workstride;
     pa.tol = tol_rel; pa.step_max = step_max; pa.inner_tol = H.inner_tol;
     pa.max_outer = H.max_outer; pa.iters = H.d.iters;
     const int cep = PCG_THREADS / M.npnp;
@@ -952,15 +1242,22 @@ int helm_solve(Helm& H, double* f, const double* bc, double* u, double tol_rel, 
     const void* kfn = refine ? (const void*)k_pcg<float, true> : (const void*)k_pcg<double, false>;
     if (lds > 64 * 1024)
       HIPCHK(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    void* kargs[] = {&pa};
-    prof_begin(s, &ev);
-    HIPCHK(hipLaunchKernel(kfn, dim3((unsigned)H.pcg_plane.size()), dim3(PCG_THREADS), kargs,
-                           lds, s));
-    prof_end(s, "k_pcg", ev);
+    const size_t npcg = H.pcg_plane.size();
+    for (int i = 0; i < nf; ++i) {
+      PcgArgs pf = pa;
+      pf.Fb = H.d.Fb + i * pFb; pf.Fi = H.d.Fi + i * pFi; pf.g = H.d.g + i * pg;
+      pf.u = us[i];
+      pf.work = H.d.work + i * npcg * H.workstride;
+      pf.iters = H.d.iters + i * npcg;
+      void* kargs[] = {&pf};
+      prof_begin(s, &ev);
+      HIPCHK(hipLaunchKernel(kfn, dim3((unsigned)npcg), dim3(PCG_THREADS), kargs, lds, s));
+      prof_end(s, "k_pcg", ev);
+    }
     HIPCHK(hipGetLastError());
     if (iters) {
       // the step-limit WARNING of src/field.cpp:713 becomes a status
-      std::vector<int> it(H.pcg_plane.size());
+      std::vector<int> it(npcg * nf);
       HIPCHK(hipMemcpyAsync(it.data(), H.d.iters, it.size() * sizeof(int),
                             hipMemcpyDeviceToHost, s));
       HIPCHK(hipStreamSynchronize(s));
@@ -973,6 +1270,12 @@ int helm_solve(Helm& H, double* f, const double* bc, double* u, double tol_rel, 
   }
   HIPCHK(hipGetLastError());
   return SEMX_OK;
+}
+
+int helm_solve(Helm& H, double* f, const double* bc, double* u, double tol_rel, int step_max,
+               int* iters, hipStream_t s) {
+  const double* bcs[1] = {bc};
+  return helm_solve_fields(H, 1, &f, bcs, &u, tol_rel, step_max, iters, s);
 }
 
 }  // namespace semx
@@ -1124,6 +1427,14 @@ int semx_helm_solve(semx_helm helm, double* f, const double* bc, double* u, doub
                     int step_max, int* iters, void* stream) {
   if (!helm || !f || !u) return SEMX_E_ARG;
   return helm_solve(helm->H, f, bc, u, tol_rel, step_max, iters, (hipStream_t)stream);
+}
+
+int semx_helm_solve_fields(semx_helm helm, int nfield, double* const* f, const double* const* bc,
+                           double* const* u, double tol_rel, int step_max, int* iters,
+                           void* stream) {
+  if (!helm) return SEMX_E_ARG;
+  return helm_solve_fields(helm->H, nfield, f, bc, u, tol_rel, step_max, iters,
+                           (hipStream_t)stream);
 }
 
 int semx_helm_set_pcg(semx_helm helm, int precision, double inner_tol, int max_outer) {

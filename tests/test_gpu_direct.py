@@ -76,3 +76,32 @@ def test_regress_laplace1(cuda):
     f = np.zeros((1, P.planeSize))
     u, _ = gpu_solve(torch, P, S, f, P.bc_lines(ex[None, :]))
     assert f"{np.abs(u[0, :P.nPlane] - ex).max():.3e}" == "1.052e-10"
+
+
+@pytest.mark.parametrize("method", [sx.DIRECT, sx.MIXED])
+@pytest.mark.parametrize("nf", [2, 3])
+def test_solve_fields_equals_separate_solves(cuda, method, nf):
+    """Several fields sharing one system (u, v, w with one MatrixSys,
+    src/matrix.cpp:85-91): the batched solve streams the band factor once
+    and returns exactly what separate solves return."""
+    import torch
+    print(f"[solve_fields] method {method} nf {nf}", flush=True)
+    P, Q, _, _, _ = make_case(6, 7, 8, 3.0, 1.0, 0.2, ("D", "N", "D", "D"))
+    S = sx.ModalMatrixSys(P, 3.0, 1.0, 0, 4, 8, method)
+    cases = [make_case(6, 7, 8, 3.0, 1.0, 0.2, ("D", "N", "D", "D"), seed=20 + i)
+             for i in range(nf)]
+    dev = torch.device("cuda:0")
+    fs, bcs, us, ref = [], [], [], []
+    for (_, _, f, bc, _) in cases:
+        u1, _ = gpu_solve(torch, P, S, f, bc)
+        ref.append(u1)
+        fs.append(torch.from_numpy(f.ravel().copy()).to(dev))
+        bcs.append(torch.from_numpy(bc.ravel().copy()).to(dev))
+        us.append(torch.zeros(f.size, dtype=torch.float64, device=dev))
+    sx.solve_fields(S, fs, bcs, us)
+    torch.cuda.synchronize()
+    for u, r in zip(us, ref):
+        g = u.cpu().numpy().reshape(r.shape)
+        d = np.abs(g[:, :P.nPlane] - r[:, :P.nPlane]).max()
+        print(f"  max |batched - separate| = {d:.3e}", flush=True)
+        assert d <= 1e-14 * np.abs(r).max()
