@@ -46,6 +46,8 @@ def main():
                     help="skip timing the JACPCG / MIXED variants")
     ap.add_argument("--dns-steps", type=int, default=20,
                     help="timed DNS steps (after N_TIME start-up steps); 0 skips the DNS")
+    ap.add_argument("--batch", type=int, default=512,
+                    help="independent 2D problems in the batched PCG line; 0 skips it")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 256 MiB (scale for HBM "
                          "byte counters) and print its timing")
@@ -244,9 +246,86 @@ def main():
         v.pop("u")
     if variants:
         result["iterative_variants"] = variants
+    if args.batch > 0:
+        result["batch"] = batch_bench(args, torch, sx, L)
     if args.dns_steps > 0:
         result["dns"] = dns_bench(args, torch, sx, L)
     print(json.dumps(result), flush=True)
+
+
+def batch_bench(args, torch, sx, L):
+    """Independent 2D problems in one launch: K x K quads, K ~ U{4..16},
+    np ~ U{5..11}, lambda2 ~ U[0.1,100], seed 4, Dirichlet, FP32 Jacobi-PCG
+    with FP64 refinement to |r| <= 1e-10 |b|; the FP64 PCG batch of the same
+    problems (to 1e-13) is the reference for the error."""
+    import ctypes as C
+    from semtex_amd.mesh import rect_mesh
+    from semtex_amd.workloads import batch_problems, single_plane_problem
+    dev = torch.device("cuda:0")
+    progress(f"batch setup ({args.batch} problems)")
+    t0 = time.time()
+    meshes, sys64, sys32, fs, bcs = [], [], [], [], []
+    dofs = 0
+    for (np_, K, lam, cs) in batch_problems(args.batch, seed=4):
+        P = sx.Mesh(np_, rect_mesh(K, K, 0, 2 * np.pi, 0, 2 * np.pi))
+        f, bc, _ = single_plane_problem(P, lam, cs)
+        S64 = sx.ModalMatrixSys(P, lam, 1.0, 0, 1, 1, sx.JACPCG)
+        S32 = sx.ModalMatrixSys(P, lam, 1.0, 0, 1, 1, sx.JACPCG)
+        S32.set_pcg(sx.PCG_FP32_REFINED, inner_tol=1e-4, max_outer=30)
+        meshes.append(P); sys64.append(S64); sys32.append(S32)
+        fs.append(torch.from_numpy(f.ravel().copy()).to(dev))
+        bcs.append(torch.from_numpy(bc.ravel().copy()).to(dev))
+        dofs += P.nglobal + P.nel * P.nint
+    B64, B32 = sx.Batch(sys64), sx.Batch(sys32)
+    u64 = [torch.zeros_like(f) for f in fs]
+    u32 = [torch.zeros_like(f) for f in fs]
+    t_setup = time.time() - t0
+    it64 = B64.solve(fs, bcs, u64, tol_rel=1e-13)
+    it32 = B32.solve(fs, bcs, u32, tol_rel=1e-10)
+    torch.cuda.synchronize()
+    num = sum(float(((a - b) ** 2).sum().item()) for a, b in zip(u32, u64))
+    den = sum(float((b ** 2).sum().item()) for b in u64)
+    rel = (num / den) ** 0.5
+    progress(f"batch setup {t_setup:.1f} s; timing")
+    stream = torch.cuda.current_stream()
+
+    def timed(Bt, us, tol, reps):
+        for _ in range(2):
+            for u in us:
+                u.zero_()
+            Bt.solve(fs, bcs, us, tol_rel=tol, iters=False)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        tot = 0.0
+        for _ in range(reps):
+            for u in us:
+                u.zero_()
+            torch.cuda.synchronize()
+            e0.record(stream)
+            Bt.solve(fs, bcs, us, tol_rel=tol, iters=False)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            tot += e0.elapsed_time(e1)
+        return tot / reps
+
+    ms32 = timed(B32, u32, 1e-10, 5)
+    ms64 = timed(B64, u64, 1e-10, 5)
+    return {
+        "metric": "DOF-solves/s",
+        "value": dofs / (ms32 * 1e-3),
+        "ms_per_step": ms32,
+        "dtype": "f32 inner PCG, f64 refinement",
+        "workload": f"{args.batch} independent 2D problems, K x K quads K~U{{4..16}}, "
+                    "np~U{5..11}, lambda2~U[0.1,100], seed 4, Dirichlet, Jacobi-PCG to 1e-10, "
+                    "one workgroup per problem, one launch",
+        "dof_total": dofs,
+        "iterations_max": it32,
+        "fp64_pcg_ms_per_step": ms64,
+        "fp64_pcg_value": dofs / (ms64 * 1e-3),
+        "rel_l2_vs_fp64": rel,
+        "setup_s": round(t_setup, 2),
+    }
 
 
 def dns_bench(args, torch, sx, L):

@@ -59,6 +59,7 @@ extern "C" {
 typedef struct semx_mesh_s* semx_mesh;
 typedef struct semx_helm_s* semx_helm;
 typedef struct semx_dns_s*  semx_dns;
+typedef struct semx_batch_s* semx_batch;
 
 /* Text for a status code.                                                  */
 const char* semx_error_string (int status);
@@ -154,6 +155,22 @@ int semx_helm_solve_fields (semx_helm helm, int nfield, double* const* f,
  * until |r| <= tol_rel |b| in FP64 -- the same stopping test as FP64.     */
 int semx_helm_set_pcg (semx_helm helm, int precision, double inner_tol,
                        int max_outer);
+
+/* --- many independent 2D problems in one launch ----------------------------
+ * No counterpart in semtex, which solves one problem per process: count
+ * systems created with nz = 1, nmodes = 1 and method SEMX_JACPCG (each on
+ * its own mesh, np, lambda2), all with the same semx_helm_set_pcg setting,
+ * are solved together, one workgroup per problem; each workgroup does all
+ * of Field::solve's JACPCG branch (src/field.cpp:564-725) for its plane:
+ * -(M f + H g) + <h,w>, x0 from u, the iteration, essential re-imposition.
+ * f, bc, u: host arrays of count device pointers (bc[i] may be NULL when
+ * mesh i has no boundary); iters as in semx_helm_solve.  The systems must
+ * outlive the batch.                                                      */
+int semx_batch_create (int count, const semx_helm* helms, semx_batch* batch);
+int semx_batch_destroy (semx_batch batch);
+int semx_batch_solve (semx_batch batch, const double* const* f,
+                      const double* const* bc, double* const* u, double tol_rel,
+                      int step_max, int* iters, void* stream);
 
 /* --- field operators (device data, caller's stream) ------------------------
  * dDFTr (femlib/fourier.c:47-171): ntrn real transforms of length tlen,

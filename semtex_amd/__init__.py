@@ -30,7 +30,7 @@ PCG_FP64, PCG_FP32_REFINED = 0, 1
 E_MAXITER = 3
 FORWARD, INVERSE = 1, -1
 
-__all__ = ["Mesh", "ModalMatrixSys", "solve", "solve_fields", "dft_r", "gradient", "smooth", "DNS",
+__all__ = ["Mesh", "ModalMatrixSys", "solve", "solve_fields", "Batch", "dft_r", "gradient", "smooth", "DNS",
            "DIRECT", "JACPCG", "MIXED",
            "PCG_FP64", "PCG_FP32_REFINED",
            "rect_mesh", "bc_lines", "side_nodes", "SemxError", "load"]
@@ -192,6 +192,44 @@ def solve_fields(msys, fs, bcs, us, tol_rel=1e-12, step_max=10000, stream=None, 
     else:
         check(st, "semx_helm_solve_fields")
     return it.value
+
+
+class Batch:
+    """Independent 2D JACPCG problems (systems made with nz=1, nmodes=1,
+    method JACPCG, equal set_pcg settings) solved in one launch, one
+    workgroup per problem."""
+
+    def __init__(self, systems):
+        self.systems = list(systems)
+        arr = (C.c_void_p * len(self.systems))(*[S.h for S in self.systems])
+        h = C.c_void_p()
+        check(load().semx_batch_create(len(self.systems), arr, C.byref(h)), "semx_batch_create")
+        self.h = h
+
+    def solve(self, fs, bcs, us, tol_rel=1e-10, step_max=10000, stream=None, iters=True):
+        n = len(self.systems)
+        assert len(fs) == n and len(us) == n and (bcs is None or len(bcs) == n)
+        arr = C.c_void_p * n
+        fa = arr(*[_ptr(t) for t in fs])
+        ua = arr(*[_ptr(t) for t in us])
+        ba = arr(*[_ptr(t) if t is not None else None for t in bcs]) if bcs is not None else None
+        it = C.c_int(0)
+        st = load().semx_batch_solve(self.h, fa, ba, ua, float(tol_rel), int(step_max),
+                                     C.byref(it) if iters else None, _stream(stream))
+        if st == E_MAXITER:
+            import warnings
+            warnings.warn("Field::solve: step limit exceeded", RuntimeWarning)
+        else:
+            check(st, "semx_batch_solve")
+        return it.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                load().semx_batch_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
 
 
 def _stream(stream):

@@ -36,6 +36,10 @@ PROTOTYPES = {
     "semx_helm_set_pcg": (C.c_int, [VP, C.c_int, C.c_double, C.c_int]),
     "semx_helm_solve_fields": (C.c_int, [VP, C.c_int, C.POINTER(VP), C.POINTER(VP),
                                          C.POINTER(VP), C.c_double, C.c_int, I, VP]),
+    "semx_batch_create": (C.c_int, [C.c_int, C.POINTER(VP), C.POINTER(VP)]),
+    "semx_batch_destroy": (C.c_int, [VP]),
+    "semx_batch_solve": (C.c_int, [VP, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP),
+                                   C.c_double, C.c_int, I, VP]),
     "semx_dft_r": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, VP]),
     "semx_gradient": (C.c_int, [VP, C.c_int, C.c_double, VP, C.c_int, VP]),
     "semx_smooth": (C.c_int, [VP, C.c_int, VP, VP]),

@@ -94,6 +94,12 @@ struct HelmDev {
   int* pcg_plane = nullptr;   // plane-list indices solved by PCG
   double* work = nullptr;     // PCG vectors, one slice per PCG plane
   int* iters = nullptr;       // PCG iterations per PCG plane
+  // cooperative band solve: hand-off flags, partial sums, chunk ranges
+  unsigned* yflag = nullptr;  // ndirect*nblkmax
+  unsigned* pflag = nullptr;  // ndirect*W*nblkmax
+  double* part = nullptr;     // ndirect*W*nblkmax*6*nb
+  int* clo = nullptr;         // W+1
+  int* err = nullptr;         // 1: a hand-off wait timed out
   // scratch
   double* Fb = nullptr;       // nplanes*nel*next
   double* Fi = nullptr;       // nplanes*nel*nint
@@ -124,6 +130,11 @@ struct Helm {
   int max_outer = 12;                 // refinement sweeps (refined)
   size_t workstride = 0;              // doubles per PCG plane
   int nf_cap = 0;                     // fields the device scratch holds
+  int coop_w = 0;                     // workgroups per mode (0: one-workgroup sweep)
+  int coop_rpt = 1;                   // panel rows per task thread
+  int nblkmax = 0;
+  unsigned coop_epoch = 0;            // launches so far (flag values grow with it)
+  std::vector<int> coop_clo;          // chunk ranges per lane
   bool on_device = false;
   HelmDev d;
 };

@@ -38,6 +38,7 @@ struct ElemArgs {
   const double* G;
   const int* eset;
   const int* emap;
+  const int* pmap;
   const int* btog;
   const int* emask;
   const double* ops;      // DV DT SDV SDT
@@ -72,15 +73,18 @@ __global__ void k_ess_gather(const double* __restrict__ bc, const int* __restric
 
 // Elements of one plane per block: CE = blockDim / (np*np) elements, thread
 // t works on element t / npnp, node t % npnp.
-__global__ __launch_bounds__(ELT_THREADS) void k_condense(ElemArgs a) {
-  extern __shared__ double esh[];
-  const int t = threadIdx.x, p = blockIdx.y;
+// One chunk of ce elements [e0, min(e0 + ce, e_end)) of plane-list entry p.
+// sHbiT: the mode's hbi staged transposed in LDS ([c][b]), or null to read
+// hbi per element from global memory.  Called uniformly by the block.
+__device__ inline void condense_chunk(const ElemArgs& a, int p, int e0, int e_end,
+                                      const double* sHbiT, double* esh) {
+  const int t = threadIdx.x;
   const int k = a.plane_of[p];
   const int np = a.np, npnp = a.npnp, next = a.next, nint = a.nint;
   const int ce = ELT_THREADS / npnp;
   const int el = t / npnp, pt = t - el * npnp;
-  const int e = blockIdx.x * ce + el;
-  const bool act = el < ce && e < a.nel;
+  const int e = e0 + el;
+  const bool act = el < ce && e < e_end;
   const int m = k >> 1;
   const double bk = a.beta * (a.base + m);
   const double hcon = a.lambda2 + bk * bk;
@@ -91,6 +95,7 @@ __global__ __launch_bounds__(ELT_THREADS) void k_condense(ElemArgs a) {
   double* sR = sU + ce * npnp;
   double* sS = sR + ce * npnp;
   double* sF = sS + ce * npnp;
+  __syncthreads();   // the previous chunk's reads of the buffers are done
   if (t < npnp) {
     sD[t] = a.ops[(svv ? 2 : 0) * npnp + t];
     sT[t] = a.ops[(svv ? 3 : 1) * npnp + t];
@@ -135,30 +140,41 @@ __global__ __launch_bounds__(ELT_THREADS) void k_condense(ElemArgs a) {
       F += T;
     }
   }
-  if (act) sF[el * npnp + pt] = -F;
+  // element values in emap order: boundary first, then interior
+  if (act) sF[el * npnp + a.pmap[pt]] = -F;
   __syncthreads();
   // interior part out; boundary part condensed: F_b - hbi F_i, plus <h,w>
   for (int idx = t; idx < ce * npnp; idx += ELT_THREADS) {
     const int le = idx / npnp, q = idx - le * npnp;
-    const int ee = blockIdx.x * ce + le;
-    if (ee >= a.nel) continue;
+    const int ee = e0 + le;
+    if (ee >= e_end) continue;
     const size_t pe = (size_t)p * a.nel + ee;
     const double* Fe = sF + le * npnp;
     if (q >= next) {
-      a.Fi[pe * nint + (q - next)] = Fe[a.emap[q]];
+      a.Fi[pe * nint + (q - next)] = Fe[q];
     } else {
       const int b = q;
+      const double* Fi = Fe + next;
       double s0 = 0.0, s1 = 0.0;
       if (a.mode_kind[m] == SEMX_DIRECT) {  // PCG modes keep the raw F_b
-        const double* hbi = a.hbi + ((size_t)m * a.nset + a.eset[ee]) * next * nint;
-        int c = 0;
-        for (; c + 1 < nint; c += 2) {
-          s0 += hbi[b * nint + c] * Fe[a.emap[next + c]];
-          s1 += hbi[b * nint + c + 1] * Fe[a.emap[next + c + 1]];
+        if (sHbiT) {
+          int c = 0;
+          for (; c + 1 < nint; c += 2) {
+            s0 += sHbiT[c * next + b] * Fi[c];
+            s1 += sHbiT[(c + 1) * next + b] * Fi[c + 1];
+          }
+          if (c < nint) s0 += sHbiT[c * next + b] * Fi[c];
+        } else {
+          const double* hbi = a.hbi + ((size_t)m * a.nset + a.eset[ee]) * next * nint;
+          int c = 0;
+          for (; c + 1 < nint; c += 2) {
+            s0 += hbi[b * nint + c] * Fi[c];
+            s1 += hbi[b * nint + c + 1] * Fi[c + 1];
+          }
+          if (c < nint) s0 += hbi[b * nint + c] * Fi[c];
         }
-        if (c < nint) s0 += hbi[b * nint + c] * Fe[a.emap[next + c]];
       }
-      double fb = Fe[a.emap[b]] - (s0 + s1);
+      double fb = Fe[b] - (s0 + s1);
       const int nm = np - 1;
       for (int cc = a.nat_ptr[ee]; cc < a.nat_ptr[ee + 1]; ++cc) {
         const int bb = a.nat_b[cc];
@@ -168,6 +184,42 @@ __global__ __launch_bounds__(ELT_THREADS) void k_condense(ElemArgs a) {
       }
       a.Fb[pe * next + b] = fb;
     }
+  }
+}
+
+// Elements of one plane per block: ce = blockDim / (np*np) elements, thread
+// t works on element t / npnp, node t % npnp.
+__global__ __launch_bounds__(ELT_THREADS) void k_condense(ElemArgs a) {
+  extern __shared__ double esh[];
+  const int ce = ELT_THREADS / a.npnp;
+  condense_chunk(a, blockIdx.y, blockIdx.x * ce, a.nel, nullptr, esh);
+}
+
+// Meshes with one geometric set: one block per (mode, group of ge
+// elements), the mode's hbi staged transposed in LDS, both planes.
+__global__ __launch_bounds__(ELT_THREADS) void k_condense_staged(ElemArgs a,
+                                                                 const int* __restrict__ mode_plane,
+                                                                 int ge) {
+  extern __shared__ double esh[];
+  const int next = a.next, nint = a.nint, npnp = a.npnp;
+  const int ce = ELT_THREADS / npnp;
+  const int m = blockIdx.y;
+  double* sHbiT = esh;
+  double* work = esh + (size_t)next * nint;
+  const bool direct = a.mode_kind[m] == SEMX_DIRECT;
+  if (direct) {
+    const double* hbi = a.hbi + (size_t)m * next * nint;
+    for (int i = threadIdx.x; i < next * nint; i += ELT_THREADS) {
+      const int b = i / nint, c = i - (i / nint) * nint;
+      sHbiT[c * next + b] = hbi[i];
+    }
+  }
+  const int e_begin = blockIdx.x * ge, e_end = min(a.nel, e_begin + ge);
+  for (int c2 = 0; c2 < 2; ++c2) {
+    const int p = mode_plane[2 * m + c2];
+    if (p < 0) continue;
+    for (int e0 = e_begin; e0 < e_end; e0 += ce)
+      condense_chunk(a, p, e0, e_end, direct ? sHbiT : nullptr, work);
   }
 }
 
@@ -411,13 +463,15 @@ struct CoopArgs {
   const int* clo;           // [W+1] chunk ranges per lane
   int* err;
   int solstride, kd, kdp, ndirect, W, nblkmax;
+  int npl;                  // planes per field and mode
+  long long fstride;        // sol offset between fields
   unsigned epoch;
 };
 
 constexpr int COOP_BT = 256;
 constexpr int COOP_TT = COOP_BT - 64;   // task threads; the last wave signals
 constexpr int COOP_RING = 16;           // accumulator blocks per lane (>= chunks + 2)
-constexpr long COOP_SPIN = 1L << 24;    // polls before a wait gives up
+constexpr long COOP_SPIN = 1L << 20;    // polls before a wait gives up
 
 __device__ inline bool coop_wait(const unsigned* f, unsigned target, int* err) {
   for (long it = 0; it < COOP_SPIN; ++it) {
@@ -449,7 +503,7 @@ __device__ bool coop_sweep(const CoopArgs& a, int md, int lane, const double* __
   const int W = a.W;
   unsigned* yflag = a.yflag + (size_t)md * a.nblkmax;
   unsigned* pflag = a.pflag + (size_t)md * W * a.nblkmax;
-  double* part = a.part + (size_t)md * W * a.nblkmax * NR * NB;
+  double* part = a.part + (size_t)md * W * a.nblkmax * NR * NB;   // NR <= 6 fits
   const bool lead = lane == 0;
   const int last = lead ? nblk - 1 : nblk - 2 - c_lo;   // last step with work
   for (int i = tid; i < COOP_RING * NR * NB; i += COOP_BT) acc[i] = 0.0;
@@ -585,9 +639,10 @@ __global__ __launch_bounds__(COOP_BT) void k_band_coop(CoopArgs a) {
   double* gy[NR];
 #pragma unroll
   for (int c = 0; c < NR; ++c) {
-    int p = a.mode_plane[2 * m + c];
+    const int fld = c / a.npl, kk = c - fld * a.npl;
+    int p = a.mode_plane[2 * m + kk];
     if (p < 0) p = a.mode_plane[2 * m];
-    gy[c] = a.sol + (size_t)p * a.solstride;
+    gy[c] = a.sol + fld * a.fstride + (size_t)p * a.solstride;
   }
   if (!coop_sweep<NB, NR, RPT, true>(a, md, lane, a.Pf + a.offP[m], a.Linv + a.offL[m], gy, n,
                                      nblk, acc, rhs, ybuf, &sflag))
@@ -651,6 +706,81 @@ __global__ __launch_bounds__(ELT_THREADS) void k_backsub(ElemArgs a) {
   }
 }
 
+// The same for meshes whose elements share one geometric-factor set: one
+// block per (direct mode, group of elements) with the mode's hii (stored
+// transposed) and hbi staged in LDS once, both planes of the mode and
+// several element chunks per block.
+__global__ __launch_bounds__(ELT_THREADS) void k_backsub_staged(ElemArgs a,
+                                                                const int* __restrict__ dmode,
+                                                                const int* __restrict__ mode_plane,
+                                                                int ge) {
+  extern __shared__ double esh[];
+  const int t = threadIdx.x;
+  const int next = a.next, nint = a.nint, npnp = a.npnp;
+  const int ce = ELT_THREADS / npnp;
+  const int m = dmode[blockIdx.y];
+  const int n = a.mode_n[m];
+  double* sHiiT = esh;                  // nint*nint, [c][q] = hii[q][c]
+  double* sHbi = sHiiT + nint * nint;   // next*nint
+  double* sub = sHbi + next * nint;     // ce*next
+  double* sFi = sub + ce * next;        // ce*nint
+  {
+    const double* hii = a.hii + (size_t)m * nint * nint;
+    const double* hbi = a.hbi + (size_t)m * next * nint;
+    for (int i = t; i < nint * nint; i += ELT_THREADS) {
+      const int q = i / nint, c = i - (i / nint) * nint;
+      sHiiT[c * nint + q] = hii[i];
+    }
+    for (int i = t; i < next * nint; i += ELT_THREADS) sHbi[i] = hbi[i];
+  }
+  const int e_begin = blockIdx.x * ge, e_end = min(a.nel, e_begin + ge);
+  for (int c2 = 0; c2 < 2; ++c2) {
+    const int p = mode_plane[2 * m + c2];
+    if (p < 0) continue;
+    const int k = a.plane_of[p];
+    for (int e0 = e_begin; e0 < e_end; e0 += ce) {
+      __syncthreads();
+      for (int idx = t; idx < ce * npnp; idx += ELT_THREADS) {
+        const int le = idx / npnp, q = idx - le * npnp;
+        const int ee = e0 + le;
+        if (ee >= e_end) continue;
+        const size_t pe = (size_t)p * a.nel + ee;
+        if (q < next) {
+          const int gid = a.btog[ee * next + q];
+          const double ub = (gid < n) ? a.sol[(size_t)p * a.solstride + gid] : 0.0;
+          sub[le * next + q] = ub;
+          a.u[(size_t)k * a.planeSize + (size_t)ee * npnp + a.emap[q]] =
+              (gid < n) ? ub
+                        : (gid >= a.nsolve ? a.g[(size_t)p * a.nzero + (gid - a.nsolve)] : 0.0);
+        } else {
+          sFi[le * nint + (q - next)] = a.Fi[pe * nint + (q - next)];
+        }
+      }
+      __syncthreads();
+      for (int idx = t; idx < ce * nint; idx += ELT_THREADS) {
+        const int le = idx / nint, q = idx - le * nint;
+        const int ee = e0 + le;
+        if (ee >= e_end) continue;
+        const double* fi = sFi + le * nint;
+        const double* ub = sub + le * next;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        int c = 0;
+        for (; c + 1 < nint; c += 2) {
+          s0 += sHiiT[c * nint + q] * fi[c];
+          s1 += sHiiT[(c + 1) * nint + q] * fi[c + 1];
+        }
+        if (c < nint) s0 += sHiiT[c * nint + q] * fi[c];
+        for (int b = 0; b + 1 < next; b += 2) {
+          s2 += sHbi[b * nint + q] * ub[b];
+          s3 += sHbi[(b + 1) * nint + q] * ub[b + 1];
+        }
+        a.u[(size_t)k * a.planeSize + (size_t)ee * npnp + a.emap[next + q]] =
+            (s0 + s1) - (s2 + s3);
+      }
+    }
+  }
+}
+
 // ===========================================================================
 // Jacobi-preconditioned CG, one persistent workgroup per plane: semtex's
 // JACPCG branch of Field::solve (src/field.cpp:627-720) with the global
@@ -707,17 +837,32 @@ __device__ inline double block_sum(double v, double* red) {
   return t;
 }
 
-// y = A x on the globally numbered vector, rows [n, nglobal) zero; returns
-// the dot product w . y (w may be x itself).
-template <typename T>
-__device__ double pcg_matvec(const PcgArgs& a, int m, int n, const T* __restrict__ x,
+// What the matrix-free operator of one plane needs.
+struct PcgOp {
+  int np, npnp, next, nint, nel, nglobal, npts;
+  double hcon;            // lambda2 + betak2
+  bool svv;               // SVV derivative operators (lambda2 > EPSDP)
+  const double* G;
+  const int* eset;
+  const int* pmap;
+  const int* btog;
+  const double* ops;
+  const int* asm_ptr;
+  const int* asm_idx;
+};
+
+// y = A x on the globally numbered vector; returns the dot product w . y
+// (w may be x itself).  MASK: unknowns >= n are treated as zero and rows
+// [n, nglobal) of y are zero (the solved system); otherwise every node takes
+// part (the lifting of essential values).
+template <typename T, bool MASK = true>
+__device__ double pcg_matvec(const PcgOp& a, int n, const T* __restrict__ x,
                              T* __restrict__ y, T* __restrict__ lb, const T* __restrict__ w,
                              T* sh, double* red) {
   const int tid = threadIdx.x, np = a.np, npnp = a.npnp, next = a.next, nint = a.nint;
   const int ce = PCG_THREADS / npnp;
-  const double bk = a.beta * (a.base + m);
-  const T hcon = (T)(a.lambda2 + bk * bk);
-  const bool svv = a.lambda2 > EPSDP;
+  const T hcon = (T)a.hcon;
+  const bool svv = a.svv;
   T* sD = sh;
   T* sT = sD + npnp;
   T* sU = sT + npnp;
@@ -740,7 +885,7 @@ __device__ double pcg_matvec(const PcgArgs& a, int m, int n, const T* __restrict
       T v;
       if (pm < next) {
         const int gid = a.btog[e * next + pm];
-        v = (gid < n) ? x[gid] : (T)0;
+        v = (!MASK || gid < n) ? x[gid] : (T)0;
       } else {
         v = x[a.nglobal + e * nint + (pm - next)];
       }
@@ -781,7 +926,7 @@ __device__ double pcg_matvec(const PcgArgs& a, int m, int n, const T* __restrict
   __syncthreads();
   for (int gI = tid; gI < a.nglobal; gI += PCG_THREADS) {
     T v = 0;
-    if (gI < n)
+    if (!MASK || gI < n)
       for (int c = a.asm_ptr[gI]; c < a.asm_ptr[gI + 1]; ++c) v += lb[a.asm_idx[c]];
     y[gI] = v;
     wy += (double)w[gI] * (double)v;
@@ -792,8 +937,8 @@ __device__ double pcg_matvec(const PcgArgs& a, int m, int n, const T* __restrict
 // PCG iterations on A x = r (r holds the residual of the initial x and is
 // updated in place) until r.r <= epsb2 or the step limit; semtex's loop.
 template <typename T>
-__device__ int pcg_core(const PcgArgs& a, int m, int n, T* x, T* r, T* p, T* q, T* lb,
-                        const T* pc, double epsb2, int step_max, T* sh, double* red) {
+__device__ int pcg_core(const PcgOp& a, int n, T* x, T* r, T* p, T* q, T* lb, const T* pc,
+                        double epsb2, int step_max, T* sh, double* red) {
   const int tid = threadIdx.x, npts = a.npts;
   double r2 = 0.0, rho1 = 0.0, rho2 = 1.0;
   for (int i = tid; i < npts; i += PCG_THREADS) {
@@ -810,7 +955,7 @@ __device__ int pcg_core(const PcgArgs& a, int m, int n, T* x, T* r, T* p, T* q, 
       const T z = pc[i] * r[i];
       p[i] = (it == 1) ? z : z + beta * p[i];
     }
-    const double pq = pcg_matvec<T>(a, m, n, p, q, lb, p, sh, red);
+    const double pq = pcg_matvec<T>(a, n, p, q, lb, p, sh, red);
     const T alpha = (T)(rho1 / pq);
     double nr2 = 0.0, nrho = 0.0;
     for (int i = tid; i < npts; i += PCG_THREADS) {
@@ -828,6 +973,96 @@ __device__ int pcg_core(const PcgArgs& a, int m, int n, T* x, T* r, T* p, T* q, 
   return (it == step_max) ? -it : it;
 }
 
+// The iteration once b (B) and x0 (X) are in place: r = b - A x0, then
+// semtex's loop in FP64, or FP64 iterative refinement around a T-precision
+// PCG on A d = r.  Vectors: X, R, B, P, Q of npts and LB of nel*next
+// doubles; the inner T vectors share the P/Q/LB space.  Returns the
+// iteration count (summed over inner solves), negative at the limit.
+template <typename T, bool REFINE>
+__device__ int pcg_run(const PcgOp& op, int n, double* X, double* R, const double* B,
+                       double* P, double* Q, double* LB, const double* pc, const float* pc32,
+                       double epsb2, int step_max, double inner_tol, int max_outer, T* sh,
+                       double* red) {
+  const int tid = threadIdx.x, npts = op.npts;
+  __syncthreads();
+  pcg_matvec<double>(op, n, X, Q, LB, X, (double*)sh, red);
+  for (int i = tid; i < npts; i += PCG_THREADS) R[i] = B[i] - Q[i];
+  __syncthreads();
+  int its;
+  if constexpr (!REFINE) {
+    its = pcg_core<double>(op, n, X, R, P, Q, LB, pc, epsb2, step_max, (double*)sh, red);
+  } else {
+    T* d = (T*)P;
+    T* rr = d + npts;
+    T* pp = rr + npts;
+    T* qq = pp + npts;
+    T* lb = qq + npts;
+    its = 0;
+    for (int outer = 0;; ++outer) {
+      double rn = 0.0;
+      for (int i = tid; i < npts; i += PCG_THREADS) rn += R[i] * R[i];
+      rn = block_sum(rn, red);
+      if (rn <= epsb2) break;
+      if (outer == max_outer) { its = -its; break; }
+      for (int i = tid; i < npts; i += PCG_THREADS) {
+        d[i] = (T)0;
+        rr[i] = (T)R[i];
+      }
+      __syncthreads();
+      const double ein = inner_tol * inner_tol * rn;
+      const int it = pcg_core<T>(op, n, d, rr, pp, qq, lb, (const T*)pc32, ein, step_max, sh,
+                                 red);
+      its += (it < 0) ? -it : it;
+      for (int i = tid; i < npts; i += PCG_THREADS) X[i] += (double)d[i];
+      __syncthreads();
+      pcg_matvec<double>(op, n, X, Q, LB, X, (double*)sh, red);
+      for (int i = tid; i < npts; i += PCG_THREADS) R[i] = B[i] - Q[i];
+      __syncthreads();
+    }
+  }
+  return its;
+}
+
+// x0: the current field, essential and pinned entries zeroed
+// (Field::local2global, src/field.cpp:928-947).  Where element copies of a
+// shared node differ, the copy written last wins (Veclib::gathr_scatr over
+// elements in order, src/element.cpp:323): the last entry of the node's
+// assembly list, which is in increasing element-storage order.
+__device__ inline void pcg_x0(const PcgOp& op, const int* emap, int n, const double* uk,
+                              double* X) {
+  const int tid = threadIdx.x, next = op.next, nint = op.nint;
+  for (int gI = tid; gI < op.nglobal; gI += PCG_THREADS) {
+    const int c = op.asm_idx[op.asm_ptr[gI + 1] - 1];
+    const int e = c / next, b = c - (c / next) * next;
+    X[gI] = (gI < n) ? uk[(size_t)e * op.npnp + emap[b]] : 0.0;
+  }
+  for (int i = tid; i < op.nel * nint; i += PCG_THREADS) {
+    const int e = i / nint, q = i - (i / nint) * nint;
+    X[op.nglobal + i] = uk[(size_t)e * op.npnp + emap[next + q]];
+  }
+}
+
+// unknowns back to the plane with the essential values re-imposed
+// (Field::global2local + setEssential, src/field.cpp:950-969, 1022-1040);
+// ess(gid) gives the value of essential node gid >= nsolve.
+template <class Ess>
+__device__ inline void pcg_store(const PcgOp& op, int n, int nsolve, const double* X,
+                                 double* uk, Ess ess) {
+  const int next = op.next, npnp = op.npnp;
+  for (int idx = threadIdx.x; idx < op.nel * npnp; idx += PCG_THREADS) {
+    const int e = idx / npnp, q = idx - (idx / npnp) * npnp;
+    const int pm = op.pmap[q];
+    double v;
+    if (pm < next) {
+      const int gid = op.btog[e * next + pm];
+      v = (gid < n) ? X[gid] : (gid >= nsolve ? ess(gid) : 0.0);
+    } else {
+      v = X[op.nglobal + e * op.nint + (pm - next)];
+    }
+    uk[idx] = v;
+  }
+}
+
 template <typename T, bool REFINE>
 __global__ __launch_bounds__(PCG_THREADS) void k_pcg(PcgArgs a) {
   extern __shared__ double pcg_lds[];
@@ -838,7 +1073,10 @@ __global__ __launch_bounds__(PCG_THREADS) void k_pcg(PcgArgs a) {
   const int k = a.plane_of[pidx];
   const int m = k >> 1;
   const int n = a.mode_n[m];
-  const int npts = a.npts, nglobal = a.nglobal, npnp = a.npnp, next = a.next;
+  const int npts = a.npts, nglobal = a.nglobal, next = a.next;
+  const double bk = a.beta * (a.base + m);
+  const PcgOp op{a.np, a.npnp, next, a.nint, a.nel, nglobal, npts, a.lambda2 + bk * bk,
+                 a.lambda2 > EPSDP, a.G, a.eset, a.pmap, a.btog, a.ops, a.asm_ptr, a.asm_idx};
   double* wk = a.work + (size_t)blockIdx.x * a.workstride;
   double* X = wk;                 // FP64 solution
   double* R = X + npts;           // FP64 residual
@@ -848,7 +1086,6 @@ __global__ __launch_bounds__(PCG_THREADS) void k_pcg(PcgArgs a) {
   double* LB = Q + npts;          // nel*next element-boundary scratch
   const double* fb = a.Fb + (size_t)pidx * a.nel * next;
   const double* fi = a.Fi + (size_t)pidx * a.nel * a.nint;
-  const double* pcm = a.pc + (size_t)m * npts;
   double* uk = a.u + (size_t)k * a.planeSize;
   // b: assembled -(M f + H g) + <h,w>, essential and pinned rows zero
   // (Field::buildRHS, src/field.cpp:859-925)
@@ -865,76 +1102,103 @@ __global__ __launch_bounds__(PCG_THREADS) void k_pcg(PcgArgs a) {
     bb += fi[i] * fi[i];
   }
   const double epsb = a.tol * sqrt(block_sum(bb, red));
-  const double epsb2 = epsb * epsb;
-  // x0: the current field, essential and pinned entries zeroed
-  // (Field::local2global, src/field.cpp:928-947).  Where element copies of
-  // a shared node differ, the copy written last wins (Veclib::gathr_scatr
-  // over elements in order, src/element.cpp:323): the last entry of the
-  // node's assembly list, which is in increasing element-storage order.
-  for (int gI = tid; gI < nglobal; gI += PCG_THREADS) {
-    const int c = a.asm_idx[a.asm_ptr[gI + 1] - 1];
-    const int e = c / next, b = c - (c / next) * next;
-    X[gI] = (gI < n) ? uk[(size_t)e * npnp + a.emap[b]] : 0.0;
-  }
-  for (int i = tid; i < a.nel * a.nint; i += PCG_THREADS) {
-    const int e = i / a.nint, q = i - (i / a.nint) * a.nint;
-    X[nglobal + i] = uk[(size_t)e * npnp + a.emap[next + q]];
-  }
-  __syncthreads();
-  // r = b - A x0
-  pcg_matvec<double>(a, m, n, X, Q, LB, X, (double*)sh, red);
-  for (int i = tid; i < npts; i += PCG_THREADS) R[i] = B[i] - Q[i];
-  __syncthreads();
-  int its;
-  if constexpr (!REFINE) {
-    its = pcg_core<double>(a, m, n, X, R, P, Q, LB, pcm, epsb2, a.step_max, (double*)sh, red);
-  } else {
-    // FP64 iterative refinement around a T-precision PCG on A d = r
-    T* d = (T*)P;                    // inner vectors share the P/Q/LB space
-    T* rr = d + npts;
-    T* pp = rr + npts;
-    T* qq = pp + npts;
-    T* lb = qq + npts;
-    const float* pc32 = a.pc32 + (size_t)m * npts;
-    its = 0;
-    for (int outer = 0;; ++outer) {
-      double rn = 0.0;
-      for (int i = tid; i < npts; i += PCG_THREADS) rn += R[i] * R[i];
-      rn = block_sum(rn, red);
-      if (rn <= epsb2) break;
-      if (outer == a.max_outer) { its = -its; break; }
-      for (int i = tid; i < npts; i += PCG_THREADS) {
-        d[i] = (T)0;
-        rr[i] = (T)R[i];
-      }
-      __syncthreads();
-      const double ein = a.inner_tol * a.inner_tol * rn;
-      const int it = pcg_core<T>(a, m, n, d, rr, pp, qq, lb, (const T*)pc32, ein, a.step_max,
-                                 sh, red);
-      its += (it < 0) ? -it : it;
-      for (int i = tid; i < npts; i += PCG_THREADS) X[i] += (double)d[i];
-      __syncthreads();
-      pcg_matvec<double>(a, m, n, X, Q, LB, X, (double*)sh, red);
-      for (int i = tid; i < npts; i += PCG_THREADS) R[i] = B[i] - Q[i];
-      __syncthreads();
-    }
-  }
-  // unknowns back to the plane with the essential values re-imposed
-  // (Field::global2local + setEssential, src/field.cpp:950-969, 1022-1040)
-  for (int idx = tid; idx < a.nel * npnp; idx += PCG_THREADS) {
-    const int e = idx / npnp, q = idx - (idx / npnp) * npnp;
-    const int pm = a.pmap[q];
-    double v;
-    if (pm < next) {
-      const int gid = a.btog[e * next + pm];
-      v = (gid < n) ? X[gid]
-        : (gid >= a.nsolve ? a.g[(size_t)pidx * a.nzero + (gid - a.nsolve)] : 0.0);
-    } else {
-      v = X[nglobal + e * a.nint + (pm - next)];
-    }
-    uk[idx] = v;
-  }
+  pcg_x0(op, a.emap, n, uk, X);
+  const int its = pcg_run<T, REFINE>(op, n, X, R, B, P, Q, LB, a.pc + (size_t)m * npts,
+                                     a.pc32 + (size_t)m * npts, epsb * epsb, a.step_max,
+                                     a.inner_tol, a.max_outer, sh, red);
+  const double* gp = a.g + (size_t)pidx * a.nzero;
+  pcg_store(op, n, a.nsolve, X, uk, [&](int gid) { return gp[gid - a.nsolve]; });
   if (tid == 0) a.iters[blockIdx.x] = its;
+}
+
+// --- independent 2D problems, one workgroup each, in one launch ----------
+struct PcgInst {
+  PcgOp op;
+  int n, nsolve, nbound;
+  const int* emap;
+  const int* ess_src;       // nglobal - nsolve: BC-line index of each essential node
+  const int* nat_ptr;       // per element: natural boundary sides
+  const int* nat_b;
+  const int* bside;
+  const double* barea;
+  const double* pc;
+  const float* pc32;
+  double* work;             // 5*npts + nel*next doubles
+};
+
+struct PcgBatchArgs {
+  const PcgInst* inst;
+  const double* const* f;   // per instance: forcing plane
+  const double* const* bc;  // per instance: BC lines (null without boundaries)
+  double* const* u;         // per instance: initial guess in, solution out
+  int* iters;
+  double tol, inner_tol;
+  int step_max, max_outer;
+};
+
+// Field::solve JACPCG for one plane from the caller's forcing and BC lines,
+// every step inside the workgroup: the right-hand side -(M f) - A g + <h,w>
+// (Field::constrain and buildRHS, src/field.cpp:728-768, 859-925, with the
+// lifting A g of the essential values g as one unmasked operator
+// application), x0, the iteration and the essential re-imposition.
+template <typename T, bool REFINE>
+__global__ __launch_bounds__(PCG_THREADS) void k_pcg_batch(PcgBatchArgs ba) {
+  extern __shared__ double pcg_lds[];
+  double* red = pcg_lds;
+  T* sh = (T*)(pcg_lds + 32);
+  const int tid = threadIdx.x;
+  const PcgInst& I = ba.inst[blockIdx.x];
+  const PcgOp& op = I.op;
+  const int npts = op.npts, nglobal = op.nglobal, next = op.next, nint = op.nint;
+  const int npnp = op.npnp, np = op.np, n = I.n, nsolve = I.nsolve;
+  const double* f = ba.f[blockIdx.x];
+  const double* bc = ba.bc[blockIdx.x];
+  double* uk = ba.u[blockIdx.x];
+  double* X = I.work;
+  double* R = X + npts;
+  double* B = R + npts;
+  double* P = B + npts;
+  double* Q = P + npts;
+  double* LB = Q + npts;
+  // lifting vector: essential values on essential nodes, zero elsewhere
+  for (int i = tid; i < npts; i += PCG_THREADS)
+    P[i] = (i >= nsolve && i < nglobal) ? bc[I.ess_src[i - nsolve]] : 0.0;
+  __syncthreads();
+  pcg_matvec<double, false>(op, n, P, Q, LB, P, (double*)sh, red);
+  // b = -(M f) - A g + <h,w>, rows of essential and pinned nodes zero
+  double bb = 0.0;
+  for (int gI = tid; gI < nglobal; gI += PCG_THREADS) {
+    double v = 0.0;
+    if (gI < n) {
+      for (int c = op.asm_ptr[gI]; c < op.asm_ptr[gI + 1]; ++c) {
+        const int e = op.asm_idx[c] / next, b = op.asm_idx[c] - (op.asm_idx[c] / next) * next;
+        const int q = I.emap[b];
+        v -= op.G[((size_t)op.eset[e] * G_N + G_Q4) * npnp + q] * f[(size_t)e * npnp + q];
+        const int nm = np - 1;
+        for (int cc = I.nat_ptr[e]; cc < I.nat_ptr[e + 1]; ++cc) {
+          const int bs = I.nat_b[cc];
+          const int ii = (b - I.bside[bs] * nm + next) % next;
+          if (ii <= nm) v += bc[bs * np + ii] * I.barea[bs * np + ii];
+        }
+      }
+      v -= Q[gI];
+    }
+    B[gI] = v;
+    bb += v * v;
+  }
+  for (int i = tid; i < op.nel * nint; i += PCG_THREADS) {
+    const int e = i / nint, q = I.emap[next + (i - (i / nint) * nint)];
+    const double v = -op.G[((size_t)op.eset[e] * G_N + G_Q4) * npnp + q] *
+                         f[(size_t)e * npnp + q] - Q[nglobal + i];
+    B[nglobal + i] = v;
+    bb += v * v;
+  }
+  const double epsb = ba.tol * sqrt(block_sum(bb, red));
+  pcg_x0(op, I.emap, n, uk, X);
+  const int its = pcg_run<T, REFINE>(op, n, X, R, B, P, Q, LB, I.pc, I.pc32, epsb * epsb,
+                                     ba.step_max, ba.inner_tol, ba.max_outer, sh, red);
+  pcg_store(op, n, nsolve, X, uk, [&](int gid) { return bc[I.ess_src[gid - nsolve]]; });
+  if (tid == 0) ba.iters[blockIdx.x] = its;
 }
 
 }  // namespace
@@ -1014,6 +1278,58 @@ void mesh_release(Mesh& M) {
   M.on_device = false;
 }
 
+// Lanes of the cooperative band solve: the lead owns the first `lead`
+// chunks of every panel, each follower `follow` further chunks.  The band
+// is split so that the workgroups of all direct modes fill the device.
+// SEMX_BAND_COOP=0 keeps the one-workgroup sweep; SEMX_COOP_LEAD and
+// SEMX_COOP_FOLLOW override the chunk counts.
+int coop_setup(Helm& H) {
+  H.coop_w = 0;
+  const int nd = (int)H.dmode.size();
+  if (nd == 0 || H.npass == 0) return SEMX_OK;
+  const char* env = getenv("SEMX_BAND_COOP");
+  if (!env || atoi(env) == 0) return SEMX_OK;
+  const int nb = H.nb, nch = H.kdp / nb;
+  int cus = 256;
+  int dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+      cus = v;
+  }
+  const int tasks = COOP_TT * 2;                       // RPT <= 2
+  int lead = std::min(nch, 4);
+  if (const char* e = getenv("SEMX_COOP_LEAD")) lead = std::max(1, std::min(nch, atoi(e)));
+  lead = std::min(lead, tasks / nb);
+  const int rest = nch - lead;
+  int follow = 0, nf = 0;
+  if (rest > 0) {
+    const int lanes_free = std::max(1, cus / nd - 1);  // followers per mode that fit
+    follow = (rest + lanes_free - 1) / lanes_free;
+    if (const char* e = getenv("SEMX_COOP_FOLLOW")) follow = std::max(1, atoi(e));
+    follow = std::min(follow, std::min(COOP_RING - 2, tasks / nb));
+    nf = (rest + follow - 1) / follow;
+  }
+  if (lead + 1 > COOP_RING) return SEMX_OK;
+  H.coop_w = 1 + nf;
+  H.coop_rpt = (std::max(lead, follow) * nb > COOP_TT) ? 2 : 1;
+  H.coop_clo.assign(H.coop_w + 1, 0);
+  H.coop_clo[1] = lead;
+  for (int w = 1; w < H.coop_w; ++w) H.coop_clo[w + 1] = std::min(nch, H.coop_clo[w] + follow);
+  int nblkmax = 0;
+  for (int m : H.dmode) nblkmax = std::max(nblkmax, H.modes[m].nblk);
+  H.nblkmax = nblkmax;
+  const size_t W = H.coop_w;
+  int st;
+  if ((st = dev_upload<unsigned>(&H.d.yflag, nullptr, (size_t)nd * nblkmax))) return st;
+  if ((st = dev_upload<unsigned>(&H.d.pflag, nullptr, (size_t)nd * W * nblkmax))) return st;
+  if ((st = dev_upload<double>(&H.d.part, nullptr, (size_t)nd * W * nblkmax * 6 * nb))) return st;
+  if ((st = dev_upload(&H.d.clo, H.coop_clo.data(), H.coop_clo.size()))) return st;
+  if ((st = dev_upload<int>(&H.d.err, nullptr, 1))) return st;
+  H.coop_epoch = 0;
+  return SEMX_OK;
+}
+
 int helm_upload(Helm& H) {
   if (H.on_device) return SEMX_OK;
   Mesh& M = *H.M;
@@ -1048,6 +1364,7 @@ int helm_upload(Helm& H) {
     if ((st = dev_upload(&H.d.pcg_plane, H.pcg_plane.data(), H.pcg_plane.size()))) return st;
   }
   if ((st = dev_upload(&H.d.plane_of, H.plane_of.data(), H.plane_of.size()))) return st;
+  if ((st = coop_setup(H))) return st;
   // host band copies are no longer needed
   std::vector<double>().swap(H.Pf);
   std::vector<double>().swap(H.Pb);
@@ -1065,6 +1382,8 @@ void helm_release(Helm& H) {
   dev_free(H.d.g); dev_free(H.d.sol); dev_free(H.d.plane_of);
   dev_free(H.d.pc32); dev_free(H.d.mode_kind); dev_free(H.d.pcg_plane); dev_free(H.d.work);
   dev_free(H.d.iters);
+  dev_free(H.d.yflag); dev_free(H.d.pflag); dev_free(H.d.part); dev_free(H.d.clo);
+  dev_free(H.d.err);
   H.nf_cap = 0;
   H.on_device = false;
 }
@@ -1126,7 +1445,7 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
   ea.np = M.np; ea.npnp = M.npnp; ea.next = M.next; ea.nint = M.nint; ea.nel = M.nel;
   ea.planeSize = M.planeSize; ea.nline = nline; ea.nsolve = M.nsolve; ea.nzero = nzero;
   ea.nset = M.nset; ea.lambda2 = H.lambda2; ea.beta = H.beta; ea.base = H.base;
-  ea.G = M.d.G; ea.eset = M.d.eset; ea.emap = M.d.emap; ea.btog = M.d.btog;
+  ea.G = M.d.G; ea.eset = M.d.eset; ea.emap = M.d.emap; ea.pmap = M.d.pmap; ea.btog = M.d.btog;
   ea.emask = M.d.emask; ea.ops = M.d.ops; ea.nat_ptr = M.d.nat_ptr; ea.nat_b = M.d.nat_b;
   ea.bside = M.d.bside; ea.barea = M.d.barea; ea.hbi = H.d.hbi; ea.hii = H.d.hii;
   ea.plane_of = H.d.plane_of; ea.mode_n = H.d.mode_n; ea.mode_kind = H.d.mode_kind;
@@ -1151,10 +1470,21 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
     HIPCHK(hipFuncSetAttribute((const void*)k_backsub,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b));
   }
+  // staged hbi when every element shares one geometric set
+  const size_t lds_cs = lds_c + (size_t)M.next * M.nint * sizeof(double);
+  const bool cstaged = M.nset == 1 && lds_cs <= 160 * 1024;
+  const int cge = 2 * ce;                        // elements per staged block
+  if (cstaged && lds_cs > 64 * 1024)
+    HIPCHK(hipFuncSetAttribute((const void*)k_condense_staged,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_cs));
   for (int i = 0; i < nf; ++i) {
     ElemArgs e = field_args(i);
     prof_begin(s, &ev);
-    hipLaunchKernelGGL(k_condense, eg, dim3(ELT_THREADS), lds_c, s, e);
+    if (cstaged)
+      hipLaunchKernelGGL(k_condense_staged, dim3((M.nel + cge - 1) / cge, H.nmodes),
+                         dim3(ELT_THREADS), lds_cs, s, e, H.d.mode_plane, cge);
+    else
+      hipLaunchKernelGGL(k_condense, eg, dim3(ELT_THREADS), lds_c, s, e);
     prof_end(s, "k_condense", ev);
   }
   if (!H.dmode.empty()) {
@@ -1204,8 +1534,43 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
     SEMX_PICK(8, 4, 512, 1)
 #undef SEMX_PICK
     if (!kfn) return SEMX_E_UNSUPPORTED;
-    HIPCHK(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    for (int l = 0; l < launches; ++l) {
+    if (H.coop_w > 0) {
+      // cooperative sweep: all fields' planes of a mode in one launch
+      const int nrc = npl * nf;
+      const void* cfn = nullptr;
+#define SEMX_COOP(NB_, NR_)                                                       \
+      if (H.nb == NB_ && nrc == NR_)                                              \
+        cfn = (H.coop_rpt == 2) ? (const void*)k_band_coop<NB_, NR_, 2>           \
+                                : (const void*)k_band_coop<NB_, NR_, 1>;
+      SEMX_COOP(32, 1) SEMX_COOP(32, 2) SEMX_COOP(32, 4) SEMX_COOP(32, 6)
+      SEMX_COOP(16, 1) SEMX_COOP(16, 2) SEMX_COOP(16, 4) SEMX_COOP(16, 6)
+      SEMX_COOP(8, 1) SEMX_COOP(8, 2) SEMX_COOP(8, 4) SEMX_COOP(8, 6)
+#undef SEMX_COOP
+      if (cfn) {
+        CoopArgs ca;
+        ca.Pf = H.d.Pf; ca.Pb = H.d.Pb; ca.Linv = H.d.Linv; ca.LinvT = H.d.LinvT;
+        ca.mode_n = H.d.mode_n; ca.offP = H.d.mode_offH; ca.offL = H.d.mode_offL;
+        ca.dmode = H.d.dmode; ca.mode_plane = H.d.mode_plane; ca.sol = H.d.sol;
+        ca.part = H.d.part; ca.yflag = H.d.yflag; ca.pflag = H.d.pflag; ca.clo = H.d.clo;
+        ca.err = H.d.err; ca.solstride = H.solstride; ca.kd = H.kd; ca.kdp = H.kdp;
+        ca.ndirect = (int)H.dmode.size(); ca.W = H.coop_w; ca.nblkmax = H.nblkmax;
+        ca.npl = npl; ca.fstride = (long long)psol;
+        ca.epoch = ++H.coop_epoch;
+        const int groups = (ca.ndirect + 7) / 8;
+        void* kargs[] = {&ca};
+        prof_begin(s, &ev);
+        const hipError_t ce2 = hipLaunchCooperativeKernel(
+            cfn, dim3((unsigned)(groups * 8 * H.coop_w)), dim3(COOP_BT), kargs, 0, s);
+        prof_end(s, "k_band_solve", ev);
+        if (ce2 != hipSuccess) {
+          fprintf(stderr, "semx: cooperative band solve: %s\n", hipGetErrorString(ce2));
+          return SEMX_E_HIP;
+        }
+        kfn = nullptr;   // done
+      }
+    }
+    if (kfn) HIPCHK(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    for (int l = 0; kfn && l < launches; ++l) {
       BandArgs bl = ba;
       bl.sol = H.d.sol + (launches > 1 ? l * psol : 0);
       void* kargs[] = {&bl};
@@ -1215,10 +1580,22 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
     }
   }
   if (!H.dmode.empty()) {
+    // staged operators when every element shares one geometric set
+    const size_t lds_s = ((size_t)M.nint * M.nint + (size_t)M.next * M.nint +
+                          (size_t)ce * (M.next + M.nint)) * sizeof(double);
+    const bool staged = M.nset == 1 && lds_s <= 160 * 1024;
+    const int ge = 2 * ce;                       // elements per block
+    if (staged && lds_s > 64 * 1024)
+      HIPCHK(hipFuncSetAttribute((const void*)k_backsub_staged,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_s));
     for (int i = 0; i < nf; ++i) {
       ElemArgs e = field_args(i);
       prof_begin(s, &ev);
-      hipLaunchKernelGGL(k_backsub, eg, dim3(ELT_THREADS), lds_b, s, e);
+      if (staged)
+        hipLaunchKernelGGL(k_backsub_staged, dim3((M.nel + ge - 1) / ge, H.dmode.size()),
+                           dim3(ELT_THREADS), lds_s, s, e, H.d.dmode, H.d.mode_plane, ge);
+      else
+        hipLaunchKernelGGL(k_backsub, eg, dim3(ELT_THREADS), lds_b, s, e);
       prof_end(s, "k_backsub", ev);
     }
   }
@@ -1269,6 +1646,16 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
     }
   }
   HIPCHK(hipGetLastError());
+  if (iters && H.coop_w > 0) {
+    // a cooperative hand-off that timed out leaves its mark here
+    int e = 0;
+    HIPCHK(hipMemcpyAsync(&e, H.d.err, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (e) {
+      fprintf(stderr, "semx: cooperative band solve: hand-off wait timed out\n");
+      return SEMX_E_HIP;
+    }
+  }
   return SEMX_OK;
 }
 
@@ -1435,6 +1822,123 @@ int semx_helm_solve_fields(semx_helm helm, int nfield, double* const* f, const d
   if (!helm) return SEMX_E_ARG;
   return helm_solve_fields(helm->H, nfield, f, bc, u, tol_rel, step_max, iters,
                            (hipStream_t)stream);
+}
+
+struct semx_batch_s {
+  std::vector<semx_helm> helms;
+  PcgInst* inst = nullptr;         // device, one per problem
+  double* work = nullptr;
+  int* iters = nullptr;
+  const double** ptrs = nullptr;   // device: f, bc, u pointer arrays
+  int count = 0, prec = SEMX_PCG_FP64, max_outer = 12;
+  double inner_tol = 1e-5;
+  size_t lds = 0;
+};
+
+static void batch_free(semx_batch_s* b) {
+  if (!b) return;
+  dev_free(b->inst); dev_free(b->work); dev_free(b->iters); dev_free(b->ptrs);
+  delete b;
+}
+
+int semx_batch_create(int count, const semx_helm* helms, semx_batch* batch) {
+  if (count < 1 || !helms || !batch) return SEMX_E_ARG;
+  *batch = nullptr;
+  semx_batch_s* b = new (std::nothrow) semx_batch_s();
+  if (!b) return SEMX_E_NOMEM;
+  b->count = count;
+  std::vector<PcgInst> inst(count);
+  std::vector<size_t> off(count + 1, 0);
+  for (int i = 0; i < count; ++i) {
+    if (!helms[i]) { batch_free(b); return SEMX_E_ARG; }
+    Helm& H = helms[i]->H;
+    if (H.nz != 1 || H.nmodes != 1 || H.modes[0].kind != SEMX_JACPCG ||
+        H.pcg_prec != helms[0]->H.pcg_prec) {
+      batch_free(b);
+      return SEMX_E_ARG;
+    }
+    int st = helm_upload(H);
+    if (st) { batch_free(b); return st; }
+    off[i + 1] = off[i] + (5 * (size_t)H.npts + (size_t)H.M->nel * H.M->next + 1) / 2 * 2;
+    b->helms.push_back(helms[i]);
+  }
+  const Helm& H0 = helms[0]->H;
+  b->prec = H0.pcg_prec; b->inner_tol = H0.inner_tol; b->max_outer = H0.max_outer;
+  int st = dev_upload<double>(&b->work, nullptr, off[count]);
+  if (!st) st = dev_upload<int>(&b->iters, nullptr, count);
+  if (!st) st = dev_upload<const double*>(&b->ptrs, nullptr, 3 * (size_t)count);
+  if (st) { batch_free(b); return st; }
+  for (int i = 0; i < count; ++i) {
+    Helm& H = helms[i]->H;
+    Mesh& M = *H.M;
+    PcgInst& I = inst[i];
+    I.op = PcgOp{M.np, M.npnp, M.next, M.nint, M.nel, M.nglobal, H.npts,
+                 H.lambda2 + H.modes[0].betak2, H.lambda2 > EPSDP, M.d.G, M.d.eset, M.d.pmap,
+                 M.d.btog, M.d.ops, M.d.asm_ptr, M.d.asm_idx};
+    I.n = H.modes[0].n; I.nsolve = M.nsolve; I.nbound = M.nbound;
+    I.emap = M.d.emap; I.ess_src = M.d.ess_src; I.nat_ptr = M.d.nat_ptr; I.nat_b = M.d.nat_b;
+    I.bside = M.d.bside; I.barea = M.d.barea; I.pc = H.d.pc; I.pc32 = H.d.pc32;
+    I.work = b->work + off[i];
+    const int cep = PCG_THREADS / M.npnp;
+    b->lds = std::max(b->lds, 32 * sizeof(double) +
+                      (2 * (size_t)M.npnp + 3 * (size_t)cep * M.npnp) * sizeof(double));
+  }
+  st = dev_upload(&b->inst, inst.data(), inst.size());
+  if (st) { batch_free(b); return st; }
+  *batch = b;
+  return SEMX_OK;
+}
+
+int semx_batch_destroy(semx_batch batch) {
+  if (!batch) return SEMX_E_ARG;
+  batch_free(batch);
+  return SEMX_OK;
+}
+
+int semx_batch_solve(semx_batch batch, const double* const* f, const double* const* bc,
+                     double* const* u, double tol_rel, int step_max, int* iters,
+                     void* stream) {
+  if (!batch || !f || !u || !(tol_rel >= 0.0) || step_max < 1) return SEMX_E_ARG;
+  semx_batch_s& b = *batch;
+  hipStream_t s = (hipStream_t)stream;
+  if (iters) *iters = 0;
+  std::vector<const double*> ptrs(3 * (size_t)b.count);
+  for (int i = 0; i < b.count; ++i) {
+    if (!f[i] || !u[i]) return SEMX_E_ARG;
+    if (b.helms[i]->H.M->nbound && (!bc || !bc[i])) return SEMX_E_ARG;
+    ptrs[i] = f[i];
+    ptrs[b.count + i] = bc ? bc[i] : nullptr;
+    ptrs[2 * b.count + i] = u[i];
+  }
+  HIPCHK(hipMemcpyAsync(b.ptrs, ptrs.data(), ptrs.size() * sizeof(double*),
+                        hipMemcpyHostToDevice, s));
+  PcgBatchArgs a;
+  a.inst = b.inst; a.f = b.ptrs; a.bc = b.ptrs + b.count;
+  a.u = (double* const*)(b.ptrs + 2 * b.count);
+  a.iters = b.iters; a.tol = tol_rel; a.inner_tol = b.inner_tol;
+  a.step_max = step_max; a.max_outer = b.max_outer;
+  const bool refine = b.prec == SEMX_PCG_FP32_REFINED;
+  const void* kfn = refine ? (const void*)k_pcg_batch<float, true>
+                           : (const void*)k_pcg_batch<double, false>;
+  if (b.lds > 64 * 1024)
+    HIPCHK(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b.lds));
+  void* kargs[] = {&a};
+  hipEvent_t ev;
+  prof_begin(s, &ev);
+  HIPCHK(hipLaunchKernel(kfn, dim3((unsigned)b.count), dim3(PCG_THREADS), kargs, b.lds, s));
+  prof_end(s, "k_pcg_batch", ev);
+  HIPCHK(hipGetLastError());
+  if (iters) {
+    std::vector<int> it(b.count);
+    HIPCHK(hipMemcpyAsync(it.data(), b.iters, it.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    int mx = 0;
+    bool limit = false;
+    for (int v : it) { limit |= v < 0; mx = std::max(mx, v < 0 ? -v : v); }
+    *iters = mx;
+    if (limit) return SEMX_E_MAXITER;
+  }
+  return SEMX_OK;
 }
 
 int semx_helm_set_pcg(semx_helm helm, int precision, double inner_tol, int max_outer) {

@@ -126,3 +126,23 @@ def taylor_green_3d(mesh, nz, beta=1.0, seed=3, amp=1e-3):
         full[:, :mesh.nPlane] = fourier_pack(f + scale * d)
         fields.append(full)
     return fields
+
+
+def batch_problems(count, seed=4):
+    """Independent 2D Dirichlet problems: K x K uniform quads on
+    [0, 2 pi]^2 with K ~ U{4..16}, np ~ U{5..11}, lambda2 ~ U[0.1, 100]
+    (numpy PCG64 stream of `seed`).  Returns [(np, K, lambda2, coeff_seed)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        K = int(rng.integers(4, 17))
+        np_ = int(rng.integers(5, 12))
+        lam = float(rng.uniform(0.1, 100.0))
+        out.append((np_, K, lam, int(rng.integers(0, 2**31 - 1))))
+    return out
+
+
+def single_plane_problem(mesh, lambda2, seed):
+    """f, BC lines and exact solution of lap u - lambda2 u = f for one
+    seeded smooth u* on a 2D mesh (nz = 1)."""
+    return helmholtz_batch(mesh, 1, lambda2, 0.0, seed)

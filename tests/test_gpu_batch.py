@@ -1,0 +1,74 @@
+"""Independent 2D problems solved together (one workgroup each) against the
+CPU FP64 oracle, through the C ABI.  Each problem's JACPCG system is the
+one semx_helm_solve uses; the batch kernel builds its right-hand side in
+the same workgroup.  Bars: converged at TOL_REL 1e-13 (FP64, or FP32 inner
+PCG with FP64 refinement): rel L2 <= 1e-10 vs the oracle DIRECT solution;
+same iteration count (+-1) as the single-system JACPCG path."""
+import numpy as np
+import pytest
+
+import semtex_amd as sx
+from semtex_amd.mesh import rect_mesh
+from semtex_amd.workloads import batch_problems, single_plane_problem
+from solve_helpers import gpu_solve, oracle_solve, rel
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+def problems(count, seed, bcs=("D", "D", "D", "D"), distort=0.0):
+    out = []
+    for (np_, K, lam, cs) in batch_problems(count, seed):
+        K = min(K, 8)
+        m = rect_mesh(K, K, 0, 2 * np.pi, 0, 2 * np.pi, bc=bcs, distort=distort)
+        P, Q = sx.Mesh(np_, m), O.Mesh(np_, m)
+        f, bc, ue = single_plane_problem(P, lam, cs)
+        out.append((P, Q, lam, f, bc, ue))
+    return out
+
+
+@pytest.mark.parametrize("prec", [sx.PCG_FP64, sx.PCG_FP32_REFINED], ids=["fp64", "fp32r"])
+def test_batch_matches_oracle(cuda, prec):
+    import torch
+    print(f"[batch] prec {prec}", flush=True)
+    probs = problems(7, seed=4) + problems(3, seed=9, bcs=("D", "N", "D", "N"), distort=0.2)
+    systems = []
+    for (P, Q, lam, f, bc, ue) in probs:
+        S = sx.ModalMatrixSys(P, lam, 1.0, 0, 1, 1, sx.JACPCG)
+        S.set_pcg(prec, inner_tol=1e-4, max_outer=30)
+        systems.append(S)
+    B = sx.Batch(systems)
+    dev = torch.device("cuda:0")
+    fs = [torch.from_numpy(p[3].ravel().copy()).to(dev) for p in probs]
+    bcs = [torch.from_numpy(p[4].ravel().copy()).to(dev) for p in probs]
+    us = [torch.zeros(p[3].size, dtype=torch.float64, device=dev) for p in probs]
+    it = B.solve(fs, bcs, us, tol_rel=1e-13)
+    torch.cuda.synchronize()
+    worst = 0.0
+    for (P, Q, lam, f, bc, ue), u, S in zip(probs, us, systems):
+        ud, _ = oracle_solve(Q, 1, lam, 1.0, f, bc, method=sx.DIRECT)
+        g = u.cpu().numpy().reshape(f.shape)
+        r = rel(g[:, :P.nPlane], ud[:, :P.nPlane])
+        worst = max(worst, r)
+        assert r <= 1e-10
+    print(f"  {len(probs)} problems, max iterations {it}, worst rel L2 vs DIRECT {worst:.3e}",
+          flush=True)
+
+
+def test_batch_iterations_match_single_solves(cuda):
+    import torch
+    probs = problems(4, seed=11)
+    dev = torch.device("cuda:0")
+    systems = [sx.ModalMatrixSys(p[0], p[2], 1.0, 0, 1, 1, sx.JACPCG) for p in probs]
+    single = []
+    for S, p in zip(systems, probs):
+        _, it = gpu_solve(torch, p[0], S, p[3], p[4], tol=1e-12)
+        single.append(it)
+    B = sx.Batch(systems)
+    fs = [torch.from_numpy(p[3].ravel().copy()).to(dev) for p in probs]
+    bcs = [torch.from_numpy(p[4].ravel().copy()).to(dev) for p in probs]
+    us = [torch.zeros(p[3].size, dtype=torch.float64, device=dev) for p in probs]
+    it = B.solve(fs, bcs, us, tol_rel=1e-12)
+    print(f"[batch] iterations batch {it}, single {single}", flush=True)
+    assert abs(it - max(single)) <= 1
