@@ -94,10 +94,9 @@ struct HelmDev {
   int* pcg_plane = nullptr;   // plane-list indices solved by PCG
   double* work = nullptr;     // PCG vectors, one slice per PCG plane
   int* iters = nullptr;       // PCG iterations per PCG plane
-  // cooperative band solve: hand-off flags, partial sums, chunk ranges
-  unsigned* yflag = nullptr;  // ndirect*nblkmax
-  unsigned* pflag = nullptr;  // ndirect*W*nblkmax
-  double* part = nullptr;     // ndirect*W*nblkmax*6*nb
+  // cooperative band solve: hand-off slots, chunk ranges
+  double* inbox = nullptr;    // 2 directions x coop_xstride, lead -> followers
+  double* part = nullptr;     // 2 directions x coop_xstride, followers -> lead
   int* clo = nullptr;         // W+1
   int* err = nullptr;         // 1: a hand-off wait timed out
   // scratch
@@ -133,7 +132,7 @@ struct Helm {
   int coop_w = 0;                     // workgroups per mode (0: one-workgroup sweep)
   int coop_rpt = 1;                   // panel rows per task thread
   int nblkmax = 0;
-  unsigned coop_epoch = 0;            // launches so far (flag values grow with it)
+  size_t coop_xstride = 0;            // hand-off slots per direction
   std::vector<int> coop_clo;          // chunk ranges per lane
   bool on_device = false;
   HelmDev d;

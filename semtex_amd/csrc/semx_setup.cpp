@@ -4,6 +4,7 @@
 // semtex's Element / Mesh / BCmgr / MatrixSys constructors; it runs once
 // per system and is excluded from solve timings.
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <map>

@@ -1,22 +1,23 @@
-// semx_solve.hip -- CDNA4 (gfx950) kernels and the C ABI of libsemx.
+// semx_solve.hip -- CDNA4 (gfx950) kernels and the C ABI of libsemx for
+// semtex's Field::solve (src/field.cpp:509-725) over every plane of a field:
 //
-// Direct (static-condensation) Helmholtz solve over all planes of a field,
-// the work of semtex's Field::solve DIRECT branch (src/field.cpp:588-625):
-//
-//   k_ess_gather   essential BC values of every plane   (Field::getEssential)
-//   k_condense     -(M f + H g), interior elimination F_b -= hbi F_i, <h,w>
-//                  (Field::constrain, Element::local2globalSumSC, Natural::sum)
-//   k_band_solve   assembly of the condensed RHS by owner gather, then the
-//                  banded Cholesky triangular solves U^T y = b, U x = y with
-//                  the whole RHS resident in LDS             (LAPACK dpbtrs)
-//   k_backsub      interior recovery u_i = hii F_i - hbi^T u_b and essential
-//                  re-imposition (Element::global2localSC, Field::setEssential)
-//
-// One workgroup owns one plane in k_band_solve: the factor of a mode is
-// streamed from HBM/L2 in blocks of NB columns; each block is a dense
-// NB x NB diagonal solve (precomputed triangular inverse) and a
-// right-looking update of the next kd rows, so every step is a parallel
-// dot product across the 1024 threads of the workgroup.
+//   k_ess_gather    essential BC values of every plane   (Field::getEssential)
+//   k_condense[_staged]
+//                   -(M f + H g), interior elimination F_b -= hbi F_i, <h,w>
+//                   (Field::constrain, Element::local2globalSumSC, Natural::sum)
+//   k_assemble      owner-gather of the condensed boundary RHS (buildRHS)
+//   k_band_solve    banded Cholesky triangular solves U^T y = b, U x = y
+//                   (LAPACK dpbtrs): one workgroup per mode, the factor
+//                   streamed as NB-column panels, the RHS in an LDS window
+//   k_band_coop     the same with several workgroups per mode in a lagged
+//                   pipeline (opt-in)
+//   k_backsub[_staged]
+//                   interior recovery u_i = hii F_i - hbi^T u_b and essential
+//                   re-imposition (Element::global2localSC, Field::setEssential)
+//   k_pcg           JACPCG, one persistent workgroup per plane (FP64, or FP32
+//                   inner PCG with FP64 refinement)
+//   k_pcg_batch     the same for many independent 2D problems in one launch,
+//                   right-hand side built in the workgroup
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -432,18 +433,23 @@ __global__ __launch_bounds__(BT) void k_band_solve(BandArgs a) {
 // The off-diagonal panel of block J (NB columns x kdp rows) is cut into
 // NB-row chunks; chunk c of block J feeds block J+1+c (forward; block
 // J-1-c backward).  Lane 0 of a mode (the lead) owns chunks [0, C1): it
-// solves each diagonal block, publishes y_J, and applies the chunks that
-// feed the next C1 blocks.  Lanes w >= 1 (followers) own further chunks;
-// each waits for y_J, applies its chunks into an LDS accumulator per
-// target block, and publishes the finished partial sum of target T once
-// it has applied its last contribution to T (from block T-1-c_lo(w)).  The
-// lead reads those sums when it reaches T, which followers finished at
-// least c_lo(w) >= C1 steps earlier, so in the steady state nobody waits.
-// Hand-offs are flags in global memory set with release and read with
-// acquire semantics at agent scope; flag values grow with every launch
-// (2 epoch forward, 2 epoch + 1 backward) so nothing is cleared between
-// solves.  Every wait is bounded: a timeout sets the error word and the
-// workgroup leaves, so the grid always drains.
+// solves each diagonal block, hands y_J to every follower, and applies the
+// chunks that feed the next C1 blocks.  Lanes w >= 1 (followers) own
+// further chunks; each applies them to y_J into an LDS accumulator per
+// target block and hands the finished partial sum of target T to the lead
+// once its last contribution to T (from block T-1-c_lo(w)) is in.  The
+// lead needs it c_lo(w) >= C1 steps after y_{T-1-c_lo} left, which hides
+// the hand-off latency.
+//
+// Hand-off slots (one per value) hold a signalling-NaN sentinel until the
+// producer stores the value; arithmetic never produces that bit pattern, so
+// a consumer knows a value has arrived by reading it, with no separate flag
+// and no store-completion wait.  Slot stores and loads are device-scope
+// relaxed atomics (coherent across XCDs without cache write-back).  Every
+// slot is reset to the sentinel by a fill kernel before each launch.  The
+// consumers load the slots of step s+PD while working on step s.  Every
+// wait is bounded: a timeout sets the error word and the workgroup leaves,
+// so the grid always drains.
 // ===========================================================================
 
 struct CoopArgs {
@@ -457,76 +463,183 @@ struct CoopArgs {
   const int* dmode;
   const int* mode_plane;
   double* sol;
-  double* part;             // [md][lane][nblkmax][NR][NB]
-  unsigned* yflag;          // [md][nblkmax]
-  unsigned* pflag;          // [md][lane][nblkmax]
+  double* inbox;            // [dir][md][W-1][nblkmax][NR*NB]  lead -> follower
+  double* part;             // [dir][md][W-1][nblkmax][NR*NB]  follower -> lead
+                            // (dir stride xstride)
   const int* clo;           // [W+1] chunk ranges per lane
   int* err;
   int solstride, kd, kdp, ndirect, W, nblkmax;
   int npl;                  // planes per field and mode
   long long fstride;        // sol offset between fields
-  unsigned epoch;
+  long long xstride;        // doubles per direction of inbox / part
 };
 
-constexpr int COOP_BT = 256;
-constexpr int COOP_TT = COOP_BT - 64;   // task threads; the last wave signals
-constexpr int COOP_RING = 16;           // accumulator blocks per lane (>= chunks + 2)
+constexpr int COOP_TT = 192;            // panel ("task") threads: the last three waves
+constexpr int COOP_RING = 32;           // accumulator blocks per lane (>= chunks + 2)
+template <int NB> constexpr int coop_ring() { return NB >= 64 ? 16 : COOP_RING; }
+constexpr int COOP_WMAX = 8;            // lanes per mode
+constexpr int COOP_PD = 2;              // steps of prefetch
 constexpr long COOP_SPIN = 1L << 20;    // polls before a wait gives up
+constexpr unsigned long long COOP_SENT = 0x7FF4DEADBEEF0001ull;   // signalling NaN
 
-__device__ inline bool coop_wait(const unsigned* f, unsigned target, int* err) {
-  for (long it = 0; it < COOP_SPIN; ++it) {
-    if (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= target) return true;
+__device__ inline double ld_dev(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_dev(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline bool is_sent(double v) {
+  return (unsigned long long)__double_as_longlong(v) == COOP_SENT;
+}
+// the value of a slot once it has arrived (v: an earlier read of it)
+__device__ inline double coop_take(const double* p, double v, int* abort) {
+  for (long it = 0; is_sent(v); ++it) {
+    if (it >= COOP_SPIN) { *abort = 1; return 0.0; }
     __builtin_amdgcn_s_sleep(1);
+    v = ld_dev(p);
   }
-  __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return false;
+  return v;
 }
 
-__device__ inline void coop_signal(unsigned* f, unsigned v) {
-  __hip_atomic_store(f, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+__global__ void k_coop_fill(double* p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = __longlong_as_double((long long)COOP_SENT);
 }
 
-// One sweep; returns false if a wait timed out (the whole workgroup agrees).
-// The first COOP_TT threads own panel rows ("tasks"); the last wave does the
-// flag traffic, so its release/acquire waits never stall on panel loads.
+// Threads per workgroup: ceil(NR*NB/64) hand-off waves (one value per
+// thread; wave 0 also solves the diagonal block) then the task waves.  The
+// two groups issue disjoint loads, so a wait on one group's data never
+// waits for the other's (memory counters are per wave).
+template <int NB, int NR>
+constexpr int coop_bt() { return 64 * ((NR * NB + 63) / 64) + COOP_TT; }
+
+// One sweep; returns false if a hand-off timed out (the workgroup agrees).
 template <int NB, int NR, int RPT, bool FWD>
 __device__ bool coop_sweep(const CoopArgs& a, int md, int lane, const double* __restrict__ P,
                            const double* __restrict__ L, double* const (&gy)[NR], int n,
-                           int nblk, double* acc, double* rhs, double* ybuf, int* sflag) {
-  constexpr int QPT = (NB * NB / COOP_BT > 0) ? NB * NB / COOP_BT : 1;
+                           int nblk, double* acc, double* rhs, double* ybuf, int* abort) {
+  constexpr int BT = coop_bt<NB, NR>();
+  constexpr int VT = BT - COOP_TT;            // hand-off threads
+  constexpr int QPT = (NB * NB / 64 > 0) ? NB * NB / 64 : 1;   // diagonal: wave 0
   constexpr int G = NB / QPT;
+  constexpr int NV = NR * NB;                 // values handed over per block
   const int tid = threadIdx.x;
-  const bool syncer = tid == COOP_TT;
-  const unsigned target = 2u * a.epoch + (FWD ? 0u : 1u);
   const int c_lo = a.clo[lane], c_hi = a.clo[lane + 1];
   const int ntask = (c_hi - c_lo) * NB;
   const int W = a.W;
-  unsigned* yflag = a.yflag + (size_t)md * a.nblkmax;
-  unsigned* pflag = a.pflag + (size_t)md * W * a.nblkmax;
-  double* part = a.part + (size_t)md * W * a.nblkmax * NR * NB;   // NR <= 6 fits
+  // first chunk of every lane, held in registers (a global load inside the
+  // step loop would make its wait cover the prefetched hand-off loads)
+  int cl[COOP_WMAX];
+#pragma unroll
+  for (int w = 0; w < COOP_WMAX; ++w) cl[w] = (w < W) ? a.clo[w] : 0x3fffffff;
+  const size_t dirofs = FWD ? 0 : (size_t)a.xstride;
+  const size_t lstride = (size_t)a.nblkmax * NV;            // per lane
+  double* inbox = a.inbox + dirofs + (size_t)md * (W - 1) * lstride;
+  double* part = a.part + dirofs + (size_t)md * (W - 1) * lstride;
   const bool lead = lane == 0;
   const int last = lead ? nblk - 1 : nblk - 2 - c_lo;   // last step with work
-  for (int i = tid; i < COOP_RING * NR * NB; i += COOP_BT) acc[i] = 0.0;
+  constexpr int RING = coop_ring<NB>();
+  for (int i = tid; i < RING * NV; i += BT) acc[i] = 0.0;
+  if (tid == 0) *abort = 0;
 
+  const bool vth = tid < NV;                  // hand-off threads: one value each
+  const int vc = vth ? tid / NB : 0, vi = vth ? tid - (tid / NB) * NB : 0;
+  double* gyv = gy[0];                        // this thread's right-hand side
+#pragma unroll
+  for (int c = 1; c < NR; ++c)
+    if (c == vc) gyv = gy[c];
+  const bool dact = tid < NB * G;             // within wave 0
+  const int da = dact ? tid / G : 0, dg = dact ? tid % G : 0;
+  const int tt = tid - VT;                    // task index base (< 0: hand-off wave)
+
+  // prefetch registers: the panel one step ahead (task waves); hand-off
+  // values, b rows and the diagonal-block inverse COOP_PD steps ahead
+  // (hand-off waves), indexed by step % COOP_PD
   double pp[RPT][NB];
+  double bv[COOP_PD], lv[COOP_PD][QPT], hv[COOP_PD][COOP_WMAX - 1];
+  auto block_of = [&](int s) { return FWD ? s : nblk - 1 - s; };
   auto load_panel = [&](int s) {
-    const int J = FWD ? s : nblk - 1 - s;
-    const double* pj = P + (size_t)J * NB * a.kdp;
+    if (s > last || ntask <= 0 || tt < 0) return;
+    const double* pj = P + (size_t)block_of(s) * NB * a.kdp;
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
-      const int t = min(tid + q * COOP_TT, max(ntask - 1, 0));
+      const int t = min(tt + q * COOP_TT, ntask - 1);
       const int r = min(c_lo * NB + t, a.kdp - 1);
 #pragma unroll
       for (int jj = 0; jj < NB; ++jj) pp[q][jj] = pj[(size_t)jj * a.kdp + r];
     }
   };
-  auto apply = [&](int s) {
+  // every hand-off thread issues the same loads on every step (indices
+  // clamped, values masked where unused), so the compiler can wait for
+  // exactly the loads a step needs and not for the ones issued after them
+  auto fetch = [&](int s, int u) {
+    if (tid >= VT) return;                    // wave-uniform
+    const int sc = min(s, last < 0 ? 0 : last);
+    const int J = block_of(sc);
+    if (lead) {
+      const int row = min(J * NB + vi, n - 1);
+      bv[u] = ld_dev(&gyv[row]);
+      const double* lj = L + (size_t)J * NB * NB + da * NB + dg * QPT;
+#pragma unroll
+      for (int q = 0; q < QPT; ++q) lv[u][q] = lj[q];
+#pragma unroll
+      for (int w = 1; w < COOP_WMAX; ++w)
+        hv[u][w - 1] = ld_dev(&part[(min(w, max(W - 1, 1)) - 1) * lstride + (size_t)sc * NV +
+                                    min(tid, NV - 1)]);
+    } else {
+      hv[u][0] = ld_dev(&inbox[(lane - 1) * lstride + (size_t)sc * NV + min(tid, NV - 1)]);
+    }
+  };
+
+  auto step = [&](int s, int u) -> bool {
+    const int J = block_of(s);
+    const int j0 = J * NB;
+    const int slot = s & (RING - 1);
+    if (lead) {
+      // A. right-hand side of block J: b - own partials - followers' partials
+      if (vth) {
+        double v = (j0 + vi < n ? bv[u] : 0.0) - acc[slot * NV + tid];
+#pragma unroll
+        for (int w = 1; w < COOP_WMAX; ++w)
+          if (s >= 1 + cl[w])
+            v -= coop_take(&part[(w - 1) * lstride + (size_t)s * NV + tid], hv[u][w - 1], abort);
+        rhs[tid] = v;
+        acc[slot * NV + tid] = 0.0;
+      }
+      if (__syncthreads_or(*abort)) return false;
+      // B. diagonal block: y_J = L_J rhs (wave 0)
+      if (tid < 64) {
+#pragma unroll
+        for (int c = 0; c < NR; ++c) {
+          double accd = 0.0;
+#pragma unroll
+          for (int q = 0; q < QPT; ++q) accd += lv[u][q] * rhs[c * NB + dg * QPT + q];
+          const double v = group_sum<G>(accd);
+          if (dact && dg == 0) ybuf[c * NB + da] = v;
+        }
+      }
+      __syncthreads();
+      // C. hand y_J to the solution vector and to every follower
+      if (vth && j0 + vi < n) st_dev(&gyv[j0 + vi], ybuf[tid]);
+      if (tid < VT)
+        for (int i = tid; i < (W - 1) * NV; i += VT) {
+          const int w = i / NV, k = i - (i / NV) * NV;
+          st_dev(&inbox[w * lstride + (size_t)s * NV + k], ybuf[k]);
+        }
+    } else {
+      // A. y_J from the lead
+      if (vth)
+        ybuf[tid] = coop_take(&inbox[(lane - 1) * lstride + (size_t)s * NV + tid], hv[u][0],
+                              abort);
+      if (__syncthreads_or(*abort)) return false;
+    }
+    // apply the owned chunks of this block's panel
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
-      const int t = tid + q * COOP_TT;
-      if (tid < COOP_TT && t < ntask) {
+      const int t = tt + q * COOP_TT;
+      if (tt >= 0 && t < ntask) {
         const int c = c_lo + t / NB, r = t - (t / NB) * NB;
-        const int tslot = (s + 1 + c) & (COOP_RING - 1);
+        const int tslot = (s + 1 + c) & (RING - 1);
         const int tr = FWD ? r : NB - 1 - r;
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
@@ -542,92 +655,41 @@ __device__ bool coop_sweep(const CoopArgs& a, int md, int lane, const double* __
         }
       }
     }
-  };
-  if (last >= 0 && tid < COOP_TT) load_panel(0);
-  __syncthreads();
-
-  for (int s = 0; s <= last; ++s) {
-    const int J = FWD ? s : nblk - 1 - s;
-    const int j0 = J * NB;
-    const int slot = s & (COOP_RING - 1);
-    if (lead) {
-      // 1. the followers' finished partial sums for this block
-      if (syncer) {
-        int ok = 1;
-        for (int w = 1; w < W && ok; ++w)
-          if (s >= 1 + a.clo[w])
-            ok = coop_wait(pflag + (size_t)w * a.nblkmax + s, target, a.err);
-        *sflag = ok;
-      }
-      __syncthreads();
-      if (!*sflag) return false;
-      // 2. right-hand side of the block
-      if (tid < NR * NB) {
-        const int c = tid / NB, i = tid - c * NB;
-        const int row = j0 + i;
-        double v = (row < n) ? gy[c][row] : 0.0;
-        v -= acc[slot * NR * NB + tid];
-        for (int w = 1; w < W; ++w)
-          if (s >= 1 + a.clo[w]) v -= part[(((size_t)w * a.nblkmax + s) * NR + c) * NB + i];
-        rhs[tid] = v;
-        acc[slot * NR * NB + tid] = 0.0;
-      }
-      __syncthreads();
-      // 3. diagonal block: y_J = L_J rhs, to LDS and to the solution vector
-      {
-        const bool dact = tid < NB * G;
-        const int da = dact ? tid / G : 0, dg = dact ? tid % G : 0;
-        const double* lj = L + (size_t)J * NB * NB + da * NB + dg * QPT;
-#pragma unroll
-        for (int c = 0; c < NR; ++c) {
-          double accd = 0.0;
-#pragma unroll
-          for (int q = 0; q < QPT; ++q) accd += lj[q] * rhs[c * NB + dg * QPT + q];
-          const double v = group_sum<G>(accd);
-          if (dact && dg == 0) {
-            ybuf[c * NB + da] = v;
-            if (j0 + da < n) gy[c][j0 + da] = v;
-          }
-        }
-      }
-      __syncthreads();
-      // 4. publish y_J; 5. apply the lead's chunks; 6. prefetch
-      if (syncer) coop_signal(yflag + s, target);
-      apply(s);
-      if (s < last && tid < COOP_TT) load_panel(s + 1);
-    } else {
-      if (syncer) *sflag = coop_wait(yflag + s, target, a.err);
-      __syncthreads();
-      if (!*sflag) return false;
-      if (tid < NR * NB) {
-        const int c = tid / NB, i = tid - c * NB;
-        const int row = j0 + i;
-        ybuf[tid] = (row < n) ? gy[c][row] : 0.0;
-      }
-      __syncthreads();
-      apply(s);
-      __syncthreads();
+    __syncthreads();
+    if (!lead) {
       // target s+1+c_lo received its last contribution from this lane
       const int T = s + 1 + c_lo;
-      const int tslot = T & (COOP_RING - 1);
-      if (T < nblk && tid < NR * NB) {
-        part[((size_t)lane * a.nblkmax + T) * NR * NB + tid] = acc[tslot * NR * NB + tid];
-        acc[tslot * NR * NB + tid] = 0.0;
+      const int tslot = T & (RING - 1);
+      if (T < nblk && vth) {
+        st_dev(&part[(lane - 1) * lstride + (size_t)T * NV + tid], acc[tslot * NV + tid]);
+        acc[tslot * NV + tid] = 0.0;
       }
-      __syncthreads();
-      if (syncer && T < nblk) coop_signal(pflag + (size_t)lane * a.nblkmax + T, target);
-      if (s < last && tid < COOP_TT) load_panel(s + 1);
+    }
+    load_panel(s + 1);
+    fetch(s + COOP_PD, u);
+    return true;
+  };
+
+  __syncthreads();
+  load_panel(0);
+#pragma unroll
+  for (int u = 0; u < COOP_PD; ++u) fetch(u, u);
+  for (int s0 = 0; s0 <= last; s0 += COOP_PD) {
+#pragma unroll
+    for (int u = 0; u < COOP_PD; ++u) {
+      if (s0 + u > last) break;
+      if (!step(s0 + u, u)) return false;
     }
   }
   return true;
 }
 
 template <int NB, int NR, int RPT>
-__global__ __launch_bounds__(COOP_BT) void k_band_coop(CoopArgs a) {
-  __shared__ double acc[COOP_RING * NR * NB];
+__global__ __launch_bounds__((coop_bt<NB, NR>())) void k_band_coop(CoopArgs a) {
+  __shared__ double acc[coop_ring<NB>() * NR * NB];
   __shared__ double rhs[NR * NB];
   __shared__ double ybuf[NR * NB];
-  __shared__ int sflag;
+  __shared__ int abort_flag;
   // blocks of one mode share an XCD: dispatch is round-robin over 8 XCDs
   const int b = blockIdx.x;
   const int md = (b / (8 * a.W)) * 8 + (b & 7);
@@ -644,15 +706,18 @@ __global__ __launch_bounds__(COOP_BT) void k_band_coop(CoopArgs a) {
     if (p < 0) p = a.mode_plane[2 * m];
     gy[c] = a.sol + fld * a.fstride + (size_t)p * a.solstride;
   }
-  if (!coop_sweep<NB, NR, RPT, true>(a, md, lane, a.Pf + a.offP[m], a.Linv + a.offL[m], gy, n,
-                                     nblk, acc, rhs, ybuf, &sflag))
-    return;
-  // the backward sweep of the lead reads, from global memory, values that
-  // other threads of this workgroup wrote during the forward sweep
-  __threadfence();
-  __syncthreads();
-  coop_sweep<NB, NR, RPT, false>(a, md, lane, a.Pb + a.offP[m], a.LinvT + a.offL[m], gy, n,
-                                 nblk, acc, rhs, ybuf, &sflag);
+  const bool ok = coop_sweep<NB, NR, RPT, true>(a, md, lane, a.Pf + a.offP[m],
+                                                a.Linv + a.offL[m], gy, n, nblk, acc, rhs,
+                                                ybuf, &abort_flag);
+  if (ok) {
+    __syncthreads();
+    coop_sweep<NB, NR, RPT, false>(a, md, lane, a.Pb + a.offP[m], a.LinvT + a.offL[m], gy, n,
+                                   nblk, acc, rhs, ybuf, &abort_flag);
+  } else if (threadIdx.x == 0) {
+    __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (ok && abort_flag && threadIdx.x == 0)
+    __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ __launch_bounds__(ELT_THREADS) void k_backsub(ElemArgs a) {
@@ -1281,14 +1346,16 @@ void mesh_release(Mesh& M) {
 // Lanes of the cooperative band solve: the lead owns the first `lead`
 // chunks of every panel, each follower `follow` further chunks.  The band
 // is split so that the workgroups of all direct modes fill the device.
-// SEMX_BAND_COOP=0 keeps the one-workgroup sweep; SEMX_COOP_LEAD and
-// SEMX_COOP_FOLLOW override the chunk counts.
+// SEMX_COOP_LEAD and SEMX_COOP_FOLLOW override the chunk counts.
 int coop_setup(Helm& H) {
   H.coop_w = 0;
   const int nd = (int)H.dmode.size();
   if (nd == 0 || H.npass == 0) return SEMX_OK;
+  // default: cooperative for bands wider than 512 (one workgroup cannot
+  // stream them fast enough); SEMX_BAND_COOP=0 / 1 forces it off / on
   const char* env = getenv("SEMX_BAND_COOP");
-  if (!env || atoi(env) == 0) return SEMX_OK;
+  const bool want = env ? atoi(env) != 0 : H.npass > 1;
+  if (!want) return SEMX_OK;
   const int nb = H.nb, nch = H.kdp / nb;
   int cus = 256;
   int dev = 0;
@@ -1297,20 +1364,21 @@ int coop_setup(Helm& H) {
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
       cus = v;
   }
-  const int tasks = COOP_TT * 2;                       // RPT <= 2
-  int lead = std::min(nch, 4);
+  const int tasks = COOP_TT * (nb >= 64 ? 1 : 2);      // RPT <= 2 (1 at 64 columns)
+  const int ring = nb >= 64 ? 16 : COOP_RING;
+  int lead = std::min(nch, std::max(1, 128 / nb));     // ~128 panel rows for the lead
   if (const char* e = getenv("SEMX_COOP_LEAD")) lead = std::max(1, std::min(nch, atoi(e)));
-  lead = std::min(lead, tasks / nb);
+  lead = std::min(lead, std::min(tasks / nb, ring - 2));
   const int rest = nch - lead;
   int follow = 0, nf = 0;
   if (rest > 0) {
-    const int lanes_free = std::max(1, cus / nd - 1);  // followers per mode that fit
-    follow = (rest + lanes_free - 1) / lanes_free;
+    const int lanes = std::max(1, std::min(COOP_WMAX - 1, cus / nd - 1));
+    follow = (rest + lanes - 1) / lanes;
     if (const char* e = getenv("SEMX_COOP_FOLLOW")) follow = std::max(1, atoi(e));
-    follow = std::min(follow, std::min(COOP_RING - 2, tasks / nb));
+    follow = std::min(follow, std::min(ring - 2, tasks / nb));
     nf = (rest + follow - 1) / follow;
   }
-  if (lead + 1 > COOP_RING) return SEMX_OK;
+  if (1 + nf > COOP_WMAX) return SEMX_OK;    // band too wide for the lanes: one-workgroup sweep
   H.coop_w = 1 + nf;
   H.coop_rpt = (std::max(lead, follow) * nb > COOP_TT) ? 2 : 1;
   H.coop_clo.assign(H.coop_w + 1, 0);
@@ -1319,14 +1387,13 @@ int coop_setup(Helm& H) {
   int nblkmax = 0;
   for (int m : H.dmode) nblkmax = std::max(nblkmax, H.modes[m].nblk);
   H.nblkmax = nblkmax;
-  const size_t W = H.coop_w;
+  // hand-off slots for up to SEMX_MAX_FIELDS fields of two planes
+  H.coop_xstride = (size_t)nd * std::max(H.coop_w - 1, 1) * nblkmax * 2 * SEMX_MAX_FIELDS * nb;
   int st;
-  if ((st = dev_upload<unsigned>(&H.d.yflag, nullptr, (size_t)nd * nblkmax))) return st;
-  if ((st = dev_upload<unsigned>(&H.d.pflag, nullptr, (size_t)nd * W * nblkmax))) return st;
-  if ((st = dev_upload<double>(&H.d.part, nullptr, (size_t)nd * W * nblkmax * 6 * nb))) return st;
+  if ((st = dev_upload<double>(&H.d.inbox, nullptr, 2 * H.coop_xstride))) return st;
+  if ((st = dev_upload<double>(&H.d.part, nullptr, 2 * H.coop_xstride))) return st;
   if ((st = dev_upload(&H.d.clo, H.coop_clo.data(), H.coop_clo.size()))) return st;
   if ((st = dev_upload<int>(&H.d.err, nullptr, 1))) return st;
-  H.coop_epoch = 0;
   return SEMX_OK;
 }
 
@@ -1382,7 +1449,7 @@ void helm_release(Helm& H) {
   dev_free(H.d.g); dev_free(H.d.sol); dev_free(H.d.plane_of);
   dev_free(H.d.pc32); dev_free(H.d.mode_kind); dev_free(H.d.pcg_plane); dev_free(H.d.work);
   dev_free(H.d.iters);
-  dev_free(H.d.yflag); dev_free(H.d.pflag); dev_free(H.d.part); dev_free(H.d.clo);
+  dev_free(H.d.inbox); dev_free(H.d.part); dev_free(H.d.clo);
   dev_free(H.d.err);
   H.nf_cap = 0;
   H.on_device = false;
@@ -1518,7 +1585,7 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
       launches = nf;
     }
     const size_t lds = (size_t)nr * (ba.ring + H.nb) * sizeof(double);
-    if (lds > 160 * 1024) return SEMX_E_UNSUPPORTED;
+    if (lds > 160 * 1024 && H.coop_w == 0) return SEMX_E_UNSUPPORTED;
     const void* kfn = nullptr;
 #define SEMX_PICK(NB_, NP_, BT_, D_)                                        \
     if (H.nb == NB_ && H.npass == NP_ && H.bt == BT_)                       \
@@ -1533,7 +1600,8 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
     SEMX_PICK(16, 2, 512, 1)
     SEMX_PICK(8, 4, 512, 1)
 #undef SEMX_PICK
-    if (!kfn) return SEMX_E_UNSUPPORTED;
+    if (!kfn && H.coop_w == 0) return SEMX_E_UNSUPPORTED;
+    bool done = false;
     if (H.coop_w > 0) {
       // cooperative sweep: all fields' planes of a mode in one launch
       const int nrc = npl * nf;
@@ -1551,26 +1619,34 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
         ca.Pf = H.d.Pf; ca.Pb = H.d.Pb; ca.Linv = H.d.Linv; ca.LinvT = H.d.LinvT;
         ca.mode_n = H.d.mode_n; ca.offP = H.d.mode_offH; ca.offL = H.d.mode_offL;
         ca.dmode = H.d.dmode; ca.mode_plane = H.d.mode_plane; ca.sol = H.d.sol;
-        ca.part = H.d.part; ca.yflag = H.d.yflag; ca.pflag = H.d.pflag; ca.clo = H.d.clo;
+        ca.inbox = H.d.inbox; ca.part = H.d.part; ca.clo = H.d.clo;
         ca.err = H.d.err; ca.solstride = H.solstride; ca.kd = H.kd; ca.kdp = H.kdp;
         ca.ndirect = (int)H.dmode.size(); ca.W = H.coop_w; ca.nblkmax = H.nblkmax;
         ca.npl = npl; ca.fstride = (long long)psol;
-        ca.epoch = ++H.coop_epoch;
+        ca.xstride = (long long)H.coop_xstride;
+        // every hand-off slot of this launch back to the sentinel
+        const size_t used = (size_t)ca.ndirect * std::max(ca.W - 1, 1) * ca.nblkmax * nrc * H.nb;
+        for (double* buf : {H.d.inbox, H.d.part})
+          for (int dir = 0; dir < 2; ++dir)
+            hipLaunchKernelGGL(k_coop_fill, dim3((unsigned)((used + 255) / 256)), dim3(256), 0,
+                               s, buf + dir * H.coop_xstride, used);
         const int groups = (ca.ndirect + 7) / 8;
         void* kargs[] = {&ca};
         prof_begin(s, &ev);
         const hipError_t ce2 = hipLaunchCooperativeKernel(
-            cfn, dim3((unsigned)(groups * 8 * H.coop_w)), dim3(COOP_BT), kargs, 0, s);
+            cfn, dim3((unsigned)(groups * 8 * H.coop_w)),
+            dim3((unsigned)(64 * ((nrc * H.nb + 63) / 64) + COOP_TT)), kargs, 0, s);
         prof_end(s, "k_band_solve", ev);
         if (ce2 != hipSuccess) {
           fprintf(stderr, "semx: cooperative band solve: %s\n", hipGetErrorString(ce2));
           return SEMX_E_HIP;
         }
-        kfn = nullptr;   // done
+        done = true;
       }
     }
-    if (kfn) HIPCHK(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    for (int l = 0; kfn && l < launches; ++l) {
+    if (!done && !kfn) return SEMX_E_UNSUPPORTED;
+    if (!done) HIPCHK(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    for (int l = 0; !done && l < launches; ++l) {
       BandArgs bl = ba;
       bl.sol = H.d.sol + (launches > 1 ? l * psol : 0);
       void* kargs[] = {&bl};
