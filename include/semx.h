@@ -178,7 +178,8 @@ int semx_batch_solve (semx_batch batch, const double* const* f,
  * in place.  sign SEMX_FORWARD: X_j = (1/tlen) sum x_k exp(-2 pi i jk/tlen)
  * packed per column as [Re0, Re_tlen/2, Re1, Im1, Re2, Im2, ...];
  * SEMX_INVERSE: the unscaled inverse of that packing.  tlen must be
- * 2^p 3^q 5^r and ntrn even (SEMX_E_ARG otherwise, as dDFTr's error).
+ * 2^p 3^q 5^r and ntrn even (SEMX_E_ARG otherwise, as dDFTr's error);
+ * lengths above 256 return SEMX_E_UNSUPPORTED.
  * Uses a library scratch buffer: concurrent calls from several host
  * threads are not supported.                                              */
 int semx_dft_r (double* data, int tlen, int ntrn, int sign, void* stream);

@@ -28,32 +28,71 @@ using namespace semx;
 
 namespace {
 
-constexpr int DFT_THREADS = 256;
-constexpr int DFT_OC = 32;          // output rows per thread
+constexpr int DFT_KB = 16;          // padding of the transform matrix columns
 constexpr int OP_THREADS = 1024;    // element-operator blocks
 
-// out[r][col] = sum_k W[r][k] in[k][col], r < N; W padded with zero rows to
-// a multiple of DFT_OC.  W is indexed uniformly across the wavefront, so its
-// entries arrive through scalar loads; the columns are coalesced.
-__global__ __launch_bounds__(DFT_THREADS) void k_dft_z(const double* __restrict__ in,
-                                                       double* __restrict__ out,
-                                                       const double* __restrict__ W,
-                                                       int N, int ncol, long long stride) {
-  const int col = blockIdx.x * DFT_THREADS + threadIdx.x;
-  const int r0 = blockIdx.y * DFT_OC;
-  if (col >= ncol) return;
-  double acc[DFT_OC];
+// out[r][col] = sum_k W[r][k] in[k][col], r < N: a register-blocked
+// product of the (padded) N x N transform matrix with a 64-column tile of
+// the field.  Threads form 16 row groups x 16 column groups; each computes
+// RP rows x 4 columns, with the k dimension staged through LDS 16 rows at a
+// time.  W has rows padded to 16*RP and columns to a multiple of 16 (zero
+// entries), so loads of rows >= N are clamped without a branch.
+constexpr int DFT_COLS = 64;
+constexpr int DFT_KC = 16;
+
+template <int RP>
+__global__ __launch_bounds__(256) void k_dft_z(const double* __restrict__ in,
+                                               double* __restrict__ out,
+                                               const double* __restrict__ W, int N, int ldw,
+                                               int ncol, long long stride) {
+  __shared__ double sX[DFT_KC][DFT_COLS];
+  __shared__ double sW[16 * RP][DFT_KC + 1];
+  const int tid = threadIdx.x;
+  const int cg = tid & 15, rg = tid >> 4;
+  const int col0 = blockIdx.x * DFT_COLS;
+  double acc[RP][4];
 #pragma unroll
-  for (int o = 0; o < DFT_OC; ++o) acc[o] = 0.0;
-  const double* w = W + (size_t)r0 * N;
-  for (int k = 0; k < N; ++k) {
-    const double x = in[(size_t)k * stride + col];
+  for (int i = 0; i < RP; ++i)
 #pragma unroll
-    for (int o = 0; o < DFT_OC; ++o) acc[o] += w[o * N + k] * x;
+    for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+  for (int k0 = 0; k0 < N; k0 += DFT_KC) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < DFT_KC * DFT_COLS / 256; ++q) {
+      const int idx = tid + q * 256;
+      const int kk = idx / DFT_COLS, c = idx - (idx / DFT_COLS) * DFT_COLS;
+      sX[kk][c] = in[(size_t)min(k0 + kk, N - 1) * stride + min(col0 + c, ncol - 1)];
+    }
+#pragma unroll
+    for (int q = 0; q < RP; ++q) {
+      const int idx = tid + q * 256;            // 16*RP rows x 16 k
+      const int r = idx / DFT_KC, kk = idx - (idx / DFT_KC) * DFT_KC;
+      sW[r][kk] = W[(size_t)r * ldw + k0 + kk];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < DFT_KC; ++kk) {
+      double xv[4], wv[RP];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) xv[j] = sX[kk][cg * 4 + j];
+#pragma unroll
+      for (int i = 0; i < RP; ++i) wv[i] = sW[rg * RP + i][kk];
+#pragma unroll
+      for (int i = 0; i < RP; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] += wv[i] * xv[j];
+    }
   }
 #pragma unroll
-  for (int o = 0; o < DFT_OC; ++o)
-    if (r0 + o < N) out[(size_t)(r0 + o) * stride + col] = acc[o];
+  for (int i = 0; i < RP; ++i) {
+    const int r = rg * RP + i;
+    if (r >= N) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int c = col0 + cg * 4 + j;
+      if (c < ncol) out[(size_t)r * stride + c] = acc[i][j];
+    }
+  }
 }
 
 // d/dx (dir 0) or d/dy (dir 1) on every element of every plane:
@@ -214,13 +253,22 @@ bool fft_length_ok(int n) {
   return n == 1;
 }
 
-// Row-major (rows padded to a multiple of DFT_OC) matrix of the real
-// transform of length N.  sign +1: semtex's forward packing with 1/N;
-// sign -1: the unscaled inverse.  Angles are reduced exactly (jk mod N)
-// and evaluated in long double.
-std::vector<double> dft_matrix(int N, int sign) {
-  const int rows = (N + DFT_OC - 1) / DFT_OC * DFT_OC;
-  std::vector<double> W((size_t)rows * N, 0.0);
+// rows per thread of k_dft_z for a transform of length N (16 row groups)
+int dft_rp(int N) {
+  const int rp = (N + 15) / 16;
+  return rp <= 1 ? 1 : rp <= 2 ? 2 : rp <= 4 ? 4 : rp <= 6 ? 6 : rp <= 8 ? 8
+         : rp <= 12 ? 12 : rp <= 16 ? 16 : -1;
+}
+
+// Row-major matrix of the real transform of length N (rows padded to
+// 16 * dft_rp(N), columns to a multiple of DFT_KB, zero entries).  sign +1:
+// semtex's forward packing with 1/N; sign -1: the unscaled inverse.  Angles
+// are reduced exactly (jk mod N) and evaluated in long double.
+std::vector<double> dft_matrix(int N, int sign, int* ldw) {
+  const int rows = 16 * dft_rp(N);
+  const int ld = (N + DFT_KB - 1) / DFT_KB * DFT_KB;
+  *ldw = ld;
+  std::vector<double> W((size_t)rows * ld, 0.0);
   const long double two_pi = 6.283185307179586476925286766559L;
   auto cs = [&](long long jk, long double* c, long double* s) {
     const long double a = two_pi * (long double)(jk % N) / (long double)N;
@@ -230,18 +278,18 @@ std::vector<double> dft_matrix(int N, int sign) {
   const int nh = N / 2;
   if (sign == 1) {
     for (int k = 0; k < N; ++k) {
-      W[0 * N + k] = (double)(1.0L / N);
-      if (N > 1) W[1 * N + k] = (double)(((k & 1) ? -1.0L : 1.0L) / N);
+      W[0 * ld + k] = (double)(1.0L / N);
+      if (N > 1) W[1 * ld + k] = (double)(((k & 1) ? -1.0L : 1.0L) / N);
       for (int j = 1; j < nh; ++j) {
         long double c, s;
         cs((long long)j * k, &c, &s);
-        W[(size_t)(2 * j) * N + k] = (double)(c / N);
-        W[(size_t)(2 * j + 1) * N + k] = (double)(-s / N);
+        W[(size_t)(2 * j) * ld + k] = (double)(c / N);
+        W[(size_t)(2 * j + 1) * ld + k] = (double)(-s / N);
       }
     }
   } else {
     for (int k = 0; k < N; ++k) {
-      double* r = &W[(size_t)k * N];
+      double* r = &W[(size_t)k * ld];
       r[0] = 1.0;
       if (N > 1) r[1] = (k & 1) ? -1.0 : 1.0;
       for (int j = 1; j < nh; ++j) {
@@ -255,31 +303,43 @@ std::vector<double> dft_matrix(int N, int sign) {
   return W;
 }
 
-struct DftPlan { int N, sign; double* W; };
+struct DftPlan { int N, sign, ldw; double* W; };
 std::vector<DftPlan> g_plans;   // device matrices, kept for the process
 
-int dft_plan(int N, int sign, const double** W) {
+int dft_plan(int N, int sign, const double** W, int* ldw) {
   for (const DftPlan& p : g_plans)
-    if (p.N == N && p.sign == sign) { *W = p.W; return SEMX_OK; }
-  std::vector<double> h = dft_matrix(N, sign);
+    if (p.N == N && p.sign == sign) { *W = p.W; *ldw = p.ldw; return SEMX_OK; }
+  int ld = 0;
+  std::vector<double> h = dft_matrix(N, sign, &ld);
   double* d = nullptr;
   int st = dev_upload(&d, h.data(), h.size());
   if (st) return st;
-  g_plans.push_back({N, sign, d});
+  g_plans.push_back({N, sign, ld, d});
   *W = d;
+  *ldw = ld;
   return SEMX_OK;
 }
 
 // out-of-place transform of ntrn columns of length N (z-stride ntrn)
 int dft_launch(const double* in, double* out, int N, int ntrn, int sign, hipStream_t s) {
   const double* W = nullptr;
-  int st = dft_plan(N, sign, &W);
+  int ldw = 0;
+  int st = dft_plan(N, sign, &W, &ldw);
   if (st) return st;
-  dim3 g((ntrn + DFT_THREADS - 1) / DFT_THREADS, (N + DFT_OC - 1) / DFT_OC);
+  const int rp = dft_rp(N);
+  dim3 g((ntrn + DFT_COLS - 1) / DFT_COLS);
   hipEvent_t ev;
   prof_begin(s, &ev);
-  hipLaunchKernelGGL(k_dft_z, g, dim3(DFT_THREADS), 0, s, in, out, W, N, ntrn,
-                     (long long)ntrn);
+  switch (rp) {
+    case 1: hipLaunchKernelGGL(k_dft_z<1>, g, dim3(256), 0, s, in, out, W, N, ldw, ntrn, (long long)ntrn); break;
+    case 2: hipLaunchKernelGGL(k_dft_z<2>, g, dim3(256), 0, s, in, out, W, N, ldw, ntrn, (long long)ntrn); break;
+    case 4: hipLaunchKernelGGL(k_dft_z<4>, g, dim3(256), 0, s, in, out, W, N, ldw, ntrn, (long long)ntrn); break;
+    case 6: hipLaunchKernelGGL(k_dft_z<6>, g, dim3(256), 0, s, in, out, W, N, ldw, ntrn, (long long)ntrn); break;
+    case 8: hipLaunchKernelGGL(k_dft_z<8>, g, dim3(256), 0, s, in, out, W, N, ldw, ntrn, (long long)ntrn); break;
+    case 12: hipLaunchKernelGGL(k_dft_z<12>, g, dim3(256), 0, s, in, out, W, N, ldw, ntrn, (long long)ntrn); break;
+    case 16: hipLaunchKernelGGL(k_dft_z<16>, g, dim3(256), 0, s, in, out, W, N, ldw, ntrn, (long long)ntrn); break;
+    default: return SEMX_E_UNSUPPORTED;
+  }
   prof_end(s, "k_dft_z", ev);
   HIPCHK(hipGetLastError());
   return SEMX_OK;

@@ -129,6 +129,7 @@ struct Helm {
   int max_outer = 12;                 // refinement sweeps (refined)
   size_t workstride = 0;              // doubles per PCG plane
   int nf_cap = 0;                     // fields the device scratch holds
+  bool coop_mode = false;             // cooperative band solve chosen at setup
   int coop_w = 0;                     // workgroups per mode (0: one-workgroup sweep)
   int coop_rpt = 1;                   // panel rows per task thread
   int nblkmax = 0;

@@ -729,6 +729,10 @@ int helm_build(Helm& H) {
     const int kd = std::max(H.kd, 1);
     // thread r owns update row r; bands wider than 512 use narrower blocks
     // and several rows per thread
+    // cooperative band solve (several workgroups per mode): by default for
+    // bands wider than 512; SEMX_BAND_COOP=0 / 1 forces it off / on
+    const char* env = getenv("SEMX_BAND_COOP");
+    H.coop_mode = env ? atoi(env) != 0 : kd > 512;
     if (kd <= 512) { H.nb = 32; H.npass = 1; H.bt = 64; while (H.bt < kd) H.bt *= 2; }
     else if (kd <= 1024) { H.nb = 16; H.npass = 2; H.bt = 512; }
     else if (kd <= 2048) { H.nb = 8; H.npass = 4; H.bt = 512; }

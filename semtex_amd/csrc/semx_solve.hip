@@ -1350,12 +1350,7 @@ void mesh_release(Mesh& M) {
 int coop_setup(Helm& H) {
   H.coop_w = 0;
   const int nd = (int)H.dmode.size();
-  if (nd == 0 || H.npass == 0) return SEMX_OK;
-  // default: cooperative for bands wider than 512 (one workgroup cannot
-  // stream them fast enough); SEMX_BAND_COOP=0 / 1 forces it off / on
-  const char* env = getenv("SEMX_BAND_COOP");
-  const bool want = env ? atoi(env) != 0 : H.npass > 1;
-  if (!want) return SEMX_OK;
+  if (nd == 0 || H.npass == 0 || !H.coop_mode) return SEMX_OK;
   const int nb = H.nb, nch = H.kdp / nb;
   int cus = 256;
   int dev = 0;
@@ -1375,10 +1370,13 @@ int coop_setup(Helm& H) {
     const int lanes = std::max(1, std::min(COOP_WMAX - 1, cus / nd - 1));
     follow = (rest + lanes - 1) / lanes;
     if (const char* e = getenv("SEMX_COOP_FOLLOW")) follow = std::max(1, atoi(e));
+    // at most COOP_WMAX - 1 followers
+    follow = std::max(follow, (rest + COOP_WMAX - 2) / (COOP_WMAX - 1));
     follow = std::min(follow, std::min(ring - 2, tasks / nb));
     nf = (rest + follow - 1) / follow;
   }
-  if (1 + nf > COOP_WMAX) return SEMX_OK;    // band too wide for the lanes: one-workgroup sweep
+  if (1 + nf > COOP_WMAX)                     // band too wide for the lanes
+    return H.bt ? SEMX_OK : SEMX_E_UNSUPPORTED;
   H.coop_w = 1 + nf;
   H.coop_rpt = (std::max(lead, follow) * nb > COOP_TT) ? 2 : 1;
   H.coop_clo.assign(H.coop_w + 1, 0);

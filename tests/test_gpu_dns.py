@@ -28,7 +28,7 @@ def dev(torch, a):
     return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64).ravel().copy()).to("cuda:0")
 
 
-@pytest.mark.parametrize("N", [2, 6, 8, 12, 30, 64, 96, 128])
+@pytest.mark.parametrize("N", [2, 6, 8, 12, 30, 64, 96, 128, 160, 256])
 def test_dft_r_matches_oracle(cuda, N):
     import torch
     print(f"[dft_r] N={N}", flush=True)
