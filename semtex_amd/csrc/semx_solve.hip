@@ -507,11 +507,12 @@ __global__ void k_coop_fill(double* p, size_t n) {
 }
 
 // Threads per workgroup: ceil(NR*NB/64) hand-off waves (one value per
-// thread; wave 0 also solves the diagonal block) then the task waves.  The
-// two groups issue disjoint loads, so a wait on one group's data never
-// waits for the other's (memory counters are per wave).
+// thread; wave 0 also solves the diagonal block), one store wave, then the
+// task waves.  Each group issues its own kind of memory operation (hand-off
+// loads, hand-off stores, panel loads), so a wait for one never waits for
+// another's (memory counters are per wave and count stores too).
 template <int NB, int NR>
-constexpr int coop_bt() { return 64 * ((NR * NB + 63) / 64) + COOP_TT; }
+constexpr int coop_bt() { return 64 * ((NR * NB + 63) / 64) + 64 + COOP_TT; }
 
 // One sweep; returns false if a hand-off timed out (the workgroup agrees).
 template <int NB, int NR, int RPT, bool FWD>
@@ -519,7 +520,8 @@ __device__ bool coop_sweep(const CoopArgs& a, int md, int lane, const double* __
                            const double* __restrict__ L, double* const (&gy)[NR], int n,
                            int nblk, double* acc, double* rhs, double* ybuf, int* abort) {
   constexpr int BT = coop_bt<NB, NR>();
-  constexpr int VT = BT - COOP_TT;            // hand-off threads
+  constexpr int VT = BT - COOP_TT - 64;       // hand-off threads
+  constexpr int ST = VT + 64;                 // end of the store wave
   constexpr int QPT = (NB * NB / 64 > 0) ? NB * NB / 64 : 1;   // diagonal: wave 0
   constexpr int G = NB / QPT;
   constexpr int NV = NR * NB;                 // values handed over per block
@@ -548,9 +550,12 @@ __device__ bool coop_sweep(const CoopArgs& a, int md, int lane, const double* __
 #pragma unroll
   for (int c = 1; c < NR; ++c)
     if (c == vc) gyv = gy[c];
+  auto& gyc = gy;
   const bool dact = tid < NB * G;             // within wave 0
   const int da = dact ? tid / G : 0, dg = dact ? tid % G : 0;
-  const int tt = tid - VT;                    // task index base (< 0: hand-off wave)
+  const int tt = tid - ST;                    // task index base (< 0: other waves)
+  const bool stw = tid >= VT && tid < ST;     // store wave
+  const int sl = tid - VT;                    // lane in the store wave
 
   // prefetch registers: the panel one step ahead (task waves); hand-off
   // values, b rows and the diagonal-block inverse COOP_PD steps ahead
@@ -619,12 +624,16 @@ __device__ bool coop_sweep(const CoopArgs& a, int md, int lane, const double* __
         }
       }
       __syncthreads();
-      // C. hand y_J to the solution vector and to every follower
-      if (vth && j0 + vi < n) st_dev(&gyv[j0 + vi], ybuf[tid]);
-      if (tid < VT)
-        for (int i = tid; i < (W - 1) * NV; i += VT) {
+      // C. hand y_J to the solution vector and to every follower (store wave)
+      if (stw)
+        for (int i = sl; i < W * NV; i += 64) {
           const int w = i / NV, k = i - (i / NV) * NV;
-          st_dev(&inbox[w * lstride + (size_t)s * NV + k], ybuf[k]);
+          if (w == 0) {
+            const int c = k / NB, r = k - (k / NB) * NB;
+            if (j0 + r < n) st_dev(&gyc[c][j0 + r], ybuf[k]);
+          } else {
+            st_dev(&inbox[(w - 1) * lstride + (size_t)s * NV + k], ybuf[k]);
+          }
         }
     } else {
       // A. y_J from the lead
@@ -660,10 +669,11 @@ __device__ bool coop_sweep(const CoopArgs& a, int md, int lane, const double* __
       // target s+1+c_lo received its last contribution from this lane
       const int T = s + 1 + c_lo;
       const int tslot = T & (RING - 1);
-      if (T < nblk && vth) {
-        st_dev(&part[(lane - 1) * lstride + (size_t)T * NV + tid], acc[tslot * NV + tid]);
-        acc[tslot * NV + tid] = 0.0;
-      }
+      if (T < nblk && stw)
+        for (int k = sl; k < NV; k += 64) {
+          st_dev(&part[(lane - 1) * lstride + (size_t)T * NV + k], acc[tslot * NV + k]);
+          acc[tslot * NV + k] = 0.0;
+        }
     }
     load_panel(s + 1);
     fetch(s + COOP_PD, u);
@@ -1633,7 +1643,7 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
         prof_begin(s, &ev);
         const hipError_t ce2 = hipLaunchCooperativeKernel(
             cfn, dim3((unsigned)(groups * 8 * H.coop_w)),
-            dim3((unsigned)(64 * ((nrc * H.nb + 63) / 64) + COOP_TT)), kargs, 0, s);
+            dim3((unsigned)(64 * ((nrc * H.nb + 63) / 64) + 64 + COOP_TT)), kargs, 0, s);
         prof_end(s, "k_band_solve", ev);
         if (ce2 != hipSuccess) {
           fprintf(stderr, "semx: cooperative band solve: %s\n", hipGetErrorString(ce2));
