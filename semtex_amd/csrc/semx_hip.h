@@ -28,10 +28,10 @@ void prof_flush();
 template <class T>
 inline int dev_upload(T** dptr, const T* src, size_t count) {
   *dptr = nullptr;
-  if (count == 0) count = 1;
-  HIPCHK(hipMalloc((void**)dptr, count * sizeof(T)));
-  if (src) HIPCHK(hipMemcpy(*dptr, src, count * sizeof(T), hipMemcpyHostToDevice));
-  else HIPCHK(hipMemset(*dptr, 0, count * sizeof(T)));
+  const size_t alloc = count ? count : 1;
+  HIPCHK(hipMalloc((void**)dptr, alloc * sizeof(T)));
+  if (src && count) HIPCHK(hipMemcpy(*dptr, src, count * sizeof(T), hipMemcpyHostToDevice));
+  else HIPCHK(hipMemset(*dptr, 0, alloc * sizeof(T)));
   return SEMX_OK;
 }
 

@@ -113,6 +113,11 @@ struct Helm {
   int base = 0, nmodes = 0, nz = 0, method = SEMX_DIRECT;
   int nb = 32;                // band-solve block size (columns per step)
   int npass = 1, bt = 1024;   // band-solve row passes, threads per block
+  int band_d = 1;             // band-solve panel prefetch depth (steps)
+  // extended panels: diagonal-block inverses folded into the panels (one
+  // product per block step); ldx rows per panel column
+  bool band_ext = false;
+  int ldx = 0;
   int parts = 1;              // lanes sharing one update row
   int kd = 0, ldab = 0, kdp = 0;
   std::vector<ModeSys> modes;

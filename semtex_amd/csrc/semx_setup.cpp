@@ -691,6 +691,28 @@ static int condense(const Mesh& M, const double* g, double lambda2, double betak
   return SEMX_OK;
 }
 
+// Inverse of the transposed diagonal block of the band factor at rows
+// [j0, j0+nb): L = (U_JJ^T)^-1 (lower triangular, identity beyond n) and
+// LT = L^T.
+static void block_inverse(const double* A, int n, int kd, int ldab, int nb, int j0,
+                          double* L, double* LT) {
+  std::vector<double> T((size_t)nb * nb, 0.0);  // T = U_JJ^T (lower), identity pad
+  for (int a = 0; a < nb; ++a)
+    for (int b = 0; b <= a; ++b) {
+      const int r = j0 + b, c = j0 + a;
+      if (c < n) T[a * nb + b] = (c - r <= kd) ? A[(size_t)r * ldab + (c - r)] : 0.0;
+      else T[a * nb + b] = (a == b) ? 1.0 : 0.0;
+    }
+  for (int col = 0; col < nb; ++col)
+    for (int a = 0; a < nb; ++a) {
+      double s = (a == col) ? 1.0 : 0.0;
+      for (int b = col; b < a; ++b) s -= T[a * nb + b] * L[b * nb + col];
+      L[a * nb + col] = (a < col) ? 0.0 : s / T[a * nb + a];
+    }
+  for (int a = 0; a < nb; ++a)
+    for (int b = 0; b < nb; ++b) LT[b * nb + a] = L[a * nb + b];
+}
+
 // Banded Cholesky on row band storage A[i*ldab + d] = A(i, i+d), d = 0..kd.
 static int band_chol_rows(int n, int kd, int ldab, double* A) {
   for (int j = 0; j < n; ++j) {
@@ -737,6 +759,11 @@ int helm_build(Helm& H) {
     else if (kd <= 1024) { H.nb = 16; H.npass = 2; H.bt = 512; }
     else if (kd <= 2048) { H.nb = 8; H.npass = 4; H.bt = 512; }
     else { H.nb = 8; H.npass = 0; H.bt = 0; }   // no direct solve for this band
+    const char* envd = getenv("SEMX_BAND_D");
+    if (envd) H.band_d = std::min(2, std::max(1, atoi(envd)));
+    // bands up to 512 wide solve with the inverses folded into the panels
+    H.band_ext = kd <= 512 && !H.coop_mode;
+    H.ldx = (H.nb + kd + 7) / 8 * 8;
     H.kdp = (kd + 63) / 64 * 64;
   }
   size_t offH = 0, offL = 0;
@@ -755,8 +782,8 @@ int helm_build(Helm& H) {
       S.nblk = (S.n + nb - 1) / nb;
       S.off_H = offH;
       S.off_L = offL;
-      offH += (size_t)S.nblk * nb * H.kdp;
-      offL += (size_t)S.nblk * nb * nb;
+      offH += (size_t)S.nblk * nb * (H.band_ext ? H.ldx : H.kdp);
+      if (!H.band_ext) offL += (size_t)S.nblk * nb * nb;
     }
   }
   H.band_doubles = offH;
@@ -801,11 +828,25 @@ int helm_build(Helm& H) {
       const int st = band_chol_rows(n, kd, ldab, A);
       if (st) { status[m] = st; return; }
       // forward panels: Pf[J][jj][r] = U(j0+jj, j0+nb+r); backward panels:
-      // Pb[J][jj][r] = U(j0-1-r, j0+jj); zero outside the band
+      // Pb[J][jj][r] = U(j0-1-r, j0+jj); zero outside the band.  With
+      // extended panels they are built per block in scratch and folded below.
+      const bool ext = H.band_ext;
+      const int ldx = H.ldx;
+      std::vector<double> pf_s, pb_s, L_s, LT_s;
+      if (ext) {
+        pf_s.assign((size_t)nb * kdp, 0.0);
+        pb_s.assign((size_t)nb * kdp, 0.0);
+        L_s.assign((size_t)nb * nb, 0.0);
+        LT_s.assign((size_t)nb * nb, 0.0);
+      }
       for (int J = 0; J < S.nblk; ++J) {
         const int j0 = J * nb;
-        double* pf = &H.Pf[S.off_H + (size_t)J * nb * kdp];
-        double* pb = &H.Pb[S.off_H + (size_t)J * nb * kdp];
+        double* pf = ext ? pf_s.data() : &H.Pf[S.off_H + (size_t)J * nb * kdp];
+        double* pb = ext ? pb_s.data() : &H.Pb[S.off_H + (size_t)J * nb * kdp];
+        if (ext) {
+          std::fill(pf_s.begin(), pf_s.end(), 0.0);
+          std::fill(pb_s.begin(), pb_s.end(), 0.0);
+        }
         for (int jj = 0; jj < nb; ++jj) {
           const int j = j0 + jj;
           if (j >= n) continue;
@@ -816,27 +857,37 @@ int helm_build(Helm& H) {
             if (db <= kd && i_b >= 0) pb[(size_t)jj * kdp + r] = A[(size_t)i_b * ldab + db];
           }
         }
+        if (!ext) continue;
+        // extended panels of block J, column kk of the window:
+        //   Xf[kk][r]      = Linv[r][kk]                      r < nb
+        //   Xf[kk][nb+r]   = sum_jj Pf[jj][r] Linv[jj][kk]    r < kd
+        // and the same with Pb and LinvT for the backward sweep
+        block_inverse(A, n, kd, ldab, nb, j0, L_s.data(), LT_s.data());
+        // stored as column pairs: entry (kk, r) at ((kk/2)*ldx + r)*2 + kk%2,
+        // so one 16-byte load gives a row its two values of a pair
+        double* xf = &H.Pf[S.off_H + (size_t)J * nb * ldx];
+        double* xb = &H.Pb[S.off_H + (size_t)J * nb * ldx];
+        auto at = [&](int kk, int r) { return ((size_t)(kk >> 1) * ldx + r) * 2 + (kk & 1); };
+        for (int kk = 0; kk < nb; ++kk) {
+          for (int r = 0; r < nb; ++r) {
+            xf[at(kk, r)] = L_s[r * nb + kk];
+            xb[at(kk, r)] = LT_s[r * nb + kk];
+          }
+          for (int r = 0; r < kd; ++r) {
+            long double sf = 0.0L, sb = 0.0L;
+            for (int jj = 0; jj < nb; ++jj) {
+              sf += (long double)pf[(size_t)jj * kdp + r] * L_s[jj * nb + kk];
+              sb += (long double)pb[(size_t)jj * kdp + r] * LT_s[jj * nb + kk];
+            }
+            xf[at(kk, nb + r)] = (double)sf;
+            xb[at(kk, nb + r)] = (double)sb;
+          }
+        }
       }
       // diagonal-block inverses: Linv_J = (U_JJ^T)^-1, lower triangular
-      for (int J = 0; J < S.nblk; ++J) {
-        const int j0 = J * nb;
-        double* L = &H.Linv[S.off_L + (size_t)J * nb * nb];
-        double* LT = &H.LinvT[S.off_L + (size_t)J * nb * nb];
-        std::vector<double> T(nb * nb, 0.0);  // T = U_JJ^T (lower), identity pad
-        for (int a = 0; a < nb; ++a)
-          for (int b = 0; b <= a; ++b) {
-            const int r = j0 + b, c = j0 + a;
-            if (c < n) T[a * nb + b] = (c - r <= kd) ? A[(size_t)r * ldab + (c - r)] : 0.0;
-            else T[a * nb + b] = (a == b) ? 1.0 : 0.0;
-          }
-        for (int col = 0; col < nb; ++col)
-          for (int a = 0; a < nb; ++a) {
-            double s = (a == col) ? 1.0 : 0.0;
-            for (int b = col; b < a; ++b) s -= T[a * nb + b] * L[b * nb + col];
-            L[a * nb + col] = (a < col) ? 0.0 : s / T[a * nb + a];
-          }
-        for (int a = 0; a < nb; ++a)
-          for (int b = 0; b < nb; ++b) LT[b * nb + a] = L[a * nb + b];
+      for (int J = 0; !ext && J < S.nblk; ++J) {
+        block_inverse(A, n, kd, ldab, nb, J * nb, &H.Linv[S.off_L + (size_t)J * nb * nb],
+                      &H.LinvT[S.off_L + (size_t)J * nb * nb]);
       }
     } else {
       // Jacobi preconditioner: inverse assembled diagonal (matrix.cpp:276-317)

@@ -427,6 +427,315 @@ __global__ __launch_bounds__(BT) void k_band_solve(BandArgs a) {
                                            ybuf, n, nblk, a.kd, a.kdp, a.ring - 1);
 }
 
+// Band solve over extended panels, which carry the diagonal-block inverse
+// folded in.  Panel of block J: NB columns of ldx rows, stored as column
+// pairs (entry (kk, r) at ((kk/2) ldx + r) 2 + kk%2, one 16-byte load per
+// row and pair, which keeps a step's loads within the wave's limit of
+// outstanding memory instructions),
+//   rows [0, NB)        Linv_J (forward) / LinvT_J (backward): row r gives
+//                       y_J[r] (x_J[r]) from the window values of block J,
+//   rows [NB, NB + kd)  Pf_J Linv_J (forward) / Pb_J LinvT_J (backward): the
+//                       update of the kd rows after (before) the block.
+// One step is a single product of the panel with the NB window values of
+// block J: rows below NB are stored as the block's solution and the rest
+// are subtracted from the window.  One barrier per step; the loads of a
+// step are its panel slice and admitted entries only, prefetched D steps
+// ahead into registers.
+constexpr int XNB = 32;
+
+template <int NPASS, int BT, int NR, int D, bool FWD>
+__device__ inline void band_ext_sweep(const double* __restrict__ P,
+                                      double* const (&ring)[NR], double* const (&gy)[NR],
+                                      int n, int nblk, int kd, int ldx, int wm) {
+  constexpr int NB = XNB;
+  const int tid = threadIdx.x;
+  const int npad = nblk * NB;
+  double pbuf[D + 1][NPASS][NB], abuf[D + 1][NR];
+  {
+    const int lo = FWD ? 0 : max(0, (nblk - 1) * NB - kd);
+    const int hi = FWD ? min(npad, NB + kd) : npad;
+    for (int i = lo + tid; i < hi; i += BT) {
+#pragma unroll
+      for (int c = 0; c < NR; ++c) ring[c][i & wm] = (i < n) ? gy[c][i] : 0.0;
+    }
+  }
+  auto admit_index = [&](int s) {
+    const int J = FWD ? s : nblk - 1 - s;
+    const int i = FWD ? J * NB + kd + tid : J * NB - kd + tid;
+    const bool ok = tid < NB && (FWD ? (i >= (NB + kd) && i < npad) : (i >= 0));
+    return ok ? i : -1;
+  };
+  auto load = [&](int s, double (&pp)[NPASS][NB], double (&ad)[NR]) {
+    const int J = FWD ? s : nblk - 1 - s;
+    const double2* pj = reinterpret_cast<const double2*>(P + (size_t)J * NB * ldx);
+#pragma unroll
+    for (int q = 0; q < NPASS; ++q) {
+      const int r = min(tid + q * BT, ldx - 1);
+#pragma unroll
+      for (int k2 = 0; k2 < NB / 2; ++k2) {
+        const double2 v = pj[(size_t)k2 * ldx + r];
+        pp[q][2 * k2] = v.x;
+        pp[q][2 * k2 + 1] = v.y;
+      }
+    }
+    const int ic = min(max(admit_index(s), 0), n - 1);
+#pragma unroll
+    for (int c = 0; c < NR; ++c) ad[c] = gy[c][ic];
+  };
+  auto step = [&](int s, double (&pp)[NPASS][NB], double (&ad)[NR]) {
+    const int J = FWD ? s : nblk - 1 - s;
+    const int j0 = J * NB;
+    const int r0 = j0 & wm;  // blocks never wrap: W is a multiple of NB
+    const int ia = admit_index(s);
+    // Every thread takes delivery of this step's registers here, on every
+    // path: consumed only inside the branches below, the loads would count
+    // as possibly outstanding at the loop head, where the compiler would
+    // then wait for all later prefetches as well.
+#pragma unroll
+    for (int q = 0; q < NPASS; ++q)
+#pragma unroll
+      for (int kk = 0; kk < NB; ++kk) asm volatile("" ::"v"(pp[q][kk]));
+#pragma unroll
+    for (int c = 0; c < NR; ++c) asm volatile("" ::"v"(ad[c]));
+    if (s > 0 && ia >= 0) {
+#pragma unroll
+      for (int c = 0; c < NR; ++c) ring[c][ia & wm] = (ia < n) ? ad[c] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NPASS; ++q) {
+      const int r = tid + q * BT;
+      const int rr = r - NB;
+      const int i = r < NB ? j0 + r : (FWD ? j0 + NB + rr : j0 - 1 - rr);
+      const bool live = r < NB ? i < n : (rr < kd && (FWD ? i < n : i >= 0));
+      if (live) {
+#pragma unroll
+        for (int c = 0; c < NR; ++c) {
+          const double* wv = ring[c] + r0;
+          double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll
+          for (int kk = 0; kk < NB; kk += 4) {
+            a0 += pp[q][kk] * wv[kk];
+            a1 += pp[q][kk + 1] * wv[kk + 1];
+            a2 += pp[q][kk + 2] * wv[kk + 2];
+            a3 += pp[q][kk + 3] * wv[kk + 3];
+          }
+          const double v = (a0 + a1) + (a2 + a3);
+          if (r < NB) gy[c][i] = v;
+          else ring[c][i & wm] -= v;
+        }
+      }
+    }
+  };
+
+  if (nblk <= 0) return;
+#pragma unroll
+  for (int d = 0; d < D; ++d) load(min(d, nblk - 1), pbuf[d], abuf[d]);
+  // whole groups of D+1 steps with no exit inside a group, then the rest
+  int s = 0;
+  for (; s + D < nblk; s += D + 1) {
+#pragma unroll
+    for (int u = 0; u <= D; ++u) {
+      const int nx = (u + D) % (D + 1);
+      load(min(s + u + D, nblk - 1), pbuf[nx], abuf[nx]);
+      step(s + u, pbuf[u], abuf[u]);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < D; ++u)
+    if (s + u < nblk) step(s + u, pbuf[u], abuf[u]);
+  // the backward sweep re-reads, from global memory, values that other
+  // threads of this workgroup wrote during the forward sweep
+  if (FWD) __threadfence();
+  __syncthreads();
+}
+
+// The same product over extended panels with the whole right-hand side
+// resident in LDS (NR vectors of nblk*NB values), solved in place: the
+// loops then touch global memory only for panel loads.  Two threads share a
+// panel row (lane pair 2t, 2t+1 = row t, column pairs of one parity each),
+// which halves the registers a prefetched step takes and doubles the
+// wavefronts that hide LDS latency; a lane swap adds the two halves.
+// Block J's solution is written over its window values one step later,
+// when no thread reads them any more, so one barrier per step suffices.
+// Value of the neighbouring lane of a pair (lane ^ 1), by a DPP quad
+// permutation [1, 0, 3, 2].
+__device__ inline double lane_swap1(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0xB1, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0xB1, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+template <int BT, int NR, int D, bool FWD>
+__device__ inline void band_res_sweep(const double* __restrict__ P, double* const (&vec)[NR],
+                                      int n, int nblk, int kd, int ldx) {
+  constexpr int NB = XNB;
+  constexpr int NH = NB / 2;            // values per thread and step
+  const int tid = threadIdx.x;
+  const int row = tid >> 1, par = tid & 1;
+  double pbuf[D + 1][NH];
+  double held[NR];          // this row's block solution, stored a step later
+  int held_i = -1;
+  auto load = [&](int s, double (&pp)[NH]) {
+    const int J = FWD ? s : nblk - 1 - s;
+    const double2* pj = reinterpret_cast<const double2*>(P + (size_t)J * NB * ldx);
+    const int r = min(row, ldx - 1);
+#pragma unroll
+    for (int k = 0; k < NH / 2; ++k) {
+      const double2 v = pj[(size_t)(2 * k + par) * ldx + r];
+      pp[2 * k] = v.x;
+      pp[2 * k + 1] = v.y;
+    }
+  };
+  auto step = [&](int s, double (&pp)[NH]) {
+    const int J = FWD ? s : nblk - 1 - s;
+    const int j0 = J * NB;
+    // Every thread takes delivery of this step's registers here, on every
+    // path: consumed only inside the branch below, the loads would count
+    // as possibly outstanding at the loop head, where the compiler would
+    // then wait for all later prefetches as well.
+#pragma unroll
+    for (int k = 0; k < NH; ++k) asm volatile("" ::"v"(pp[k]));
+    __syncthreads();
+    // the previous block's solution (its window is no longer read)
+    if (held_i >= 0 && par == 0) {
+#pragma unroll
+      for (int c = 0; c < NR; ++c) vec[c][held_i] = held[c];
+    }
+    held_i = -1;
+    const int rr = row - NB;
+    const int i = row < NB ? j0 + row : (FWD ? j0 + NB + rr : j0 - 1 - rr);
+    const bool live = row < NB || (rr < kd && (FWD ? i < nblk * NB : i >= 0));
+    // column pair 2k+par of the window: values 4k + 2 par, 4k + 2 par + 1
+    double acc[NR][2];
+#pragma unroll
+    for (int c = 0; c < NR; ++c) acc[c][0] = acc[c][1] = 0.0;
+    if (live) {
+#pragma unroll
+      for (int c = 0; c < NR; ++c) {
+        const double2* wv = reinterpret_cast<const double2*>(vec[c] + j0) + par;
+#pragma unroll
+        for (int k0 = 0; k0 < NH / 2; k0 += 4) {
+          double2 w[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) w[k] = wv[2 * (k0 + k)];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            acc[c][0] += pp[2 * (k0 + k)] * w[k].x;
+            acc[c][1] += pp[2 * (k0 + k) + 1] * w[k].y;
+          }
+        }
+      }
+    }
+    double v[NR];
+#pragma unroll
+    for (int c = 0; c < NR; ++c) {
+      const double part = acc[c][0] + acc[c][1];
+      v[c] = part + lane_swap1(part);
+    }
+    if (live && par == 0) {
+#pragma unroll
+      for (int c = 0; c < NR; ++c) {
+        if (row < NB) held[c] = v[c];
+        else vec[c][i] -= v[c];
+      }
+      if (row < NB) held_i = i;
+    }
+  };
+
+  if (nblk <= 0) return;
+#pragma unroll
+  for (int d = 0; d < D; ++d) load(min(d, nblk - 1), pbuf[d]);
+  // whole groups of D+1 steps with no exit inside a group (an exit between
+  // the steps of a group would leave its prefetches unconsumed on a path
+  // back to the loop head), then the remaining steps
+  int s = 0;
+  for (; s + D < nblk; s += D + 1) {
+#pragma unroll
+    for (int u = 0; u <= D; ++u) {
+      const int nx = (u + D) % (D + 1);
+      load(min(s + u + D, nblk - 1), pbuf[nx]);
+      step(s + u, pbuf[u]);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < D; ++u)
+    if (s + u < nblk) step(s + u, pbuf[u]);
+  __syncthreads();
+  if (held_i >= 0 && par == 0) {
+#pragma unroll
+    for (int c = 0; c < NR; ++c) vec[c][held_i] = held[c];
+  }
+  __syncthreads();
+}
+
+// Workgroup-uniform values read from device tables are moved to scalar
+// registers: left in vector registers, their loads would stay in the
+// in-order memory counter, and every wait for them inside a prefetching
+// loop would also wait for all panel loads issued since.
+__device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline long long uni64(long long v) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
+  return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+template <int NPASS, int BT, int NR, int D>
+__global__ __launch_bounds__(BT) void k_band_ext(BandArgs a) {
+  extern __shared__ double lds[];
+  const int m = uni(a.dmode[blockIdx.x]);
+  const int n = uni(a.mode_n[m]);
+  const long long offp = uni64(a.offP[m]);
+  const int nblk = (n + XNB - 1) / XNB;
+  double* ring[NR];
+  double* gy[NR];
+#pragma unroll
+  for (int c = 0; c < NR; ++c) {
+    const int fld = c / a.npl, kk = c - fld * a.npl;
+    int p = a.mode_plane[2 * m + kk];
+    if (p < 0) p = a.mode_plane[2 * m];
+    p = uni(p);
+    ring[c] = lds + (size_t)c * a.ring;
+    gy[c] = a.sol + fld * a.fstride + (size_t)p * a.solstride;
+  }
+  band_ext_sweep<NPASS, BT, NR, D, true>(a.Pf + offp, ring, gy, n, nblk, a.kd, a.kdp,
+                                         a.ring - 1);
+  band_ext_sweep<NPASS, BT, NR, D, false>(a.Pb + offp, ring, gy, n, nblk, a.kd, a.kdp,
+                                          a.ring - 1);
+}
+
+template <int BT, int NR, int D>
+__global__ __launch_bounds__(BT) void k_band_res(BandArgs a) {
+  extern __shared__ double lds[];
+  const int m = uni(a.dmode[blockIdx.x]);
+  const int n = uni(a.mode_n[m]);
+  const long long offp = uni64(a.offP[m]);
+  const int nblk = (n + XNB - 1) / XNB;
+  const int npad = nblk * XNB;
+  double* vec[NR];
+  double* gy[NR];
+#pragma unroll
+  for (int c = 0; c < NR; ++c) {
+    const int fld = c / a.npl, kk = c - fld * a.npl;
+    int p = a.mode_plane[2 * m + kk];
+    if (p < 0) p = a.mode_plane[2 * m];
+    p = uni(p);
+    vec[c] = lds + (size_t)c * a.ring;
+    gy[c] = a.sol + fld * a.fstride + (size_t)p * a.solstride;
+  }
+  for (int i = threadIdx.x; i < npad; i += BT) {
+#pragma unroll
+    for (int c = 0; c < NR; ++c) vec[c][i] = (i < n) ? gy[c][i] : 0.0;
+  }
+  band_res_sweep<BT, NR, D, true>(a.Pf + offp, vec, n, nblk, a.kd, a.kdp);
+  band_res_sweep<BT, NR, D, false>(a.Pb + offp, vec, n, nblk, a.kd, a.kdp);
+  for (int i = threadIdx.x; i < n; i += BT) {
+#pragma unroll
+    for (int c = 0; c < NR; ++c) gy[c][i] = vec[c][i];
+  }
+}
+
 // ===========================================================================
 // Cooperative band solve: W workgroups per mode in a lagged pipeline.
 //
@@ -1595,8 +1904,62 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
     const size_t lds = (size_t)nr * (ba.ring + H.nb) * sizeof(double);
     if (lds > 160 * 1024 && H.coop_w == 0) return SEMX_E_UNSUPPORTED;
     const void* kfn = nullptr;
+    unsigned bt = (unsigned)H.bt;
+    size_t lds_x = lds;
+    if (H.band_ext) {
+      // extended panels: ldx rows, one thread per row up to 512 threads
+      ba.kdp = H.ldx;
+      const int xbt = std::min(512, (H.ldx + 63) / 64 * 64);
+      const int xnp = (H.ldx + xbt - 1) / xbt;
+      bt = (unsigned)xbt;
+#define SEMX_XPICK(NP_, BT_, D_)                                                 \
+      if (xnp == NP_ && xbt == BT_ && H.band_d == D_)                            \
+        kfn = (nr == 6)   ? (const void*)k_band_ext<NP_, BT_, 6, D_>             \
+              : (nr == 4) ? (const void*)k_band_ext<NP_, BT_, 4, D_>             \
+              : (nr == 2) ? (const void*)k_band_ext<NP_, BT_, 2, D_>             \
+                          : (const void*)k_band_ext<NP_, BT_, 1, D_>;
+#define SEMX_XPICKD(NP_, BT_) SEMX_XPICK(NP_, BT_, 1) SEMX_XPICK(NP_, BT_, 2)
+      SEMX_XPICKD(1, 64) SEMX_XPICKD(1, 128) SEMX_XPICKD(1, 192) SEMX_XPICKD(1, 256)
+      SEMX_XPICKD(1, 320) SEMX_XPICKD(1, 384) SEMX_XPICKD(1, 448) SEMX_XPICKD(1, 512)
+      SEMX_XPICK(2, 512, 1)
+#undef SEMX_XPICKD
+#undef SEMX_XPICK
+      if (!kfn) return SEMX_E_UNSUPPORTED;
+      // the whole right-hand side resident in LDS when it fits (one or two
+      // right-hand sides per launch; two threads per panel row)
+      int npadmax = 0;
+      for (int m : H.dmode) npadmax = std::max(npadmax, H.modes[m].nblk * XNB);
+      const int rbt = (2 * H.ldx + 63) / 64 * 64;
+      const int rnr = nr <= 2 ? nr : npl;
+      const size_t lds_res = (size_t)rnr * npadmax * sizeof(double);
+      if (lds_res <= 160 * 1024 && rbt <= 1024 && rnr <= 2) {
+        const void* rfn = nullptr;
+#define SEMX_RPICK(BT_, D_)                                                      \
+        if (rbt == BT_ && H.band_d == D_)                                        \
+          rfn = (rnr == 2) ? (const void*)k_band_res<BT_, 2, D_>                 \
+                           : (const void*)k_band_res<BT_, 1, D_>;
+#define SEMX_RPICKD(BT_) SEMX_RPICK(BT_, 1) SEMX_RPICK(BT_, 2)
+        SEMX_RPICKD(128) SEMX_RPICKD(192) SEMX_RPICKD(256) SEMX_RPICKD(320)
+        SEMX_RPICKD(384) SEMX_RPICKD(448) SEMX_RPICKD(512) SEMX_RPICKD(576)
+        SEMX_RPICKD(640) SEMX_RPICKD(704) SEMX_RPICKD(768) SEMX_RPICKD(832)
+        SEMX_RPICKD(896) SEMX_RPICKD(960) SEMX_RPICKD(1024)
+#undef SEMX_RPICKD
+#undef SEMX_RPICK
+        if (rfn) {
+          kfn = rfn;
+          ba.ring = npadmax;
+          lds_x = lds_res;
+          bt = (unsigned)rbt;
+          if (rnr != nr) {
+            nr = rnr;
+            launches = nf;
+          }
+        }
+      }
+    }
+    if (!H.band_ext) {
 #define SEMX_PICK(NB_, NP_, BT_, D_)                                        \
-    if (H.nb == NB_ && H.npass == NP_ && H.bt == BT_)                       \
+    if (H.nb == NB_ && H.npass == NP_ && H.bt == BT_)                         \
       kfn = (nr == 6)   ? (const void*)k_band_solve<NB_, NP_, BT_, 6, D_>   \
             : (nr == 4) ? (const void*)k_band_solve<NB_, NP_, BT_, 4, D_>   \
             : (nr == 2) ? (const void*)k_band_solve<NB_, NP_, BT_, 2, D_>   \
@@ -1608,6 +1971,7 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
     SEMX_PICK(16, 2, 512, 1)
     SEMX_PICK(8, 4, 512, 1)
 #undef SEMX_PICK
+    }
     if (!kfn && H.coop_w == 0) return SEMX_E_UNSUPPORTED;
     bool done = false;
     if (H.coop_w > 0) {
@@ -1653,13 +2017,13 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
       }
     }
     if (!done && !kfn) return SEMX_E_UNSUPPORTED;
-    if (!done) HIPCHK(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (!done) HIPCHK(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_x));
     for (int l = 0; !done && l < launches; ++l) {
       BandArgs bl = ba;
       bl.sol = H.d.sol + (launches > 1 ? l * psol : 0);
       void* kargs[] = {&bl};
       prof_begin(s, &ev);
-      HIPCHK(hipLaunchKernel(kfn, dim3((unsigned)H.dmode.size()), dim3(H.bt), kargs, lds, s));
+      HIPCHK(hipLaunchKernel(kfn, dim3((unsigned)H.dmode.size()), dim3(bt), kargs, lds_x, s));
       prof_end(s, "k_band_solve", ev);
     }
   }
