@@ -197,8 +197,9 @@ __global__ __launch_bounds__(ELT_THREADS) void k_condense(ElemArgs a) {
 }
 
 // Meshes with one geometric set: one block per (mode, group of ge
-// elements), the mode's hbi staged transposed in LDS, both planes.
-__global__ __launch_bounds__(ELT_THREADS) void k_condense_staged(ElemArgs a,
+// elements), the mode's hbi staged transposed in LDS, both planes.  At most
+// 64 registers, so that two blocks share a CU.
+__global__ __launch_bounds__(ELT_THREADS, 8) void k_condense_staged(ElemArgs a,
                                                                  const int* __restrict__ mode_plane,
                                                                  int ge) {
   extern __shared__ double esh[];
@@ -1091,9 +1092,11 @@ __global__ __launch_bounds__(ELT_THREADS) void k_backsub(ElemArgs a) {
 }
 
 // The same for meshes whose elements share one geometric-factor set: one
-// block per (direct mode, group of elements) with the mode's hii (stored
-// transposed) and hbi staged in LDS once, both planes of the mode and
-// several element chunks per block.
+// block per (direct mode, group of ge elements) with the mode's hii (stored
+// transposed) and hbi staged in LDS once.  All of the block's element
+// values, both planes, are gathered in one phase (one round of memory
+// latency), then each thread forms one interior row of one element for
+// both planes, each staged coefficient read serving the two.
 __global__ __launch_bounds__(ELT_THREADS) void k_backsub_staged(ElemArgs a,
                                                                 const int* __restrict__ dmode,
                                                                 const int* __restrict__ mode_plane,
@@ -1101,13 +1104,12 @@ __global__ __launch_bounds__(ELT_THREADS) void k_backsub_staged(ElemArgs a,
   extern __shared__ double esh[];
   const int t = threadIdx.x;
   const int next = a.next, nint = a.nint, npnp = a.npnp;
-  const int ce = ELT_THREADS / npnp;
   const int m = dmode[blockIdx.y];
   const int n = a.mode_n[m];
   double* sHiiT = esh;                  // nint*nint, [c][q] = hii[q][c]
   double* sHbi = sHiiT + nint * nint;   // next*nint
-  double* sub = sHbi + next * nint;     // ce*next
-  double* sFi = sub + ce * next;        // ce*nint
+  double* sub = sHbi + next * nint;     // 2*ge*next, [plane][element][b]
+  double* sFi = sub + 2 * ge * next;    // 2*ge*nint, [plane][element][c]
   {
     const double* hii = a.hii + (size_t)m * nint * nint;
     const double* hbi = a.hbi + (size_t)m * next * nint;
@@ -1118,50 +1120,54 @@ __global__ __launch_bounds__(ELT_THREADS) void k_backsub_staged(ElemArgs a,
     for (int i = t; i < next * nint; i += ELT_THREADS) sHbi[i] = hbi[i];
   }
   const int e_begin = blockIdx.x * ge, e_end = min(a.nel, e_begin + ge);
-  for (int c2 = 0; c2 < 2; ++c2) {
-    const int p = mode_plane[2 * m + c2];
-    if (p < 0) continue;
-    const int k = a.plane_of[p];
-    for (int e0 = e_begin; e0 < e_end; e0 += ce) {
-      __syncthreads();
-      for (int idx = t; idx < ce * npnp; idx += ELT_THREADS) {
-        const int le = idx / npnp, q = idx - le * npnp;
-        const int ee = e0 + le;
-        if (ee >= e_end) continue;
-        const size_t pe = (size_t)p * a.nel + ee;
-        if (q < next) {
-          const int gid = a.btog[ee * next + q];
-          const double ub = (gid < n) ? a.sol[(size_t)p * a.solstride + gid] : 0.0;
-          sub[le * next + q] = ub;
-          a.u[(size_t)k * a.planeSize + (size_t)ee * npnp + a.emap[q]] =
-              (gid < n) ? ub
-                        : (gid >= a.nsolve ? a.g[(size_t)p * a.nzero + (gid - a.nsolve)] : 0.0);
-        } else {
-          sFi[le * nint + (q - next)] = a.Fi[pe * nint + (q - next)];
-        }
+  const int ne = e_end - e_begin;
+  const int p0 = mode_plane[2 * m], p1 = mode_plane[2 * m + 1];
+  const int npl = p1 >= 0 ? 2 : 1;
+  for (int idx = t; idx < ne * npnp; idx += ELT_THREADS) {
+    const int le = idx / npnp, q = idx - le * npnp;
+    const int ee = e_begin + le;
+    if (q < next) {
+      const int gid = a.btog[ee * next + q];
+      for (int pl = 0; pl < npl; ++pl) {
+        const int p = pl ? p1 : p0;
+        const int k = a.plane_of[p];
+        const double ub = (gid < n) ? a.sol[(size_t)p * a.solstride + gid] : 0.0;
+        sub[(pl * ge + le) * next + q] = ub;
+        a.u[(size_t)k * a.planeSize + (size_t)ee * npnp + a.emap[q]] =
+            (gid < n) ? ub
+                      : (gid >= a.nsolve ? a.g[(size_t)p * a.nzero + (gid - a.nsolve)] : 0.0);
       }
-      __syncthreads();
-      for (int idx = t; idx < ce * nint; idx += ELT_THREADS) {
-        const int le = idx / nint, q = idx - le * nint;
-        const int ee = e0 + le;
-        if (ee >= e_end) continue;
-        const double* fi = sFi + le * nint;
-        const double* ub = sub + le * next;
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-        int c = 0;
-        for (; c + 1 < nint; c += 2) {
-          s0 += sHiiT[c * nint + q] * fi[c];
-          s1 += sHiiT[(c + 1) * nint + q] * fi[c + 1];
-        }
-        if (c < nint) s0 += sHiiT[c * nint + q] * fi[c];
-        for (int b = 0; b + 1 < next; b += 2) {
-          s2 += sHbi[b * nint + q] * ub[b];
-          s3 += sHbi[(b + 1) * nint + q] * ub[b + 1];
-        }
-        a.u[(size_t)k * a.planeSize + (size_t)ee * npnp + a.emap[next + q]] =
-            (s0 + s1) - (s2 + s3);
+    } else {
+      for (int pl = 0; pl < npl; ++pl) {
+        const int p = pl ? p1 : p0;
+        sFi[(pl * ge + le) * nint + (q - next)] =
+            a.Fi[((size_t)p * a.nel + ee) * nint + (q - next)];
       }
     }
+  }
+  __syncthreads();
+  const int k0 = a.plane_of[p0], k1 = npl > 1 ? a.plane_of[p1] : k0;
+  for (int idx = t; idx < ne * nint; idx += ELT_THREADS) {
+    const int le = idx / nint, q = idx - le * nint;
+    const int ee = e_begin + le;
+    const double* fi0 = sFi + le * nint;
+    const double* fi1 = sFi + (ge + le) * nint;
+    const double* ub0 = sub + le * next;
+    const double* ub1 = sub + (ge + le) * next;
+    double s0 = 0.0, s1 = 0.0, t0 = 0.0, t1 = 0.0;
+    for (int c = 0; c < nint; ++c) {
+      const double h = sHiiT[c * nint + q];
+      s0 += h * fi0[c];
+      s1 += h * fi1[c];
+    }
+    for (int b = 0; b < next; ++b) {
+      const double h = sHbi[b * nint + q];
+      t0 += h * ub0[b];
+      t1 += h * ub1[b];
+    }
+    const size_t off = (size_t)ee * npnp + a.emap[next + q];
+    a.u[(size_t)k0 * a.planeSize + off] = s0 - t0;
+    if (npl > 1) a.u[(size_t)k1 * a.planeSize + off] = s1 - t1;
   }
 }
 
@@ -2029,10 +2035,12 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
   }
   if (!H.dmode.empty()) {
     // staged operators when every element shares one geometric set
+    // elements per block: enough blocks for two per CU, at most 2 ce
+    int ge = (int)((M.nel * H.dmode.size() + 511) / 512);
+    ge = std::max(1, std::min(2 * ce, ge));
     const size_t lds_s = ((size_t)M.nint * M.nint + (size_t)M.next * M.nint +
-                          (size_t)ce * (M.next + M.nint)) * sizeof(double);
+                          2 * (size_t)ge * (M.next + M.nint)) * sizeof(double);
     const bool staged = M.nset == 1 && lds_s <= 160 * 1024;
-    const int ge = 2 * ce;                       // elements per block
     if (staged && lds_s > 64 * 1024)
       HIPCHK(hipFuncSetAttribute((const void*)k_backsub_staged,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_s));
