@@ -196,32 +196,138 @@ __global__ __launch_bounds__(ELT_THREADS) void k_condense(ElemArgs a) {
   condense_chunk(a, blockIdx.y, blockIdx.x * ce, a.nel, nullptr, esh);
 }
 
+// Synchronisation of one wavefront's LDS traffic.
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Meshes with one geometric set: one block per (mode, group of ge
-// elements), the mode's hbi staged transposed in LDS, both planes.  At most
-// 64 registers, so that two blocks share a CU.
+// elements), the mode's hbi staged transposed in LDS.  Each wavefront forms
+// the forcing of one (element, plane) at a time in emap order -- mass term,
+// and on elements with essential nodes the lifting -H g by the tensor-
+// product operator in a private LDS slice with wavefront synchronisation
+// only -- and after one block barrier all threads condense the block's
+// elements, both planes: interior part out, F_b - hbi F_i plus the natural
+// boundary terms.
+constexpr int CND_WAVES = ELT_THREADS / 64;
+
 __global__ __launch_bounds__(ELT_THREADS, 8) void k_condense_staged(ElemArgs a,
                                                                  const int* __restrict__ mode_plane,
                                                                  int ge) {
   extern __shared__ double esh[];
-  const int next = a.next, nint = a.nint, npnp = a.npnp;
-  const int ce = ELT_THREADS / npnp;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int np = a.np, next = a.next, nint = a.nint, npnp = a.npnp;
   const int m = blockIdx.y;
-  double* sHbiT = esh;
-  double* work = esh + (size_t)next * nint;
   const bool direct = a.mode_kind[m] == SEMX_DIRECT;
+  const double bk = a.beta * (a.base + m);
+  const double hcon = a.lambda2 + bk * bk;
+  const bool svv = a.lambda2 > EPSDP;
+  double* sHbiT = esh;                          // nint*next, [c][b]
+  double* sD = sHbiT + (size_t)next * nint;     // npnp
+  double* sT = sD + npnp;                       // npnp
+  double* sF = sT + npnp;                       // 2*ge*npnp, [plane][element], emap order
+  double* sW = sF + 2 * (size_t)ge * npnp;      // per wave 3*npnp: U, R, S
   if (direct) {
     const double* hbi = a.hbi + (size_t)m * next * nint;
-    for (int i = threadIdx.x; i < next * nint; i += ELT_THREADS) {
+    for (int i = t; i < next * nint; i += ELT_THREADS) {
       const int b = i / nint, c = i - (i / nint) * nint;
       sHbiT[c * next + b] = hbi[i];
     }
   }
+  for (int i = t; i < npnp; i += ELT_THREADS) {
+    sD[i] = a.ops[(svv ? 2 : 0) * npnp + i];
+    sT[i] = a.ops[(svv ? 3 : 1) * npnp + i];
+  }
+  __syncthreads();
   const int e_begin = blockIdx.x * ge, e_end = min(a.nel, e_begin + ge);
-  for (int c2 = 0; c2 < 2; ++c2) {
-    const int p = mode_plane[2 * m + c2];
-    if (p < 0) continue;
-    for (int e0 = e_begin; e0 < e_end; e0 += ce)
-      condense_chunk(a, p, e0, e_end, direct ? sHbiT : nullptr, work);
+  const int ne = e_end - e_begin;
+  const int p0 = mode_plane[2 * m], p1 = mode_plane[2 * m + 1];
+  const int npl = p1 >= 0 ? 2 : 1;
+  const double* g = a.G;                        // the one geometric set
+  double* sU = sW + (size_t)wave * 3 * npnp;
+  double* sR = sU + npnp;
+  double* sS = sR + npnp;
+  for (int item = wave; item < ne * npl; item += CND_WAVES) {
+    const int pl = item / ne, le = item - pl * ne;
+    const int ee = e_begin + le;
+    const int p = pl ? p1 : p0;
+    const int k = a.plane_of[p];
+    double* Fe = sF + (size_t)(pl * ge + le) * npnp;
+    const bool em = a.emask[ee] != 0;           // uniform over the wavefront
+    if (em) {
+      for (int pt = lane; pt < npnp; pt += 64) sU[pt] = 0.0;
+      wave_sync();
+      for (int pt = lane; pt < next; pt += 64) {
+        const int gid = a.btog[ee * next + pt];
+        sU[a.emap[pt]] = (gid >= a.nsolve) ? a.g[(size_t)p * a.nzero + (gid - a.nsolve)] : 0.0;
+      }
+      wave_sync();
+      for (int pt = lane; pt < npnp; pt += 64) {
+        const int i = pt / np, j = pt - (pt / np) * np;
+        double r = 0.0, s2 = 0.0;
+        for (int q = 0; q < np; ++q) {
+          r += sU[i * np + q] * sT[q * np + j];
+          s2 += sD[i * np + q] * sU[q * np + j];
+        }
+        const double q1 = g[G_Q1 * npnp + pt], q2 = g[G_Q2 * npnp + pt], q3 = g[G_Q3 * npnp + pt];
+        sR[pt] = q1 * r + q3 * s2;
+        sS[pt] = q2 * s2 + q3 * r;
+      }
+      wave_sync();
+    }
+    for (int pt = lane; pt < npnp; pt += 64) {
+      double F = g[G_Q4 * npnp + pt] * a.f[(size_t)k * a.planeSize + (size_t)ee * npnp + pt];
+      if (em) {
+        const int i = pt / np, j = pt - (pt / np) * np;
+        double T = g[G_Q4 * npnp + pt] * sU[pt] * hcon;
+        double s1 = 0.0, s3 = 0.0;
+        for (int q = 0; q < np; ++q) {
+          s1 += sT[i * np + q] * sS[q * np + j];
+          s3 += sR[i * np + q] * sD[q * np + j];
+        }
+        T += s1;
+        T += s3;
+        F += T;
+      }
+      Fe[a.pmap[pt]] = -F;
+    }
+    wave_sync();   // this wavefront's slice is free for its next item
+  }
+  __syncthreads();
+  const int nm = np - 1;
+  for (int idx = t; idx < npl * ne * npnp; idx += ELT_THREADS) {
+    const int it = idx / npnp, q = idx - it * npnp;
+    const int pl = it / ne, le = it - pl * ne;
+    const int ee = e_begin + le;
+    const int p = pl ? p1 : p0;
+    const int k = a.plane_of[p];
+    const size_t pe = (size_t)p * a.nel + ee;
+    const double* Fe = sF + (size_t)(pl * ge + le) * npnp;
+    if (q >= next) {
+      a.Fi[pe * nint + (q - next)] = Fe[q];
+    } else {
+      const int b = q;
+      const double* Fi = Fe + next;
+      double s0 = 0.0, s1 = 0.0;
+      if (direct) {   // PCG modes keep the raw F_b
+        int c = 0;
+        for (; c + 1 < nint; c += 2) {
+          s0 += sHbiT[c * next + b] * Fi[c];
+          s1 += sHbiT[(c + 1) * next + b] * Fi[c + 1];
+        }
+        if (c < nint) s0 += sHbiT[c * next + b] * Fi[c];
+      }
+      double fb = Fe[b] - (s0 + s1);
+      for (int cc = a.nat_ptr[ee]; cc < a.nat_ptr[ee + 1]; ++cc) {
+        const int bb = a.nat_b[cc];
+        const int side = a.bside[bb];
+        const int ii = (b - side * nm + next) % next;
+        if (ii <= nm) fb += a.bc[(size_t)k * a.nline + bb * np + ii] * a.barea[bb * np + ii];
+      }
+      a.Fb[pe * next + b] = fb;
+    }
   }
 }
 
@@ -1860,10 +1966,14 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
     HIPCHK(hipFuncSetAttribute((const void*)k_backsub,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b));
   }
-  // staged hbi when every element shares one geometric set
-  const size_t lds_cs = lds_c + (size_t)M.next * M.nint * sizeof(double);
+  // staged hbi when every element shares one geometric set; elements per
+  // block: enough blocks for two per CU, at most 2 ce
+  int cge = (M.nel * H.nmodes + 511) / 512;
+  cge = std::max(1, std::min(2 * ce, cge));
+  const size_t lds_cs = ((size_t)M.next * M.nint + 2 * (size_t)M.npnp +
+                         2 * (size_t)cge * M.npnp + 3 * (size_t)CND_WAVES * M.npnp) *
+                        sizeof(double);
   const bool cstaged = M.nset == 1 && lds_cs <= 160 * 1024;
-  const int cge = 2 * ce;                        // elements per staged block
   if (cstaged && lds_cs > 64 * 1024)
     HIPCHK(hipFuncSetAttribute((const void*)k_condense_staged,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_cs));
