@@ -113,7 +113,8 @@ struct Helm {
   int base = 0, nmodes = 0, nz = 0, method = SEMX_DIRECT;
   int nb = 32;                // band-solve block size (columns per step)
   int npass = 1, bt = 1024;   // band-solve row passes, threads per block
-  int band_d = 1;             // band-solve panel prefetch depth (steps)
+  int band_d = 1;             // band-solve panel prefetch depth (steps); depth 2
+                              // measured no faster on the headline workload
   // extended panels: diagonal-block inverses folded into the panels (one
   // product per block step); ldx rows per panel column
   bool band_ext = false;

@@ -759,8 +759,6 @@ int helm_build(Helm& H) {
     else if (kd <= 1024) { H.nb = 16; H.npass = 2; H.bt = 512; }
     else if (kd <= 2048) { H.nb = 8; H.npass = 4; H.bt = 512; }
     else { H.nb = 8; H.npass = 0; H.bt = 0; }   // no direct solve for this band
-    const char* envd = getenv("SEMX_BAND_D");
-    if (envd) H.band_d = std::min(2, std::max(1, atoi(envd)));
     // bands up to 512 wide solve with the inverses folded into the panels
     H.band_ext = kd <= 512 && !H.coop_mode;
     H.ldx = (H.nb + kd + 7) / 8 * 8;

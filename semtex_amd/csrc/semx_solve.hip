@@ -2034,7 +2034,7 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
               : (nr == 4) ? (const void*)k_band_ext<NP_, BT_, 4, D_>             \
               : (nr == 2) ? (const void*)k_band_ext<NP_, BT_, 2, D_>             \
                           : (const void*)k_band_ext<NP_, BT_, 1, D_>;
-#define SEMX_XPICKD(NP_, BT_) SEMX_XPICK(NP_, BT_, 1) SEMX_XPICK(NP_, BT_, 2)
+#define SEMX_XPICKD(NP_, BT_) SEMX_XPICK(NP_, BT_, 1)
       SEMX_XPICKD(1, 64) SEMX_XPICKD(1, 128) SEMX_XPICKD(1, 192) SEMX_XPICKD(1, 256)
       SEMX_XPICKD(1, 320) SEMX_XPICKD(1, 384) SEMX_XPICKD(1, 448) SEMX_XPICKD(1, 512)
       SEMX_XPICK(2, 512, 1)
@@ -2054,7 +2054,7 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
         if (rbt == BT_ && H.band_d == D_)                                        \
           rfn = (rnr == 2) ? (const void*)k_band_res<BT_, 2, D_>                 \
                            : (const void*)k_band_res<BT_, 1, D_>;
-#define SEMX_RPICKD(BT_) SEMX_RPICK(BT_, 1) SEMX_RPICK(BT_, 2)
+#define SEMX_RPICKD(BT_) SEMX_RPICK(BT_, 1)
         SEMX_RPICKD(128) SEMX_RPICKD(192) SEMX_RPICKD(256) SEMX_RPICKD(320)
         SEMX_RPICKD(384) SEMX_RPICKD(448) SEMX_RPICKD(512) SEMX_RPICKD(576)
         SEMX_RPICKD(640) SEMX_RPICKD(704) SEMX_RPICKD(768) SEMX_RPICKD(832)
