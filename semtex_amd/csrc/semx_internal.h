@@ -88,6 +88,7 @@ struct HelmDev {
   int* mode_plane = nullptr;  // per mode: up to 2 plane-list indices (-1 none)
   long long* mode_offH = nullptr;
   long long* mode_offL = nullptr;
+  int* xrows = nullptr;       // extended-panel live rows, see Helm::xrows
   double* pc = nullptr;       // JACPCG inverse diagonals, nmodes*npts
   float* pc32 = nullptr;      // the same in FP32
   int* mode_kind = nullptr;   // per mode: DIRECT or JACPCG
@@ -119,6 +120,10 @@ struct Helm {
   // product per block step); ldx rows per panel column
   bool band_ext = false;
   int ldx = 0;
+  // live rows (32 + envelope height) of each extended panel, forward and
+  // backward: xrows[(mode*xblk + J)*2 + dir], dir 0 forward
+  std::vector<int> xrows;
+  int xblk = 0;
   int parts = 1;              // lanes sharing one update row
   int kd = 0, ldab = 0, kdp = 0;
   std::vector<ModeSys> modes;

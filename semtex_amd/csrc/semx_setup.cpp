@@ -786,6 +786,12 @@ int helm_build(Helm& H) {
   }
   H.band_doubles = offH;
   H.inv_doubles = offL;
+  if (H.band_ext) {
+    H.xblk = 0;
+    for (int m = 0; m < nmodes; ++m)
+      if (H.modes[m].kind == SEMX_DIRECT) H.xblk = std::max(H.xblk, H.modes[m].nblk);
+    H.xrows.assign((size_t)nmodes * std::max(H.xblk, 1) * 2, nb);
+  }
   H.Pf.assign(offH, 0.0);
   H.Pb.assign(offH, 0.0);
   H.Linv.assign(offL, 0.0);
@@ -881,6 +887,16 @@ int helm_build(Helm& H) {
             xb[at(kk, nb + r)] = (double)sb;
           }
         }
+        // live rows: beyond the block's envelope the folded rows are exact
+        // zeros (the factor keeps the matrix's envelope)
+        int hf = 0, hb = 0;
+        for (int r = 0; r < kd; ++r)
+          for (int kk = 0; kk < nb; ++kk) {
+            if (xf[at(kk, nb + r)] != 0.0) hf = r + 1;
+            if (xb[at(kk, nb + r)] != 0.0) hb = r + 1;
+          }
+        H.xrows[((size_t)m * H.xblk + J) * 2] = nb + hf;
+        H.xrows[((size_t)m * H.xblk + J) * 2 + 1] = nb + hb;
       }
       // diagonal-block inverses: Linv_J = (U_JJ^T)^-1, lower triangular
       for (int J = 0; !ext && J < S.nblk; ++J) {

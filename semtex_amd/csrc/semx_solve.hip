@@ -348,6 +348,8 @@ struct BandArgs {
   int solstride, kd, kdp, nel, next, ring;
   int npl;                  // right-hand sides per field and mode (1 or 2 planes)
   long long fstride;        // sol offset between fields
+  const int* xrows;         // extended panels: live rows per block and direction
+  int xblk;                 // block stride of xrows per mode
 };
 
 template <int G>
@@ -676,7 +678,8 @@ __device__ inline double lane_swap1(double x) {
 
 template <int BT, int NR, int D, bool FWD>
 __device__ inline void band_res_sweep(const double* __restrict__ P, double* const (&vec)[NR],
-                                      int n, int nblk, int kd, int ldx) {
+                                      const int* __restrict__ xrows, int n, int nblk, int kd,
+                                      int ldx) {
   constexpr int NB = XNB;
   constexpr int NH = NB / 2;            // values per thread and step
   const int tid = threadIdx.x;
@@ -684,10 +687,12 @@ __device__ inline void band_res_sweep(const double* __restrict__ P, double* cons
   double pbuf[D + 1][NH];
   double held[NR];          // this row's block solution, stored a step later
   int held_i = -1;
+  // rows past a block's live rows load the last live row's address (the
+  // same cache lines, no extra memory traffic) and skip the product
   auto load = [&](int s, double (&pp)[NH]) {
     const int J = FWD ? s : nblk - 1 - s;
     const double2* pj = reinterpret_cast<const double2*>(P + (size_t)J * NB * ldx);
-    const int r = min(row, ldx - 1);
+    const int r = min(row, xrows[2 * J + (FWD ? 0 : 1)] - 1);
 #pragma unroll
     for (int k = 0; k < NH / 2; ++k) {
       const double2 v = pj[(size_t)(2 * k + par) * ldx + r];
@@ -713,7 +718,8 @@ __device__ inline void band_res_sweep(const double* __restrict__ P, double* cons
     held_i = -1;
     const int rr = row - NB;
     const int i = row < NB ? j0 + row : (FWD ? j0 + NB + rr : j0 - 1 - rr);
-    const bool live = row < NB || (rr < kd && (FWD ? i < nblk * NB : i >= 0));
+    const bool live = row < NB || (row < xrows[2 * J + (FWD ? 0 : 1)] && rr < kd &&
+                                   (FWD ? i < nblk * NB : i >= 0));
     // column pair 2k+par of the window: values 4k + 2 par, 4k + 2 par + 1
     double acc[NR][2];
 #pragma unroll
@@ -831,12 +837,15 @@ __global__ __launch_bounds__(BT) void k_band_res(BandArgs a) {
     vec[c] = lds + (size_t)c * a.ring;
     gy[c] = a.sol + fld * a.fstride + (size_t)p * a.solstride;
   }
+  int* xr = reinterpret_cast<int*>(lds + (size_t)NR * a.ring);
+  for (int i = threadIdx.x; i < 2 * nblk; i += BT) xr[i] = a.xrows[(size_t)m * 2 * a.xblk + i];
   for (int i = threadIdx.x; i < npad; i += BT) {
 #pragma unroll
     for (int c = 0; c < NR; ++c) vec[c][i] = (i < n) ? gy[c][i] : 0.0;
   }
-  band_res_sweep<BT, NR, D, true>(a.Pf + offp, vec, n, nblk, a.kd, a.kdp);
-  band_res_sweep<BT, NR, D, false>(a.Pb + offp, vec, n, nblk, a.kd, a.kdp);
+  __syncthreads();   // the row table is read by the first prefetches
+  band_res_sweep<BT, NR, D, true>(a.Pf + offp, vec, xr, n, nblk, a.kd, a.kdp);
+  band_res_sweep<BT, NR, D, false>(a.Pb + offp, vec, xr, n, nblk, a.kd, a.kdp);
   for (int i = threadIdx.x; i < n; i += BT) {
 #pragma unroll
     for (int c = 0; c < NR; ++c) gy[c][i] = vec[c][i];
@@ -1846,6 +1855,7 @@ int helm_upload(Helm& H) {
   if ((st = dev_upload(&H.d.dmode, H.dmode.data(), H.dmode.size()))) return st;
   if ((st = dev_upload(&H.d.mode_plane, H.mode_plane.data(), H.mode_plane.size()))) return st;
   if ((st = dev_upload(&H.d.Linv, H.Linv.data(), H.Linv.size()))) return st;
+  if ((st = dev_upload(&H.d.xrows, H.xrows.data(), H.xrows.size()))) return st;
   if ((st = dev_upload(&H.d.LinvT, H.LinvT.data(), H.LinvT.size()))) return st;
   if ((st = dev_upload(&H.d.mode_n, mn.data(), mn.size()))) return st;
   if ((st = dev_upload(&H.d.mode_offH, oh.data(), oh.size()))) return st;
@@ -1873,7 +1883,7 @@ int helm_upload(Helm& H) {
 void helm_release(Helm& H) {
   dev_free(H.d.hbi); dev_free(H.d.hii); dev_free(H.d.Pf); dev_free(H.d.Pb);
   dev_free(H.d.dmode); dev_free(H.d.mode_plane);
-  dev_free(H.d.Linv); dev_free(H.d.LinvT); dev_free(H.d.mode_n); dev_free(H.d.mode_offH);
+  dev_free(H.d.Linv); dev_free(H.d.LinvT); dev_free(H.d.xrows); dev_free(H.d.mode_n); dev_free(H.d.mode_offH);
   dev_free(H.d.mode_offL); dev_free(H.d.pc); dev_free(H.d.Fb); dev_free(H.d.Fi);
   dev_free(H.d.g); dev_free(H.d.sol); dev_free(H.d.plane_of);
   dev_free(H.d.pc32); dev_free(H.d.mode_kind); dev_free(H.d.pcg_plane); dev_free(H.d.work);
@@ -2006,6 +2016,8 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
     ba.ring = 64;
     while (ba.ring < H.kd + 2 * H.nb) ba.ring *= 2;
     ba.fstride = (long long)psol;
+    ba.xrows = H.d.xrows;
+    ba.xblk = H.xblk;
     int npl = 1;
     for (int m : H.dmode) if (H.mode_plane[2 * m + 1] >= 0) npl = 2;
     ba.npl = npl;
@@ -2047,7 +2059,8 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
       for (int m : H.dmode) npadmax = std::max(npadmax, H.modes[m].nblk * XNB);
       const int rbt = (2 * H.ldx + 63) / 64 * 64;
       const int rnr = nr <= 2 ? nr : npl;
-      const size_t lds_res = (size_t)rnr * npadmax * sizeof(double);
+      const size_t lds_res = (size_t)rnr * npadmax * sizeof(double) +
+                             ((size_t)2 * H.xblk * sizeof(int) + 7) / 8 * 8;
       if (lds_res <= 160 * 1024 && rbt <= 1024 && rnr <= 2) {
         const void* rfn = nullptr;
 #define SEMX_RPICK(BT_, D_)                                                      \
