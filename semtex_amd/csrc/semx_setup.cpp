@@ -752,15 +752,18 @@ int helm_build(Helm& H) {
     // thread r owns update row r; bands wider than 512 use narrower blocks
     // and several rows per thread
     // cooperative band solve (several workgroups per mode): by default for
-    // bands wider than 512; SEMX_BAND_COOP=0 / 1 forces it off / on
+    // bands wider than 1008 (narrower ones stream extended panels faster
+    // with one workgroup per mode); SEMX_BAND_COOP=0 / 1 forces it off / on
     const char* env = getenv("SEMX_BAND_COOP");
-    H.coop_mode = env ? atoi(env) != 0 : kd > 512;
+    H.coop_mode = env ? atoi(env) != 0 : kd > 1008;
     if (kd <= 512) { H.nb = 32; H.npass = 1; H.bt = 64; while (H.bt < kd) H.bt *= 2; }
     else if (kd <= 1024) { H.nb = 16; H.npass = 2; H.bt = 512; }
     else if (kd <= 2048) { H.nb = 8; H.npass = 4; H.bt = 512; }
     else { H.nb = 8; H.npass = 0; H.bt = 0; }   // no direct solve for this band
-    // bands up to 512 wide solve with the inverses folded into the panels
-    H.band_ext = kd <= 512 && !H.coop_mode;
+    // bands up to 1008 wide solve with the inverses folded into the panels
+    // (32-column blocks up to 512, 16-column blocks beyond: one thread per
+    // panel row, at most 1024)
+    H.band_ext = kd <= 1008 && !H.coop_mode;
     H.ldx = (H.nb + kd + 7) / 8 * 8;
     H.kdp = (kd + 63) / 64 * 64;
   }

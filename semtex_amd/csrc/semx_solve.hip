@@ -552,11 +552,12 @@ __global__ __launch_bounds__(BT) void k_band_solve(BandArgs a) {
 // ahead into registers.
 constexpr int XNB = 32;
 
-template <int NPASS, int BT, int NR, int D, bool FWD>
+template <int NB, int NPASS, int BT, int NR, int D, bool FWD>
 __device__ inline void band_ext_sweep(const double* __restrict__ P,
                                       double* const (&ring)[NR], double* const (&gy)[NR],
-                                      int n, int nblk, int kd, int ldx, int wm) {
-  constexpr int NB = XNB;
+                                      const int* __restrict__ xrows, int n, int nblk, int kd,
+                                      int ldx, int wm) {
+  constexpr int WC = (NR >= 4 || NB < 16) ? 4 : 8;   // window values per read batch
   const int tid = threadIdx.x;
   const int npad = nblk * NB;
   double pbuf[D + 1][NPASS][NB], abuf[D + 1][NR];
@@ -574,12 +575,15 @@ __device__ inline void band_ext_sweep(const double* __restrict__ P,
     const bool ok = tid < NB && (FWD ? (i >= (NB + kd) && i < npad) : (i >= 0));
     return ok ? i : -1;
   };
+  // rows past a block's live rows load the last live row's address (the
+  // same cache lines, no extra memory traffic) and skip the product
   auto load = [&](int s, double (&pp)[NPASS][NB], double (&ad)[NR]) {
     const int J = FWD ? s : nblk - 1 - s;
     const double2* pj = reinterpret_cast<const double2*>(P + (size_t)J * NB * ldx);
+    const int rlast = xrows[2 * J + (FWD ? 0 : 1)] - 1;
 #pragma unroll
     for (int q = 0; q < NPASS; ++q) {
-      const int r = min(tid + q * BT, ldx - 1);
+      const int r = min(tid + q * BT, rlast);
 #pragma unroll
       for (int k2 = 0; k2 < NB / 2; ++k2) {
         const double2 v = pj[(size_t)k2 * ldx + r];
@@ -596,6 +600,7 @@ __device__ inline void band_ext_sweep(const double* __restrict__ P,
     const int j0 = J * NB;
     const int r0 = j0 & wm;  // blocks never wrap: W is a multiple of NB
     const int ia = admit_index(s);
+    const int rows = xrows[2 * J + (FWD ? 0 : 1)];
     // Every thread takes delivery of this step's registers here, on every
     // path: consumed only inside the branches below, the loads would count
     // as possibly outstanding at the loop head, where the compiler would
@@ -616,20 +621,25 @@ __device__ inline void band_ext_sweep(const double* __restrict__ P,
       const int r = tid + q * BT;
       const int rr = r - NB;
       const int i = r < NB ? j0 + r : (FWD ? j0 + NB + rr : j0 - 1 - rr);
-      const bool live = r < NB ? i < n : (rr < kd && (FWD ? i < n : i >= 0));
+      const bool live = r < NB ? i < n : (r < rows && rr < kd && (FWD ? i < n : i >= 0));
       if (live) {
+        // one right-hand side at a time, its window read in batches of WC
 #pragma unroll
         for (int c = 0; c < NR; ++c) {
-          const double* wv = ring[c] + r0;
-          double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+          double acc0 = 0.0, acc1 = 0.0;
 #pragma unroll
-          for (int kk = 0; kk < NB; kk += 4) {
-            a0 += pp[q][kk] * wv[kk];
-            a1 += pp[q][kk + 1] * wv[kk + 1];
-            a2 += pp[q][kk + 2] * wv[kk + 2];
-            a3 += pp[q][kk + 3] * wv[kk + 3];
+          for (int h = 0; h < NB; h += WC) {
+            const double2* wv = reinterpret_cast<const double2*>(ring[c] + r0 + h);
+            double2 w[WC / 2];
+#pragma unroll
+            for (int k = 0; k < WC / 2; ++k) w[k] = wv[k];
+#pragma unroll
+            for (int k = 0; k < WC / 2; ++k) {
+              acc0 += pp[q][h + 2 * k] * w[k].x;
+              acc1 += pp[q][h + 2 * k + 1] * w[k].y;
+            }
           }
-          const double v = (a0 + a1) + (a2 + a3);
+          const double v = acc0 + acc1;
           if (r < NB) gy[c][i] = v;
           else ring[c][i & wm] -= v;
         }
@@ -794,13 +804,13 @@ __device__ inline long long uni64(long long v) {
   return (long long)(((unsigned long long)hi << 32) | lo);
 }
 
-template <int NPASS, int BT, int NR, int D>
+template <int NB, int NPASS, int BT, int NR, int D>
 __global__ __launch_bounds__(BT) void k_band_ext(BandArgs a) {
   extern __shared__ double lds[];
   const int m = uni(a.dmode[blockIdx.x]);
   const int n = uni(a.mode_n[m]);
   const long long offp = uni64(a.offP[m]);
-  const int nblk = (n + XNB - 1) / XNB;
+  const int nblk = (n + NB - 1) / NB;
   double* ring[NR];
   double* gy[NR];
 #pragma unroll
@@ -812,10 +822,13 @@ __global__ __launch_bounds__(BT) void k_band_ext(BandArgs a) {
     ring[c] = lds + (size_t)c * a.ring;
     gy[c] = a.sol + fld * a.fstride + (size_t)p * a.solstride;
   }
-  band_ext_sweep<NPASS, BT, NR, D, true>(a.Pf + offp, ring, gy, n, nblk, a.kd, a.kdp,
-                                         a.ring - 1);
-  band_ext_sweep<NPASS, BT, NR, D, false>(a.Pb + offp, ring, gy, n, nblk, a.kd, a.kdp,
-                                          a.ring - 1);
+  int* xr = reinterpret_cast<int*>(lds + (size_t)NR * a.ring);
+  for (int i = threadIdx.x; i < 2 * nblk; i += BT) xr[i] = a.xrows[(size_t)m * 2 * a.xblk + i];
+  __syncthreads();   // the row table is read by the first prefetches
+  band_ext_sweep<NB, NPASS, BT, NR, D, true>(a.Pf + offp, ring, gy, xr, n, nblk, a.kd, a.kdp,
+                                             a.ring - 1);
+  band_ext_sweep<NB, NPASS, BT, NR, D, false>(a.Pb + offp, ring, gy, xr, n, nblk, a.kd, a.kdp,
+                                              a.ring - 1);
 }
 
 template <int BT, int NR, int D>
@@ -2035,29 +2048,31 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
     unsigned bt = (unsigned)H.bt;
     size_t lds_x = lds;
     if (H.band_ext) {
-      // extended panels: ldx rows, one thread per row up to 512 threads
+      // extended panels: ldx rows, one thread per row up to 512 threads,
+      // two beyond (16-column blocks for bands wider than 512)
       ba.kdp = H.ldx;
       const int xbt = std::min(512, (H.ldx + 63) / 64 * 64);
       const int xnp = (H.ldx + xbt - 1) / xbt;
       bt = (unsigned)xbt;
-#define SEMX_XPICK(NP_, BT_, D_)                                                 \
-      if (xnp == NP_ && xbt == BT_ && H.band_d == D_)                            \
-        kfn = (nr == 6)   ? (const void*)k_band_ext<NP_, BT_, 6, D_>             \
-              : (nr == 4) ? (const void*)k_band_ext<NP_, BT_, 4, D_>             \
-              : (nr == 2) ? (const void*)k_band_ext<NP_, BT_, 2, D_>             \
-                          : (const void*)k_band_ext<NP_, BT_, 1, D_>;
-#define SEMX_XPICKD(NP_, BT_) SEMX_XPICK(NP_, BT_, 1)
-      SEMX_XPICKD(1, 64) SEMX_XPICKD(1, 128) SEMX_XPICKD(1, 192) SEMX_XPICKD(1, 256)
-      SEMX_XPICKD(1, 320) SEMX_XPICKD(1, 384) SEMX_XPICKD(1, 448) SEMX_XPICKD(1, 512)
-      SEMX_XPICK(2, 512, 1)
-#undef SEMX_XPICKD
+      lds_x = (size_t)nr * (ba.ring + H.nb) * sizeof(double) +
+              ((size_t)2 * H.xblk * sizeof(int) + 7) / 8 * 8;
+#define SEMX_XPICK(NB_, NP_, BT_)                                                \
+      if (H.nb == NB_ && xnp == NP_ && xbt == BT_)                               \
+        kfn = (nr == 6)   ? (const void*)k_band_ext<NB_, NP_, BT_, 6, 1>         \
+              : (nr == 4) ? (const void*)k_band_ext<NB_, NP_, BT_, 4, 1>         \
+              : (nr == 2) ? (const void*)k_band_ext<NB_, NP_, BT_, 2, 1>         \
+                          : (const void*)k_band_ext<NB_, NP_, BT_, 1, 1>;
+      SEMX_XPICK(32, 1, 64) SEMX_XPICK(32, 1, 128) SEMX_XPICK(32, 1, 192)
+      SEMX_XPICK(32, 1, 256) SEMX_XPICK(32, 1, 320) SEMX_XPICK(32, 1, 384)
+      SEMX_XPICK(32, 1, 448) SEMX_XPICK(32, 1, 512) SEMX_XPICK(32, 2, 512)
+      SEMX_XPICK(16, 2, 512)
 #undef SEMX_XPICK
       if (!kfn) return SEMX_E_UNSUPPORTED;
       // the whole right-hand side resident in LDS when it fits (one or two
       // right-hand sides per launch; two threads per panel row)
       int npadmax = 0;
       for (int m : H.dmode) npadmax = std::max(npadmax, H.modes[m].nblk * XNB);
-      const int rbt = (2 * H.ldx + 63) / 64 * 64;
+      const int rbt = H.nb == XNB ? (2 * H.ldx + 63) / 64 * 64 : 0;
       const int rnr = nr <= 2 ? nr : npl;
       const size_t lds_res = (size_t)rnr * npadmax * sizeof(double) +
                              ((size_t)2 * H.xblk * sizeof(int) + 7) / 8 * 8;
