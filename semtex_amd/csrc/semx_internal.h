@@ -120,6 +120,7 @@ struct Helm {
   // product per block step); ldx rows per panel column
   bool band_ext = false;
   int ldx = 0;
+  bool xcoop = false;         // cooperative sweep over extended panels
   // live rows (32 + envelope height) of each extended panel, forward and
   // backward: xrows[(mode*xblk + J)*2 + dir], dir 0 forward
   std::vector<int> xrows;

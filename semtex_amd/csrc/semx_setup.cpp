@@ -763,7 +763,11 @@ int helm_build(Helm& H) {
     // bands up to 1008 wide solve with the inverses folded into the panels
     // (32-column blocks up to 512, 16-column blocks beyond: one thread per
     // panel row, at most 1024)
-    H.band_ext = kd <= 1008 && !H.coop_mode;
+    // SEMX_BAND_XCOOP=1: the cooperative sweep over extended panels
+    const char* envx = getenv("SEMX_BAND_XCOOP");
+    H.xcoop = envx && atoi(envx) != 0 && kd <= 1008;
+    if (H.xcoop) H.coop_mode = true;
+    H.band_ext = kd <= 1008 && (!H.coop_mode || H.xcoop);
     H.ldx = (H.nb + kd + 7) / 8 * 8;
     H.kdp = (kd + 63) / 64 * 64;
   }

@@ -1168,6 +1168,243 @@ __global__ __launch_bounds__((coop_bt<NB, NR>())) void k_band_coop(CoopArgs a) {
     __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// ===========================================================================
+// Cooperative sweep over extended panels.  The panel of block J is cut by
+// rows: the lead (lane 0) owns rows [0, NB + C1 NB) -- the block inverse
+// and the near chunks -- and each follower w owns the chunks [c_lo(w),
+// c_hi(w)) of the update rows (chunk c feeds block J+1+c).  Per step:
+//   A  lead: window of block J = b - own partial sums - followers' partial
+//      sums (hand-off waves); the store wave writes the previous block's
+//      solution out.  Follower: the window of block J from the lead; the
+//      store wave hands over the partial sum of block s + c_lo, complete
+//      since the previous step.
+//   B  lead: the store wave publishes the window to every follower; task
+//      waves form the block solution (rows < NB) and the near updates in one
+//      product with the window.  Follower: task waves apply their chunks.
+// The window goes out before the block is solved, so the followers' work
+// is off the lead's critical path.  Hand-off slots, sentinel and bounded
+// waits are those of the cooperative sweep above.
+// ===========================================================================
+template <int NB, int NR, int RPT, bool FWD>
+__device__ bool xcoop_sweep(const CoopArgs& a, int md, int lane, const int (&cl)[COOP_WMAX + 1],
+                            const double* __restrict__ P, double* const (&gy)[NR], int n,
+                            int nblk, double* acc, double* rhs, double* ybuf, int* abort) {
+  constexpr int BT = coop_bt<NB, NR>();
+  constexpr int VT = BT - COOP_TT - 64;       // hand-off threads
+  constexpr int ST = VT + 64;                 // end of the store wave
+  constexpr int NV = NR * NB;                 // values handed over per block
+  constexpr int RING = coop_ring<NB>();
+  const int tid = threadIdx.x;
+  const int W = a.W, ldx = a.kdp, kd = a.kd;
+  int c_lo = 0, c_hi = cl[1];
+#pragma unroll
+  for (int w = 1; w < COOP_WMAX; ++w)
+    if (w == lane) { c_lo = cl[w]; c_hi = cl[w + 1]; }
+  const bool lead = lane == 0;
+  const int prow0 = lead ? 0 : NB + c_lo * NB;                      // first owned panel row
+  const int nrow = lead ? NB + c_hi * NB : (c_hi - c_lo) * NB;     // owned panel rows
+  const size_t dirofs = FWD ? 0 : (size_t)a.xstride;
+  const size_t lstride = (size_t)a.nblkmax * NV;            // per lane
+  double* inbox = a.inbox + dirofs + (size_t)md * (W - 1) * lstride;
+  double* part = a.part + dirofs + (size_t)md * (W - 1) * lstride;
+  const int last = lead ? nblk - 1 : nblk - 2 - c_lo;      // last step with work
+  for (int i = tid; i < RING * NV; i += BT) acc[i] = 0.0;
+  if (tid == 0) *abort = 0;
+
+  const bool vth = tid < NV;                  // hand-off threads: one value each
+  const int vc = vth ? tid / NB : 0, vi = vth ? tid - (tid / NB) * NB : 0;
+  double* gyv = gy[0];                        // this thread's right-hand side
+#pragma unroll
+  for (int c = 1; c < NR; ++c)
+    if (c == vc) gyv = gy[c];
+  auto& gyc = gy;
+  const int tt = tid - ST;                    // task index base (< 0: other waves)
+  const bool stw = tid >= VT && tid < ST;     // store wave
+  const int sl = tid - VT;                    // lane in the store wave
+
+  double pp[RPT][NB];
+  double bv[COOP_PD], hv[COOP_PD][COOP_WMAX - 1];
+  auto block_of = [&](int s) { return FWD ? s : nblk - 1 - s; };
+  // task waves: the owned rows of one block's panel, 16-byte loads, rows
+  // clamped (every task thread issues the same loads on every step)
+  auto load_panel = [&](int s) {
+    if (tt < 0) return;                       // wave-uniform
+    const int sc = min(s, max(last, 0));
+    const double2* pj = reinterpret_cast<const double2*>(P + (size_t)block_of(sc) * NB * ldx);
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int t = min(tt + q * COOP_TT, max(nrow - 1, 0));
+      const int prow = min(prow0 + t, ldx - 1);
+#pragma unroll
+      for (int k2 = 0; k2 < NB / 2; ++k2) {
+        const double2 v = pj[(size_t)k2 * ldx + prow];
+        pp[q][2 * k2] = v.x;
+        pp[q][2 * k2 + 1] = v.y;
+      }
+    }
+  };
+  auto fetch = [&](int s, int u) {
+    if (tid >= VT) return;                    // wave-uniform
+    const int sc = min(s, max(last, 0));
+    const int J = block_of(sc);
+    if (lead) {
+      const int row = min(J * NB + vi, n - 1);
+      bv[u] = ld_dev(&gyv[row]);
+#pragma unroll
+      for (int w = 1; w < COOP_WMAX; ++w)
+        hv[u][w - 1] = ld_dev(&part[(min(w, max(W - 1, 1)) - 1) * lstride + (size_t)sc * NV +
+                                    min(tid, NV - 1)]);
+    } else {
+      hv[u][0] = ld_dev(&inbox[(lane - 1) * lstride + (size_t)sc * NV + min(tid, NV - 1)]);
+    }
+  };
+  // the previous block's solution out (lead) / the partial sum of block T
+  // to the lead (follower); store wave
+  auto flush = [&](int s) {
+    if (!stw) return;
+    if (lead) {
+      const int J = block_of(s);
+      for (int k = sl; k < NV; k += 64) {
+        const int c = k / NB, r = k - (k / NB) * NB;
+        if (J * NB + r < n) st_dev(&gyc[c][J * NB + r], ybuf[k]);
+      }
+    } else {
+      const int T = s + 1 + c_lo;
+      if (T < nblk) {
+        const int tslot = T & (RING - 1);
+        for (int k = sl; k < NV; k += 64) {
+          st_dev(&part[(lane - 1) * lstride + (size_t)T * NV + k], acc[tslot * NV + k]);
+          acc[tslot * NV + k] = 0.0;
+        }
+      }
+    }
+  };
+
+  auto step = [&](int s, int u) -> bool {
+    const int J = block_of(s);
+    const int j0 = J * NB;
+    const int slot = s & (RING - 1);
+    // every task thread takes delivery of its panel registers here
+    if (tt >= 0) {
+#pragma unroll
+      for (int q = 0; q < RPT; ++q)
+#pragma unroll
+        for (int k = 0; k < NB; ++k) asm volatile("" ::"v"(pp[q][k]));
+    }
+    // A
+    if (lead) {
+      if (vth) {
+        double v = (j0 + vi < n ? bv[u] : 0.0) - acc[slot * NV + tid];
+#pragma unroll
+        for (int w = 1; w < COOP_WMAX; ++w)   // followers only (cl[W] ends the chunks)
+          if (w < W && s >= 1 + cl[w])
+            v -= coop_take(&part[(w - 1) * lstride + (size_t)s * NV + tid], hv[u][w - 1], abort);
+        rhs[tid] = v;
+        acc[slot * NV + tid] = 0.0;
+      }
+    } else if (vth) {
+      rhs[tid] = coop_take(&inbox[(lane - 1) * lstride + (size_t)s * NV + tid], hv[u][0], abort);
+    }
+    if (s > 0) flush(s - 1);
+    if (__syncthreads_or(*abort)) return false;
+    // B
+    if (lead && stw)
+      for (int i = sl; i < (W - 1) * NV; i += 64) {
+        const int w = i / NV, k = i - (i / NV) * NV;
+        st_dev(&inbox[w * lstride + (size_t)s * NV + k], rhs[k]);
+      }
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int t = tt + q * COOP_TT;
+      if (tt >= 0 && t < nrow) {
+        const int prow = prow0 + t;
+        const int ur = prow - NB;             // update row (< 0: block solution row)
+        if (ur < kd) {
+          const int c = ur / NB, r = ur - (ur / NB) * NB;
+          const int tslot = (s + 1 + c) & (RING - 1);
+          const int tr = FWD ? r : NB - 1 - r;
+#pragma unroll
+          for (int k = 0; k < NR; ++k) {
+            const double2* wv = reinterpret_cast<const double2*>(rhs + k * NB);
+            double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+            for (int k2 = 0; k2 < NB / 2; ++k2) {
+              const double2 w2 = wv[k2];
+              a0 += pp[q][2 * k2] * w2.x;
+              a1 += pp[q][2 * k2 + 1] * w2.y;
+            }
+            const double v = a0 + a1;
+            if (ur < 0) ybuf[k * NB + prow] = v;
+            else acc[(tslot * NR + k) * NB + tr] += v;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    load_panel(s + 1);
+    fetch(s + COOP_PD, u);
+    return true;
+  };
+
+  __syncthreads();
+  load_panel(0);
+#pragma unroll
+  for (int u = 0; u < COOP_PD; ++u) fetch(u, u);
+  // whole groups of COOP_PD steps with no exit inside a group, then the rest
+  int s0 = 0;
+  for (; s0 + COOP_PD - 1 <= last; s0 += COOP_PD) {
+#pragma unroll
+    for (int u = 0; u < COOP_PD; ++u)
+      if (!step(s0 + u, u)) return false;
+  }
+#pragma unroll
+  for (int u = 0; u < COOP_PD; ++u)
+    if (s0 + u <= last && !step(s0 + u, u)) return false;
+  if (last >= 0) flush(last);
+  __syncthreads();
+  return true;
+}
+
+template <int NB, int NR, int RPT>
+__global__ __launch_bounds__((coop_bt<NB, NR>())) void k_band_xcoop(CoopArgs a) {
+  __shared__ double acc[coop_ring<NB>() * NR * NB];
+  __shared__ __align__(16) double rhs[NR * NB];   // read as 16-byte pairs
+  __shared__ double ybuf[NR * NB];
+  __shared__ int abort_flag;
+  // blocks of one mode share an XCD: dispatch is round-robin over 8 XCDs
+  const int b = blockIdx.x;
+  const int md = (b / (8 * a.W)) * 8 + (b & 7);
+  const int lane = (b >> 3) % a.W;
+  if (md >= a.ndirect) return;
+  const int m = uni(a.dmode[md]);
+  const int n = uni(a.mode_n[m]);
+  const long long offp = uni64(a.offP[m]);
+  const int nblk = (n + NB - 1) / NB;
+  int cl[COOP_WMAX + 1];
+#pragma unroll
+  for (int w = 0; w <= COOP_WMAX; ++w) cl[w] = (w <= a.W) ? uni(a.clo[w]) : 0x3fffffff;
+  double* gy[NR];
+#pragma unroll
+  for (int c = 0; c < NR; ++c) {
+    const int fld = c / a.npl, kk = c - fld * a.npl;
+    int p = a.mode_plane[2 * m + kk];
+    if (p < 0) p = a.mode_plane[2 * m];
+    p = uni(p);
+    gy[c] = a.sol + fld * a.fstride + (size_t)p * a.solstride;
+  }
+  const bool ok = xcoop_sweep<NB, NR, RPT, true>(a, md, lane, cl, a.Pf + offp, gy, n, nblk,
+                                                 acc, rhs, ybuf, &abort_flag);
+  if (ok) {
+    __syncthreads();
+    const bool ok2 = xcoop_sweep<NB, NR, RPT, false>(a, md, lane, cl, a.Pb + offp, gy, n, nblk,
+                                                     acc, rhs, ybuf, &abort_flag);
+    if (!ok2 && threadIdx.x == 0)
+      __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else if (threadIdx.x == 0) {
+    __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 __global__ __launch_bounds__(ELT_THREADS) void k_backsub(ElemArgs a) {
   extern __shared__ double esh[];
   const int t = threadIdx.x, p = blockIdx.y;
@@ -1804,7 +2041,9 @@ int coop_setup(Helm& H) {
   H.coop_w = 0;
   const int nd = (int)H.dmode.size();
   if (nd == 0 || H.npass == 0 || !H.coop_mode) return SEMX_OK;
-  const int nb = H.nb, nch = H.kdp / nb;
+  // chunks of NB update rows; with extended panels the lead also owns the
+  // NB rows of the block inverse
+  const int nb = H.nb, nch = H.xcoop ? (H.kd + nb - 1) / nb : H.kdp / nb;
   int cus = 256;
   int dev = 0;
   if (hipGetDevice(&dev) == hipSuccess) {
@@ -1816,7 +2055,7 @@ int coop_setup(Helm& H) {
   const int ring = nb >= 64 ? 16 : COOP_RING;
   int lead = std::min(nch, std::max(1, 128 / nb));     // ~128 panel rows for the lead
   if (const char* e = getenv("SEMX_COOP_LEAD")) lead = std::max(1, std::min(nch, atoi(e)));
-  lead = std::min(lead, std::min(tasks / nb, ring - 2));
+  lead = std::min(lead, std::min(tasks / nb - (H.xcoop ? 1 : 0), ring - 2));
   const int rest = nch - lead;
   int follow = 0, nf = 0;
   if (rest > 0) {
@@ -1831,7 +2070,7 @@ int coop_setup(Helm& H) {
   if (1 + nf > COOP_WMAX)                     // band too wide for the lanes
     return H.bt ? SEMX_OK : SEMX_E_UNSUPPORTED;
   H.coop_w = 1 + nf;
-  H.coop_rpt = (std::max(lead, follow) * nb > COOP_TT) ? 2 : 1;
+  H.coop_rpt = (std::max(lead + (H.xcoop ? 1 : 0), follow) * nb > COOP_TT) ? 2 : 1;
   H.coop_clo.assign(H.coop_w + 1, 0);
   H.coop_clo[1] = lead;
   for (int w = 1; w < H.coop_w; ++w) H.coop_clo[w + 1] = std::min(nch, H.coop_clo[w] + follow);
@@ -2124,8 +2363,10 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
       const void* cfn = nullptr;
 #define SEMX_COOP(NB_, NR_)                                                       \
       if (H.nb == NB_ && nrc == NR_)                                              \
-        cfn = (H.coop_rpt == 2) ? (const void*)k_band_coop<NB_, NR_, 2>           \
-                                : (const void*)k_band_coop<NB_, NR_, 1>;
+        cfn = H.xcoop ? ((H.coop_rpt == 2) ? (const void*)k_band_xcoop<NB_, NR_, 2>    \
+                                           : (const void*)k_band_xcoop<NB_, NR_, 1>)   \
+                      : ((H.coop_rpt == 2) ? (const void*)k_band_coop<NB_, NR_, 2>     \
+                                           : (const void*)k_band_coop<NB_, NR_, 1>);
       SEMX_COOP(32, 1) SEMX_COOP(32, 2) SEMX_COOP(32, 4) SEMX_COOP(32, 6)
       SEMX_COOP(16, 1) SEMX_COOP(16, 2) SEMX_COOP(16, 4) SEMX_COOP(16, 6)
       SEMX_COOP(8, 1) SEMX_COOP(8, 2) SEMX_COOP(8, 4) SEMX_COOP(8, 6)
@@ -2136,7 +2377,8 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
         ca.mode_n = H.d.mode_n; ca.offP = H.d.mode_offH; ca.offL = H.d.mode_offL;
         ca.dmode = H.d.dmode; ca.mode_plane = H.d.mode_plane; ca.sol = H.d.sol;
         ca.inbox = H.d.inbox; ca.part = H.d.part; ca.clo = H.d.clo;
-        ca.err = H.d.err; ca.solstride = H.solstride; ca.kd = H.kd; ca.kdp = H.kdp;
+        ca.err = H.d.err; ca.solstride = H.solstride; ca.kd = H.kd;
+        ca.kdp = H.xcoop ? H.ldx : H.kdp;
         ca.ndirect = (int)H.dmode.size(); ca.W = H.coop_w; ca.nblkmax = H.nblkmax;
         ca.npl = npl; ca.fstride = (long long)psol;
         ca.xstride = (long long)H.coop_xstride;

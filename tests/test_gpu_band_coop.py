@@ -19,14 +19,22 @@ CASES = [
     ("periodic-kd607-singular", 16, 9, 4, 0.0, 1.0, 0.0, PERIODIC),
     ("periodic-kd1059", 20, 12, 2, 3.0, 1.0, 0.0, PERIODIC),
 ]
-LAYOUTS = [("default", None, None), ("one-chunk-lanes", "1", "1"), ("lead2-follow3", "2", "3")]
+# (name, lead chunks, follower chunks, extended panels): the cooperative
+# sweep over separate diagonal-block inverses, and the one over extended
+# panels (SEMX_BAND_XCOOP; bands up to 1008, wider ones fall back to the
+# former)
+LAYOUTS = [("default", None, None, "0"), ("one-chunk-lanes", "1", "1", "0"),
+           ("lead2-follow3", "2", "3", "0"), ("xcoop-default", None, None, "1"),
+           ("xcoop-one-chunk-lanes", "1", "1", "1"), ("xcoop-lead3-follow2", "3", "2", "1")]
 
 
 @pytest.fixture
 def coop_env(request):
-    name, lead, follow = request.param
-    old = {k: os.environ.get(k) for k in ("SEMX_BAND_COOP", "SEMX_COOP_LEAD", "SEMX_COOP_FOLLOW")}
+    name, lead, follow, xcoop = request.param
+    keys = ("SEMX_BAND_COOP", "SEMX_COOP_LEAD", "SEMX_COOP_FOLLOW", "SEMX_BAND_XCOOP")
+    old = {k: os.environ.get(k) for k in keys}
     os.environ["SEMX_BAND_COOP"] = "1"
+    os.environ["SEMX_BAND_XCOOP"] = xcoop
     for k, v in (("SEMX_COOP_LEAD", lead), ("SEMX_COOP_FOLLOW", follow)):
         if v is None:
             os.environ.pop(k, None)
@@ -59,7 +67,8 @@ def test_coop_band_parity(cuda, coop_env, case):
     assert r <= 1e-12
 
 
-@pytest.mark.parametrize("coop_env", LAYOUTS[:1], ids=["default"], indirect=True)
+@pytest.mark.parametrize("coop_env", [LAYOUTS[0], LAYOUTS[3]], ids=["default", "xcoop"],
+                         indirect=True)
 def test_coop_band_fields(cuda, coop_env):
     """Three fields sharing the system in one cooperative sweep."""
     import torch
