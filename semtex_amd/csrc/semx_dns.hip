@@ -31,6 +31,83 @@ namespace {
 constexpr int DFT_KB = 16;          // padding of the transform matrix columns
 constexpr int OP_THREADS = 1024;    // element-operator blocks
 
+// The same product on the FP64 matrix cores (v_mfma_f64_16x16x4: A 16x4,
+// B 4x16, D 16x16; lane l holds A[l%16][l/16], B[l/16][l%16] and, in
+// register r, D[4r + l/16][l%16]).  W (MT 16-row tiles by K padded to 4) is
+// staged in LDS with an odd row stride, so the 16 rows an A operand reads
+// fall in distinct banks; each wave owns DFT_MCT 16-column tiles of the
+// block's columns and all MT row tiles, and takes its B operands (four input
+// rows by 16 columns per k step) straight from memory, DFT_MPF steps ahead
+// in a register ring.  Input rows at or beyond N are clamped (their W
+// columns are zero), as are columns beyond ntrn (not stored).  For N <= 128.
+constexpr int DFT_MCT = 2;                       // column tiles per wave
+constexpr int DFT_MPF = 4;                       // k steps of B prefetch
+constexpr int DFT_MCOLS = 4 * 16 * DFT_MCT;      // columns per block
+typedef double dft_d4 __attribute__((ext_vector_type(4)));
+
+template <int MT>
+__global__ __launch_bounds__(256) void k_dft_mfma(const double* __restrict__ in,
+                                                  double* __restrict__ out,
+                                                  const double* __restrict__ W, int N, int ldw,
+                                                  int ntrn, long long ld) {
+  extern __shared__ double sWm[];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int KP = (N + 3) & ~3;
+  const int ldk = KP + 1;
+  const int li = lane & 15, lk = lane >> 4;
+  int col[DFT_MCT];
+#pragma unroll
+  for (int ct = 0; ct < DFT_MCT; ++ct)
+    col[ct] = min(blockIdx.x * DFT_MCOLS + (wave * DFT_MCT + ct) * 16 + li, ntrn - 1);
+  // B operands DFT_MPF k steps ahead in a register ring (rows clamped)
+  const int KS = KP / 4;
+  auto load_b = [&](int ks, double (&b)[DFT_MCT]) {
+    const size_t row = (size_t)min(4 * ks + lk, N - 1) * ld;
+#pragma unroll
+    for (int ct = 0; ct < DFT_MCT; ++ct) b[ct] = in[row + col[ct]];
+  };
+  double bq[DFT_MPF][DFT_MCT];
+#pragma unroll
+  for (int u = 0; u < DFT_MPF; ++u) load_b(u, bq[u]);
+  for (int r = tid >> 6; r < MT * 16; r += 4)
+    for (int k = lane; k < KP; k += 64) sWm[r * ldk + k] = W[(size_t)r * ldw + k];
+  __syncthreads();
+  dft_d4 acc[MT][DFT_MCT];
+#pragma unroll
+  for (int rt = 0; rt < MT; ++rt)
+#pragma unroll
+    for (int ct = 0; ct < DFT_MCT; ++ct) acc[rt][ct] = dft_d4{0.0, 0.0, 0.0, 0.0};
+  const double* wa = sWm + li * ldk + lk;
+  for (int k0 = 0; k0 < KS; k0 += DFT_MPF) {
+#pragma unroll
+    for (int u = 0; u < DFT_MPF; ++u) {
+      const int ks = k0 + u;
+      if (ks < KS) {
+#pragma unroll
+        for (int rt = 0; rt < MT; ++rt) {
+          const double a = wa[rt * 16 * ldk + 4 * ks];
+#pragma unroll
+          for (int ct = 0; ct < DFT_MCT; ++ct)
+            acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bq[u][ct], acc[rt][ct], 0, 0, 0);
+        }
+      }
+      load_b(ks + DFT_MPF, bq[u]);
+    }
+  }
+#pragma unroll
+  for (int ct = 0; ct < DFT_MCT; ++ct) {
+    const int c = blockIdx.x * DFT_MCOLS + (wave * DFT_MCT + ct) * 16 + li;
+    if (c >= ntrn) continue;
+#pragma unroll
+    for (int rt = 0; rt < MT; ++rt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = rt * 16 + 4 * r + lk;
+        if (row < N) out[(size_t)row * ld + c] = acc[rt][ct][r];
+      }
+  }
+}
+
 // out[r][col] = sum_k W[r][k] in[k][col], r < N: a register-blocked
 // product of the (padded) N x N transform matrix with a 64-column tile of
 // the field.  Threads form 16 row groups x 16 column groups; each computes
@@ -326,6 +403,34 @@ int dft_launch(const double* in, double* out, int N, int ntrn, int sign, hipStre
   int ldw = 0;
   int st = dft_plan(N, sign, &W, &ldw);
   if (st) return st;
+  if (N <= 128 && ntrn > 0) {
+    // matrix cores: W staged whole in LDS
+    const int MT = (N + 15) / 16;
+    const size_t lds = (size_t)MT * 16 * (((N + 3) & ~3) + 1) * sizeof(double);
+    dim3 gm((ntrn + DFT_MCOLS - 1) / DFT_MCOLS);
+    const void* kfn = nullptr;
+    switch (MT) {
+      case 1: kfn = (const void*)k_dft_mfma<1>; break;
+      case 2: kfn = (const void*)k_dft_mfma<2>; break;
+      case 3: kfn = (const void*)k_dft_mfma<3>; break;
+      case 4: kfn = (const void*)k_dft_mfma<4>; break;
+      case 5: kfn = (const void*)k_dft_mfma<5>; break;
+      case 6: kfn = (const void*)k_dft_mfma<6>; break;
+      case 7: kfn = (const void*)k_dft_mfma<7>; break;
+      default: kfn = (const void*)k_dft_mfma<8>; break;
+    }
+    if (lds > 64 * 1024)
+      HIPCHK(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    long long ldl = ntrn;
+    void* kargs[] = {(void*)&in, (void*)&out, (void*)&W, (void*)&N, (void*)&ldw, (void*)&ntrn,
+                     (void*)&ldl};
+    hipEvent_t evm;
+    prof_begin(s, &evm);
+    HIPCHK(hipLaunchKernel(kfn, gm, dim3(256), kargs, lds, s));
+    prof_end(s, "k_dft_z", evm);
+    HIPCHK(hipGetLastError());
+    return SEMX_OK;
+  }
   const int rp = dft_rp(N);
   dim3 g((ntrn + DFT_COLS - 1) / DFT_COLS);
   hipEvent_t ev;
