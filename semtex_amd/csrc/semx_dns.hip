@@ -210,6 +210,58 @@ __global__ __launch_bounds__(OP_THREADS) void k_grad_xy(double* __restrict__ dat
   u[pt] = B * sx + A * rx;
 }
 
+// One x or y term of the nonlinear forcing, fused (dns/nonlinear.cpp:
+// 36-211): MODE 0 (convective) N -= b * d(a)/dx_dir, MODE 1 (conservative)
+// N -= d(a b)/dx_dir, the derivative by the same element operator as
+// k_grad_xy, read out of place so the operand needs no copy, and applied in
+// the epilogue.  Entries of a plane beyond its elements (the padding up to
+// planeSize) take the values the unfused sequence gives them:
+// N -= a b.
+template <int MODE>
+__global__ __launch_bounds__(OP_THREADS) void k_grad_nl(double* __restrict__ N,
+                                                        const double* __restrict__ a,
+                                                        const double* __restrict__ b,
+                                                        const double* __restrict__ G,
+                                                        const int* __restrict__ eset,
+                                                        const double* __restrict__ ops, int np,
+                                                        int nel, int planeSize, int dir) {
+  extern __shared__ double gsh[];
+  const int npnp = np * np;
+  const int ce = OP_THREADS / npnp;
+  const int t = threadIdx.x;
+  const int el = t / npnp, pt = t - el * npnp;
+  const int e = blockIdx.x * ce + el;
+  const bool act = el < ce && e < nel;
+  double* sD = gsh;
+  double* sT = sD + npnp;
+  double* sU = sT + npnp;
+  if (t < npnp) {
+    sD[t] = ops[t];
+    sT[t] = ops[npnp + t];
+  }
+  const size_t pbase = (size_t)blockIdx.y * planeSize;
+  if (blockIdx.x == 0)
+    for (int k = nel * npnp + t; k < planeSize; k += OP_THREADS)
+      N[pbase + k] -= a[pbase + k] * b[pbase + k];
+  const size_t off = pbase + (size_t)e * npnp + pt;
+  if (act) sU[el * npnp + pt] = MODE == 0 ? a[off] : a[off] * b[off];
+  __syncthreads();
+  if (!act) return;
+  const int i = pt / np, j = pt - (pt / np) * np;
+  const double* su = sU + el * npnp;
+  double A = 0.0, B = 0.0;
+  for (int q = 0; q < np; ++q) {
+    A += su[i * np + q] * sT[q * np + j];
+    B += sD[i * np + q] * su[q * np + j];
+  }
+  const double* g = G + (size_t)eset[e] * G_N * npnp;
+  const double rx = g[(dir ? G_DRDY : G_DRDX) * npnp + pt];
+  const double sx = g[(dir ? G_DSDY : G_DSDX) * npnp + pt];
+  const double d = B * sx + A * rx;
+  if (MODE == 0) N[off] -= b[off] * d;
+  else N[off] -= d;
+}
+
 // z-derivative of Fourier planes: mode k Re <- -beta k Im, Im <- beta k Re;
 // mode 0 and the Nyquist plane become zero.  src may equal dst.
 __global__ void k_grad_z(const double* src, double* dst, int nmodes, int planeSize, double beta) {
@@ -450,6 +502,25 @@ int dft_launch(const double* in, double* out, int N, int ntrn, int sign, hipStre
   return SEMX_OK;
 }
 
+// N -= b d(a)/dx_dir (mode 0) or N -= d(a b)/dx_dir (mode 1), dir 0 or 1
+int grad_nl_launch(Mesh& M, int nz, double* N, const double* a, const double* b, int dir,
+                   int mode, hipStream_t s) {
+  hipEvent_t ev;
+  const int ce = OP_THREADS / M.npnp;
+  dim3 g((M.nel + ce - 1) / ce, nz);
+  const size_t lds = (2 + (size_t)ce) * M.npnp * sizeof(double);
+  prof_begin(s, &ev);
+  if (mode == 0)
+    hipLaunchKernelGGL(k_grad_nl<0>, g, dim3(OP_THREADS), lds, s, N, a, b, M.d.G, M.d.eset,
+                       M.d.ops, M.np, M.nel, M.planeSize, dir);
+  else
+    hipLaunchKernelGGL(k_grad_nl<1>, g, dim3(OP_THREADS), lds, s, N, a, b, M.d.G, M.d.eset,
+                       M.d.ops, M.np, M.nel, M.planeSize, dir);
+  prof_end(s, "k_grad_nl", ev);
+  HIPCHK(hipGetLastError());
+  return SEMX_OK;
+}
+
 int grad_launch(Mesh& M, int nz, double beta, double* data, int dir, hipStream_t s) {
   hipEvent_t ev;
   if (dir < 2) {
@@ -573,18 +644,21 @@ int dns_nonlinear(semx_dns_s& D, hipStream_t s) {
                              M.planeSize, D.beta);
           CHK(ew_launch_check());
           CHK(dns_transform(D, D.tmp, -1, s));
-        } else {
-          EW(k_copy, D.tmp, D.U[i]);
-          CHK(grad_launch(M, D.nz, D.beta, D.tmp, j, s));
+          EW(k_times_minus, N, D.U[j], D.tmp);
+        } else {   // fused: no copy of u_i, the product in the epilogue
+          CHK(grad_nl_launch(M, D.nz, N, D.U[i], D.U[j], j, 0, s));
         }
-        EW(k_times_minus, N, D.U[j], D.tmp);
       }
       if (cons) {   // n_i -= d(u_i u_j)/dx_j
-        EW(k_times, D.tmp, D.U[i], D.U[j]);
-        if (j == 2) CHK(dns_transform(D, D.tmp, +1, s));
-        CHK(grad_launch(M, D.nz, D.beta, D.tmp, j, s));
-        if (j == 2) CHK(dns_transform(D, D.tmp, -1, s));
-        EW(k_sub, N, D.tmp);
+        if (j == 2) {
+          EW(k_times, D.tmp, D.U[i], D.U[j]);
+          CHK(dns_transform(D, D.tmp, +1, s));
+          CHK(grad_launch(M, D.nz, D.beta, D.tmp, j, s));
+          CHK(dns_transform(D, D.tmp, -1, s));
+          EW(k_sub, N, D.tmp);
+        } else {   // fused: the product formed on load, the difference in the epilogue
+          CHK(grad_nl_launch(M, D.nz, N, D.U[i], D.U[j], j, 1, s));
+        }
       }
     }
     if (D.advection == 0) EW(k_scale, N, 0.5);
