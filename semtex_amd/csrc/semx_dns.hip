@@ -210,13 +210,16 @@ __global__ __launch_bounds__(OP_THREADS) void k_grad_xy(double* __restrict__ dat
   u[pt] = B * sx + A * rx;
 }
 
-// One x or y term of the nonlinear forcing, fused (dns/nonlinear.cpp:
-// 36-211): MODE 0 (convective) N -= b * d(a)/dx_dir, MODE 1 (conservative)
-// N -= d(a b)/dx_dir, the derivative by the same element operator as
-// k_grad_xy, read out of place so the operand needs no copy, and applied in
-// the epilogue.  Entries of a plane beyond its elements (the padding up to
-// planeSize) take the values the unfused sequence gives them:
-// N -= a b.
+// An x or y derivative (the element operator of k_grad_xy) read out of
+// place, so the operand needs no copy, with its use applied in the
+// epilogue:
+//   MODE 0  N -= b * d(a)/dx_dir     convective term (dns/nonlinear.cpp:36-211)
+//   MODE 1  N -= d(a b)/dx_dir       conservative term
+//   MODE 2  N  = d(a)/dx_dir         setPForce (dns/integrate.cpp:159-160)
+//   MODE 3  N += c * d(a)/dx_dir     project (dns/integrate.cpp:174-186)
+// Entries of a plane beyond its elements (the padding up to planeSize) take
+// the values the unfused copy-derivative-update sequence gives them (the
+// derivative leaves them as they were): N -= a b, N -= a b, N = a, N += c a.
 template <int MODE>
 __global__ __launch_bounds__(OP_THREADS) void k_grad_nl(double* __restrict__ N,
                                                         const double* __restrict__ a,
@@ -224,7 +227,8 @@ __global__ __launch_bounds__(OP_THREADS) void k_grad_nl(double* __restrict__ N,
                                                         const double* __restrict__ G,
                                                         const int* __restrict__ eset,
                                                         const double* __restrict__ ops, int np,
-                                                        int nel, int planeSize, int dir) {
+                                                        int nel, int planeSize, int dir,
+                                                        double c) {
   extern __shared__ double gsh[];
   const int npnp = np * np;
   const int ce = OP_THREADS / npnp;
@@ -241,10 +245,14 @@ __global__ __launch_bounds__(OP_THREADS) void k_grad_nl(double* __restrict__ N,
   }
   const size_t pbase = (size_t)blockIdx.y * planeSize;
   if (blockIdx.x == 0)
-    for (int k = nel * npnp + t; k < planeSize; k += OP_THREADS)
-      N[pbase + k] -= a[pbase + k] * b[pbase + k];
+    for (int k = nel * npnp + t; k < planeSize; k += OP_THREADS) {
+      const size_t q = pbase + k;
+      if (MODE <= 1) N[q] -= a[q] * b[q];
+      else if (MODE == 2) N[q] = a[q];
+      else N[q] += c * a[q];
+    }
   const size_t off = pbase + (size_t)e * npnp + pt;
-  if (act) sU[el * npnp + pt] = MODE == 0 ? a[off] : a[off] * b[off];
+  if (act) sU[el * npnp + pt] = MODE == 1 ? a[off] * b[off] : a[off];
   __syncthreads();
   if (!act) return;
   const int i = pt / np, j = pt - (pt / np) * np;
@@ -259,7 +267,9 @@ __global__ __launch_bounds__(OP_THREADS) void k_grad_nl(double* __restrict__ N,
   const double sx = g[(dir ? G_DSDY : G_DSDX) * npnp + pt];
   const double d = B * sx + A * rx;
   if (MODE == 0) N[off] -= b[off] * d;
-  else N[off] -= d;
+  else if (MODE == 1) N[off] -= d;
+  else if (MODE == 2) N[off] = d;
+  else N[off] += c * d;
 }
 
 // z-derivative of Fourier planes: mode k Re <- -beta k Im, Im <- beta k Re;
@@ -502,20 +512,24 @@ int dft_launch(const double* in, double* out, int N, int ntrn, int sign, hipStre
   return SEMX_OK;
 }
 
-// N -= b d(a)/dx_dir (mode 0) or N -= d(a b)/dx_dir (mode 1), dir 0 or 1
+// The fused derivative of k_grad_nl (modes 0-3 there), dir 0 or 1
 int grad_nl_launch(Mesh& M, int nz, double* N, const double* a, const double* b, int dir,
-                   int mode, hipStream_t s) {
+                   int mode, hipStream_t s, double c = 0.0) {
   hipEvent_t ev;
   const int ce = OP_THREADS / M.npnp;
   dim3 g((M.nel + ce - 1) / ce, nz);
   const size_t lds = (2 + (size_t)ce) * M.npnp * sizeof(double);
   prof_begin(s, &ev);
-  if (mode == 0)
-    hipLaunchKernelGGL(k_grad_nl<0>, g, dim3(OP_THREADS), lds, s, N, a, b, M.d.G, M.d.eset,
-                       M.d.ops, M.np, M.nel, M.planeSize, dir);
-  else
-    hipLaunchKernelGGL(k_grad_nl<1>, g, dim3(OP_THREADS), lds, s, N, a, b, M.d.G, M.d.eset,
-                       M.d.ops, M.np, M.nel, M.planeSize, dir);
+#define SEMX_GNL(MODE_)                                                                     \
+  hipLaunchKernelGGL(k_grad_nl<MODE_>, g, dim3(OP_THREADS), lds, s, N, a, b, M.d.G, M.d.eset, \
+                     M.d.ops, M.np, M.nel, M.planeSize, dir, c)
+  switch (mode) {
+    case 0: SEMX_GNL(0); break;
+    case 1: SEMX_GNL(1); break;
+    case 2: SEMX_GNL(2); break;
+    default: SEMX_GNL(3); break;
+  }
+#undef SEMX_GNL
   prof_end(s, "k_grad_nl", ev);
   HIPCHK(hipGetLastError());
   return SEMX_OK;
@@ -707,10 +721,16 @@ int dns_step(semx_dns_s& D, hipStream_t s) {
   // swapData (U[i], Us[0][i]) and roll of the forcing levels
   for (int i = 0; i < ncom; ++i) std::swap(D.U[i], D.Us[0][i]);
   roll(D.Uf, nord, ncom);
-  // setPForce
+  // setPForce: Uf[0][i] = d(Us[0][i])/dx_i, read out of place
   for (int i = 0; i < D.ndim; ++i) {
-    EW(k_copy, D.Uf[0][i], D.Us[0][i]);
-    CHK(grad_launch(M, D.nz, D.beta, D.Uf[0][i], i, s));
+    if (i < 2) {
+      CHK(grad_nl_launch(M, D.nz, D.Uf[0][i], D.Us[0][i], nullptr, i, 2, s));
+    } else {
+      dim3 g((M.planeSize + 255) / 256, D.nz / 2);
+      hipLaunchKernelGGL(k_grad_z, g, dim3(256), 0, s, D.Us[0][i], D.Uf[0][i], D.nz / 2,
+                         M.planeSize, D.beta);
+      CHK(ew_launch_check());
+    }
   }
   EW(k_divergence_sum, D.Uf[0][0], D.Uf[0][1], D.ndim > 2 ? D.Uf[0][2] : nullptr, D.dt);
   // pressure solve
@@ -722,10 +742,16 @@ int dns_step(semx_dns_s& D, hipStream_t s) {
       std::swap(D.Us[0][i], D.Uf[0][i]);
       EW(k_scale, D.Uf[0][i], a);
     }
-    for (int i = 0; i < D.ndim; ++i) {
-      EW(k_copy, D.Us[0][0], D.P);
-      CHK(grad_launch(M, D.nz, D.beta, D.Us[0][0], i, s));
-      EW(k_axpy, D.Uf[0][i], b, D.Us[0][0]);
+    for (int i = 0; i < D.ndim; ++i) {   // Uf[0][i] += b dP/dx_i
+      if (i < 2) {
+        CHK(grad_nl_launch(M, D.nz, D.Uf[0][i], D.P, nullptr, i, 3, s, b));
+      } else {
+        dim3 g((M.planeSize + 255) / 256, D.nz / 2);
+        hipLaunchKernelGGL(k_grad_z, g, dim3(256), 0, s, D.P, D.Us[0][0], D.nz / 2,
+                           M.planeSize, D.beta);
+        CHK(ew_launch_check());
+        EW(k_axpy, D.Uf[0][i], b, D.Us[0][0]);
+      }
     }
   }
   for (int i = 0; i < ncom; ++i) EW(k_copy, D.Us[0][i], D.U[i]);
