@@ -1222,12 +1222,12 @@ __device__ bool xcoop_sweep(const CoopArgs& a, int md, int lane, const int (&cl)
   const bool stw = tid >= VT && tid < ST;     // store wave
   const int sl = tid - VT;                    // lane in the store wave
 
-  double pp[RPT][NB];
+  double pp[COOP_PD][RPT][NB];   // panel rows, COOP_PD steps ahead
   double bv[COOP_PD], hv[COOP_PD][COOP_WMAX - 1];
   auto block_of = [&](int s) { return FWD ? s : nblk - 1 - s; };
   // task waves: the owned rows of one block's panel, 16-byte loads, rows
   // clamped (every task thread issues the same loads on every step)
-  auto load_panel = [&](int s) {
+  auto load_panel = [&](int s, int u) {
     if (tt < 0) return;                       // wave-uniform
     const int sc = min(s, max(last, 0));
     const double2* pj = reinterpret_cast<const double2*>(P + (size_t)block_of(sc) * NB * ldx);
@@ -1238,8 +1238,8 @@ __device__ bool xcoop_sweep(const CoopArgs& a, int md, int lane, const int (&cl)
 #pragma unroll
       for (int k2 = 0; k2 < NB / 2; ++k2) {
         const double2 v = pj[(size_t)k2 * ldx + prow];
-        pp[q][2 * k2] = v.x;
-        pp[q][2 * k2 + 1] = v.y;
+        pp[u][q][2 * k2] = v.x;
+        pp[u][q][2 * k2 + 1] = v.y;
       }
     }
   };
@@ -1289,7 +1289,7 @@ __device__ bool xcoop_sweep(const CoopArgs& a, int md, int lane, const int (&cl)
 #pragma unroll
       for (int q = 0; q < RPT; ++q)
 #pragma unroll
-        for (int k = 0; k < NB; ++k) asm volatile("" ::"v"(pp[q][k]));
+        for (int k = 0; k < NB; ++k) asm volatile("" ::"v"(pp[u][q][k]));
     }
     // A
     if (lead) {
@@ -1330,8 +1330,8 @@ __device__ bool xcoop_sweep(const CoopArgs& a, int md, int lane, const int (&cl)
 #pragma unroll
             for (int k2 = 0; k2 < NB / 2; ++k2) {
               const double2 w2 = wv[k2];
-              a0 += pp[q][2 * k2] * w2.x;
-              a1 += pp[q][2 * k2 + 1] * w2.y;
+              a0 += pp[u][q][2 * k2] * w2.x;
+              a1 += pp[u][q][2 * k2 + 1] * w2.y;
             }
             const double v = a0 + a1;
             if (ur < 0) ybuf[k * NB + prow] = v;
@@ -1341,15 +1341,17 @@ __device__ bool xcoop_sweep(const CoopArgs& a, int md, int lane, const int (&cl)
       }
     }
     __syncthreads();
-    load_panel(s + 1);
+    load_panel(s + COOP_PD, u);
     fetch(s + COOP_PD, u);
     return true;
   };
 
   __syncthreads();
-  load_panel(0);
 #pragma unroll
-  for (int u = 0; u < COOP_PD; ++u) fetch(u, u);
+  for (int u = 0; u < COOP_PD; ++u) {
+    load_panel(u, u);
+    fetch(u, u);
+  }
   // whole groups of COOP_PD steps with no exit inside a group, then the rest
   int s0 = 0;
   for (; s0 + COOP_PD - 1 <= last; s0 += COOP_PD) {
