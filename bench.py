@@ -170,6 +170,24 @@ def main():
     progress(f"direct solve {ms:.4f} ms")
     gms, gsame = graph_timed(S, args.steps)
     progress(f"direct solve as a HIP graph replay {gms:.4f} ms (equal to eager: {gsame})")
+    # three fields sharing the system (u, v, w of a DNS step): one call
+    fs3 = [tf.clone() for _ in range(3)]
+    us3 = [torch.zeros_like(tu) for _ in range(3)]
+    for _ in range(3):
+        sx.solve_fields(S, fs3, [tb] * 3, us3, iters=False)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        for u3 in us3:
+            u3.zero_()
+        sx.solve_fields(S, fs3, [tb] * 3, us3, iters=False)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    f3ms = e0.elapsed_time(e1) / args.steps
+    f3same = all(bool(torch.equal(u3, us3[0])) for u3 in us3)
+    progress(f"three fields in one call {f3ms:.4f} ms")
     u, _ = fresh_solve(S)
     err_exact = float(np.abs(u[:, :P.nPlane] - ue[:, :P.nPlane]).max())
 
@@ -225,6 +243,10 @@ def main():
         },
         "kernel_ms": kprof,
         "max_abs_err_vs_exact": err_exact,
+        "fields3": {"ms_per_step": f3ms, "value": 3 * npts * nplanes / (f3ms * 1e-3),
+                    "fields_equal": f3same,
+                    "what": "three fields with the same forcing solved by one solve_fields "
+                            "call (Field::solve of u, v, w on one ModalMatrixSys)"},
         "graph": {"ms_per_step": gms, "value": npts * nplanes / (gms * 1e-3),
                   "equals_eager": gsame,
                   "what": "the same solve (zero fill + launches) captured once as a HIP graph "

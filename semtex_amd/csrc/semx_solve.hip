@@ -2331,13 +2331,61 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
 #undef SEMX_RPICKD
 #undef SEMX_RPICK
         if (rfn) {
-          kfn = rfn;
-          ba.ring = npadmax;
-          lds_x = lds_res;
-          bt = (unsigned)rbt;
-          if (rnr != nr) {
-            nr = rnr;
-            launches = nf;
+          // two candidates: the resident sweep (rnr right-hand sides, one
+          // launch per field when rnr < nr) and the ring-window sweep (all
+          // nr in one launch).  Which streams the factor faster depends on
+          // the band, the field count and the part, so the first call for
+          // each field count times both (SEMX_AUTOTUNE=0: resident).
+          const int r_launches = (rnr != nr) ? nf : launches;
+          if (H.band_pick[nf] == 0) {
+            const char* at = getenv("SEMX_AUTOTUNE");
+            if (at && atoi(at) == 0) {
+              H.band_pick[nf] = 1;
+            } else {
+              double* tune = nullptr;
+              int st2 = dev_upload<double>(&tune, nullptr, (size_t)nf * psol);
+              if (st2) return st2;
+              float best[3] = {0.f, 0.f, 0.f};
+              hipEvent_t e0, e1;
+              HIPCHK(hipEventCreate(&e0));
+              HIPCHK(hipEventCreate(&e1));
+              for (int cand = 1; cand <= 2; ++cand) {
+                const void* fn = cand == 1 ? rfn : kfn;
+                const size_t cl = cand == 1 ? lds_res : lds_x;
+                const unsigned cbt = cand == 1 ? (unsigned)rbt : bt;
+                const int cn = cand == 1 ? r_launches : launches;
+                BandArgs bc = ba;
+                bc.ring = cand == 1 ? npadmax : ba.ring;
+                HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)cl));
+                for (int rep = 0; rep < 3; ++rep) {
+                  if (rep == 1) HIPCHK(hipEventRecord(e0, s));
+                  for (int l = 0; l < cn; ++l) {
+                    bc.sol = tune + (cn > 1 ? l * psol : 0);
+                    void* kargs[] = {&bc};
+                    HIPCHK(hipLaunchKernel(fn, dim3((unsigned)H.dmode.size()), dim3(cbt), kargs,
+                                           cl, s));
+                  }
+                }
+                HIPCHK(hipEventRecord(e1, s));
+                HIPCHK(hipEventSynchronize(e1));
+                HIPCHK(hipEventElapsedTime(&best[cand], e0, e1));
+              }
+              HIPCHK(hipEventDestroy(e0));
+              HIPCHK(hipEventDestroy(e1));
+              dev_free(tune);
+              H.band_pick[nf] = best[1] <= best[2] ? 1 : 2;
+            }
+          }
+          if (H.band_pick[nf] == 1) {
+            kfn = rfn;
+            ba.ring = npadmax;
+            lds_x = lds_res;
+            bt = (unsigned)rbt;
+            if (rnr != nr) {
+              nr = rnr;
+              launches = nf;
+            }
           }
         }
       }
