@@ -1693,7 +1693,124 @@ __device__ double pcg_matvec(const PcgOp& a, int n, const T* __restrict__ x,
 
 // PCG iterations on A x = r (r holds the residual of the initial x and is
 // updated in place) until r.r <= epsb2 or the step limit; semtex's loop.
-template <typename T>
+// The same operator with each wavefront working through its own elements
+// (e = wave, wave + 16, ...) in a private LDS slice with wavefront-level
+// synchronisation only, the next element's values already in registers
+// while the current one is computed; one block barrier then separates the
+// element phase from the owner-gather assembly.  Fewer block barriers, at
+// the cost of idle lanes when np^2 is not a multiple of 64: faster for the
+// small, many-iteration problems of the batch kernel, slower for one large
+// system per workgroup.  LDS: (2 + 3 PCG_WAVES) np^2 values of T.
+constexpr int PCG_WAVES = PCG_THREADS / 64;
+constexpr int PCG_NPT = 3;                    // nodes per lane (np <= 13)
+
+template <typename T, bool MASK = true>
+__device__ double pcg_matvec_wave(const PcgOp& a, int n, const T* __restrict__ x,
+                                  T* __restrict__ y, T* __restrict__ lb,
+                                  const T* __restrict__ w, T* sh, double* red) {
+  const int tid = threadIdx.x, np = a.np, npnp = a.npnp, next = a.next, nint = a.nint;
+  const int lane = tid & 63, wave = tid >> 6;
+  const T hcon = (T)a.hcon;
+  const bool svv = a.svv;
+  T* sD = sh;
+  T* sT = sD + npnp;
+  T* sU = sT + npnp + wave * 3 * npnp;
+  T* sR = sU + npnp;
+  T* sS = sR + npnp;
+  __syncthreads();   // the previous use of the shared operators is complete
+  if (tid < npnp) {
+    sD[tid] = (T)a.ops[(svv ? 2 : 0) * npnp + tid];
+    sT[tid] = (T)a.ops[(svv ? 3 : 1) * npnp + tid];
+  }
+  __syncthreads();
+  // this lane's nodes: pt = lane + 64 k
+  auto gather = [&](int e, T (&v)[PCG_NPT]) {
+    const int ec = min(e, a.nel - 1);
+#pragma unroll
+    for (int k = 0; k < PCG_NPT; ++k) {
+      const int pmk = a.pmap[min(lane + 64 * k, npnp - 1)];
+      if (pmk < next) {
+        const int gid = a.btog[ec * next + pmk];
+        const T xv = x[gid];
+        v[k] = (!MASK || gid < n) ? xv : (T)0;
+      } else {
+        v[k] = x[a.nglobal + ec * nint + (pmk - next)];
+      }
+    }
+  };
+  double wy = 0.0;
+  T cur[PCG_NPT], nxt[PCG_NPT];
+  gather(wave, cur);
+  for (int e = wave; e < a.nel; e += PCG_WAVES) {
+    gather(e + PCG_WAVES, nxt);
+#pragma unroll
+    for (int k = 0; k < PCG_NPT; ++k)
+      if (lane + 64 * k < npnp) sU[lane + 64 * k] = cur[k];
+    wave_sync();
+    const double* g = a.G + (size_t)a.eset[e] * G_N * npnp;
+#pragma unroll 1
+    for (int k = 0; k < PCG_NPT; ++k) {
+      const int pt = lane + 64 * k;
+      if (pt < npnp) {
+        const int i = pt / np, j = pt - (pt / np) * np;
+        T r = 0, s2 = 0;
+        for (int q = 0; q < np; ++q) {
+          r += sU[i * np + q] * sT[q * np + j];
+          s2 += sD[i * np + q] * sU[q * np + j];
+        }
+        const T q1 = (T)g[G_Q1 * npnp + pt], q2 = (T)g[G_Q2 * npnp + pt];
+        const T q3 = (T)g[G_Q3 * npnp + pt];
+        sR[pt] = q1 * r + q3 * s2;
+        sS[pt] = q2 * s2 + q3 * r;
+      }
+    }
+    wave_sync();
+#pragma unroll 1
+    for (int k = 0; k < PCG_NPT; ++k) {
+      const int pt = lane + 64 * k;
+      if (pt < npnp) {
+        const int i = pt / np, j = pt - (pt / np) * np;
+        const int pmk = a.pmap[pt];
+        T out = (T)g[G_Q4 * npnp + pt] * sU[pt] * hcon;
+        T s1 = 0, s3 = 0;
+        for (int q = 0; q < np; ++q) {
+          s1 += sT[i * np + q] * sS[q * np + j];
+          s3 += sR[i * np + q] * sD[q * np + j];
+        }
+        out += s1;
+        out += s3;
+        if (pmk < next) {
+          lb[e * next + pmk] = out;
+        } else {
+          const int idx = a.nglobal + e * nint + (pmk - next);
+          y[idx] = out;
+          wy += (double)w[idx] * (double)out;
+        }
+      }
+    }
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < PCG_NPT; ++k) cur[k] = nxt[k];
+  }
+  __syncthreads();
+  for (int gI = tid; gI < a.nglobal; gI += PCG_THREADS) {
+    T v = 0;
+    if (!MASK || gI < n)
+      for (int c = a.asm_ptr[gI]; c < a.asm_ptr[gI + 1]; ++c) v += lb[a.asm_idx[c]];
+    y[gI] = v;
+    wy += (double)w[gI] * (double)v;
+  }
+  return block_sum(wy, red);
+}
+
+template <typename T, bool MASK, bool WV>
+__device__ inline double pcg_op(const PcgOp& a, int n, const T* x, T* y, T* lb, const T* w,
+                                T* sh, double* red) {
+  if constexpr (WV) return pcg_matvec_wave<T, MASK>(a, n, x, y, lb, w, sh, red);
+  else return pcg_matvec<T, MASK>(a, n, x, y, lb, w, sh, red);
+}
+
+template <typename T, bool WV = false>
 __device__ int pcg_core(const PcgOp& a, int n, T* x, T* r, T* p, T* q, T* lb, const T* pc,
                         double epsb2, int step_max, T* sh, double* red) {
   const int tid = threadIdx.x, npts = a.npts;
@@ -1712,7 +1829,7 @@ __device__ int pcg_core(const PcgOp& a, int n, T* x, T* r, T* p, T* q, T* lb, co
       const T z = pc[i] * r[i];
       p[i] = (it == 1) ? z : z + beta * p[i];
     }
-    const double pq = pcg_matvec<T>(a, n, p, q, lb, p, sh, red);
+    const double pq = pcg_op<T, true, WV>(a, n, p, q, lb, p, sh, red);
     const T alpha = (T)(rho1 / pq);
     double nr2 = 0.0, nrho = 0.0;
     for (int i = tid; i < npts; i += PCG_THREADS) {
@@ -1735,19 +1852,19 @@ __device__ int pcg_core(const PcgOp& a, int n, T* x, T* r, T* p, T* q, T* lb, co
 // PCG on A d = r.  Vectors: X, R, B, P, Q of npts and LB of nel*next
 // doubles; the inner T vectors share the P/Q/LB space.  Returns the
 // iteration count (summed over inner solves), negative at the limit.
-template <typename T, bool REFINE>
+template <typename T, bool REFINE, bool WV = false>
 __device__ int pcg_run(const PcgOp& op, int n, double* X, double* R, const double* B,
                        double* P, double* Q, double* LB, const double* pc, const float* pc32,
                        double epsb2, int step_max, double inner_tol, int max_outer, T* sh,
                        double* red) {
   const int tid = threadIdx.x, npts = op.npts;
   __syncthreads();
-  pcg_matvec<double>(op, n, X, Q, LB, X, (double*)sh, red);
+  pcg_op<double, true, WV>(op, n, X, Q, LB, X, (double*)sh, red);
   for (int i = tid; i < npts; i += PCG_THREADS) R[i] = B[i] - Q[i];
   __syncthreads();
   int its;
   if constexpr (!REFINE) {
-    its = pcg_core<double>(op, n, X, R, P, Q, LB, pc, epsb2, step_max, (double*)sh, red);
+    its = pcg_core<double, WV>(op, n, X, R, P, Q, LB, pc, epsb2, step_max, (double*)sh, red);
   } else {
     T* d = (T*)P;
     T* rr = d + npts;
@@ -1767,12 +1884,12 @@ __device__ int pcg_run(const PcgOp& op, int n, double* X, double* R, const doubl
       }
       __syncthreads();
       const double ein = inner_tol * inner_tol * rn;
-      const int it = pcg_core<T>(op, n, d, rr, pp, qq, lb, (const T*)pc32, ein, step_max, sh,
+      const int it = pcg_core<T, WV>(op, n, d, rr, pp, qq, lb, (const T*)pc32, ein, step_max, sh,
                                  red);
       its += (it < 0) ? -it : it;
       for (int i = tid; i < npts; i += PCG_THREADS) X[i] += (double)d[i];
       __syncthreads();
-      pcg_matvec<double>(op, n, X, Q, LB, X, (double*)sh, red);
+      pcg_op<double, true, WV>(op, n, X, Q, LB, X, (double*)sh, red);
       for (int i = tid; i < npts; i += PCG_THREADS) R[i] = B[i] - Q[i];
       __syncthreads();
     }
@@ -1921,7 +2038,7 @@ __global__ __launch_bounds__(PCG_THREADS) void k_pcg_batch(PcgBatchArgs ba) {
   for (int i = tid; i < npts; i += PCG_THREADS)
     P[i] = (i >= nsolve && i < nglobal) ? bc[I.ess_src[i - nsolve]] : 0.0;
   __syncthreads();
-  pcg_matvec<double, false>(op, n, P, Q, LB, P, (double*)sh, red);
+  pcg_op<double, false, true>(op, n, P, Q, LB, P, (double*)sh, red);
   // b = -(M f) - A g + <h,w>, rows of essential and pinned nodes zero
   double bb = 0.0;
   for (int gI = tid; gI < nglobal; gI += PCG_THREADS) {
@@ -1952,7 +2069,7 @@ __global__ __launch_bounds__(PCG_THREADS) void k_pcg_batch(PcgBatchArgs ba) {
   }
   const double epsb = ba.tol * sqrt(block_sum(bb, red));
   pcg_x0(op, I.emap, n, uk, X);
-  const int its = pcg_run<T, REFINE>(op, n, X, R, B, P, Q, LB, I.pc, I.pc32, epsb * epsb,
+  const int its = pcg_run<T, REFINE, true>(op, n, X, R, B, P, Q, LB, I.pc, I.pc32, epsb * epsb,
                                      ba.step_max, ba.inner_tol, ba.max_outer, sh, red);
   pcg_store(op, n, nsolve, X, uk, [&](int gid) { return bc[I.ess_src[gid - nsolve]]; });
   if (tid == 0) ba.iters[blockIdx.x] = its;
@@ -2765,9 +2882,10 @@ int semx_batch_create(int count, const semx_helm* helms, semx_batch* batch) {
     I.emap = M.d.emap; I.ess_src = M.d.ess_src; I.nat_ptr = M.d.nat_ptr; I.nat_b = M.d.nat_b;
     I.bside = M.d.bside; I.barea = M.d.barea; I.pc = H.d.pc; I.pc32 = H.d.pc32;
     I.work = b->work + off[i];
-    const int cep = PCG_THREADS / M.npnp;
+    // wavefront-level element operator: a private slice per wavefront
+    if (M.np > 13) { batch_free(b); return SEMX_E_UNSUPPORTED; }
     b->lds = std::max(b->lds, 32 * sizeof(double) +
-                      (2 * (size_t)M.npnp + 3 * (size_t)cep * M.npnp) * sizeof(double));
+                      (2 * (size_t)M.npnp + 3 * (size_t)PCG_WAVES * M.npnp) * sizeof(double));
   }
   st = dev_upload(&b->inst, inst.data(), inst.size());
   if (st) { batch_free(b); return st; }
