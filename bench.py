@@ -417,7 +417,7 @@ def dns_bench(args, torch, sx, L):
     torch.cuda.synchronize()
     prof = {}
     for name in ("k_band_solve", "k_condense", "k_assemble", "k_backsub", "k_dft_z",
-                 "k_grad_xy", "k_grad_z", "k_smooth", "dns_nonlinear"):
+                 "k_grad_xy", "k_grad_nl", "k_grad_z", "k_smooth", "dns_nonlinear"):
         tot = C.c_double(); cnt = C.c_longlong()
         L.semx_profile_query(name.encode(), C.byref(tot), C.byref(cnt))
         if cnt.value:
@@ -425,6 +425,11 @@ def dns_bench(args, torch, sx, L):
                           "launches_per_step": cnt.value / args.dns_steps}
     L.semx_profile_enable(0)
     umax = float(u.abs().max().item())
+    # algorithmic HBM bytes of the band sweeps, which bound the step: the
+    # velocity and the pressure system, every mode's factor read by the
+    # forward and the backward sweep (nsolve x nband doubles per mode)
+    band_bytes = 2 * (nz // 2) * 2 * P.nsolve * P.nband * 8
+    band_ms = prof.get("k_band_solve", {}).get("ms_per_step", float("nan"))
     return {
         "metric": "DNS steps/s",
         "value": 1e3 / ms,
@@ -437,6 +442,13 @@ def dns_bench(args, torch, sx, L):
         "steps_total": D.steps,
         "max_abs_u_fourier": umax,
         "kernel_ms": prof,
+        "band_hbm": {"bytes_per_step": band_bytes,
+                     "achieved_gb_s": band_bytes / (band_ms * 1e-3) / 1e9,
+                     "frac_of_peak_in_sweeps": band_bytes / (band_ms * 1e-3) / 8e12,
+                     "frac_of_peak_whole_step": band_bytes / (ms * 1e-3) / 8e12,
+                     "what": "algorithmic band-factor bytes (2 systems x modes x 2 sweeps x "
+                             "nsolve x nband x 8 B) over the sweeps' time and over the whole "
+                             "step (field traffic not counted)"},
     }
 
 

@@ -650,6 +650,7 @@ int dns_nonlinear(semx_dns_s& D, hipStream_t s) {
   const bool cons = D.advection == 0 || (D.advection == 1 && !D.toggle);
   for (int i = 0; i < D.ncom; ++i) {
     double* N = D.Uf[0][i];
+    bool zdefer = false;   // the conservative z term waits in tmp (Fourier space)
     for (int j = 0; j < D.ndim; ++j) {
       if (conv) {   // n_i -= u_j d(u_i)/dx_j
         if (j == 2) {
@@ -665,18 +666,23 @@ int dns_nonlinear(semx_dns_s& D, hipStream_t s) {
       }
       if (cons) {   // n_i -= d(u_i u_j)/dx_j
         if (j == 2) {
+          // d(u_i w)/dz is formed in Fourier space and, the transform being
+          // linear, subtracted there after N's own forward transform below
+          // (one inverse transform fewer than subtracting it in physical
+          // space); tmp holds it until then
           EW(k_times, D.tmp, D.U[i], D.U[j]);
           CHK(dns_transform(D, D.tmp, +1, s));
           CHK(grad_launch(M, D.nz, D.beta, D.tmp, j, s));
-          CHK(dns_transform(D, D.tmp, -1, s));
-          EW(k_sub, N, D.tmp);
+          zdefer = true;
         } else {   // fused: the product formed on load, the difference in the epilogue
           CHK(grad_nl_launch(M, D.nz, N, D.U[i], D.U[j], j, 1, s));
         }
       }
     }
+    const double half = D.advection == 0 ? 0.5 : 1.0;
     if (D.advection == 0) EW(k_scale, N, 0.5);
     CHK(dns_transform(D, D.Uf[0][i], +1, s));
+    if (zdefer) EW(k_axpy, D.Uf[0][i], -half, D.tmp);
     CHK(smooth_launch(M, D.nz, D.Uf[0][i], s));
   }
   if (D.advection == 1) D.toggle = 1 - D.toggle;
