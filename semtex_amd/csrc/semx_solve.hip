@@ -2452,12 +2452,13 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
           // launch per field when rnr < nr) and the ring-window sweep (all
           // nr in one launch).  Which streams the factor faster depends on
           // the band, the field count and the part, so the first call for
-          // each field count times both (SEMX_AUTOTUNE=0: resident).
+          // each field count times both (SEMX_AUTOTUNE=0 / 2 force one).
           const int r_launches = (rnr != nr) ? nf : launches;
           if (H.band_pick[nf] == 0) {
+            // SEMX_AUTOTUNE=0: the resident sweep, =2: the ring sweep
             const char* at = getenv("SEMX_AUTOTUNE");
-            if (at && atoi(at) == 0) {
-              H.band_pick[nf] = 1;
+            if (at && (atoi(at) == 0 || atoi(at) == 2)) {
+              H.band_pick[nf] = atoi(at) == 0 ? 1 : 2;
             } else {
               double* tune = nullptr;
               int st2 = dev_upload<double>(&tune, nullptr, (size_t)nf * psol);

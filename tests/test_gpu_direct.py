@@ -105,3 +105,40 @@ def test_solve_fields_equals_separate_solves(cuda, method, nf):
         d = np.abs(g[:, :P.nPlane] - r[:, :P.nPlane]).max()
         print(f"  max |batched - separate| = {d:.3e}", flush=True)
         assert d <= 1e-14 * np.abs(r).max()
+
+
+@pytest.mark.parametrize("pick", ["0", "2"], ids=["resident", "ring"])
+def test_solve_fields_band_variants(cuda, pick):
+    """Both band sweeps the first-call autotuner chooses between -- one
+    LDS-resident sweep per field, or one ring-window sweep for all fields --
+    return what separate solves return, whichever the timing would pick."""
+    import os
+    import torch
+    old = os.environ.get("SEMX_AUTOTUNE")
+    os.environ["SEMX_AUTOTUNE"] = pick
+    try:
+        print(f"[band variants] SEMX_AUTOTUNE={pick}", flush=True)
+        P, Q, _, _, _ = make_case(6, 7, 8, 3.0, 1.0, 0.2, ("D", "N", "D", "D"))
+        S = sx.ModalMatrixSys(P, 3.0, 1.0, 0, 4, 8, sx.DIRECT)
+        cases = [make_case(6, 7, 8, 3.0, 1.0, 0.2, ("D", "N", "D", "D"), seed=40 + i)
+                 for i in range(3)]
+        dev = torch.device("cuda:0")
+        fs, bcs, us, ref = [], [], [], []
+        for (_, _, f, bc, _) in cases:
+            u1, _ = gpu_solve(torch, P, S, f, bc)
+            ref.append(u1)
+            fs.append(torch.from_numpy(f.ravel().copy()).to(dev))
+            bcs.append(torch.from_numpy(bc.ravel().copy()).to(dev))
+            us.append(torch.zeros(f.size, dtype=torch.float64, device=dev))
+        sx.solve_fields(S, fs, bcs, us)
+        torch.cuda.synchronize()
+        for u, r in zip(us, ref):
+            g = u.cpu().numpy().reshape(r.shape)
+            d = np.abs(g[:, :P.nPlane] - r[:, :P.nPlane]).max()
+            print(f"  max |batched - separate| = {d:.3e}", flush=True)
+            assert d <= 1e-14 * np.abs(r).max()
+    finally:
+        if old is None:
+            os.environ.pop("SEMX_AUTOTUNE", None)
+        else:
+            os.environ["SEMX_AUTOTUNE"] = old
