@@ -22,6 +22,8 @@ CASES = [
     ("periodic-kd607-singular", 16, 9, 4, 0.0, 1.0, 0.0, PERIODIC),
     ("periodic-kd1059", 20, 12, 2, 3.0, 1.0, 0.0, PERIODIC),
     ("uniform32-np11-k0", 32, 11, 1, 0.0, 1.0, 0.0, ("D", "D", "D", "D")),
+    # 3x3 elements of np=7: 441 values per plane, padded to 442
+    ("odd-plane-3d", 3, 7, 8, 2.0, 1.0, 0.2, ("D", "N", "D", "D")),
 ]
 
 

@@ -27,6 +27,8 @@ CASES = [
     ("distorted-3d", 4, 6, 8, 1.0, 1.3, 0.3, ("D", "N", "D", "D")),
     ("uniform-16x16-np9-nz8", 16, 9, 8, 20000.0, 1.0, 0.0, DDDD),
     ("periodic-singular", 8, 7, 4, 0.0, 1.0, 0.0, PERIODIC),
+    # 3x3 elements of np=7: 441 values per plane, padded to 442
+    ("odd-plane-3d", 3, 7, 8, 2.0, 1.0, 0.2, ("D", "N", "D", "D")),
 ]
 IDS = [c[0] for c in CASES]
 
