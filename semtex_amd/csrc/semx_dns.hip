@@ -326,11 +326,6 @@ __global__ void k_times_minus(size_t n, double* __restrict__ t, const double* __
   if (i < n) t[i] -= a[i] * b[i];
 }
 
-__global__ void k_sub(size_t n, double* __restrict__ t, const double* __restrict__ a) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) t[i] -= a[i];
-}
-
 __global__ void k_fill(size_t n, double* __restrict__ t, double a) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) t[i] = a;
