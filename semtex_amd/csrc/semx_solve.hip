@@ -350,7 +350,22 @@ struct BandArgs {
   long long fstride;        // sol offset between fields
   const int* xrows;         // extended panels: live rows per block and direction
   int xblk;                 // block stride of xrows per mode
+  // right-hand sides split over fsplit workgroups per mode (each NR of
+  // them), the workgroups of one mode on one XCD so that their reads of the
+  // same panels share its L2; fsplit = 1: one workgroup per mode
+  int fsplit = 1, ndm = 0;
 };
+
+// mode and right-hand-side group of this workgroup (md < 0: padding block).
+// Dispatch is round-robin over the 8 XCDs, so blocks b and b + 8 k share one.
+__device__ inline void band_block(const BandArgs& a, int& md, int& fsel) {
+  const int b = blockIdx.x;
+  if (a.fsplit <= 1) { md = b; fsel = 0; return; }
+  const int F = a.fsplit;
+  md = (b / (8 * F)) * 8 + (b & 7);
+  fsel = (b >> 3) % F;
+  if (md >= a.ndm) md = -1;
+}
 
 template <int G>
 __device__ inline double group_sum(double v) {
@@ -807,7 +822,10 @@ __device__ inline long long uni64(long long v) {
 template <int NB, int NPASS, int BT, int NR, int D>
 __global__ __launch_bounds__(BT) void k_band_ext(BandArgs a) {
   extern __shared__ double lds[];
-  const int m = uni(a.dmode[blockIdx.x]);
+  int md, fsel;
+  band_block(a, md, fsel);
+  if (md < 0) return;
+  const int m = uni(a.dmode[md]);
   const int n = uni(a.mode_n[m]);
   const long long offp = uni64(a.offP[m]);
   const int nblk = (n + NB - 1) / NB;
@@ -815,7 +833,8 @@ __global__ __launch_bounds__(BT) void k_band_ext(BandArgs a) {
   double* gy[NR];
 #pragma unroll
   for (int c = 0; c < NR; ++c) {
-    const int fld = c / a.npl, kk = c - fld * a.npl;
+    const int cg = fsel * NR + c;             // right-hand side across the split
+    const int fld = cg / a.npl, kk = cg - fld * a.npl;
     int p = a.mode_plane[2 * m + kk];
     if (p < 0) p = a.mode_plane[2 * m];
     p = uni(p);
@@ -834,7 +853,10 @@ __global__ __launch_bounds__(BT) void k_band_ext(BandArgs a) {
 template <int BT, int NR, int D>
 __global__ __launch_bounds__(BT) void k_band_res(BandArgs a) {
   extern __shared__ double lds[];
-  const int m = uni(a.dmode[blockIdx.x]);
+  int md, fsel;
+  band_block(a, md, fsel);
+  if (md < 0) return;
+  const int m = uni(a.dmode[md]);
   const int n = uni(a.mode_n[m]);
   const long long offp = uni64(a.offP[m]);
   const int nblk = (n + XNB - 1) / XNB;
@@ -843,7 +865,8 @@ __global__ __launch_bounds__(BT) void k_band_res(BandArgs a) {
   double* gy[NR];
 #pragma unroll
   for (int c = 0; c < NR; ++c) {
-    const int fld = c / a.npl, kk = c - fld * a.npl;
+    const int cg = fsel * NR + c;             // right-hand side across the split
+    const int fld = cg / a.npl, kk = cg - fld * a.npl;
     int p = a.mode_plane[2 * m + kk];
     if (p < 0) p = a.mode_plane[2 * m];
     p = uni(p);
@@ -2405,108 +2428,118 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
     const void* kfn = nullptr;
     unsigned bt = (unsigned)H.bt;
     size_t lds_x = lds;
+    unsigned grid = (unsigned)H.dmode.size();
     if (H.band_ext) {
-      // extended panels: ldx rows, one thread per row up to 512 threads,
-      // two beyond (16-column blocks for bands wider than 512)
+      // Extended panels.  Candidate sweeps:
+      //   1  LDS-resident right-hand sides (32-column blocks, two threads per
+      //      panel row), one workgroup per mode and field (split)
+      //   2  LDS ring window, all fields in one workgroup per mode
+      //   3  LDS ring window, one workgroup per mode and field (split)
+      // A split puts a mode's workgroups on one XCD, so they read the same
+      // panels through its L2.  The first call for a field count times the
+      // candidates and keeps the fastest (SEMX_AUTOTUNE=0 / 2 / 3: 1 / 2 / 3).
       ba.kdp = H.ldx;
+      ba.ndm = (int)H.dmode.size();
       const int xbt = std::min(512, (H.ldx + 63) / 64 * 64);
       const int xnp = (H.ldx + xbt - 1) / xbt;
-      bt = (unsigned)xbt;
-      lds_x = (size_t)nr * (ba.ring + H.nb) * sizeof(double) +
-              ((size_t)2 * H.xblk * sizeof(int) + 7) / 8 * 8;
+      const size_t xr_b = ((size_t)2 * H.xblk * sizeof(int) + 7) / 8 * 8;
+      auto ext_fn = [&](int r) -> const void* {
+        const void* f = nullptr;
 #define SEMX_XPICK(NB_, NP_, BT_)                                                \
-      if (H.nb == NB_ && xnp == NP_ && xbt == BT_)                               \
-        kfn = (nr == 6)   ? (const void*)k_band_ext<NB_, NP_, BT_, 6, 1>         \
-              : (nr == 4) ? (const void*)k_band_ext<NB_, NP_, BT_, 4, 1>         \
-              : (nr == 2) ? (const void*)k_band_ext<NB_, NP_, BT_, 2, 1>         \
-                          : (const void*)k_band_ext<NB_, NP_, BT_, 1, 1>;
-      SEMX_XPICK(32, 1, 64) SEMX_XPICK(32, 1, 128) SEMX_XPICK(32, 1, 192)
-      SEMX_XPICK(32, 1, 256) SEMX_XPICK(32, 1, 320) SEMX_XPICK(32, 1, 384)
-      SEMX_XPICK(32, 1, 448) SEMX_XPICK(32, 1, 512) SEMX_XPICK(32, 2, 512)
-      SEMX_XPICK(16, 2, 512)
+        if (H.nb == NB_ && xnp == NP_ && xbt == BT_)                             \
+          f = (r == 6)   ? (const void*)k_band_ext<NB_, NP_, BT_, 6, 1>          \
+              : (r == 4) ? (const void*)k_band_ext<NB_, NP_, BT_, 4, 1>          \
+              : (r == 2) ? (const void*)k_band_ext<NB_, NP_, BT_, 2, 1>          \
+              : (r == 1) ? (const void*)k_band_ext<NB_, NP_, BT_, 1, 1> : nullptr;
+        SEMX_XPICK(32, 1, 64) SEMX_XPICK(32, 1, 128) SEMX_XPICK(32, 1, 192)
+        SEMX_XPICK(32, 1, 256) SEMX_XPICK(32, 1, 320) SEMX_XPICK(32, 1, 384)
+        SEMX_XPICK(32, 1, 448) SEMX_XPICK(32, 1, 512) SEMX_XPICK(32, 2, 512)
+        SEMX_XPICK(16, 2, 512)
 #undef SEMX_XPICK
-      if (!kfn) return SEMX_E_UNSUPPORTED;
-      // the whole right-hand side resident in LDS when it fits (one or two
-      // right-hand sides per launch; two threads per panel row)
+        return f;
+      };
       int npadmax = 0;
       for (int m : H.dmode) npadmax = std::max(npadmax, H.modes[m].nblk * XNB);
       const int rbt = H.nb == XNB ? (2 * H.ldx + 63) / 64 * 64 : 0;
-      const int rnr = nr <= 2 ? nr : npl;
-      const size_t lds_res = (size_t)rnr * npadmax * sizeof(double) +
-                             ((size_t)2 * H.xblk * sizeof(int) + 7) / 8 * 8;
-      if (lds_res <= 160 * 1024 && rbt <= 1024 && rnr <= 2) {
-        const void* rfn = nullptr;
-#define SEMX_RPICK(BT_, D_)                                                      \
-        if (rbt == BT_ && H.band_d == D_)                                        \
-          rfn = (rnr == 2) ? (const void*)k_band_res<BT_, 2, D_>                 \
-                           : (const void*)k_band_res<BT_, 1, D_>;
-#define SEMX_RPICKD(BT_) SEMX_RPICK(BT_, 1)
-        SEMX_RPICKD(128) SEMX_RPICKD(192) SEMX_RPICKD(256) SEMX_RPICKD(320)
-        SEMX_RPICKD(384) SEMX_RPICKD(448) SEMX_RPICKD(512) SEMX_RPICKD(576)
-        SEMX_RPICKD(640) SEMX_RPICKD(704) SEMX_RPICKD(768) SEMX_RPICKD(832)
-        SEMX_RPICKD(896) SEMX_RPICKD(960) SEMX_RPICKD(1024)
-#undef SEMX_RPICKD
+      const void* rfn = nullptr;
+#define SEMX_RPICK(BT_)                                                          \
+      if (rbt == BT_)                                                            \
+        rfn = (npl == 2) ? (const void*)k_band_res<BT_, 2, 1>                    \
+                         : (const void*)k_band_res<BT_, 1, 1>;
+      SEMX_RPICK(128) SEMX_RPICK(192) SEMX_RPICK(256) SEMX_RPICK(320) SEMX_RPICK(384)
+      SEMX_RPICK(448) SEMX_RPICK(512) SEMX_RPICK(576) SEMX_RPICK(640) SEMX_RPICK(704)
+      SEMX_RPICK(768) SEMX_RPICK(832) SEMX_RPICK(896) SEMX_RPICK(960) SEMX_RPICK(1024)
 #undef SEMX_RPICK
-        if (rfn) {
-          // two candidates: the resident sweep (rnr right-hand sides, one
-          // launch per field when rnr < nr) and the ring-window sweep (all
-          // nr in one launch).  Which streams the factor faster depends on
-          // the band, the field count and the part, so the first call for
-          // each field count times both (SEMX_AUTOTUNE=0 / 2 force one).
-          const int r_launches = (rnr != nr) ? nf : launches;
-          if (H.band_pick[nf] == 0) {
-            // SEMX_AUTOTUNE=0: the resident sweep, =2: the ring sweep
-            const char* at = getenv("SEMX_AUTOTUNE");
-            if (at && (atoi(at) == 0 || atoi(at) == 2)) {
-              H.band_pick[nf] = atoi(at) == 0 ? 1 : 2;
-            } else {
-              double* tune = nullptr;
-              int st2 = dev_upload<double>(&tune, nullptr, (size_t)nf * psol);
-              if (st2) return st2;
-              float best[3] = {0.f, 0.f, 0.f};
-              hipEvent_t e0, e1;
-              HIPCHK(hipEventCreate(&e0));
-              HIPCHK(hipEventCreate(&e1));
-              for (int cand = 1; cand <= 2; ++cand) {
-                const void* fn = cand == 1 ? rfn : kfn;
-                const size_t cl = cand == 1 ? lds_res : lds_x;
-                const unsigned cbt = cand == 1 ? (unsigned)rbt : bt;
-                const int cn = cand == 1 ? r_launches : launches;
-                BandArgs bc = ba;
-                bc.ring = cand == 1 ? npadmax : ba.ring;
-                HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)cl));
-                for (int rep = 0; rep < 3; ++rep) {
-                  if (rep == 1) HIPCHK(hipEventRecord(e0, s));
-                  for (int l = 0; l < cn; ++l) {
-                    bc.sol = tune + (cn > 1 ? l * psol : 0);
-                    void* kargs[] = {&bc};
-                    HIPCHK(hipLaunchKernel(fn, dim3((unsigned)H.dmode.size()), dim3(cbt), kargs,
-                                           cl, s));
-                  }
-                }
-                HIPCHK(hipEventRecord(e1, s));
-                HIPCHK(hipEventSynchronize(e1));
-                HIPCHK(hipEventElapsedTime(&best[cand], e0, e1));
-              }
-              HIPCHK(hipEventDestroy(e0));
-              HIPCHK(hipEventDestroy(e1));
-              dev_free(tune);
-              H.band_pick[nf] = best[1] <= best[2] ? 1 : 2;
+      struct Cand { const void* fn; size_t lds; unsigned bt; int F; int ring; };
+      Cand cand[4] = {};
+      const int gpad = (ba.ndm + 7) / 8 * 8;
+      {
+        const size_t l1 = (size_t)npl * npadmax * sizeof(double) + xr_b;
+        if (rfn && l1 <= 160 * 1024) cand[1] = {rfn, l1, (unsigned)rbt, nf, npadmax};
+        const size_t l2 = (size_t)npl * nf * (ba.ring + H.nb) * sizeof(double) + xr_b;
+        const void* f2 = ext_fn(npl * nf);
+        if (f2 && l2 <= 160 * 1024) cand[2] = {f2, l2, (unsigned)xbt, 1, ba.ring};
+        const size_t l3 = (size_t)npl * (ba.ring + H.nb) * sizeof(double) + xr_b;
+        const void* f3 = nf > 1 ? ext_fn(npl) : nullptr;
+        if (f3 && l3 <= 160 * 1024) cand[3] = {f3, l3, (unsigned)xbt, nf, ba.ring};
+      }
+      int navail = 0, only = 0;
+      for (int c = 1; c <= 3; ++c)
+        if (cand[c].fn) { ++navail; only = c; }
+      if (navail == 0) return SEMX_E_UNSUPPORTED;
+      int& pick = H.band_pick[nf];
+      if (pick == 0 || !cand[pick].fn) {
+        const char* at = getenv("SEMX_AUTOTUNE");
+        const int force = at ? atoi(at) : -1;
+        if (force >= 0 && force <= 3) {
+          pick = force == 0 ? 1 : force;
+          if (!cand[pick].fn) pick = only;
+        } else if (navail == 1) {
+          pick = only;
+        } else {
+          // time every candidate on a zeroed copy of the right-hand sides
+          double* tune = nullptr;
+          int st2 = dev_upload<double>(&tune, nullptr, (size_t)nf * psol);
+          if (st2) return st2;
+          hipEvent_t e0, e1;
+          HIPCHK(hipEventCreate(&e0));
+          HIPCHK(hipEventCreate(&e1));
+          float best = 0.f;
+          for (int c = 1; c <= 3; ++c) {
+            if (!cand[c].fn) continue;
+            BandArgs bc = ba;
+            bc.sol = tune;
+            bc.ring = cand[c].ring;
+            bc.fsplit = cand[c].F;
+            const unsigned gc = cand[c].F > 1 ? (unsigned)(gpad * cand[c].F) : (unsigned)bc.ndm;
+            HIPCHK(hipFuncSetAttribute(cand[c].fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)cand[c].lds));
+            for (int rep = 0; rep < 3; ++rep) {
+              if (rep == 1) HIPCHK(hipEventRecord(e0, s));
+              void* kargs[] = {&bc};
+              HIPCHK(hipLaunchKernel(cand[c].fn, dim3(gc), dim3(cand[c].bt), kargs,
+                                     cand[c].lds, s));
             }
+            HIPCHK(hipEventRecord(e1, s));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            if (pick == 0 || ms < best) { pick = c; best = ms; }
           }
-          if (H.band_pick[nf] == 1) {
-            kfn = rfn;
-            ba.ring = npadmax;
-            lds_x = lds_res;
-            bt = (unsigned)rbt;
-            if (rnr != nr) {
-              nr = rnr;
-              launches = nf;
-            }
-          }
+          HIPCHK(hipEventDestroy(e0));
+          HIPCHK(hipEventDestroy(e1));
+          dev_free(tune);
         }
       }
+      const Cand& cc = cand[pick];
+      kfn = cc.fn;
+      lds_x = cc.lds;
+      bt = cc.bt;
+      ba.ring = cc.ring;
+      ba.fsplit = cc.F;
+      grid = cc.F > 1 ? (unsigned)(gpad * cc.F) : (unsigned)ba.ndm;
+      nr = pick == 2 ? npl * nf : npl;
+      launches = 1;
     }
     if (!H.band_ext) {
 #define SEMX_PICK(NB_, NP_, BT_, D_)                                        \
@@ -2577,7 +2610,7 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
       bl.sol = H.d.sol + (launches > 1 ? l * psol : 0);
       void* kargs[] = {&bl};
       prof_begin(s, &ev);
-      HIPCHK(hipLaunchKernel(kfn, dim3((unsigned)H.dmode.size()), dim3(bt), kargs, lds_x, s));
+      HIPCHK(hipLaunchKernel(kfn, dim3(grid), dim3(bt), kargs, lds_x, s));
       prof_end(s, "k_band_solve", ev);
     }
   }

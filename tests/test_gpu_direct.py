@@ -109,11 +109,12 @@ def test_solve_fields_equals_separate_solves(cuda, method, nf):
         assert d <= 1e-14 * np.abs(r).max()
 
 
-@pytest.mark.parametrize("pick", ["0", "2"], ids=["resident", "ring"])
+@pytest.mark.parametrize("pick", ["0", "2", "3"], ids=["resident-split", "ring", "ring-split"])
 def test_solve_fields_band_variants(cuda, pick):
-    """Both band sweeps the first-call autotuner chooses between -- one
-    LDS-resident sweep per field, or one ring-window sweep for all fields --
-    return what separate solves return, whichever the timing would pick."""
+    """Every band sweep the first-call autotuner chooses between -- the
+    LDS-resident sweep with one workgroup per mode and field, one ring-window
+    sweep carrying all fields, or the ring-window sweep split per field --
+    returns what separate solves return, whichever the timing would pick."""
     import os
     import torch
     old = os.environ.get("SEMX_AUTOTUNE")
