@@ -122,8 +122,9 @@ struct Helm {
   int ldx = 0;
   bool xcoop = false;         // cooperative sweep over extended panels
   // band-sweep variant per field count, chosen by timing on first use:
-  // 0 undecided, 1 right-hand side resident in LDS (one launch per field
-  // beyond two right-hand sides), 2 LDS ring window (all fields in one sweep)
+  // 0 undecided, 1 right-hand sides resident in LDS (one workgroup per mode
+  // and field), 2 LDS ring window (all fields in one workgroup per mode),
+  // 3 ring window split per field, 4 / 5 as 1 / 3 split per plane
   int band_pick[SEMX_MAX_FIELDS + 1] = {0};
   // live rows (32 + envelope height) of each extended panel, forward and
   // backward: xrows[(mode*xblk + J)*2 + dir], dir 0 forward

@@ -826,6 +826,19 @@ __global__ __launch_bounds__(BT) void k_band_ext(BandArgs a) {
   band_block(a, md, fsel);
   if (md < 0) return;
   const int m = uni(a.dmode[md]);
+  // a mode without a second plane sweeps its first plane in that place; a
+  // workgroup of a split whose right-hand sides are all such repeats has
+  // nothing of its own to do (and must not touch a plane another
+  // workgroup is solving)
+  if (a.fsplit > 1) {
+    bool own = false;
+#pragma unroll
+    for (int c = 0; c < NR; ++c) {
+      const int cg = fsel * NR + c;
+      own |= a.mode_plane[2 * m + (cg - (cg / a.npl) * a.npl)] >= 0;
+    }
+    if (!uni(own ? 1 : 0)) return;
+  }
   const int n = uni(a.mode_n[m]);
   const long long offp = uni64(a.offP[m]);
   const int nblk = (n + NB - 1) / NB;
@@ -857,6 +870,19 @@ __global__ __launch_bounds__(BT) void k_band_res(BandArgs a) {
   band_block(a, md, fsel);
   if (md < 0) return;
   const int m = uni(a.dmode[md]);
+  // a mode without a second plane sweeps its first plane in that place; a
+  // workgroup of a split whose right-hand sides are all such repeats has
+  // nothing of its own to do (and must not touch a plane another
+  // workgroup is solving)
+  if (a.fsplit > 1) {
+    bool own = false;
+#pragma unroll
+    for (int c = 0; c < NR; ++c) {
+      const int cg = fsel * NR + c;
+      own |= a.mode_plane[2 * m + (cg - (cg / a.npl) * a.npl)] >= 0;
+    }
+    if (!uni(own ? 1 : 0)) return;
+  }
   const int n = uni(a.mode_n[m]);
   const long long offp = uni64(a.offP[m]);
   const int nblk = (n + XNB - 1) / XNB;
@@ -2435,9 +2461,10 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
       //      panel row), one workgroup per mode and field (split)
       //   2  LDS ring window, all fields in one workgroup per mode
       //   3  LDS ring window, one workgroup per mode and field (split)
+      //   4, 5  as 1 and 3 with one workgroup per mode and plane
       // A split puts a mode's workgroups on one XCD, so they read the same
       // panels through its L2.  The first call for a field count times the
-      // candidates and keeps the fastest (SEMX_AUTOTUNE=0 / 2 / 3: 1 / 2 / 3).
+      // candidates and keeps the fastest (SEMX_AUTOTUNE=0 / 2..5 force 1 / 2..5).
       ba.kdp = H.ldx;
       ba.ndm = (int)H.dmode.size();
       const int xbt = std::min(512, (H.ldx + 63) / 64 * 64);
@@ -2471,7 +2498,8 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
       SEMX_RPICK(768) SEMX_RPICK(832) SEMX_RPICK(896) SEMX_RPICK(960) SEMX_RPICK(1024)
 #undef SEMX_RPICK
       struct Cand { const void* fn; size_t lds; unsigned bt; int F; int ring; };
-      Cand cand[4] = {};
+      constexpr int NCAND = 5;
+      Cand cand[NCAND + 1] = {};
       const int gpad = (ba.ndm + 7) / 8 * 8;
       {
         const size_t l1 = (size_t)npl * npadmax * sizeof(double) + xr_b;
@@ -2482,16 +2510,30 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
         const size_t l3 = (size_t)npl * (ba.ring + H.nb) * sizeof(double) + xr_b;
         const void* f3 = nf > 1 ? ext_fn(npl) : nullptr;
         if (f3 && l3 <= 160 * 1024) cand[3] = {f3, l3, (unsigned)xbt, nf, ba.ring};
+        if (npl == 2) {
+          const void* r1 = nullptr;
+#define SEMX_RPICK1(BT_) if (rbt == BT_) r1 = (const void*)k_band_res<BT_, 1, 1>;
+          SEMX_RPICK1(128) SEMX_RPICK1(192) SEMX_RPICK1(256) SEMX_RPICK1(320)
+          SEMX_RPICK1(384) SEMX_RPICK1(448) SEMX_RPICK1(512) SEMX_RPICK1(576)
+          SEMX_RPICK1(640) SEMX_RPICK1(704) SEMX_RPICK1(768) SEMX_RPICK1(832)
+          SEMX_RPICK1(896) SEMX_RPICK1(960) SEMX_RPICK1(1024)
+#undef SEMX_RPICK1
+          const size_t l4 = (size_t)npadmax * sizeof(double) + xr_b;
+          if (r1 && l4 <= 160 * 1024) cand[4] = {r1, l4, (unsigned)rbt, 2 * nf, npadmax};
+          const size_t l5 = (size_t)(ba.ring + H.nb) * sizeof(double) + xr_b;
+          const void* f5 = ext_fn(1);
+          if (f5 && l5 <= 160 * 1024) cand[5] = {f5, l5, (unsigned)xbt, 2 * nf, ba.ring};
+        }
       }
       int navail = 0, only = 0;
-      for (int c = 1; c <= 3; ++c)
+      for (int c = 1; c <= NCAND; ++c)
         if (cand[c].fn) { ++navail; only = c; }
       if (navail == 0) return SEMX_E_UNSUPPORTED;
       int& pick = H.band_pick[nf];
       if (pick == 0 || !cand[pick].fn) {
         const char* at = getenv("SEMX_AUTOTUNE");
         const int force = at ? atoi(at) : -1;
-        if (force >= 0 && force <= 3) {
+        if (force >= 0 && force <= NCAND) {
           pick = force == 0 ? 1 : force;
           if (!cand[pick].fn) pick = only;
         } else if (navail == 1) {
@@ -2505,7 +2547,7 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
           HIPCHK(hipEventCreate(&e0));
           HIPCHK(hipEventCreate(&e1));
           float best = 0.f;
-          for (int c = 1; c <= 3; ++c) {
+          for (int c = 1; c <= NCAND; ++c) {
             if (!cand[c].fn) continue;
             BandArgs bc = ba;
             bc.sol = tune;
@@ -2538,7 +2580,7 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
       ba.ring = cc.ring;
       ba.fsplit = cc.F;
       grid = cc.F > 1 ? (unsigned)(gpad * cc.F) : (unsigned)ba.ndm;
-      nr = pick == 2 ? npl * nf : npl;
+      nr = pick == 2 ? npl * nf : pick >= 4 ? 1 : npl;
       launches = 1;
     }
     if (!H.band_ext) {

@@ -109,7 +109,9 @@ def test_solve_fields_equals_separate_solves(cuda, method, nf):
         assert d <= 1e-14 * np.abs(r).max()
 
 
-@pytest.mark.parametrize("pick", ["0", "2", "3"], ids=["resident-split", "ring", "ring-split"])
+@pytest.mark.parametrize("pick", ["0", "2", "3", "4", "5"],
+                         ids=["resident-split", "ring", "ring-split", "resident-plane-split",
+                              "ring-plane-split"])
 def test_solve_fields_band_variants(cuda, pick):
     """Every band sweep the first-call autotuner chooses between -- the
     LDS-resident sweep with one workgroup per mode and field, one ring-window
