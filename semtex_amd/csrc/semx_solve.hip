@@ -2462,9 +2462,10 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
       //   2  LDS ring window, all fields in one workgroup per mode
       //   3  LDS ring window, one workgroup per mode and field (split)
       //   4, 5  as 1 and 3 with one workgroup per mode and plane
+      //   6  as 5 with one panel row per thread (16-column blocks)
       // A split puts a mode's workgroups on one XCD, so they read the same
       // panels through its L2.  The first call for a field count times the
-      // candidates and keeps the fastest (SEMX_AUTOTUNE=0 / 2..5 force 1 / 2..5).
+      // candidates and keeps the fastest (SEMX_AUTOTUNE=0 / 2..6 force 1 / 2..6).
       ba.kdp = H.ldx;
       ba.ndm = (int)H.dmode.size();
       const int xbt = std::min(512, (H.ldx + 63) / 64 * 64);
@@ -2498,7 +2499,7 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
       SEMX_RPICK(768) SEMX_RPICK(832) SEMX_RPICK(896) SEMX_RPICK(960) SEMX_RPICK(1024)
 #undef SEMX_RPICK
       struct Cand { const void* fn; size_t lds; unsigned bt; int F; int ring; };
-      constexpr int NCAND = 5;
+      constexpr int NCAND = 6;
       Cand cand[NCAND + 1] = {};
       const int gpad = (ba.ndm + 7) / 8 * 8;
       {
@@ -2523,6 +2524,9 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
           const size_t l5 = (size_t)(ba.ring + H.nb) * sizeof(double) + xr_b;
           const void* f5 = ext_fn(1);
           if (f5 && l5 <= 160 * 1024) cand[5] = {f5, l5, (unsigned)xbt, 2 * nf, ba.ring};
+          // one row per thread: 16-column blocks whose panels fit 960 rows
+          if (H.nb == 16 && H.ldx > 896 && H.ldx <= 960 && l5 <= 160 * 1024)
+            cand[6] = {(const void*)k_band_ext<16, 1, 960, 1, 1>, l5, 960u, 2 * nf, ba.ring};
         }
       }
       int navail = 0, only = 0;

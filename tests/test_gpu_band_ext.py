@@ -77,3 +77,29 @@ def test_ext_band_fields_wide(cuda, no_coop, nf):
         d = np.abs(g[:, :P.nPlane] - r[:, :P.nPlane]).max()
         print(f"  max |batched - separate| = {d:.3e}", flush=True)
         assert d <= 1e-14 * np.abs(r).max()
+
+
+@pytest.mark.parametrize("pick", ["2", "3", "5", "6"],
+                         ids=["ring", "ring-field-split", "ring-plane-split", "ring-plane-split-1row"])
+def test_ext_band_dns_width_candidates(cuda, no_coop, pick):
+    """The band of the DNS systems (24x24 periodic quads, np=9: kd = 911,
+    16-column blocks of 928 panel rows) through every ring-window candidate
+    the autotuner chooses between, against the oracle."""
+    import torch
+    old = os.environ.get("SEMX_AUTOTUNE")
+    os.environ["SEMX_AUTOTUNE"] = pick
+    try:
+        print(f"[ext band dns width] SEMX_AUTOTUNE={pick}", flush=True)
+        P, Q, f, bcl, ue = make_case(24, 9, 4, 3.0, 1.0, 0.0, PERIODIC)
+        S = sx.ModalMatrixSys(P, 3.0, 1.0, 0, 2, 4, sx.DIRECT)
+        print(f"  band {S.info['ldab']}", flush=True)
+        ug, _ = gpu_solve(torch, P, S, f, bcl)
+        uo, _ = oracle_solve(Q, 4, 3.0, 1.0, f, bcl)
+        r = rel(ug[:, :P.nPlane], uo[:, :P.nPlane])
+        print(f"  rel L2 gpu vs oracle = {r:.3e}", flush=True)
+        assert r <= 1e-12
+    finally:
+        if old is None:
+            os.environ.pop("SEMX_AUTOTUNE", None)
+        else:
+            os.environ["SEMX_AUTOTUNE"] = old
