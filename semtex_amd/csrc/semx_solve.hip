@@ -2465,7 +2465,7 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
       //   6  as 5 with one panel row per thread (16-column blocks)
       // A split puts a mode's workgroups on one XCD, so they read the same
       // panels through its L2.  The first call for a field count times the
-      // candidates and keeps the fastest (SEMX_AUTOTUNE=0 / 2..6 force 1 / 2..6).
+      // candidates and keeps the fastest (SEMX_AUTOTUNE=0 / 2..7 force 1 / 2..7).
       ba.kdp = H.ldx;
       ba.ndm = (int)H.dmode.size();
       const int xbt = std::min(512, (H.ldx + 63) / 64 * 64);
@@ -2499,7 +2499,7 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
       SEMX_RPICK(768) SEMX_RPICK(832) SEMX_RPICK(896) SEMX_RPICK(960) SEMX_RPICK(1024)
 #undef SEMX_RPICK
       struct Cand { const void* fn; size_t lds; unsigned bt; int F; int ring; };
-      constexpr int NCAND = 6;
+      constexpr int NCAND = 7;
       Cand cand[NCAND + 1] = {};
       const int gpad = (ba.ndm + 7) / 8 * 8;
       {
@@ -2511,6 +2511,9 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
         const size_t l3 = (size_t)npl * (ba.ring + H.nb) * sizeof(double) + xr_b;
         const void* f3 = nf > 1 ? ext_fn(npl) : nullptr;
         if (f3 && l3 <= 160 * 1024) cand[3] = {f3, l3, (unsigned)xbt, nf, ba.ring};
+        // per field with one panel row per thread (16-column blocks, panels <= 960 rows)
+        if (nf > 1 && npl == 2 && H.nb == 16 && H.ldx > 896 && H.ldx <= 960 && l3 <= 160 * 1024)
+          cand[7] = {(const void*)k_band_ext<16, 1, 960, 2, 1>, l3, 960u, nf, ba.ring};
         if (npl == 2) {
           const void* r1 = nullptr;
 #define SEMX_RPICK1(BT_) if (rbt == BT_) r1 = (const void*)k_band_res<BT_, 1, 1>;
@@ -2584,7 +2587,7 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
       ba.ring = cc.ring;
       ba.fsplit = cc.F;
       grid = cc.F > 1 ? (unsigned)(gpad * cc.F) : (unsigned)ba.ndm;
-      nr = pick == 2 ? npl * nf : pick >= 4 ? 1 : npl;
+      nr = pick == 2 ? npl * nf : (pick >= 4 && pick <= 6) ? 1 : npl;
       launches = 1;
     }
     if (!H.band_ext) {

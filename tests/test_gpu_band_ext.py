@@ -79,25 +79,52 @@ def test_ext_band_fields_wide(cuda, no_coop, nf):
         assert d <= 1e-14 * np.abs(r).max()
 
 
-@pytest.mark.parametrize("pick", ["2", "3", "5", "6"],
-                         ids=["ring", "ring-field-split", "ring-plane-split", "ring-plane-split-1row"])
+_DNS_WIDTH_ORACLE = {}
+
+
+def _dns_width_case(seed):
+    """24x24 periodic quads, np=9, Nz=4, and its oracle solution (cached)."""
+    P, Q, f, bcl, _ = make_case(24, 9, 4, 3.0, 1.0, 0.0, PERIODIC, seed=seed)
+    if seed not in _DNS_WIDTH_ORACLE:
+        _DNS_WIDTH_ORACLE[seed] = oracle_solve(Q, 4, 3.0, 1.0, f, bcl)[0]
+    return P, f, bcl, _DNS_WIDTH_ORACLE[seed]
+
+
+@pytest.mark.parametrize("pick", ["2", "3", "5", "6", "7"],
+                         ids=["ring", "ring-field-split", "ring-plane-split", "ring-plane-split-1row",
+                              "ring-field-split-1row"])
 def test_ext_band_dns_width_candidates(cuda, no_coop, pick):
     """The band of the DNS systems (24x24 periodic quads, np=9: kd = 911,
     16-column blocks of 928 panel rows) through every ring-window candidate
-    the autotuner chooses between, against the oracle."""
+    the autotuner chooses between, for one field and for three fields in one
+    call, against the oracle."""
     import torch
     old = os.environ.get("SEMX_AUTOTUNE")
     os.environ["SEMX_AUTOTUNE"] = pick
     try:
         print(f"[ext band dns width] SEMX_AUTOTUNE={pick}", flush=True)
-        P, Q, f, bcl, ue = make_case(24, 9, 4, 3.0, 1.0, 0.0, PERIODIC)
+        P, f, bcl, uo = _dns_width_case(40)
         S = sx.ModalMatrixSys(P, 3.0, 1.0, 0, 2, 4, sx.DIRECT)
         print(f"  band {S.info['ldab']}", flush=True)
         ug, _ = gpu_solve(torch, P, S, f, bcl)
-        uo, _ = oracle_solve(Q, 4, 3.0, 1.0, f, bcl)
         r = rel(ug[:, :P.nPlane], uo[:, :P.nPlane])
-        print(f"  rel L2 gpu vs oracle = {r:.3e}", flush=True)
+        print(f"  one field: rel L2 gpu vs oracle = {r:.3e}", flush=True)
         assert r <= 1e-12
+        dev = torch.device("cuda:0")
+        fs, bcs, us, refs = [], [], [], []
+        for seed in (41, 42, 43):
+            _, f3, bc3, uo3 = _dns_width_case(seed)
+            fs.append(torch.from_numpy(f3.ravel().copy()).to(dev))
+            bcs.append(torch.from_numpy(bc3.ravel().copy()).to(dev))
+            us.append(torch.zeros(f3.size, dtype=torch.float64, device=dev))
+            refs.append(uo3)
+        sx.solve_fields(S, fs, bcs, us)
+        torch.cuda.synchronize()
+        for u, ref in zip(us, refs):
+            g = u.cpu().numpy().reshape(ref.shape)
+            r = rel(g[:, :P.nPlane], ref[:, :P.nPlane])
+            print(f"  three fields: rel L2 gpu vs oracle = {r:.3e}", flush=True)
+            assert r <= 1e-12
     finally:
         if old is None:
             os.environ.pop("SEMX_AUTOTUNE", None)
