@@ -219,7 +219,8 @@ __global__ __launch_bounds__(OP_THREADS) void k_grad_xy(double* __restrict__ dat
 //   MODE 3  N += c * d(a)/dx_dir     project (dns/integrate.cpp:174-186)
 // Entries of a plane beyond its elements (the padding up to planeSize) take
 // the values the unfused copy-derivative-update sequence gives them (the
-// derivative leaves them as they were): N -= a b, N -= a b, N = a, N += c a.
+// derivative leaves them as they were): N -= a b, N -= a b, N = a,
+// N = sc N + c a.
 template <int MODE>
 __global__ __launch_bounds__(OP_THREADS) void k_grad_nl(double* __restrict__ N,
                                                         const double* __restrict__ a,
@@ -228,7 +229,7 @@ __global__ __launch_bounds__(OP_THREADS) void k_grad_nl(double* __restrict__ N,
                                                         const int* __restrict__ eset,
                                                         const double* __restrict__ ops, int np,
                                                         int nel, int planeSize, int dir,
-                                                        double c) {
+                                                        double c, double sc) {
   extern __shared__ double gsh[];
   const int npnp = np * np;
   const int ce = OP_THREADS / npnp;
@@ -249,7 +250,7 @@ __global__ __launch_bounds__(OP_THREADS) void k_grad_nl(double* __restrict__ N,
       const size_t q = pbase + k;
       if (MODE <= 1) N[q] -= a[q] * b[q];
       else if (MODE == 2) N[q] = a[q];
-      else N[q] += c * a[q];
+      else N[q] = sc * N[q] + c * a[q];
     }
   const size_t off = pbase + (size_t)e * npnp + pt;
   if (act) sU[el * npnp + pt] = MODE == 1 ? a[off] * b[off] : a[off];
@@ -269,7 +270,7 @@ __global__ __launch_bounds__(OP_THREADS) void k_grad_nl(double* __restrict__ N,
   if (MODE == 0) N[off] -= b[off] * d;
   else if (MODE == 1) N[off] -= d;
   else if (MODE == 2) N[off] = d;
-  else N[off] += c * d;
+  else N[off] = sc * N[off] + c * d;
 }
 
 // z-derivative of Fourier planes: mode k Re <- -beta k Im, Im <- beta k Re;
@@ -288,6 +289,19 @@ __global__ void k_grad_z(const double* src, double* dst, int nmodes, int planeSi
     dst[re] = -bk * m;
     dst[im] = bk * r;
   }
+}
+
+// dst = sc dst + c d(src)/dz over the Fourier planes (k_grad_z's derivative).
+__global__ void k_grad_z_axpby(const double* __restrict__ src, double* __restrict__ dst,
+                               int nmodes, int planeSize, double beta, double sc, double c) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  if (idx >= (size_t)planeSize || k >= nmodes) return;
+  const size_t re = (size_t)(2 * k) * planeSize + idx, im = re + planeSize;
+  const double bk = beta * k;
+  const double dr = k == 0 ? 0.0 : -bk * src[im], di = k == 0 ? 0.0 : bk * src[re];
+  dst[re] = sc * dst[re] + c * dr;
+  dst[im] = sc * dst[im] + c * di;
 }
 
 // Mass-weighted average of element-boundary values over the copies of each
@@ -344,6 +358,12 @@ __global__ void k_scale(size_t n, double* __restrict__ t, double a) {
 __global__ void k_axpy(size_t n, double* __restrict__ t, double a, const double* __restrict__ x) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) t[i] += a * x[i];
+}
+
+__global__ void k_axpby(size_t n, double* __restrict__ t, double b, double a,
+                        const double* __restrict__ x) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) t[i] = b * t[i] + a * x[i];
 }
 
 struct WaveArgs {
@@ -509,7 +529,7 @@ int dft_launch(const double* in, double* out, int N, int ntrn, int sign, hipStre
 
 // The fused derivative of k_grad_nl (modes 0-3 there), dir 0 or 1
 int grad_nl_launch(Mesh& M, int nz, double* N, const double* a, const double* b, int dir,
-                   int mode, hipStream_t s, double c = 0.0) {
+                   int mode, hipStream_t s, double c = 0.0, double sc = 1.0) {
   hipEvent_t ev;
   const int ce = OP_THREADS / M.npnp;
   dim3 g((M.nel + ce - 1) / ce, nz);
@@ -517,7 +537,7 @@ int grad_nl_launch(Mesh& M, int nz, double* N, const double* a, const double* b,
   prof_begin(s, &ev);
 #define SEMX_GNL(MODE_)                                                                     \
   hipLaunchKernelGGL(k_grad_nl<MODE_>, g, dim3(OP_THREADS), lds, s, N, a, b, M.d.G, M.d.eset, \
-                     M.d.ops, M.np, M.nel, M.planeSize, dir, c)
+                     M.d.ops, M.np, M.nel, M.planeSize, dir, c, sc)
   switch (mode) {
     case 0: SEMX_GNL(0); break;
     case 1: SEMX_GNL(1); break;
@@ -675,9 +695,13 @@ int dns_nonlinear(semx_dns_s& D, hipStream_t s) {
       }
     }
     const double half = D.advection == 0 ? 0.5 : 1.0;
-    if (D.advection == 0) EW(k_scale, N, 0.5);
+    // the skew-symmetric halving, exact in binary and commuting with the
+    // linear transform, rides on the deferred z term when there is one
+    const bool fold = D.advection == 0 && zdefer;
+    if (D.advection == 0 && !fold) EW(k_scale, N, 0.5);
     CHK(dns_transform(D, D.Uf[0][i], +1, s));
-    if (zdefer) EW(k_axpy, D.Uf[0][i], -half, D.tmp);
+    if (fold) EW(k_axpby, D.Uf[0][i], 0.5, -half, D.tmp);
+    else if (zdefer) EW(k_axpy, D.Uf[0][i], -half, D.tmp);
     CHK(smooth_launch(M, D.nz, D.Uf[0][i], s));
   }
   if (D.advection == 1) D.toggle = 1 - D.toggle;
@@ -741,17 +765,16 @@ int dns_step(semx_dns_s& D, hipStream_t s) {
     const double a = -1.0 / (D.dt * D.kinvis), b = 1.0 / D.kinvis;
     for (int i = 0; i < ncom; ++i) {
       std::swap(D.Us[0][i], D.Uf[0][i]);
-      EW(k_scale, D.Uf[0][i], a);
+      if (i >= D.ndim) EW(k_scale, D.Uf[0][i], a);
     }
-    for (int i = 0; i < D.ndim; ++i) {   // Uf[0][i] += b dP/dx_i
+    for (int i = 0; i < D.ndim; ++i) {   // Uf[0][i] = a Uf[0][i] + b dP/dx_i
       if (i < 2) {
-        CHK(grad_nl_launch(M, D.nz, D.Uf[0][i], D.P, nullptr, i, 3, s, b));
+        CHK(grad_nl_launch(M, D.nz, D.Uf[0][i], D.P, nullptr, i, 3, s, b, a));
       } else {
         dim3 g((M.planeSize + 255) / 256, D.nz / 2);
-        hipLaunchKernelGGL(k_grad_z, g, dim3(256), 0, s, D.P, D.Us[0][0], D.nz / 2,
-                           M.planeSize, D.beta);
+        hipLaunchKernelGGL(k_grad_z_axpby, g, dim3(256), 0, s, D.P, D.Uf[0][i], D.nz / 2,
+                           M.planeSize, D.beta, a, b);
         CHK(ew_launch_check());
-        EW(k_axpy, D.Uf[0][i], b, D.Us[0][0]);
       }
     }
   }
