@@ -658,8 +658,18 @@ int dns_nonlinear(semx_dns_s& D, hipStream_t s) {
   prof_begin(s, &ev);
   for (int i = 0; i < D.ncom; ++i) {
     EW(k_fill, D.Uf[0][i], 0.0);
-    EW(k_copy, D.Us[0][i], D.U[i]);
-    CHK(dns_transform(D, D.U[i], -1, s));
+    if (D.nz > 2) {
+      // the transform is out of place: its input stays as the Fourier copy
+      // Us[0] and the previous Us[0] buffer becomes the spare (no copy)
+      double* f = D.U[i];
+      CHK(dft_launch(f, D.spare, D.nz, M.planeSize, -1, s));
+      D.U[i] = D.spare;
+      D.spare = D.Us[0][i];
+      D.Us[0][i] = f;
+    } else {
+      EW(k_copy, D.Us[0][i], D.U[i]);
+      CHK(dns_transform(D, D.U[i], -1, s));
+    }
   }
   const bool conv = D.advection == 0 || (D.advection == 1 && D.toggle);
   const bool cons = D.advection == 0 || (D.advection == 1 && !D.toggle);
