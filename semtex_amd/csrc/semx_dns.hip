@@ -216,11 +216,12 @@ __global__ __launch_bounds__(OP_THREADS) void k_grad_xy(double* __restrict__ dat
 //   MODE 0  N -= b * d(a)/dx_dir     convective term (dns/nonlinear.cpp:36-211)
 //   MODE 1  N -= d(a b)/dx_dir       conservative term
 //   MODE 2  N  = d(a)/dx_dir         setPForce (dns/integrate.cpp:159-160)
-//   MODE 3  N += c * d(a)/dx_dir     project (dns/integrate.cpp:174-186)
+//   MODE 3  N = sc N + c d(a)/dx_dir project (dns/integrate.cpp:174-186)
 // Entries of a plane beyond its elements (the padding up to planeSize) take
 // the values the unfused copy-derivative-update sequence gives them (the
 // derivative leaves them as they were): N -= a b, N -= a b, N = a,
-// N = sc N + c a.
+// N = sc N + c a.  In modes 0 and 1, sc = 0 starts N from zero (N is not
+// read: the first term of a nonlinear product needs no cleared field).
 template <int MODE>
 __global__ __launch_bounds__(OP_THREADS) void k_grad_nl(double* __restrict__ N,
                                                         const double* __restrict__ a,
@@ -248,7 +249,7 @@ __global__ __launch_bounds__(OP_THREADS) void k_grad_nl(double* __restrict__ N,
   if (blockIdx.x == 0)
     for (int k = nel * npnp + t; k < planeSize; k += OP_THREADS) {
       const size_t q = pbase + k;
-      if (MODE <= 1) N[q] -= a[q] * b[q];
+      if (MODE <= 1) N[q] = (sc == 0.0 ? 0.0 : N[q]) - a[q] * b[q];
       else if (MODE == 2) N[q] = a[q];
       else N[q] = sc * N[q] + c * a[q];
     }
@@ -267,8 +268,8 @@ __global__ __launch_bounds__(OP_THREADS) void k_grad_nl(double* __restrict__ N,
   const double rx = g[(dir ? G_DRDY : G_DRDX) * npnp + pt];
   const double sx = g[(dir ? G_DSDY : G_DSDX) * npnp + pt];
   const double d = B * sx + A * rx;
-  if (MODE == 0) N[off] -= b[off] * d;
-  else if (MODE == 1) N[off] -= d;
+  if (MODE == 0) N[off] = (sc == 0.0 ? 0.0 : N[off]) - b[off] * d;
+  else if (MODE == 1) N[off] = (sc == 0.0 ? 0.0 : N[off]) - d;
   else if (MODE == 2) N[off] = d;
   else N[off] = sc * N[off] + c * d;
 }
@@ -657,7 +658,6 @@ int dns_nonlinear(semx_dns_s& D, hipStream_t s) {
   hipEvent_t ev;
   prof_begin(s, &ev);
   for (int i = 0; i < D.ncom; ++i) {
-    EW(k_fill, D.Uf[0][i], 0.0);
     if (D.nz > 2) {
       // the transform is out of place: its input stays as the Fourier copy
       // Us[0] and the previous Us[0] buffer becomes the spare (no copy)
@@ -676,6 +676,7 @@ int dns_nonlinear(semx_dns_s& D, hipStream_t s) {
   for (int i = 0; i < D.ncom; ++i) {
     double* N = D.Uf[0][i];
     bool zdefer = false;   // the conservative z term waits in tmp (Fourier space)
+    double start = 0.0;    // the first x/y term stores N instead of updating it
     for (int j = 0; j < D.ndim; ++j) {
       if (conv) {   // n_i -= u_j d(u_i)/dx_j
         if (j == 2) {
@@ -686,7 +687,8 @@ int dns_nonlinear(semx_dns_s& D, hipStream_t s) {
           CHK(dns_transform(D, D.tmp, -1, s));
           EW(k_times_minus, N, D.U[j], D.tmp);
         } else {   // fused: no copy of u_i, the product in the epilogue
-          CHK(grad_nl_launch(M, D.nz, N, D.U[i], D.U[j], j, 0, s));
+          CHK(grad_nl_launch(M, D.nz, N, D.U[i], D.U[j], j, 0, s, 0.0, start));
+          start = 1.0;
         }
       }
       if (cons) {   // n_i -= d(u_i u_j)/dx_j
@@ -700,7 +702,8 @@ int dns_nonlinear(semx_dns_s& D, hipStream_t s) {
           CHK(grad_launch(M, D.nz, D.beta, D.tmp, j, s));
           zdefer = true;
         } else {   // fused: the product formed on load, the difference in the epilogue
-          CHK(grad_nl_launch(M, D.nz, N, D.U[i], D.U[j], j, 1, s));
+          CHK(grad_nl_launch(M, D.nz, N, D.U[i], D.U[j], j, 1, s, 0.0, start));
+          start = 1.0;
         }
       }
     }
