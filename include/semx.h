@@ -42,6 +42,9 @@ extern "C" {
 #define SEMX_E_JACOBIAN   6   /* nonpositive Jacobian: element.cpp:1358     */
 #define SEMX_E_MESH       7   /* inconsistent mesh connectivity             */
 #define SEMX_E_UNSUPPORTED 8  /* configuration not supported                */
+#define SEMX_E_TIMEOUT    9   /* a bounded device-side wait expired: the    *
+                               * results of the solves since the last check *
+                               * are invalid (no counterpart in semtex)     */
 
 /* --- solver kinds (femlib/cfemdef.h:43-52 SolverKind) -------------------- */
 #define SEMX_DIRECT  0        /* static condensation + banded Cholesky      */
@@ -136,6 +139,16 @@ int semx_helm_element (semx_helm helm, int m, int elmt, double* hbb,
 int semx_helm_solve (semx_helm helm, double* f, const double* bc, double* u,
                      double tol_rel, int step_max, int* iters, void* stream);
 
+/* Waits for stream and reports whether any solve of this system since the
+ * last check had a cooperative band sweep whose bounded hand-off wait
+ * expired (SEMX_E_TIMEOUT; its solution is wrong), then clears the mark.
+ * No counterpart in semtex, whose solves are synchronous.  A solve called
+ * with iters != NULL and semx_dns_run make this check themselves, so a
+ * timeout is reported by the first synchronising call after it, never
+ * lost; callers that only issue asynchronous solves (iters = NULL, graph
+ * replays) call this at their own synchronisation points.                */
+int semx_helm_check (semx_helm helm, void* stream);
+
 /* Field::solve for nfield (1..3) fields that share one ModalMatrixSys, as
  * the velocity components u, v, w do in semtex (the MatrixSys cache,
  * src/matrix.cpp:85-91, and Solve per component, dns/integrate.cpp:
@@ -209,7 +222,9 @@ int semx_dns_create (semx_mesh mesh, int nz, double beta, double dt,
 int semx_dns_destroy (semx_dns dns);
 /* nstep steps; u, v, w (NULL in 2D), p: device fields of nz*planeSize in
  * Fourier space, read at entry and written at exit.  The history carries
- * over between calls, as integrate's static storage does.                */
+ * over between calls, as integrate's static storage does.  Returns after
+ * the stream has finished the steps, with SEMX_E_TIMEOUT if a cooperative
+ * band sweep among them timed out (semx_helm_check).                     */
 int semx_dns_run (semx_dns dns, int nstep, double* u, double* v, double* w,
                   double* p, void* stream);
 /* info[4] = steps taken, nz, velocity components, norder                  */

@@ -1191,12 +1191,14 @@ int orc_solve_plane (const orc_sys* S, double* forcing, const double* bc,
 }
 
 static int field_solve_planes (const orc_mesh* M, int nz, double lambda2, double beta,
-                        int method, double* f, const double* bc, double* u,
+                        int base, int method, double* f, const double* bc, double* u,
                         double tol_rel, int step_max, int nthreads, int nrep,
                         double* t_solve, int* iters)
 /* Field::solve over planes (field.cpp:564-723) with one MatrixSys per
- * mode (matrix.cpp:33-110; SVV z-weights are 1 with SVV off).  iters (may
- * be NULL) receives each plane's PCG iteration count, -1 at the limit.   */
+ * mode base..base+nz/2-1 (matrix.cpp:33-110; SVV z-weights are 1 with SVV
+ * off).  The Nyquist plane 1 is skipped only on the planes that start at
+ * mode 0 (the ROOTONLY skip of field.cpp:566).  iters (may be NULL)
+ * receives each plane's PCG iteration count, -1 at the limit.            */
 {
   const int nmodes = (nz > 1) ? nz / 2 : 1;
   const int nline = M->nbound * M->np;
@@ -1208,8 +1210,8 @@ static int field_solve_planes (const orc_mesh* M, int nz, double lambda2, double
 #pragma omp parallel for schedule(dynamic, 1)
   for (m = 0; m < nmodes; m++) {
     /* MIXED: DIRECT for mode 0, JACPCG otherwise (matrix.cpp:93-96) */
-    const int kind = (method == 2) ? (m == 0 ? ORC_DIRECT : ORC_JACPCG) : method;
-    const double bk = beta * m;
+    const int kind = (method == 2) ? (base + m == 0 ? ORC_DIRECT : ORC_JACPCG) : method;
+    const double bk = beta * (base + m);
     sys[m] = orc_sys_create (M, lambda2, bk * bk, kind);
   }
   {
@@ -1227,7 +1229,7 @@ static int field_solve_planes (const orc_mesh* M, int nz, double lambda2, double
 #pragma omp parallel for schedule(dynamic, 1) reduction(+:warn)
       for (k = 0; k < nz; k++) {
         int it;
-        if (nz > 1 && k == 1) continue;
+        if (nz > 1 && k == 1 && base == 0) continue;
         it = orc_solve_plane (sys[k >> 1], f + (size_t) k * M->planeSize,
                               bc + (size_t) k * nline, u + (size_t) k * M->planeSize,
                               tol_rel, step_max);
@@ -1249,7 +1251,7 @@ int orc_field_solve (const orc_mesh* M, int nz, double lambda2, double beta,
                      double tol_rel, int step_max, int nthreads, int nrep,
                      double* t_solve)
 {
-  return field_solve_planes (M, nz, lambda2, beta, method, f, bc, u, tol_rel, step_max,
+  return field_solve_planes (M, nz, lambda2, beta, 0, method, f, bc, u, tol_rel, step_max,
                       nthreads, nrep, t_solve, NULL);
 }
 
@@ -1257,8 +1259,16 @@ int orc_field_solve_iters (const orc_mesh* M, int nz, double lambda2, double bet
                            int method, double* f, const double* bc, double* u,
                            double tol_rel, int step_max, int* iters)
 {
-  return field_solve_planes (M, nz, lambda2, beta, method, f, bc, u, tol_rel, step_max,
+  return field_solve_planes (M, nz, lambda2, beta, 0, method, f, bc, u, tol_rel, step_max,
                       0, 1, NULL, iters);
+}
+
+int orc_field_solve_base (const orc_mesh* M, int nz, double lambda2, double beta,
+                          int base, int method, double* f, const double* bc, double* u,
+                          double tol_rel, int step_max, int* iters)
+{
+  return field_solve_planes (M, nz, lambda2, beta, base, method, f, bc, u, tol_rel,
+                      step_max, 0, 1, NULL, iters);
 }
 
 /* ======================================================================== *

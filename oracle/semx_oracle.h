@@ -108,6 +108,13 @@ int orc_field_solve (const orc_mesh* M, int nz, double lambda2, double beta,
 int orc_field_solve_iters (const orc_mesh* M, int nz, double lambda2, double beta,
                            int method, double* f, const double* bc, double* u,
                            double tol_rel, int step_max, int* iters);
+/* The same for the planes of modes base..base+nz/2-1 (one MPI process's
+ * share in semtex, Geometry::baseMode): plane k uses mode base + k>>1 and
+ * plane 1 is solved unless base == 0 (field.cpp:566 skips it on the root
+ * process only).                                                          */
+int orc_field_solve_base (const orc_mesh* M, int nz, double lambda2, double beta,
+                          int base, int method, double* f, const double* bc, double* u,
+                          double tol_rel, int step_max, int* iters);
 
 /* --- Fourier transform & field operators -------------------------------- */
 /* dDFTr (fourier.c:47-171) semantics: ntrn lines of tlen reals strided by

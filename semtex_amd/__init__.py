@@ -167,6 +167,13 @@ def solve(msys, f, bc, u, tol_rel=1e-12, step_max=10000, stream=None, iters=True
     return it.value
 
 
+def helm_check(msys, stream=None):
+    """Wait for the stream and raise SemxError(SEMX_E_TIMEOUT) if any solve
+    of this system since the last check had a cooperative band sweep whose
+    bounded hand-off wait expired (semx_helm_check)."""
+    check(load().semx_helm_check(msys.h, _stream(stream)), "semx_helm_check")
+
+
 def solve_fields(msys, fs, bcs, us, tol_rel=1e-12, step_max=10000, stream=None, iters=True):
     """Field::solve for up to 3 fields sharing msys (as u, v, w do): one
     sweep of each mode's factor for all of them.  Lists of device tensors;

@@ -29,9 +29,12 @@ def _run(cmd):
 
 
 def build(verbose: bool = True) -> str:
+    """Compile every source (in parallel, one compiler process each) and
+    link libsemx.so."""
     objs = []
     bdir = os.path.join(HERE, "_build")
     os.makedirs(bdir, exist_ok=True)
+    procs = []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         obj = os.path.join(bdir, src + ".o")
@@ -39,8 +42,12 @@ def build(verbose: bool = True) -> str:
             cmd = [HIPCC, "-c", path, "-o", obj] + COMMON + DEVICE_FLAGS
         else:
             cmd = ["g++", "-c", path, "-o", obj, "-pthread"] + COMMON
-        _run(cmd)
+        print(" ".join(cmd), flush=True)
+        procs.append((src, subprocess.Popen(cmd)))
         objs.append(obj)
+    failed = [src for src, p in procs if p.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, "compile " + " ".join(failed))
     tmp = OUT + ".tmp"
     _run([HIPCC, "-shared", "-o", tmp] + objs + DEVICE_FLAGS + ["-pthread"])
     os.replace(tmp, OUT)

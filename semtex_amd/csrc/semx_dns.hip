@@ -931,7 +931,12 @@ int semx_dns_run(semx_dns dns, int nstep, double* u, double* v, double* w, doubl
   for (int i = 0; i < D.ncom; ++i)
     HIPCHK(hipMemcpyAsync(ext[i], D.U[i], bytes, hipMemcpyDeviceToDevice, s));
   HIPCHK(hipMemcpyAsync(p, D.P, bytes, hipMemcpyDeviceToDevice, s));
-  return SEMX_OK;
+  // the solves of the steps ran without host synchronisation: any expired
+  // cooperative hand-off wait among them is reported here
+  int st = SEMX_OK;
+  for (Helm* h : {D.vel, D.pre})
+    if (h && !st) st = helm_check(*h, s);
+  return st;
 }
 
 int semx_dns_info(semx_dns dns, int* info) {

@@ -941,10 +941,12 @@ int helm_build(Helm& H) {
     pool.emplace_back([&, t]() { for (int m = t; m < nmodes; m += nth) work(m); });
   for (auto& th : pool) th.join();
   for (int st : status) if (st) return st;
-  // plane list: every plane except the Nyquist plane 1 (field.cpp:566)
+  // plane list: every plane except the Nyquist plane 1, which only the
+  // planes that start at mode 0 hold (ROOTONLY skip, field.cpp:566); with
+  // base_mode > 0 plane 1 is Im of mode base_mode and is solved
   H.plane_of.clear();
   for (int k = 0; k < H.nz; ++k)
-    if (!(H.nz > 1 && k == 1)) H.plane_of.push_back(k);
+    if (!(H.nz > 1 && k == 1 && H.base == 0)) H.plane_of.push_back(k);
   H.nplanes = (int)H.plane_of.size();
   H.solstride = M.nglobal + (M.nglobal % 2);
   // band-solve work list: one workgroup per direct mode, solving every

@@ -2728,21 +2728,32 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
       bool limit = false;
       for (int v : it) { limit |= v < 0; mx = std::max(mx, v < 0 ? -v : v); }
       *iters = mx;
-      if (limit) return SEMX_E_MAXITER;
+      if (limit) {
+        const int st = helm_check(H, s);
+        return st ? st : SEMX_E_MAXITER;
+      }
     }
   }
   HIPCHK(hipGetLastError());
-  if (iters && H.coop_w > 0) {
-    // a cooperative hand-off that timed out leaves its mark here
-    int e = 0;
-    HIPCHK(hipMemcpyAsync(&e, H.d.err, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    if (e) {
-      fprintf(stderr, "semx: cooperative band solve: hand-off wait timed out\n");
-      return SEMX_E_HIP;
-    }
-  }
+  if (iters) return helm_check(H, s);
   return SEMX_OK;
+}
+
+// The cooperative band sweeps set a sticky error word when a bounded
+// hand-off wait expires (their result is then wrong).  Every host
+// synchronisation point of the solve path reads it here, after the stream
+// has drained, and clears it, so a timeout is reported exactly once, by the
+// first check after it happened, whatever the caller passed for iters.
+int helm_check(Helm& H, hipStream_t s) {
+  HIPCHK(hipStreamSynchronize(s));
+  if (H.coop_w <= 0 || !H.d.err) return SEMX_OK;
+  int e = 0;
+  HIPCHK(hipMemcpy(&e, H.d.err, sizeof(int), hipMemcpyDeviceToHost));
+  if (!e) return SEMX_OK;
+  const int zero = 0;
+  HIPCHK(hipMemcpy(H.d.err, &zero, sizeof(int), hipMemcpyHostToDevice));
+  fprintf(stderr, "semx: cooperative band solve: hand-off wait timed out\n");
+  return SEMX_E_TIMEOUT;
 }
 
 int helm_solve(Helm& H, double* f, const double* bc, double* u, double tol_rel, int step_max,
@@ -2771,6 +2782,7 @@ const char* semx_error_string(int status) {
     case SEMX_E_JACOBIAN: return "element Jacobian nonpositive";
     case SEMX_E_MESH: return "inconsistent mesh connectivity";
     case SEMX_E_UNSUPPORTED: return "unsupported configuration";
+    case SEMX_E_TIMEOUT: return "a bounded device-side wait expired (results invalid)";
     default: return "unknown status";
   }
 }
@@ -2900,6 +2912,12 @@ int semx_helm_solve(semx_helm helm, double* f, const double* bc, double* u, doub
                     int step_max, int* iters, void* stream) {
   if (!helm || !f || !u) return SEMX_E_ARG;
   return helm_solve(helm->H, f, bc, u, tol_rel, step_max, iters, (hipStream_t)stream);
+}
+
+int semx_helm_check(semx_helm helm, void* stream) {
+  if (!helm) return SEMX_E_ARG;
+  if (!helm->H.on_device) return SEMX_OK;
+  return helm_check(helm->H, (hipStream_t)stream);
 }
 
 int semx_helm_solve_fields(semx_helm helm, int nfield, double* const* f, const double* const* bc,

@@ -40,6 +40,9 @@ def lib():
                                       C.c_int, C.c_int, _D]
         L.orc_field_solve_iters.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double,
                                             C.c_int, _D, _D, _D, C.c_double, C.c_int, _I]
+        L.orc_field_solve_base.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double,
+                                           C.c_int, C.c_int, _D, _D, _D, C.c_double,
+                                           C.c_int, _I]
         L.orc_dftr.argtypes = [_D, C.c_int, C.c_int, C.c_int]
         L.orc_gradient.argtypes = [C.c_void_p, C.c_int, C.c_double, _D, C.c_int]
         L.orc_smooth.argtypes = [C.c_void_p, C.c_int, _D]
@@ -163,6 +166,17 @@ def field_solve_iters(mesh, nz, lambda2, beta, method, f, bc, u, tol=1e-12,
     lib().orc_field_solve_iters(C.c_void_p(mesh.h), nz, lambda2, beta, method, dp(f),
                                 dp(bc), dp(u), tol, step_max,
                                 it.ctypes.data_as(C.POINTER(C.c_int)))
+    return it
+
+
+def field_solve_base(mesh, nz, lambda2, beta, base, method, f, bc, u, tol=1e-12,
+                     step_max=10000):
+    """Field::solve on the planes of modes base..base+nz/2-1; returns each
+    plane's PCG iteration count."""
+    it = np.zeros(nz, dtype=np.int32)
+    lib().orc_field_solve_base(C.c_void_p(mesh.h), nz, lambda2, beta, base, method,
+                               dp(f), dp(bc), dp(u), tol, step_max,
+                               it.ctypes.data_as(C.POINTER(C.c_int)))
     return it
 
 

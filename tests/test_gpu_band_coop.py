@@ -60,6 +60,14 @@ def test_coop_band_parity(cuda, coop_env, case):
     ug2, _ = gpu_solve(torch, P, S, f, bcl)     # a second launch (next flag epoch)
     uo, _ = oracle_solve(Q, nz, lam2, beta, f, bcl)
     assert np.array_equal(ug, ug2)
+    # an asynchronous solve (iters = NULL) reports through the next check:
+    # no hand-off wait of these launches expired
+    tf = torch.from_numpy(np.ascontiguousarray(f).ravel().copy()).to("cuda:0")
+    tb = torch.from_numpy(np.ascontiguousarray(bcl).ravel().copy()).to("cuda:0")
+    tu = torch.zeros_like(tf)
+    sx.solve(S, tf, tb, tu, iters=False)
+    sx.helm_check(S)
+    assert np.array_equal(tu.cpu().numpy().reshape(f.shape), ug)
     if S.info["singular0"]:
         drop_mode0_constant(P, ug, uo)
     r = rel(ug[:, :P.nPlane], uo[:, :P.nPlane])

@@ -6,6 +6,7 @@ import pytest
 
 import semtex_amd as sx
 from semtex_amd.mesh import rect_mesh
+from semtex_amd.workloads import helmholtz_batch
 from solve_helpers import PERIODIC, drop_mode0_constant, gpu_solve, make_case, oracle_solve, rel
 
 pytestmark = pytest.mark.gpu
@@ -147,3 +148,26 @@ def test_solve_fields_band_variants(cuda, pick):
             os.environ.pop("SEMX_AUTOTUNE", None)
         else:
             os.environ["SEMX_AUTOTUNE"] = old
+
+
+def test_config0_square8_np8(cuda):
+    """The plain CPU reference workload: 8x8 uniform quads on [0,2pi]^2,
+    np=8, lambda2=1, Dirichlet from the seed-0 manufactured solution,
+    DIRECT.  rel L2 <= 1e-12 vs the oracle; the manufactured solution is
+    trigonometric of degree 3 on elements of width pi/4, so np=8 resolves
+    it to about 1e-7."""
+    import torch
+    from semtex_amd.workloads import workload_mesh
+    np_, m, nz = workload_mesh("square8_np8")
+    P = sx.Mesh(np_, m)
+    from oracle_lib import Mesh as OMesh
+    Q = OMesh(np_, m)
+    f, bcl, ue = helmholtz_batch(P, nz, 1.0, 0.0, seed=0)
+    S = sx.ModalMatrixSys(P, 1.0, 0.0, 0, 1, nz, sx.DIRECT)
+    ug, _ = gpu_solve(torch, P, S, f, bcl)
+    uo, _ = oracle_solve(Q, nz, 1.0, 0.0, f, bcl)
+    r = rel(ug[:, :P.nPlane], uo[:, :P.nPlane])
+    err = np.abs(ug[:, :P.nPlane] - ue[:, :P.nPlane]).max()
+    print(f"[config0] rel L2 vs oracle {r:.3e}, |u-u*|_inf {err:.3e}", flush=True)
+    assert r <= TOL
+    assert err < 1e-6

@@ -195,3 +195,38 @@ def test_dns_matches_oracle_3d(cuda, nx):
         r = rel(g[keep, :P.nPlane], po[keep, :P.nPlane])
         print(f"  p: rel L2 vs oracle (mode-0 constant removed) {r:.3e}", flush=True)
         assert r <= 1e-12
+
+
+@pytest.mark.parametrize("case", ["taylor4", "taylor5"])
+def test_regress_taylor_z_on_gpu(cuda, case):
+    """regress/taylor4.ok and taylor5.ok: 2D Taylor flow in the x-z (y-z)
+    plane on 2x2 periodic elements of [0,2]^2, np=11, Nz=8, Lz=2, nu=0.01,
+    dt=0.02, N_TIME=2, alternating skew-symmetric terms, 20 steps.  Fields
+    are transformed on the GPU (semx_dft_r) both ways; the Nyquist plane is
+    zeroed before the inverse transform as Domain::transform does
+    (src/domain.cpp:249-259).  Error norms to all printed digits."""
+    import torch
+    from test_oracle import TAYLOR_Z, norm_text, taylor_z_physical
+    print(f"[{case}]", flush=True)
+    m = rect_mesh(2, 2, 0, 2, 0, 2, periodic_x=True, periodic_y=True)
+    P = sx.Mesh(11, m)
+    nz, nu, dt, nstep = 8, 0.01, 0.02, 20
+    ps = P.planeSize
+    u, v, w, _ = taylor_z_physical(case, P, nz, 0.0, nu)
+    tu, tv, tw = dev(torch, u), dev(torch, v), dev(torch, w)
+    tp = dev(torch, np.zeros((nz, ps)))
+    for t in (tu, tv, tw):
+        sx.dft_r(t, nz, ps, 1)
+    D = sx.DNS(P, nz, np.pi, dt, nu, norder=2, advection=1, tol_rel=1e-12, step_max=500)
+    D.run(nstep, tu, tv, tw, tp)
+    exact = taylor_z_physical(case, P, nz, nstep * dt, nu)
+    got = []
+    for t, e in zip((tu, tv, tw, tp), exact):
+        t.view(nz, ps)[1].zero_()
+        sx.dft_r(t, nz, ps, -1)
+        torch.cuda.synchronize()
+        f = t.cpu().numpy().reshape(nz, ps)
+        got.append(norm_text(np.abs(f[:, :P.nPlane] - e[:, :P.nPlane]).max()))
+    print(f"  norms u v w p: {got}", flush=True)
+    assert tuple(got) == TAYLOR_Z[case][1]
+    assert D.steps == nstep

@@ -100,3 +100,32 @@ def test_pcg_tight_vs_direct(cuda, case, method, prec):
     e = rel(solved(P, ug), solved(P, ud))
     print(f"  iterations {it}, rel L2 vs DIRECT {e:.3e}", flush=True)
     assert e <= 1e-10
+
+
+@pytest.mark.parametrize("method", [sx.DIRECT, sx.MIXED], ids=["direct", "mixed"])
+def test_base_mode_share(cuda, method):
+    """A system for modes base..base+nmodes-1 (one process's share in
+    semtex, Geometry::baseMode = 2): plane 1 is Im of mode 2 and is solved
+    (field.cpp:566 skips plane 1 on the root process only).  rel L2 vs the
+    oracle's base-mode solve <= 1e-12 (DIRECT) / 1e-10 (MIXED, all modes
+    JACPCG since none is global mode 0)."""
+    import torch
+    import oracle_lib as O
+    from semtex_amd.mesh import rect_mesh
+    print(f"[base mode] method {method}", flush=True)
+    m = rect_mesh(3, 3, distort=0.2, bc=("D", "N", "D", "D"))
+    np_, base, nmodes, beta, lam = 6, 2, 2, 1.3, 0.5
+    nz = 2 * nmodes
+    P, Q = sx.Mesh(np_, m), O.Mesh(np_, m)
+    rng = np.random.default_rng(8)
+    f = np.zeros((nz, P.planeSize)); f[:, :P.nPlane] = rng.standard_normal((nz, P.nPlane))
+    bc = np.ascontiguousarray(rng.standard_normal((nz, P.nbound * np_)))
+    S = sx.ModalMatrixSys(P, lam, beta, base, nmodes, nz, method)
+    ug, _ = gpu_solve(torch, P, S, f, bc, tol=1e-13)
+    uo = np.zeros_like(f)
+    O.field_solve_base(Q, nz, lam, beta, base, sx.DIRECT, np.ascontiguousarray(f.copy()),
+                       bc, uo)
+    assert np.abs(ug[1, :P.nPlane]).max() > 0
+    r = rel(ug[:, :P.nPlane], uo[:, :P.nPlane])
+    print(f"  rel L2 vs oracle {r:.3e}", flush=True)
+    assert r <= (1e-12 if method == sx.DIRECT else 1e-10)

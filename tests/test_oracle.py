@@ -253,6 +253,32 @@ def test_pcg_matches_direct_3d():
     assert np.linalg.norm(u1 - u2) <= 1e-10 * np.linalg.norm(u1)
 
 
+def test_base_mode_planes_equal_full_field_planes():
+    """One process's share of modes (Geometry::baseMode > 0): its planes are
+    those of the full field shifted by 2*base, and its plane 1 is Im of mode
+    base, which is solved (field.cpp:566 skips plane 1 on the root only).
+    Pinned against the base-0 solve of the whole field, for DIRECT and
+    MIXED (DIRECT only for global mode 0, matrix.cpp:93-96)."""
+    m = rect_mesh(3, 3, distort=0.2)
+    M = O.Mesh(6, m)
+    base, nmodes, beta, lam = 2, 2, 1.3, 0.5
+    nzf = 2 * (base + nmodes)
+    rng = np.random.default_rng(8)
+    f = rng.standard_normal((nzf, M.planeSize)); f[1] = 0
+    bc = rng.standard_normal((nzf, M.nbound * 6)); bc[1] = 0
+    for method, tol in ((0, 1e-12), (2, 1e-10)):
+        uf = np.zeros_like(f)
+        O.field_solve(M, nzf, lam, beta, method, f.copy(), bc.copy(), uf, tol=1e-13)
+        sl = slice(2 * base, nzf)
+        ub = np.zeros((2 * nmodes, M.planeSize))
+        it = O.field_solve_base(M, 2 * nmodes, lam, beta, base, method,
+                                np.ascontiguousarray(f[sl]), np.ascontiguousarray(bc[sl]),
+                                ub, tol=1e-13)
+        assert np.abs(ub[1]).max() > 0
+        assert (it > 0).all() if method == 2 else (it == 0).all()
+        assert np.linalg.norm(ub - uf[sl]) <= tol * np.linalg.norm(uf[sl])
+
+
 # --------------------------------------------------------------------------
 # Reference regression pins (regress/*.ok, norm_inf vs the exact solution,
 # `noise-level` meaning < 1e-12: utility/compare.cpp:85,229-235)
@@ -411,3 +437,71 @@ def test_regress_taylor(nz):
     assert f"{np.abs(p[0, :M.nPlane] - pe).max():.3e}" == "4.224e-01"
     if nz > 1:
         assert np.abs(w).max() < 1e-12 and np.abs(u[2:]).max() < 1e-12
+
+
+# --------------------------------------------------------------------------
+# a17/a18 with z-variation: regress/taylor4.ok and taylor5.ok (2D Taylor
+# flow in the x-z and y-z planes, mesh/taylor4, mesh/taylor5): these are the
+# pins of every z-coupled term of the nonlinear products, setPForce and the
+# projection.  The reference's comparison (utility/compare.cpp:212-235)
+# reads the physical-space fields that Domain::transform writes, which zeroes
+# the Nyquist plane before the inverse transform (src/domain.cpp:249-259).
+# --------------------------------------------------------------------------
+
+TAYLOR_Z = {
+    # case: (exact u, v, w, p as functions of (a, z, t) with a = x or y;
+    #        which in-plane coordinate; golden norms u v w p)
+    "taylor4": ("x", ("1.099e-05", "noise-level", "1.176e-05", "2.089e-01")),
+    "taylor5": ("y", ("noise-level", "1.099e-05", "1.176e-05", "2.135e-01")),
+}
+
+
+def taylor_z_exact(case, x, y, z, t, nu):
+    a = x if TAYLOR_Z[case][0] == "x" else y
+    f1 = np.exp(-2 * np.pi ** 2 * nu * t)
+    f2 = np.exp(-4 * np.pi ** 2 * nu * t)
+    s = -np.cos(np.pi * a) * np.sin(np.pi * z) * f1
+    w = np.sin(np.pi * a) * np.cos(np.pi * z) * f1
+    p = -0.25 * (np.cos(2 * np.pi * a) + np.cos(2 * np.pi * z)) * f2
+    zero = np.zeros_like(s)
+    return (s, zero, w, p) if TAYLOR_Z[case][0] == "x" else (zero, s, w, p)
+
+
+def taylor_z_physical(case, M, nz, t, nu):
+    """Physical planes (nz, planeSize) of u, v, w, p at time t; z_k = k Lz/nz
+    with Lz = 2 (BETA = 2 pi / Lz = pi)."""
+    out = [np.zeros((nz, M.planeSize)) for _ in range(4)]
+    for k in range(nz):
+        z = 2.0 * k / nz
+        for f, e in zip(out, taylor_z_exact(case, M.x, M.y, z, t, nu)):
+            f[k, :M.nPlane] = e
+    return out
+
+
+def norm_text(e):
+    """utility/compare.cpp:232-235 with -n: below 1e-12 prints noise-level."""
+    return "noise-level" if e < 1e-12 else f"{e:.3e}"
+
+
+@pytest.mark.parametrize("case", ["taylor4", "taylor5"])
+def test_regress_taylor_z(case):
+    m = rect_mesh(2, 2, 0, 2, 0, 2, periodic_x=True, periodic_y=True)
+    M = O.Mesh(11, m)
+    nz, nu, dt, nstep = 8, 0.01, 0.02, 20
+    ps = M.planeSize
+    u, v, w, _ = taylor_z_physical(case, M, nz, 0.0, nu)
+    for f in (u, v, w):
+        O.dftr(f, nz, ps, 1)
+    p = np.zeros((nz, ps))
+    D = O.lib().orc_dns_create(C.c_void_p(M.h), nz, np.pi, dt, nu, 2, 1, 1e-12, 500, 0)
+    t = C.c_double()
+    O.lib().orc_dns_run(C.c_void_p(D), nstep, O.dp(u), O.dp(v), O.dp(w), O.dp(p),
+                        C.byref(t))
+    O.lib().orc_dns_destroy(C.c_void_p(D))
+    exact = taylor_z_physical(case, M, nz, nstep * dt, nu)
+    got = []
+    for f, e in zip((u, v, w, p), exact):
+        f[1] = 0.0                                  # Domain::transform
+        O.dftr(f, nz, ps, -1)
+        got.append(norm_text(np.abs(f[:, :M.nPlane] - e[:, :M.nPlane]).max()))
+    assert tuple(got) == TAYLOR_Z[case][1]
