@@ -45,6 +45,8 @@ extern "C" {
 #define SEMX_E_TIMEOUT    9   /* a bounded device-side wait expired: the    *
                                * results of the solves since the last check *
                                * are invalid (no counterpart in semtex)     */
+#define SEMX_E_SCRATCH   10   /* request exceeds the static scratch of an   *
+                               * allocation-free call                       */
 
 /* --- solver kinds (femlib/cfemdef.h:43-52 SolverKind) -------------------- */
 #define SEMX_DIRECT  0        /* static condensation + banded Cholesky      */
@@ -196,6 +198,21 @@ int semx_batch_solve (semx_batch batch, const double* const* f,
  * Uses a library scratch buffer: concurrent calls from several host
  * threads are not supported.                                              */
 int semx_dft_r (double* data, int tlen, int ntrn, int sign, void* stream);
+/* The same transform, allocating nothing: for callers that cannot call
+ * hipMalloc (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation).  No counterpart in semtex.  It works in a static
+ * device array of the library (the transform matrix and a staging copy of
+ * the field, about 48.5 MiB, held on every device the library runs on until
+ * the process ends) and otherwise behaves as semx_dft_r.  Limits: tlen <= 256
+ * and tlen*ntrn <= SEMX_DFT_NOALLOC_MAX; larger requests return
+ * SEMX_E_SCRATCH.  Every call on one device shares that device's array:
+ * calls that overlap on one device (several streams or host threads) race.
+ * The array's device address is looked up by semx_device_init (call it once
+ * per device before capturing), else by the first call on a device.
+ * Results agree with semx_dft_r to rounding (its transform matrix is formed
+ * on the device in FP64 rather than on the host in long double).           */
+#define SEMX_DFT_NOALLOC_MAX ((size_t)6 << 20)   /* doubles (48 MiB)        */
+int semx_dft_r_noalloc (double* data, int tlen, int ntrn, int sign, void* stream);
 /* AuxField::gradient (src/auxfield.cpp:514-665) on nz planes of data (in
  * place): dir 0 = d/dx, 1 = d/dy (Element::grad, src/element.cpp:552-597,
  * per plane), 2 = d/dz of Fourier planes (mode k: Re <- -beta k Im,

@@ -253,6 +253,14 @@ def dft_r(data, tlen, ntrn, sign, stream=None):
           "semx_dft_r")
 
 
+def dft_r_noalloc(data, tlen, ntrn, sign, stream=None):
+    """dDFTr in place through the allocation-free entry point (static device
+    scratch; tlen <= 256 and tlen*ntrn <= SEMX_DFT_NOALLOC_MAX)."""
+    assert data.numel() >= tlen * ntrn
+    check(load().semx_dft_r_noalloc(_ptr(data), int(tlen), int(ntrn), int(sign),
+                                    _stream(stream)), "semx_dft_r_noalloc")
+
+
 def gradient(mesh, nz, beta, data, direction, stream=None):
     """AuxField::gradient in place (direction 0: x, 1: y, 2: z in Fourier space)."""
     assert data.numel() == nz * mesh.planeSize

@@ -42,6 +42,7 @@ PROTOTYPES = {
     "semx_batch_solve": (C.c_int, [VP, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP),
                                    C.c_double, C.c_int, I, VP]),
     "semx_dft_r": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, VP]),
+    "semx_dft_r_noalloc": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, VP]),
     "semx_gradient": (C.c_int, [VP, C.c_int, C.c_double, VP, C.c_int, VP]),
     "semx_smooth": (C.c_int, [VP, C.c_int, VP, VP]),
     "semx_dns_create": (C.c_int, [VP, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,

@@ -18,6 +18,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <utility>
 #include <vector>
@@ -458,6 +459,36 @@ std::vector<double> dft_matrix(int N, int sign, int* ldw) {
   return W;
 }
 
+// The matrix of dft_matrix(N, sign) formed on the device, one thread per
+// entry of its rows x ldw storage: the same packing, padding and scaling,
+// with the angle reduced exactly (jk mod N) and evaluated by sincospi in
+// FP64 (within a few units in the last place of the long-double values).
+__global__ void k_dft_matrix(double* __restrict__ W, int N, int sign, int rows, int ldw) {
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= rows * ldw) return;
+  const int r = idx / ldw, c = idx - (idx / ldw) * ldw;
+  const int nh = N / 2;
+  double v = 0.0;
+  if (r < N && c < N) {
+    // sign +1: row r is output coefficient r, column c input point c;
+    // sign -1: row r is output point r, column c packed coefficient c
+    const int pk = (sign == 1) ? r : c, pt = (sign == 1) ? c : r;
+    const double sc = (sign == 1) ? 1.0 / N : 2.0;
+    if (pk == 0) {
+      v = (sign == 1) ? 1.0 / N : 1.0;
+    } else if (pk == 1) {
+      v = ((pt & 1) ? -1.0 : 1.0) * ((sign == 1) ? 1.0 / N : 1.0);
+    } else if (pk / 2 < nh) {
+      const long long m = ((long long)(pk / 2) * pt) % N;
+      double sn, cn;
+      sincospi(2.0 * (double)m / (double)N, &sn, &cn);
+      v = (pk & 1) ? -sn : cn;
+      v = (sign == 1) ? v / N : sc * v;
+    }
+  }
+  W[idx] = v;
+}
+
 struct DftPlan { int N, sign, ldw; double* W; };
 std::vector<DftPlan> g_plans;   // device matrices, kept for the process
 
@@ -475,12 +506,10 @@ int dft_plan(int N, int sign, const double** W, int* ldw) {
   return SEMX_OK;
 }
 
-// out-of-place transform of ntrn columns of length N (z-stride ntrn)
-int dft_launch(const double* in, double* out, int N, int ntrn, int sign, hipStream_t s) {
-  const double* W = nullptr;
-  int ldw = 0;
-  int st = dft_plan(N, sign, &W, &ldw);
-  if (st) return st;
+// out-of-place transform of ntrn columns of length N (z-stride ntrn) with
+// the transform matrix W (leading dimension ldw) already on the device
+int dft_launch_w(const double* in, double* out, const double* W, int ldw, int N, int ntrn,
+                 hipStream_t s) {
   if (N <= 128 && ntrn > 0) {
     // matrix cores: W staged whole in LDS
     const int MT = (N + 15) / 16;
@@ -526,6 +555,15 @@ int dft_launch(const double* in, double* out, int N, int ntrn, int sign, hipStre
   prof_end(s, "k_dft_z", ev);
   HIPCHK(hipGetLastError());
   return SEMX_OK;
+}
+
+// the same with the process-wide plan of (N, sign)
+int dft_launch(const double* in, double* out, int N, int ntrn, int sign, hipStream_t s) {
+  const double* W = nullptr;
+  int ldw = 0;
+  int st = dft_plan(N, sign, &W, &ldw);
+  if (st) return st;
+  return dft_launch_w(in, out, W, ldw, N, ntrn, s);
 }
 
 // The fused derivative of k_grad_nl (modes 0-3 there), dir 0 or 1
@@ -828,26 +866,106 @@ int make_system(Mesh& M, double lambda2, double beta, int nz, int method, Helm**
 
 }  // namespace
 
+// --- device memory of the allocation-free transform -------------------------
+// semx_dft_r_noalloc works in this array: the transform matrix (at most
+// 256 x 256 values, for tlen <= 256) and then the transformed field (at most
+// SEMX_DFT_NOALLOC_MAX values).  Every device the library runs on holds its
+// own copy until the process ends.  Host code never uses the array's name
+// (a host-side stand-in, not a device address): it uses the device address
+// obtained once per device by hipGetSymbolAddress.
+constexpr size_t NOALLOC_W = 256 * 256;
+static __device__ __align__(256) double scratch[NOALLOC_W + SEMX_DFT_NOALLOC_MAX];
+
+namespace {
+
+constexpr int NOALLOC_DEVICES = 64;
+std::mutex g_noalloc_mu;
+double* g_noalloc_ptr[NOALLOC_DEVICES] = {};
+
+}  // namespace
+
+namespace semx {
+
+// The current device's address of the array, looked up on its first use
+// on that device (semx_device_init, or the first allocation-free call).
+int noalloc_scratch(double** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_DEVICES) return SEMX_E_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      fprintf(stderr, "semx: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorString(e));
+      return SEMX_E_HIP;
+    }
+    g_noalloc_ptr[dev] = (double*)ptr;
+  }
+  if (out) *out = g_noalloc_ptr[dev];
+  return SEMX_OK;
+}
+
+}  // namespace semx
+
+namespace {
+
+// Arguments of a transform as dDFTr checks them (femlib/fourier.c:47-62);
+// 1 = nothing to do.
+int dft_args(const double* data, int tlen, int ntrn, int sign) {
+  if (!data || tlen < 1 || ntrn < 0 || (sign != 1 && sign != -1)) return SEMX_E_ARG;
+  if (tlen < 2 || ntrn == 0) return -1;
+  if (!fft_length_ok(tlen) || (ntrn & 1)) return SEMX_E_ARG;
+  return SEMX_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int semx_dft_r(double* data, int tlen, int ntrn, int sign, void* stream) {
-  if (!data || tlen < 1 || ntrn < 0 || (sign != 1 && sign != -1)) return SEMX_E_ARG;
-  if (tlen < 2 || ntrn == 0) return SEMX_OK;
-  if (!fft_length_ok(tlen) || (ntrn & 1)) return SEMX_E_ARG;
+  int st = dft_args(data, tlen, ntrn, sign);
+  if (st) return st < 0 ? SEMX_OK : st;
   hipStream_t s = (hipStream_t)stream;
-  static double* scratch = nullptr;
+  static double* stage = nullptr;
   static size_t cap = 0;
   const size_t need = (size_t)tlen * ntrn;
   if (need > cap) {
     HIPCHK(hipStreamSynchronize(s));
-    dev_free(scratch);
-    int st = dev_upload<double>(&scratch, nullptr, need);
+    dev_free(stage);
+    st = dev_upload<double>(&stage, nullptr, need);
     if (st) { cap = 0; return st; }
     cap = need;
   }
-  int st = dft_launch(data, scratch, tlen, ntrn, sign, s);
+  st = dft_launch(data, stage, tlen, ntrn, sign, s);
   if (st) return st;
-  HIPCHK(hipMemcpyAsync(data, scratch, need * sizeof(double), hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync(data, stage, need * sizeof(double), hipMemcpyDeviceToDevice, s));
+  return SEMX_OK;
+}
+
+int semx_dft_r_noalloc(double* data, int tlen, int ntrn, int sign, void* stream) {
+  int st = dft_args(data, tlen, ntrn, sign);
+  if (st) return st < 0 ? SEMX_OK : st;
+  const size_t need = (size_t)tlen * ntrn;
+  if (tlen > 256 || need > SEMX_DFT_NOALLOC_MAX) {
+    fprintf(stderr,
+            "semx_dft_r_noalloc: %d x %d values exceed the static scratch "
+            "(tlen <= 256, tlen*ntrn <= %zu)\n",
+            tlen, ntrn, (size_t)SEMX_DFT_NOALLOC_MAX);
+    return SEMX_E_SCRATCH;
+  }
+  double* base = nullptr;
+  if ((st = noalloc_scratch(&base))) return st;
+  hipStream_t s = (hipStream_t)stream;
+  const int rows = 16 * dft_rp(tlen);
+  const int ldw = (tlen + DFT_KB - 1) / DFT_KB * DFT_KB;
+  double* W = base;
+  double* out = base + NOALLOC_W;
+  hipLaunchKernelGGL(k_dft_matrix, dim3((rows * ldw + 255) / 256), dim3(256), 0, s, W, tlen,
+                     sign, rows, ldw);
+  HIPCHK(hipGetLastError());
+  if ((st = dft_launch_w(data, out, W, ldw, tlen, ntrn, s))) return st;
+  HIPCHK(hipMemcpyAsync(data, out, need * sizeof(double), hipMemcpyDeviceToDevice, s));
   return SEMX_OK;
 }
 

@@ -49,6 +49,9 @@ int helm_solve(Helm& H, double* f, const double* bc, double* u, double tol_rel, 
 int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* bcs,
                       double* const* us, double tol_rel, int step_max, int* iters,
                       hipStream_t s);
+// Device address (current device) of the static scratch array of the
+// allocation-free entry points; looked up once per device.
+int noalloc_scratch(double** out);
 // Waits for the stream, then reports (SEMX_E_TIMEOUT) and clears a
 // cooperative band sweep's expired hand-off wait since the last check.
 int helm_check(Helm& H, hipStream_t s);

@@ -2783,6 +2783,7 @@ const char* semx_error_string(int status) {
     case SEMX_E_MESH: return "inconsistent mesh connectivity";
     case SEMX_E_UNSUPPORTED: return "unsupported configuration";
     case SEMX_E_TIMEOUT: return "a bounded device-side wait expired (results invalid)";
+    case SEMX_E_SCRATCH: return "request larger than the static scratch of an allocation-free call";
     default: return "unknown status";
   }
 }
@@ -2793,7 +2794,8 @@ int semx_device_init(int device) {
   if (device < 0 || device >= n) return SEMX_E_ARG;
   HIPCHK(hipSetDevice(device));
   HIPCHK(hipFree(nullptr));
-  return SEMX_OK;
+  // the device address of the allocation-free calls' static scratch
+  return noalloc_scratch(nullptr);
 }
 
 int semx_mesh_create(int np, int nel, const double* vx, const double* vy, const int* nbr,
