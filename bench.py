@@ -46,6 +46,8 @@ def main():
                     help="skip timing the JACPCG / MIXED variants")
     ap.add_argument("--dns-steps", type=int, default=20,
                     help="timed DNS steps (after N_TIME start-up steps); 0 skips the DNS")
+    ap.add_argument("--config2", type=int, default=1,
+                    help="1: time the 32x32 np=11 Nz=128 MIXED solve in FP64, FP32 and BF16 PCG")
     ap.add_argument("--batch", type=int, default=512,
                     help="independent 2D problems in the batched PCG line; 0 skips it")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -300,11 +302,95 @@ def main():
         v.pop("u")
     if variants:
         result["iterative_variants"] = variants
+    if args.config2:
+        result["config2"] = config2_bench(args, torch, sx, L)
     if args.batch > 0:
         result["batch"] = batch_bench(args, torch, sx, L)
     if args.dns_steps > 0:
         result["dns"] = dns_bench(args, torch, sx, L)
     print(json.dumps(result), flush=True)
+
+
+# inner-iteration bytes per PCG iteration per plane: s (3 nPlane + 17 npts),
+# the BLAS-1 and gather/scatter traffic of semtex's loop (src/field.cpp:
+# 676-703) with s bytes per vector value (the BF16 operator reads FP32
+# vectors, so s = 4 for it too)
+PCG_VALUE_BYTES = {"fp64": 8, "fp32_refined": 4, "bf16_refined": 4}
+
+
+def config2_bench(args, torch, sx, L):
+    """The mixed-precision workload: 32x32 quads np=11 on [0,2pi]^2, Nz=128
+    (127 planes), lambda2_k = k^2 (beta = 1), Dirichlet, seeded smooth
+    solutions; MIXED (mode 0 FP64 DIRECT, modes 1..63 Jacobi-PCG) with the
+    PCG in FP64, in FP32 with FP64 refinement, and with the element operator
+    on the BF16 matrix cores (FP32 vectors, FP64 refinement), each to
+    |r| <= 1e-10 |b|.  The error is measured against the FP64 PCG solution
+    at TOL_REL 1e-13."""
+    import ctypes as C
+    from semtex_amd.workloads import helmholtz_batch, workload_mesh
+    dev = torch.device("cuda:0")
+    np_, m, nz = workload_mesh("square32_np11_nz128")
+    progress("config2 setup (mode 0 factorisation on the host)")
+    t0 = time.time()
+    P = sx.Mesh(np_, m)
+    S = sx.ModalMatrixSys(P, 0.0, 1.0, 0, nz // 2, nz, sx.MIXED)
+    t_setup = time.time() - t0
+    f, bc, ue = helmholtz_batch(P, nz, 0.0, 1.0, seed=2)
+    tf = torch.from_numpy(f.ravel().copy()).to(dev)
+    tb = torch.from_numpy(bc.ravel().copy()).to(dev)
+    tu = torch.zeros_like(tf)
+    npts = P.nglobal + P.nel * P.nint
+    nplanes = nz - 1
+    stream = torch.cuda.current_stream()
+    settings = {"fp64": (sx.PCG_FP64, 1e-5, 1), "fp32_refined": (sx.PCG_FP32_REFINED, 1e-4, 40),
+                "bf16_refined": (sx.PCG_BF16_REFINED, 1e-2, 100)}
+    S.set_pcg(sx.PCG_FP64)
+    tu.zero_()
+    sx.solve(S, tf, tb, tu, tol_rel=1e-13, step_max=20000)
+    torch.cuda.synchronize()
+    uref = tu.clone()
+    out = {"workload": "32x32 quads np=11 on [0,2pi]^2, Nz=128 (127 planes), lambda2_k=k^2, "
+                       "Dirichlet, seeded smooth solutions; MIXED: mode 0 FP64 DIRECT, modes "
+                       "1..63 Jacobi-PCG to |r| <= 1e-10 |b|",
+           "dof_per_plane": npts, "planes": nplanes, "setup_s": round(t_setup, 2),
+           "bytes_per_iteration_per_plane_formula": "s*(3*nPlane + 17*npts), s = value bytes"}
+    for label, (prec, itol, mo) in settings.items():
+        S.set_pcg(prec, inner_tol=itol, max_outer=mo)
+        progress(f"config2 {label}")
+        tu.zero_()
+        it = sx.solve(S, tf, tb, tu, tol_rel=1e-10, step_max=20000)
+        torch.cuda.synchronize()
+        pits = S.plane_iters()
+        err = float(torch.linalg.norm(tu - uref) / torch.linalg.norm(uref))
+        reps = 3
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            tu.zero_()
+            sx.solve(S, tf, tb, tu, tol_rel=1e-10, step_max=20000, iters=False)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        L.semx_profile_reset()
+        L.semx_profile_enable(1)
+        tu.zero_()
+        sx.solve(S, tf, tb, tu, tol_rel=1e-10, step_max=20000, iters=False)
+        torch.cuda.synchronize()
+        tot = C.c_double(); cnt = C.c_longlong()
+        L.semx_profile_query(b"k_pcg", C.byref(tot), C.byref(cnt))
+        L.semx_profile_enable(0)
+        pcg_ms = tot.value / max(cnt.value, 1)
+        sb = PCG_VALUE_BYTES[label]
+        byts = float(np.abs(pits).sum()) * sb * (3 * P.nPlane + 17 * npts)
+        out[label] = {"ms_per_step": ms, "value": npts * nplanes / (ms * 1e-3),
+                      "iterations_max": it, "iterations_sum": int(np.abs(pits).sum()),
+                      "rel_l2_vs_fp64_pcg_1e-13": err, "k_pcg_ms": pcg_ms,
+                      "k_pcg_inner_bytes": byts,
+                      "k_pcg_inner_gb_s": byts / (pcg_ms * 1e-3) / 1e9}
+    for label in ("fp32_refined", "bf16_refined"):
+        out[label]["speedup_vs_fp64_pcg"] = out["fp64"]["ms_per_step"] / out[label]["ms_per_step"]
+    return out
 
 
 def batch_bench(args, torch, sx, L):
@@ -318,7 +404,7 @@ def batch_bench(args, torch, sx, L):
     dev = torch.device("cuda:0")
     progress(f"batch setup ({args.batch} problems)")
     t0 = time.time()
-    meshes, sys64, sys32, fs, bcs = [], [], [], [], []
+    meshes, sys64, sys32, sys16, fs, bcs = [], [], [], [], [], []
     dofs = 0
     for (np_, K, lam, cs) in batch_problems(args.batch, seed=4):
         P = sx.Mesh(np_, rect_mesh(K, K, 0, 2 * np.pi, 0, 2 * np.pi))
@@ -326,20 +412,28 @@ def batch_bench(args, torch, sx, L):
         S64 = sx.ModalMatrixSys(P, lam, 1.0, 0, 1, 1, sx.JACPCG)
         S32 = sx.ModalMatrixSys(P, lam, 1.0, 0, 1, 1, sx.JACPCG)
         S32.set_pcg(sx.PCG_FP32_REFINED, inner_tol=1e-4, max_outer=30)
-        meshes.append(P); sys64.append(S64); sys32.append(S32)
+        S16 = sx.ModalMatrixSys(P, lam, 1.0, 0, 1, 1, sx.JACPCG)
+        S16.set_pcg(sx.PCG_BF16_REFINED, inner_tol=1e-2, max_outer=80)
+        meshes.append(P); sys64.append(S64); sys32.append(S32); sys16.append(S16)
         fs.append(torch.from_numpy(f.ravel().copy()).to(dev))
         bcs.append(torch.from_numpy(bc.ravel().copy()).to(dev))
         dofs += P.nglobal + P.nel * P.nint
-    B64, B32 = sx.Batch(sys64), sx.Batch(sys32)
+    B64, B32, B16 = sx.Batch(sys64), sx.Batch(sys32), sx.Batch(sys16)
     u64 = [torch.zeros_like(f) for f in fs]
     u32 = [torch.zeros_like(f) for f in fs]
+    u16 = [torch.zeros_like(f) for f in fs]
     t_setup = time.time() - t0
     it64 = B64.solve(fs, bcs, u64, tol_rel=1e-13)
     it32 = B32.solve(fs, bcs, u32, tol_rel=1e-10)
+    it16 = B16.solve(fs, bcs, u16, tol_rel=1e-10)
     torch.cuda.synchronize()
-    num = sum(float(((a - b) ** 2).sum().item()) for a, b in zip(u32, u64))
-    den = sum(float((b ** 2).sum().item()) for b in u64)
-    rel = (num / den) ** 0.5
+
+    def rel_to_fp64(us):
+        num = sum(float(((a - b) ** 2).sum().item()) for a, b in zip(us, u64))
+        den = sum(float((b ** 2).sum().item()) for b in u64)
+        return (num / den) ** 0.5
+    rel = rel_to_fp64(u32)
+    rel16 = rel_to_fp64(u16)
     progress(f"batch setup {t_setup:.1f} s; timing")
     stream = torch.cuda.current_stream()
 
@@ -364,6 +458,7 @@ def batch_bench(args, torch, sx, L):
         return tot / reps
 
     ms32 = timed(B32, u32, 1e-10, 5)
+    ms16 = timed(B16, u16, 1e-10, 5)
     ms64 = timed(B64, u64, 1e-10, 5)
     return {
         "metric": "DOF-solves/s",
@@ -378,6 +473,13 @@ def batch_bench(args, torch, sx, L):
         "fp64_pcg_ms_per_step": ms64,
         "fp64_pcg_value": dofs / (ms64 * 1e-3),
         "rel_l2_vs_fp64": rel,
+        "speedup_vs_fp64_pcg": ms64 / ms32,
+        "bf16_refined": {"ms_per_step": ms16, "value": dofs / (ms16 * 1e-3),
+                         "iterations_max": it16, "rel_l2_vs_fp64": rel16,
+                         "speedup_vs_fp64_pcg": ms64 / ms16,
+                         "what": "the same batch with the inner element operator on the BF16 "
+                                 "matrix cores (np padded to 16, FP32 accumulation), FP32 "
+                                 "vectors, FP64 refinement"},
         "setup_s": round(t_setup, 2),
     }
 

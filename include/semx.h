@@ -56,6 +56,10 @@ extern "C" {
 /* --- JACPCG iteration precision (semx_helm_set_pcg) ------------------- */
 #define SEMX_PCG_FP64          0   /* FP64 vectors, semtex's loop exactly    */
 #define SEMX_PCG_FP32_REFINED  1   /* FP32 inner PCG, FP64 refinement        */
+#define SEMX_PCG_BF16_REFINED  2   /* FP32 inner PCG whose element operator   *
+                                    * runs on the BF16 matrix cores (np <= 16,*
+                                    * padded to 16, FP32 accumulation), FP64  *
+                                    * refinement                              */
 
 /* --- transform direction (femlib/cfemdef.h TransformKind) --------------- */
 #define SEMX_FORWARD  1
@@ -151,6 +155,13 @@ int semx_helm_solve (semx_helm helm, double* f, const double* bc, double* u,
  * replays) call this at their own synchronisation points.                */
 int semx_helm_check (semx_helm helm, void* stream);
 
+/* Per-plane JACPCG iteration counts of the last solve of the system (its
+ * first field): iters[nz] (host), summed over inner solves with refinement,
+ * negative where the step limit was hit, 0 for planes solved directly and
+ * for the skipped Nyquist plane.  semtex reports the count per plane in its
+ * verbose output (src/field.cpp:714-718).  Waits for stream.             */
+int semx_helm_iters (semx_helm helm, int* iters, void* stream);
+
 /* Field::solve for nfield (1..3) fields that share one ModalMatrixSys, as
  * the velocity components u, v, w do in semtex (the MatrixSys cache,
  * src/matrix.cpp:85-91, and Solve per component, dns/integrate.cpp:
@@ -167,7 +178,13 @@ int semx_helm_solve_fields (semx_helm helm, int nfield, double* const* f,
  * SEMX_PCG_FP32_REFINED has no counterpart in semtex: the residual
  * r = b - A x is formed in FP64, A d = r is solved by FP32 Jacobi-PCG to
  * relative residual inner_tol, x += d, repeated (at most max_outer times)
- * until |r| <= tol_rel |b| in FP64 -- the same stopping test as FP64.     */
+ * until |r| <= tol_rel |b| in FP64 -- the same stopping test as FP64.
+ * SEMX_PCG_BF16_REFINED is the same refinement with the inner iteration's
+ * element operator (Element::HelmholtzOp, src/element.cpp:1621-1650) on the
+ * BF16 matrix cores: operands rounded to BF16, FP32 accumulation; the FP64
+ * residuals use the FP64 operator.  Its inner operator is accurate to about
+ * 1e-2 relative, so inner_tol of order 1e-2 and a larger max_outer suit it.
+ * SEMX_E_UNSUPPORTED for np > 16.                                         */
 int semx_helm_set_pcg (semx_helm helm, int precision, double inner_tol,
                        int max_outer);
 

@@ -26,13 +26,13 @@ from ._capi import SemxError, check, load
 from .mesh import bc_lines, rect_mesh, side_nodes
 
 DIRECT, JACPCG, MIXED = 0, 1, 2
-PCG_FP64, PCG_FP32_REFINED = 0, 1
+PCG_FP64, PCG_FP32_REFINED, PCG_BF16_REFINED = 0, 1, 2
 E_MAXITER = 3
 FORWARD, INVERSE = 1, -1
 
 __all__ = ["Mesh", "ModalMatrixSys", "solve", "solve_fields", "Batch", "dft_r", "gradient", "smooth", "DNS",
            "DIRECT", "JACPCG", "MIXED",
-           "PCG_FP64", "PCG_FP32_REFINED",
+           "PCG_FP64", "PCG_FP32_REFINED", "PCG_BF16_REFINED",
            "rect_mesh", "bc_lines", "side_nodes", "SemxError", "load"]
 
 
@@ -112,10 +112,18 @@ class ModalMatrixSys:
                          factor_kB=int(info[6]), singular0=int(info[7]))
 
     def set_pcg(self, precision=PCG_FP64, inner_tol=1e-5, max_outer=12):
-        """Precision of the JACPCG planes (FP64, or FP32 inner PCG under FP64
-        iterative refinement)."""
+        """Precision of the JACPCG planes (FP64, or FP32 inner PCG -- element
+        operator in FP32 or on the BF16 matrix cores -- under FP64 iterative
+        refinement)."""
         check(load().semx_helm_set_pcg(self.h, int(precision), float(inner_tol),
                                        int(max_outer)), "semx_helm_set_pcg")
+
+    def plane_iters(self, stream=None):
+        """Per-plane JACPCG iteration counts of the last solve (0 for planes
+        solved directly and for the Nyquist plane; negative at the limit)."""
+        it = np.zeros(self.nz, dtype=np.int32)
+        check(load().semx_helm_iters(self.h, _ip(it), _stream(stream)), "semx_helm_iters")
+        return it
 
     def element(self, m, e):
         M = self.mesh

@@ -34,6 +34,7 @@ PROTOTYPES = {
     "semx_helm_element": (C.c_int, [VP, C.c_int, C.c_int, D, D, D]),
     "semx_helm_solve": (C.c_int, [VP, VP, VP, VP, C.c_double, C.c_int, I, VP]),
     "semx_helm_check": (C.c_int, [VP, VP]),
+    "semx_helm_iters": (C.c_int, [VP, I, VP]),
     "semx_helm_set_pcg": (C.c_int, [VP, C.c_int, C.c_double, C.c_int]),
     "semx_helm_solve_fields": (C.c_int, [VP, C.c_int, C.POINTER(VP), C.POINTER(VP),
                                          C.POINTER(VP), C.c_double, C.c_int, I, VP]),

@@ -24,6 +24,7 @@ namespace semx {
 void prof_begin(hipStream_t s, hipEvent_t* a);
 void prof_end(hipStream_t s, const char* name, hipEvent_t a);
 void prof_flush();
+bool prof_enabled();
 
 template <class T>
 inline int dev_upload(T** dptr, const T* src, size_t count) {

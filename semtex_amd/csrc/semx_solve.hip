@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <type_traits>
 
 #include "semx_hip.h"
 
@@ -1852,14 +1853,164 @@ __device__ double pcg_matvec_wave(const PcgOp& a, int n, const T* __restrict__ x
   return block_sum(wy, red);
 }
 
-template <typename T, bool MASK, bool WV>
-__device__ inline double pcg_op(const PcgOp& a, int n, const T* x, T* y, T* lb, const T* w,
-                                T* sh, double* red) {
-  if constexpr (WV) return pcg_matvec_wave<T, MASK>(a, n, x, y, lb, w, sh, red);
-  else return pcg_matvec<T, MASK>(a, n, x, y, lb, w, sh, red);
+// The operator on the BF16 matrix cores (SEMX_PCG_BF16_REFINED): each
+// wavefront takes one element at a time, zero-padded to 16 x 16, and forms
+// the four tensor-product contractions of HelmholtzOp (src/element.cpp:
+// 1621-1650; R = U T, S = D U, then T S' + R' D with R' = Q1 R + Q3 S,
+// S' = Q2 S + Q3 R) as four v_mfma_f32_16x16x16_bf16 with FP32
+// accumulation: operands rounded to BF16 (nearest even), everything else --
+// the geometric factors, the mass term (lambda2 + betak2) Q4 U and the
+// sums -- in FP32.  Lane l holds, for v = 0..3, A operands at (row l%16,
+// column 4(l/16)+v), B operands and results at (row 4(l/16)+v, column
+// l%16); U and R' change from the second layout to the first through a
+// private LDS tile (stride 20 floats) with wavefront synchronisation only.
+// The next element's values are gathered while the current one is formed.
+// np <= 16; LDS: PCG_WAVES * 2 * 16 * 20 floats.
+typedef short mf_bf4 __attribute__((ext_vector_type(4)));
+typedef float mf_f4 __attribute__((ext_vector_type(4)));
+constexpr int MF_LD = 20;                   // LDS tile row stride (floats)
+constexpr int MF_TILE = 16 * MF_LD;
+
+__device__ inline short bf16_rne(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
 }
 
-template <typename T, bool WV = false>
+__device__ inline mf_bf4 bf16x4(float a, float b, float c, float d) {
+  mf_bf4 r;
+  r[0] = bf16_rne(a); r[1] = bf16_rne(b); r[2] = bf16_rne(c); r[3] = bf16_rne(d);
+  return r;
+}
+
+template <bool MASK = true>
+__device__ double pcg_matvec_mfma(const PcgOp& a, int n, const float* __restrict__ x,
+                                  float* __restrict__ y, float* __restrict__ lb,
+                                  const float* __restrict__ w, float* sh, double* red) {
+  const int tid = threadIdx.x, np = a.np, npnp = a.npnp, next = a.next, nint = a.nint;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int c = lane & 15, rb = 4 * (lane >> 4);
+  float* tU = sh + wave * 2 * MF_TILE;
+  float* tR = tU + MF_TILE;
+  const double* D = a.ops + (a.svv ? 2 : 0) * npnp;
+  const double* Tm = a.ops + (a.svv ? 3 : 1) * npnp;
+  // operator operands of this lane, zero outside np x np
+  auto ent = [&](const double* M, int i, int j) {
+    return (i < np && j < np) ? (float)M[i * np + j] : 0.0f;
+  };
+  const mf_bf4 aD = bf16x4(ent(D, c, rb), ent(D, c, rb + 1), ent(D, c, rb + 2), ent(D, c, rb + 3));
+  const mf_bf4 aT = bf16x4(ent(Tm, c, rb), ent(Tm, c, rb + 1), ent(Tm, c, rb + 2),
+                           ent(Tm, c, rb + 3));
+  const mf_bf4 bT = bf16x4(ent(Tm, rb, c), ent(Tm, rb + 1, c), ent(Tm, rb + 2, c),
+                           ent(Tm, rb + 3, c));
+  const mf_bf4 bD = bf16x4(ent(D, rb, c), ent(D, rb + 1, c), ent(D, rb + 2, c), ent(D, rb + 3, c));
+  // this lane's four nodes (row rb+v, column c): emap index or -1
+  int pm[4], pt[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const bool in = rb + v < np && c < np;
+    pt[v] = in ? (rb + v) * np + c : 0;
+    pm[v] = in ? a.pmap[pt[v]] : -1;
+  }
+  const float hcon = (float)a.hcon;
+  auto gather = [&](int e, float (&u)[4]) {
+    const int ec = min(e, a.nel - 1);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      float xv = 0.0f;
+      if (pm[v] >= 0) {
+        if (pm[v] < next) {
+          const int gid = a.btog[ec * next + pm[v]];
+          const float t = x[gid];
+          xv = (!MASK || gid < n) ? t : 0.0f;
+        } else {
+          xv = x[a.nglobal + ec * nint + (pm[v] - next)];
+        }
+      }
+      u[v] = xv;
+    }
+  };
+  double wy = 0.0;
+  float cur[4], nxt[4];
+  gather(wave, cur);
+  for (int e = wave; e < a.nel; e += PCG_WAVES) {
+    gather(e + PCG_WAVES, nxt);
+    const double* g = a.G + (size_t)a.eset[e] * G_N * npnp;
+    float q1[4], q2[4], q3[4], q4[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const bool in = pm[v] >= 0;
+      q1[v] = in ? (float)g[G_Q1 * npnp + pt[v]] : 0.0f;
+      q2[v] = in ? (float)g[G_Q2 * npnp + pt[v]] : 0.0f;
+      q3[v] = in ? (float)g[G_Q3 * npnp + pt[v]] : 0.0f;
+      q4[v] = in ? (float)g[G_Q4 * npnp + pt[v]] : 0.0f;
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) tU[(rb + v) * MF_LD + c] = cur[v];
+    wave_sync();
+    const mf_f4 ua = *(const mf_f4*)&tU[c * MF_LD + rb];
+    const mf_f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    const mf_f4 S = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
+        aD, bf16x4(cur[0], cur[1], cur[2], cur[3]), zero, 0, 0, 0);
+    const mf_f4 R = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
+        bf16x4(ua[0], ua[1], ua[2], ua[3]), bT, zero, 0, 0, 0);
+    float rp[4], sp[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      rp[v] = q1[v] * R[v] + q3[v] * S[v];
+      sp[v] = q2[v] * S[v] + q3[v] * R[v];
+      tR[(rb + v) * MF_LD + c] = rp[v];
+    }
+    wave_sync();
+    const mf_f4 ra = *(const mf_f4*)&tR[c * MF_LD + rb];
+    mf_f4 o = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aT, bf16x4(sp[0], sp[1], sp[2], sp[3]),
+                                                        zero, 0, 0, 0);
+    o = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4(ra[0], ra[1], ra[2], ra[3]), bD, o, 0,
+                                                  0, 0);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      if (pm[v] < 0) continue;
+      const float out = o[v] + q4[v] * cur[v] * hcon;
+      if (pm[v] < next) {
+        lb[e * next + pm[v]] = out;
+      } else {
+        const int idx = a.nglobal + e * nint + (pm[v] - next);
+        y[idx] = out;
+        wy += (double)w[idx] * (double)out;
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) cur[v] = nxt[v];
+  }
+  __syncthreads();
+  for (int gI = tid; gI < a.nglobal; gI += PCG_THREADS) {
+    float v = 0;
+    if (!MASK || gI < n)
+      for (int cc = a.asm_ptr[gI]; cc < a.asm_ptr[gI + 1]; ++cc) v += lb[a.asm_idx[cc]];
+    y[gI] = v;
+    wy += (double)w[gI] * (double)v;
+  }
+  return block_sum(wy, red);
+}
+
+// Element-operator variants: the block-wide and wavefront-level scalar
+// operators (any T) and the BF16 matrix-core operator (T = float).
+enum { OPK_BLOCK = 0, OPK_WAVE = 1, OPK_MFMA = 2 };
+
+template <typename T, bool MASK, int OPK>
+__device__ inline double pcg_op(const PcgOp& a, int n, const T* x, T* y, T* lb, const T* w,
+                                T* sh, double* red) {
+  if constexpr (OPK == OPK_MFMA) {
+    static_assert(std::is_same<T, float>::value, "the BF16 operator works on FP32 vectors");
+    return pcg_matvec_mfma<MASK>(a, n, x, y, lb, w, sh, red);
+  } else if constexpr (OPK == OPK_WAVE) {
+    return pcg_matvec_wave<T, MASK>(a, n, x, y, lb, w, sh, red);
+  } else {
+    return pcg_matvec<T, MASK>(a, n, x, y, lb, w, sh, red);
+  }
+}
+
+template <typename T, int OPK = OPK_BLOCK>
 __device__ int pcg_core(const PcgOp& a, int n, T* x, T* r, T* p, T* q, T* lb, const T* pc,
                         double epsb2, int step_max, T* sh, double* red) {
   const int tid = threadIdx.x, npts = a.npts;
@@ -1878,7 +2029,7 @@ __device__ int pcg_core(const PcgOp& a, int n, T* x, T* r, T* p, T* q, T* lb, co
       const T z = pc[i] * r[i];
       p[i] = (it == 1) ? z : z + beta * p[i];
     }
-    const double pq = pcg_op<T, true, WV>(a, n, p, q, lb, p, sh, red);
+    const double pq = pcg_op<T, true, OPK>(a, n, p, q, lb, p, sh, red);
     const T alpha = (T)(rho1 / pq);
     double nr2 = 0.0, nrho = 0.0;
     for (int i = tid; i < npts; i += PCG_THREADS) {
@@ -1901,19 +2052,21 @@ __device__ int pcg_core(const PcgOp& a, int n, T* x, T* r, T* p, T* q, T* lb, co
 // PCG on A d = r.  Vectors: X, R, B, P, Q of npts and LB of nel*next
 // doubles; the inner T vectors share the P/Q/LB space.  Returns the
 // iteration count (summed over inner solves), negative at the limit.
-template <typename T, bool REFINE, bool WV = false>
+// OPK: the operator of the inner T-precision iteration; OP64: the one of
+// the FP64 residuals.
+template <typename T, bool REFINE, int OPK = OPK_BLOCK, int OP64 = OPK_BLOCK>
 __device__ int pcg_run(const PcgOp& op, int n, double* X, double* R, const double* B,
                        double* P, double* Q, double* LB, const double* pc, const float* pc32,
                        double epsb2, int step_max, double inner_tol, int max_outer, T* sh,
                        double* red) {
   const int tid = threadIdx.x, npts = op.npts;
   __syncthreads();
-  pcg_op<double, true, WV>(op, n, X, Q, LB, X, (double*)sh, red);
+  pcg_op<double, true, OP64>(op, n, X, Q, LB, X, (double*)sh, red);
   for (int i = tid; i < npts; i += PCG_THREADS) R[i] = B[i] - Q[i];
   __syncthreads();
   int its;
   if constexpr (!REFINE) {
-    its = pcg_core<double, WV>(op, n, X, R, P, Q, LB, pc, epsb2, step_max, (double*)sh, red);
+    its = pcg_core<double, OP64>(op, n, X, R, P, Q, LB, pc, epsb2, step_max, (double*)sh, red);
   } else {
     T* d = (T*)P;
     T* rr = d + npts;
@@ -1933,12 +2086,12 @@ __device__ int pcg_run(const PcgOp& op, int n, double* X, double* R, const doubl
       }
       __syncthreads();
       const double ein = inner_tol * inner_tol * rn;
-      const int it = pcg_core<T, WV>(op, n, d, rr, pp, qq, lb, (const T*)pc32, ein, step_max, sh,
-                                 red);
+      const int it = pcg_core<T, OPK>(op, n, d, rr, pp, qq, lb, (const T*)pc32, ein, step_max,
+                                      sh, red);
       its += (it < 0) ? -it : it;
       for (int i = tid; i < npts; i += PCG_THREADS) X[i] += (double)d[i];
       __syncthreads();
-      pcg_op<double, true, WV>(op, n, X, Q, LB, X, (double*)sh, red);
+      pcg_op<double, true, OP64>(op, n, X, Q, LB, X, (double*)sh, red);
       for (int i = tid; i < npts; i += PCG_THREADS) R[i] = B[i] - Q[i];
       __syncthreads();
     }
@@ -1986,7 +2139,7 @@ __device__ inline void pcg_store(const PcgOp& op, int n, int nsolve, const doubl
   }
 }
 
-template <typename T, bool REFINE>
+template <typename T, bool REFINE, int OPK = OPK_BLOCK>
 __global__ __launch_bounds__(PCG_THREADS) void k_pcg(PcgArgs a) {
   extern __shared__ double pcg_lds[];
   double* red = pcg_lds;               // 32 doubles
@@ -2026,7 +2179,7 @@ __global__ __launch_bounds__(PCG_THREADS) void k_pcg(PcgArgs a) {
   }
   const double epsb = a.tol * sqrt(block_sum(bb, red));
   pcg_x0(op, a.emap, n, uk, X);
-  const int its = pcg_run<T, REFINE>(op, n, X, R, B, P, Q, LB, a.pc + (size_t)m * npts,
+  const int its = pcg_run<T, REFINE, OPK>(op, n, X, R, B, P, Q, LB, a.pc + (size_t)m * npts,
                                      a.pc32 + (size_t)m * npts, epsb * epsb, a.step_max,
                                      a.inner_tol, a.max_outer, sh, red);
   const double* gp = a.g + (size_t)pidx * a.nzero;
@@ -2064,7 +2217,7 @@ struct PcgBatchArgs {
 // (Field::constrain and buildRHS, src/field.cpp:728-768, 859-925, with the
 // lifting A g of the essential values g as one unmasked operator
 // application), x0, the iteration and the essential re-imposition.
-template <typename T, bool REFINE>
+template <typename T, bool REFINE, int OPK = OPK_WAVE>
 __global__ __launch_bounds__(PCG_THREADS) void k_pcg_batch(PcgBatchArgs ba) {
   extern __shared__ double pcg_lds[];
   double* red = pcg_lds;
@@ -2087,7 +2240,7 @@ __global__ __launch_bounds__(PCG_THREADS) void k_pcg_batch(PcgBatchArgs ba) {
   for (int i = tid; i < npts; i += PCG_THREADS)
     P[i] = (i >= nsolve && i < nglobal) ? bc[I.ess_src[i - nsolve]] : 0.0;
   __syncthreads();
-  pcg_op<double, false, true>(op, n, P, Q, LB, P, (double*)sh, red);
+  pcg_op<double, false, OPK_WAVE>(op, n, P, Q, LB, P, (double*)sh, red);
   // b = -(M f) - A g + <h,w>, rows of essential and pinned nodes zero
   double bb = 0.0;
   for (int gI = tid; gI < nglobal; gI += PCG_THREADS) {
@@ -2118,8 +2271,9 @@ __global__ __launch_bounds__(PCG_THREADS) void k_pcg_batch(PcgBatchArgs ba) {
   }
   const double epsb = ba.tol * sqrt(block_sum(bb, red));
   pcg_x0(op, I.emap, n, uk, X);
-  const int its = pcg_run<T, REFINE, true>(op, n, X, R, B, P, Q, LB, I.pc, I.pc32, epsb * epsb,
-                                     ba.step_max, ba.inner_tol, ba.max_outer, sh, red);
+  const int its = pcg_run<T, REFINE, OPK, OPK_WAVE>(op, n, X, R, B, P, Q, LB, I.pc, I.pc32,
+                                                   epsb * epsb, ba.step_max, ba.inner_tol,
+                                                   ba.max_outer, sh, red);
   pcg_store(op, n, nsolve, X, uk, [&](int gid) { return bc[I.ess_src[gid - nsolve]]; });
   if (tid == 0) ba.iters[blockIdx.x] = its;
 }
@@ -2132,6 +2286,7 @@ namespace semx {
 struct ProfRec { const char* name; hipEvent_t a, b; };
 struct ProfAcc { const char* name; double ms; long long count; };
 static bool g_prof = false;
+bool prof_enabled() { return g_prof; }
 static std::vector<ProfRec> g_pending;
 static std::vector<ProfAcc> g_acc;
 
@@ -2699,10 +2854,15 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
     pa.tol = tol_rel; pa.step_max = step_max; pa.inner_tol = H.inner_tol;
     pa.max_outer = H.max_outer; pa.iters = H.d.iters;
     const int cep = PCG_THREADS / M.npnp;
-    const bool refine = H.pcg_prec == SEMX_PCG_FP32_REFINED;
-    const size_t lds = 32 * sizeof(double) +
+    size_t lds = 32 * sizeof(double) +
         (2 * (size_t)M.npnp + 3 * (size_t)cep * M.npnp) * sizeof(double);
-    const void* kfn = refine ? (const void*)k_pcg<float, true> : (const void*)k_pcg<double, false>;
+    const void* kfn = (const void*)k_pcg<double, false>;
+    if (H.pcg_prec == SEMX_PCG_FP32_REFINED) {
+      kfn = (const void*)k_pcg<float, true>;
+    } else if (H.pcg_prec == SEMX_PCG_BF16_REFINED) {
+      kfn = (const void*)k_pcg<float, true, OPK_MFMA>;
+      lds = std::max(lds, 32 * sizeof(double) + (size_t)PCG_WAVES * 2 * MF_TILE * sizeof(float));
+    }
     if (lds > 64 * 1024)
       HIPCHK(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const size_t npcg = H.pcg_plane.size();
@@ -2916,6 +3076,20 @@ int semx_helm_solve(semx_helm helm, double* f, const double* bc, double* u, doub
   return helm_solve(helm->H, f, bc, u, tol_rel, step_max, iters, (hipStream_t)stream);
 }
 
+int semx_helm_iters(semx_helm helm, int* iters, void* stream) {
+  if (!helm || !iters) return SEMX_E_ARG;
+  Helm& H = helm->H;
+  for (int k = 0; k < H.nz; ++k) iters[k] = 0;
+  if (H.pcg_plane.empty() || !H.d.iters) return SEMX_OK;
+  hipStream_t s = (hipStream_t)stream;
+  std::vector<int> it(H.pcg_plane.size());
+  HIPCHK(hipMemcpyAsync(it.data(), H.d.iters, it.size() * sizeof(int), hipMemcpyDeviceToHost,
+                        s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (size_t i = 0; i < it.size(); ++i) iters[H.plane_of[H.pcg_plane[i]]] = it[i];
+  return SEMX_OK;
+}
+
 int semx_helm_check(semx_helm helm, void* stream) {
   if (!helm) return SEMX_E_ARG;
   if (!helm->H.on_device) return SEMX_OK;
@@ -2989,6 +3163,9 @@ int semx_batch_create(int count, const semx_helm* helms, semx_batch* batch) {
     if (M.np > 13) { batch_free(b); return SEMX_E_UNSUPPORTED; }
     b->lds = std::max(b->lds, 32 * sizeof(double) +
                       (2 * (size_t)M.npnp + 3 * (size_t)PCG_WAVES * M.npnp) * sizeof(double));
+    if (b->prec == SEMX_PCG_BF16_REFINED)
+      b->lds = std::max(b->lds, 32 * sizeof(double) +
+                        (size_t)PCG_WAVES * 2 * MF_TILE * sizeof(float));
   }
   st = dev_upload(&b->inst, inst.data(), inst.size());
   if (st) { batch_free(b); return st; }
@@ -3024,9 +3201,9 @@ int semx_batch_solve(semx_batch batch, const double* const* f, const double* con
   a.u = (double* const*)(b.ptrs + 2 * b.count);
   a.iters = b.iters; a.tol = tol_rel; a.inner_tol = b.inner_tol;
   a.step_max = step_max; a.max_outer = b.max_outer;
-  const bool refine = b.prec == SEMX_PCG_FP32_REFINED;
-  const void* kfn = refine ? (const void*)k_pcg_batch<float, true>
-                           : (const void*)k_pcg_batch<double, false>;
+  const void* kfn = (const void*)k_pcg_batch<double, false>;
+  if (b.prec == SEMX_PCG_FP32_REFINED) kfn = (const void*)k_pcg_batch<float, true>;
+  else if (b.prec == SEMX_PCG_BF16_REFINED) kfn = (const void*)k_pcg_batch<float, true, OPK_MFMA>;
   if (b.lds > 64 * 1024)
     HIPCHK(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b.lds));
   void* kargs[] = {&a};
@@ -3050,13 +3227,16 @@ int semx_batch_solve(semx_batch batch, const double* const* f, const double* con
 
 int semx_helm_set_pcg(semx_helm helm, int precision, double inner_tol, int max_outer) {
   if (!helm) return SEMX_E_ARG;
-  if (precision != SEMX_PCG_FP64 && precision != SEMX_PCG_FP32_REFINED) return SEMX_E_ARG;
-  if (precision == SEMX_PCG_FP32_REFINED &&
-      (!(inner_tol > 0.0 && inner_tol < 1.0) || max_outer < 1))
+  if (precision != SEMX_PCG_FP64 && precision != SEMX_PCG_FP32_REFINED &&
+      precision != SEMX_PCG_BF16_REFINED)
     return SEMX_E_ARG;
+  const bool refine = precision != SEMX_PCG_FP64;
+  if (refine && (!(inner_tol > 0.0 && inner_tol < 1.0) || max_outer < 1)) return SEMX_E_ARG;
   Helm& H = helm->H;
+  // the matrix-core operator pads elements to 16 x 16
+  if (precision == SEMX_PCG_BF16_REFINED && H.M->np > 16) return SEMX_E_UNSUPPORTED;
   H.pcg_prec = precision;
-  if (precision == SEMX_PCG_FP32_REFINED) { H.inner_tol = inner_tol; H.max_outer = max_outer; }
+  if (refine) { H.inner_tol = inner_tol; H.max_outer = max_outer; }
   return SEMX_OK;
 }
 
