@@ -527,6 +527,20 @@ def dns_bench(args, torch, sx, L):
                           "launches_per_step": cnt.value / args.dns_steps}
     L.semx_profile_enable(0)
     umax = float(u.abs().max().item())
+    # the same steps replayed from HIP graphs: one per buffer-role assignment
+    # (a cycle of 88 steps here), captured during the untimed warm-up
+    D.set_graph(True)
+    D.run(100, u, v, w, p)
+    torch.cuda.synchronize()
+    ng0, _ = D.graph_info()
+    e0.record(stream)
+    D.run(args.dns_steps, u, v, w, p)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    gms = e0.elapsed_time(e1) / args.dns_steps
+    ng1, nrep = D.graph_info()
+    D.set_graph(False)
+    progress(f"dns eager {ms:.3f} ms/step, graph replay {gms:.3f} ms/step ({ng1} graphs)")
     # algorithmic HBM bytes of the band sweeps, which bound the step: the
     # velocity and the pressure system, every mode's factor read by the
     # forward and the backward sweep (nsolve x nband doubles per mode)
@@ -544,6 +558,10 @@ def dns_bench(args, torch, sx, L):
         "steps_total": D.steps,
         "max_abs_u_fourier": umax,
         "kernel_ms": prof,
+        "graph": {"ms_per_step": gms, "value": 1e3 / gms, "graphs": ng1,
+                  "captured_in_timed_region": ng1 - ng0,
+                  "what": "the same steps replayed from HIP graphs (semx_dns_set_graph), one "
+                          "graph per role assignment of the step's buffers"},
         "band_hbm": {"bytes_per_step": band_bytes,
                      "achieved_gb_s": band_bytes / (band_ms * 1e-3) / 1e9,
                      "frac_of_peak_in_sweeps": band_bytes / (band_ms * 1e-3) / 8e12,

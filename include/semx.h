@@ -263,6 +263,19 @@ int semx_dns_run (semx_dns dns, int nstep, double* u, double* v, double* w,
                   double* p, void* stream);
 /* info[4] = steps taken, nz, velocity components, norder                  */
 int semx_dns_info (semx_dns dns, int* info);
+/* Replay steady time steps of semx_dns_run from HIP graphs (on != 0) or
+ * launch every kernel (0, the default).  No counterpart in semtex, whose
+ * time loop (dns/integrate.cpp:148-219) calls its operators one by one.
+ * A step permutes the roles of its buffers (transform output and spare,
+ * swapData, the rolled time levels), so a graph is captured, on first use,
+ * for each role assignment the steps meet (a cycle of 88 steps for N_TIME =
+ * 3 with the skew-symmetric terms, 182 with the alternating form) and
+ * replayed whenever that assignment recurs.  Start-up steps, the first
+ * steady step and steps taken while kernel timing is on run eagerly.
+ * Replayed steps equal eager ones bit for bit.                           */
+int semx_dns_set_graph (semx_dns dns, int on);
+/* Graphs captured so far and steps replayed from them.                     */
+int semx_dns_graph_info (semx_dns dns, int* ngraphs, long long* replays);
 
 /* --- kernel timing -------------------------------------------------------
  * No counterpart in semtex (dclock(), veclib/util.c:65-85, is never called

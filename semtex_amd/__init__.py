@@ -310,6 +310,19 @@ class DNS:
         check(load().semx_dns_info(self.h, _ip(info)))
         return int(info[0])
 
+    def set_graph(self, on=True):
+        """Replay steady time steps from HIP graphs (one per buffer-role
+        assignment of the step, captured on first use)."""
+        check(load().semx_dns_set_graph(self.h, int(bool(on))), "semx_dns_set_graph")
+
+    def graph_info(self):
+        """(graphs captured, steps replayed from graphs)."""
+        ng = C.c_int(0)
+        rp = C.c_longlong(0)
+        check(load().semx_dns_graph_info(self.h, C.byref(ng), C.byref(rp)),
+              "semx_dns_graph_info")
+        return ng.value, rp.value
+
     def __del__(self):
         try:
             if getattr(self, "h", None):

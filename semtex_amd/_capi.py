@@ -51,6 +51,8 @@ PROTOTYPES = {
     "semx_dns_destroy": (C.c_int, [VP]),
     "semx_dns_run": (C.c_int, [VP, C.c_int, VP, VP, VP, VP, VP]),
     "semx_dns_info": (C.c_int, [VP, I]),
+    "semx_dns_set_graph": (C.c_int, [VP, C.c_int]),
+    "semx_dns_graph_info": (C.c_int, [VP, C.POINTER(C.c_int), C.POINTER(C.c_longlong)]),
     "semx_malloc": (C.c_int, [C.POINTER(VP), C.c_size_t]),
     "semx_free": (C.c_int, [VP]),
     "semx_memcpy_htod": (C.c_int, [VP, VP, C.c_size_t]),

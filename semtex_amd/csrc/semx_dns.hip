@@ -639,6 +639,29 @@ void extrapolation(int n, double* c) {
 // DNS state: the fields of integrate() held on the device.
 // ===========================================================================
 
+// The buffer roles of the DNS state: which allocation each field of the
+// step uses.  The step permutes them (transform output <-> spare,
+// swapData, the rolled time levels) and returns to a given assignment only
+// after a full cycle of steps (88 steps for N_TIME = 3 with the
+// skew-symmetric terms, 182 with the alternating form).
+struct DnsRoles {
+  double* U[3];
+  double* P;
+  double* Us[3][3];
+  double* Uf[3][3];
+  double* tmp;
+  double* spare;
+  int toggle;
+  bool operator==(const DnsRoles& o) const { return std::memcmp(this, &o, sizeof(*this)) == 0; }
+};
+
+// One steady step captured for one assignment of roles: replaying it leaves
+// the roles as `after`.
+struct DnsGraph {
+  DnsRoles before, after;
+  hipGraphExec_t exec;
+};
+
 struct semx_dns_s {
   Mesh* M = nullptr;
   int nz = 1, ncom = 2, ndim = 2, nord = 2, advection = 0, step = 0, step_max = 500, toggle = 1;
@@ -654,6 +677,10 @@ struct semx_dns_s {
   Helm* vel = nullptr;                  // DIRECT, lambda2 = alpha0/(dt nu)
   Helm* pre = nullptr;                  // DIRECT, lambda2 = 0
   Helm* start[2] = {nullptr, nullptr};  // JACPCG start-up velocity systems
+  bool graphs_on = false;               // replay steady steps from HIP graphs
+  hipStream_t capture = nullptr;        // stream the steps are captured on
+  std::vector<DnsGraph> graphs;         // one per role assignment met
+  long long replays = 0;
 };
 
 namespace {
@@ -840,8 +867,79 @@ int dns_step(semx_dns_s& D, hipStream_t s) {
   return SEMX_OK;
 }
 
+DnsRoles roles_of(const semx_dns_s& D) {
+  DnsRoles r;
+  std::memset(&r, 0, sizeof(r));
+  for (int i = 0; i < 3; ++i) r.U[i] = D.U[i];
+  r.P = D.P;
+  for (int q = 0; q < 3; ++q)
+    for (int i = 0; i < 3; ++i) { r.Us[q][i] = D.Us[q][i]; r.Uf[q][i] = D.Uf[q][i]; }
+  r.tmp = D.tmp;
+  r.spare = D.spare;
+  r.toggle = D.toggle;
+  return r;
+}
+
+void set_roles(semx_dns_s& D, const DnsRoles& r) {
+  for (int i = 0; i < 3; ++i) D.U[i] = r.U[i];
+  D.P = r.P;
+  for (int q = 0; q < 3; ++q)
+    for (int i = 0; i < 3; ++i) { D.Us[q][i] = r.Us[q][i]; D.Uf[q][i] = r.Uf[q][i]; }
+  D.tmp = r.tmp;
+  D.spare = r.spare;
+  D.toggle = r.toggle;
+}
+
+// A step from a HIP graph.  Start-up steps (order < N_TIME, JACPCG
+// systems) and the first steady step (which runs the band-sweep autotuner
+// and allocates the solve scratch) run eagerly, as do all steps while
+// kernel timing is on.  A steady step whose role assignment has not been
+// met is captured on the library's capture stream (the host state
+// advances as in an eager step), instantiated and launched on the caller's
+// stream; later steps with that assignment replay it.  Every kernel of the
+// step takes its buffers by value, so one graph per assignment is exact.
+// If a capture fails, graphs are switched off and the step runs eagerly.
+int dns_step_graph(semx_dns_s& D, hipStream_t s) {
+  if (!D.graphs_on || D.step < D.nord || prof_enabled()) return dns_step(D, s);
+  const DnsRoles cur = roles_of(D);
+  for (const DnsGraph& g : D.graphs) {
+    if (g.before == cur) {
+      HIPCHK(hipGraphLaunch(g.exec, s));
+      set_roles(D, g.after);
+      D.step += 1;
+      D.replays += 1;
+      return SEMX_OK;
+    }
+  }
+  if (!D.capture) HIPCHK(hipStreamCreateWithFlags(&D.capture, hipStreamNonBlocking));
+  const int step0 = D.step;
+  HIPCHK(hipStreamBeginCapture(D.capture, hipStreamCaptureModeRelaxed));
+  const int st = dns_step(D, D.capture);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(D.capture, &graph);
+  hipGraphExec_t exec = nullptr;
+  hipError_t ei = hipErrorUnknown;
+  if (!st && ec == hipSuccess && graph) ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (st || ec != hipSuccess || ei != hipSuccess) {
+    (void)hipGetLastError();
+    fprintf(stderr, "semx: DNS step capture failed (%s); stepping without graphs\n",
+            st ? semx_error_string(st) : hipGetErrorString(ec != hipSuccess ? ec : ei));
+    D.graphs_on = false;
+    set_roles(D, cur);
+    D.step = step0;
+    return dns_step(D, s);
+  }
+  D.graphs.push_back({cur, roles_of(D), exec});
+  HIPCHK(hipGraphLaunch(exec, s));
+  D.replays += 1;
+  return SEMX_OK;
+}
+
 void dns_free(semx_dns_s* D) {
   if (!D) return;
+  for (DnsGraph& g : D->graphs) (void)hipGraphExecDestroy(g.exec);
+  if (D->capture) (void)hipStreamDestroy(D->capture);
   for (Helm* h : {D->vel, D->pre, D->start[0], D->start[1]}) {
     if (!h) continue;
     helm_release(*h);
@@ -1045,7 +1143,7 @@ int semx_dns_run(semx_dns dns, int nstep, double* u, double* v, double* w, doubl
   for (int i = 0; i < D.ncom; ++i)
     HIPCHK(hipMemcpyAsync(D.U[i], ext[i], bytes, hipMemcpyDeviceToDevice, s));
   HIPCHK(hipMemcpyAsync(D.P, p, bytes, hipMemcpyDeviceToDevice, s));
-  for (int k = 0; k < nstep; ++k) CHK(dns_step(D, s));
+  for (int k = 0; k < nstep; ++k) CHK(dns_step_graph(D, s));
   for (int i = 0; i < D.ncom; ++i)
     HIPCHK(hipMemcpyAsync(ext[i], D.U[i], bytes, hipMemcpyDeviceToDevice, s));
   HIPCHK(hipMemcpyAsync(p, D.P, bytes, hipMemcpyDeviceToDevice, s));
@@ -1060,6 +1158,19 @@ int semx_dns_run(semx_dns dns, int nstep, double* u, double* v, double* w, doubl
 int semx_dns_info(semx_dns dns, int* info) {
   if (!dns || !info) return SEMX_E_ARG;
   info[0] = dns->step; info[1] = dns->nz; info[2] = dns->ncom; info[3] = dns->nord;
+  return SEMX_OK;
+}
+
+int semx_dns_set_graph(semx_dns dns, int on) {
+  if (!dns) return SEMX_E_ARG;
+  dns->graphs_on = on != 0;
+  return SEMX_OK;
+}
+
+int semx_dns_graph_info(semx_dns dns, int* ngraphs, long long* replays) {
+  if (!dns || !ngraphs || !replays) return SEMX_E_ARG;
+  *ngraphs = (int)dns->graphs.size();
+  *replays = dns->replays;
   return SEMX_OK;
 }
 
