@@ -193,16 +193,25 @@ def main():
     u, _ = fresh_solve(S)
     err_exact = float(np.abs(u[:, :P.nPlane] - ue[:, :P.nPlane]).max())
 
-    # algorithmic bytes of k_band_solve per launch: every mode's factor read
-    # for the forward and for the backward sweep, plus the condensed RHS in
-    # and the boundary solution out, per plane
+    # algorithmic bytes of k_band_solve per launch: every mode's band factor
+    # (nsolve x nband doubles) read once -- the Re and Im planes share it --
+    # plus the condensed RHS in and the boundary solution out, per plane;
+    # the two-sweep count (the factor streamed by the forward and again by
+    # the backward sweep, as dpbtrs does) is reported beside it
     n0 = S.info["nsolve0"]
     ldab = S.info["ldab"]
-    bytes_factor = (nz // 2) * 2 * n0 * ldab * 8
+    bytes_factor = (nz // 2) * n0 * ldab * 8
     bytes_rhs = nplanes * (P.nel * P.next + n0) * 8
     band_bytes = bytes_factor + bytes_rhs
+    band_bytes2 = 2 * bytes_factor + bytes_rhs
     band_ms = kprof["k_band_solve"]
     achieved = band_bytes / (band_ms * 1e-3) / 1e9
+    # the whole all-mode solve's algorithmic bytes: factor once per mode,
+    # one condensed element-matrix set per mode (one geometric set), forcing
+    # in and solution out per plane, BC lines
+    nline = P.nbound * P.np
+    solve_bytes = 8 * ((nz // 2) * (n0 * ldab + P.nsets * (P.next * P.nint + P.nint ** 2))
+                       + nplanes * (2 * P.nPlane + nline))
 
     # HBM bytes per k_band_solve launch from the committed rocprofv3 counter
     # summary (FETCH_SIZE / WRITE_SIZE scaled by a plain copy kernel)
@@ -242,6 +251,9 @@ def main():
             "traffic": traffic,
             "bytes_per_launch": band_bytes,
             "avg_launch_ms": band_ms,
+            "frac_two_sweeps": band_bytes2 / (band_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "whole_solve_frac": solve_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "whole_solve_bytes": solve_bytes,
         },
         "kernel_ms": kprof,
         "max_abs_err_vs_exact": err_exact,
@@ -284,10 +296,20 @@ def main():
         _, tcpu = O.field_solve(Q, nz, lam2, beta, 0, fo, bc, uo, nthreads=cores, nrep=nrep)
         rel = float(np.linalg.norm(u[:, :P.nPlane] - uo[:, :P.nPlane]) /
                     np.linalg.norm(uo[:, :P.nPlane]))
+        cpu_model = "unknown"
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
         result["cpu_baseline"] = {
             "value": dofs / tcpu,
             "unit": "DOF-solves/s",
             "cores": cores,
+            "cpu_model": cpu_model,
+            "host_threads": os.cpu_count(),
             "kind": "port",
             "sample": f"the same all-mode solve repeated {nrep}x (setup excluded), "
                       f"FP64 oracle, OpenMP over planes; {tcpu * 1e3:.2f} ms per solve",
@@ -434,6 +456,14 @@ def batch_bench(args, torch, sx, L):
         return (num / den) ** 0.5
     rel = rel_to_fp64(u32)
     rel16 = rel_to_fp64(u16)
+
+    def tuner(Bt):
+        ops, tms = Bt.info()
+        picks = {}
+        for P, o in zip(meshes, ops):
+            picks.setdefault(o, set()).add((P.np, P.nel))
+        return {"tune_ms": round(tms, 1),
+                "classes_per_operator": {o: len(c) for o, c in picks.items()}}
     progress(f"batch setup {t_setup:.1f} s; timing")
     stream = torch.cuda.current_stream()
 
@@ -474,6 +504,10 @@ def batch_bench(args, torch, sx, L):
         "fp64_pcg_value": dofs / (ms64 * 1e-3),
         "rel_l2_vs_fp64": rel,
         "speedup_vs_fp64_pcg": ms64 / ms32,
+        "tuner": {"fp64": tuner(B64), "fp32_refined": tuner(B32), "bf16_refined": tuner(B16),
+                  "what": "element operator per (np, element count) class chosen by the batch "
+                          "tuner on the first solve (block-wide or wavefront-level scalar, or "
+                          "BF16 matrix cores)"},
         "bf16_refined": {"ms_per_step": ms16, "value": dofs / (ms16 * 1e-3),
                          "iterations_max": it16, "rel_l2_vs_fp64": rel16,
                          "speedup_vs_fp64_pcg": ms64 / ms16,
@@ -546,6 +580,11 @@ def dns_bench(args, torch, sx, L):
     # forward and the backward sweep (nsolve x nband doubles per mode)
     band_bytes = 2 * (nz // 2) * 2 * P.nsolve * P.nband * 8
     band_ms = prof.get("k_band_solve", {}).get("ms_per_step", float("nan"))
+    # the whole step's algorithmic bytes: both systems' factors once per
+    # mode, and about 46 compulsory passes over a field (nonlinear terms 6,
+    # waveProp 21, setPForce 4, pressure 2, projection 7, velocity 6)
+    field_bytes = nz * P.planeSize * 8
+    step_bytes = 2 * (nz // 2) * P.nsolve * P.nband * 8 + 46 * field_bytes
     return {
         "metric": "DNS steps/s",
         "value": 1e3 / ms,
@@ -562,6 +601,10 @@ def dns_bench(args, torch, sx, L):
                   "captured_in_timed_region": ng1 - ng0,
                   "what": "the same steps replayed from HIP graphs (semx_dns_set_graph), one "
                           "graph per role assignment of the step's buffers"},
+        "step_hbm": {"bytes_per_step": step_bytes,
+                     "frac_of_peak": step_bytes / (ms * 1e-3) / 8e12,
+                     "what": "both systems' band factors once per mode plus 46 field passes "
+                             "over the measured step time"},
         "band_hbm": {"bytes_per_step": band_bytes,
                      "achieved_gb_s": band_bytes / (band_ms * 1e-3) / 1e9,
                      "frac_of_peak_in_sweeps": band_bytes / (band_ms * 1e-3) / 8e12,

@@ -96,6 +96,22 @@ int semx_device_init (int device);
 int semx_mesh_create (int np, int nel, const double* vx, const double* vy,
                       const int* nbr, const int* skind, int svv_mn,
                       double svv_eps, double kinvis, semx_mesh* mesh);
+/* The same mesh numbered as semtex numbers it: btog and bmask (nel*next
+ * each, element by element, sides 0..3 CCW, each side from its first corner
+ * -- the bmap / mask columns of session.num), NGLOBAL and NSOLVE, as
+ * BCmgr::buildnum reads them from the file that enumerate writes
+ * (src/bcmgr.cpp:586-824, utility/enumerate.cpp:426-524).  With the same
+ * numbering the band layout and the pinned node of a singular mode-0
+ * system (src/matrix.cpp:167-169) are semtex's.  SEMX_E_MESH when the
+ * numbering does not join exactly the element-boundary nodes the
+ * connectivity joins, mask exactly the essential ones, or number the
+ * unknowns below nsolve and the essential nodes from nsolve up.          */
+int semx_mesh_create_numbered (int np, int nel, const double* vx,
+                               const double* vy, const int* nbr,
+                               const int* skind, int svv_mn, double svv_eps,
+                               double kinvis, const int* btog,
+                               const int* bmask, int nglobal, int nsolve,
+                               semx_mesh* mesh);
 int semx_mesh_destroy (semx_mesh mesh);
 /* info[10] = np nel next nint nglobal nsolve nband nbound nPlane planeSize,
  * nband including the diagonal (utility/enumerate.cpp:903-943).           */
@@ -121,6 +137,18 @@ int semx_mesh_nsets (semx_mesh mesh, int* nsets);
 int semx_helm_create (semx_mesh mesh, double lambda2, double beta,
                       int base_mode, int nmodes, int nz, int method,
                       semx_helm* helm);
+/* The same with semtex's Fourier-direction spectral vanishing viscosity
+ * (SVV_MZ, SVV_EPSZ, KINVIS): when lambda2 > EPSDP, betak2 of mode k is
+ * weighted by 1 + SVV_EPSZ/KINVIS exp(-((N-k)/(SVV_MZ-k))^2) for k > SVV_MZ
+ * (SVV::coeffs_z, src/svv.cpp:82-123, applied in ModalMatrixSys,
+ * src/matrix.cpp:78-83), N = nmode_total - 1 with nmode_total the field's
+ * number of Fourier modes (Geometry::nMode, src/geometry.h:42; it exceeds
+ * nmodes when the system holds one process's share).  svv_mz < 0 or
+ * SVV_EPSZ <= EPSDP leaves the weights at 1.                              */
+int semx_helm_create_svv (semx_mesh mesh, double lambda2, double beta,
+                          int base_mode, int nmodes, int nz, int method,
+                          int svv_mz, double svv_epsz, double kinvis,
+                          int nmode_total, semx_helm* helm);
 int semx_helm_destroy (semx_helm helm);
 /* info[8] = nmodes nz method nsolve(mode0) nband nblk(block size)
  *           bytes_per_solve_factor(MB*1000, int) singular(mode0)           */
@@ -200,6 +228,15 @@ int semx_helm_set_pcg (semx_helm helm, int precision, double inner_tol,
  * outlive the batch.                                                      */
 int semx_batch_create (int count, const semx_helm* helms, semx_batch* batch);
 int semx_batch_destroy (semx_batch batch);
+/* The element operator each problem's inner iteration uses (opk[count]:
+ * 0 block-wide scalar, 1 wavefront-level scalar, 2 BF16 matrix cores; -1
+ * before the first solve) and the host time the batch tuner took (ms).
+ * The first semx_batch_solve tunes: per class of problems with the same np
+ * and element count it times each operator the precision allows on the
+ * class's first problem (40 iterations, into a scratch solution) and keeps
+ * the fastest per iteration; it synchronises the stream, so make it before
+ * capturing a graph.  SEMX_BATCH_OP=block|wave|mfma forces one operator.  */
+int semx_batch_info (semx_batch batch, int* opk, double* tune_ms);
 int semx_batch_solve (semx_batch batch, const double* const* f,
                       const double* const* bc, double* const* u, double tol_rel,
                       int step_max, int* iters, void* stream);
@@ -246,13 +283,29 @@ int semx_smooth (semx_mesh mesh, int nz, double* data, void* stream);
  * (advection 1, :214-411) nonlinear terms, waveProp, setPForce, project and
  * the pressure / viscous Field::solve calls (DIRECT; JACPCG start-up
  * systems while step < norder = N_TIME), Cartesian, on a mesh without
- * boundaries (periodic in x and y; SEMX_E_UNSUPPORTED otherwise).  nz = 1
+ * boundaries (periodic in x and y; SEMX_E_UNSUPPORTED otherwise: a mesh
+ * with boundaries is set up by semx_dns_create_bc).  nz = 1
  * is 2D (u, v); nz > 1 is 3D3C with Fourier z (nz = 2 or 2^p 3^q 5^r).
  * create builds the velocity (lambda2 = alpha0/(dt kinvis)) and pressure
  * (lambda2 = 0) systems and the device history (Us, Uf of integrate).    */
 int semx_dns_create (semx_mesh mesh, int nz, double beta, double dt,
                      double kinvis, int norder, int advection, double tol_rel,
                      int step_max, semx_dns* dns);
+/* The same on a mesh with boundaries (dns/dns.cpp:55-133 with BCmgr,
+ * src/bcmgr.cpp): vmesh numbers the velocity, pmesh the pressure -- two
+ * numberings of one element mesh with one boundary list (the same
+ * vertices, neighbours and sides; SEMX_E_MESH otherwise), as semtex's
+ * session.num holds one numbering per field mask.  bcu, bcv, bcw (device,
+ * nz*nbound*np each in Fourier space, as semx_helm_solve's bc; bcw NULL
+ * when nz = 1) are the velocity BC lines: values on essential sides,
+ * fluxes on natural ones, fixed in time; they are copied.  The pressure
+ * takes the computed high-order flux on its natural sides (KIO91:
+ * BCmgr::maintainFourier and evaluateCNBCp, src/bcmgr.cpp:1039-1337, with
+ * u.n and flux levels kept across steps) and p = 0 on its essential ones. */
+int semx_dns_create_bc (semx_mesh vmesh, semx_mesh pmesh, int nz, double beta,
+                        double dt, double kinvis, int norder, int advection,
+                        double tol_rel, int step_max, const double* bcu,
+                        const double* bcv, const double* bcw, semx_dns* dns);
 int semx_dns_destroy (semx_dns dns);
 /* nstep steps; u, v, w (NULL in 2D), p: device fields of nz*planeSize in
  * Fourier space, read at entry and written at exit.  The history carries
@@ -276,6 +329,37 @@ int semx_dns_info (semx_dns dns, int* info);
 int semx_dns_set_graph (semx_dns dns, int on);
 /* Graphs captured so far and steps replayed from them.                     */
 int semx_dns_graph_info (semx_dns dns, int* ngraphs, long long* replays);
+
+/* --- field files ------------------------------------------------------------
+ * semtex's field-file format (.fld, .rst, .chk): writeField
+ * (src/misc.cpp:106-180) and Domain's stream input (src/domain.cpp:
+ * 329-398).  A dump is a 10-line header (Session, Created, "Nr, Ns, Nz,
+ * Elements", Step, Time, Time step, Kinvis, Beta, Fields written, Format;
+ * each value left-justified in 25 columns) followed, for each field named,
+ * by its nz physical-space planes of nPlane doubles in the writer's byte
+ * order.
+ * semx_field_write: nfield device fields in Fourier space (nz*planeSize
+ * each, named by the letters of names) written as one dump, each
+ * transformed to physical space on a scratch copy with the Nyquist plane
+ * zeroed first (Domain::dump, src/domain.cpp:188-248, and
+ * Domain::transform, :250-264); append != 0 adds the dump to the end of
+ * the file (a .fld written every IO_FLD steps).
+ * semx_field_read: dump number `dump` (0 = first) of a file read into the
+ * device fields whose names the file holds (others are left as they are;
+ * fields of the file not asked for are skipped), transformed forward with
+ * the Nyquist plane zeroed (Domain::restart, src/domain.cpp:152-180);
+ * byte-swapped when the file's byte order differs.  step, time (may be
+ * NULL) receive the header's values, fields_in_file (may be NULL, 64
+ * chars) its field names.  SEMX_E_MESH when np, nz or the element count
+ * differ from the mesh's (readField, src/misc.cpp:196-205).               */
+int semx_field_write (const char* path, int append, semx_mesh mesh, int nz,
+                      int nfield, const char* names, double* const* fields,
+                      const char* session, int step, double time, double dt,
+                      double kinvis, double beta, void* stream);
+int semx_field_read (const char* path, int dump, semx_mesh mesh, int nz,
+                     int nfield, const char* names, double* const* fields,
+                     int* step, double* time, char* fields_in_file,
+                     void* stream);
 
 /* --- kernel timing -------------------------------------------------------
  * No counterpart in semtex (dclock(), veclib/util.c:65-85, is never called

@@ -10,6 +10,7 @@
  */
 #include "semx_oracle.h"
 
+#include <limits.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -203,6 +204,7 @@ struct orc_mesh {
   int    *hasQ3, *hasdrdx, *hasdsdx, *hasdrdy, *hasdsdy;
   int     nglobal, nsolve, nband;
   int    *btog, *bmask, *emask;
+  int    *prov;             /* Mesh::buildMap numbering (before sortGid)   */
   int     nbound, *belmt, *bside, *bkind;
   double *barea, *imass;
 };
@@ -408,6 +410,8 @@ static void mesh_number (orc_mesh* M, const int* nbr, const int* skind)
       }
     }
   free (par); free (cid); free (sfirst);
+  M->prov = xcalloc (nel * next, sizeof (int));
+  memcpy (M->prov, prov, nel * next * sizeof (int));
 
   /* -- Essential mask on global nodes.                                    */
   ess = xcalloc (ng, sizeof (int));
@@ -599,7 +603,7 @@ void orc_mesh_destroy (orc_mesh* M)
   free (M->Q1); free (M->Q2); free (M->Q3); free (M->Q4);
   free (M->drdx); free (M->dsdx); free (M->drdy); free (M->dsdy);
   free (M->hasQ3); free (M->hasdrdx); free (M->hasdsdx); free (M->hasdrdy); free (M->hasdsdy);
-  free (M->btog); free (M->bmask); free (M->emask);
+  free (M->btog); free (M->bmask); free (M->emask); free (M->prov);
   free (M->belmt); free (M->bside); free (M->bkind); free (M->barea); free (M->imass);
   free (M);
 }
@@ -621,6 +625,295 @@ void orc_mesh_bsides (const orc_mesh* M, int* belmt, int* bside)
 {
   memcpy (belmt, M->belmt, M->nbound * sizeof (int));
   memcpy (bside, M->bside, M->nbound * sizeof (int));
+}
+
+int orc_mesh_renumber (orc_mesh* M, const int* btog, const int* bmask, int nglobal,
+                       int nsolve)
+/* Replace the numbering by a given one (session.num as BCmgr::buildnum
+ * reads it, src/bcmgr.cpp:586-824): same partition of element-boundary
+ * slots into global nodes, same essential mask, unknowns below nsolve.
+ * Recomputes emask, the band (utility/enumerate.cpp:903-943) and the
+ * inverse global mass (src/bcmgr.cpp:806-822).  Returns 0, or 1 for a
+ * numbering that does not fit the mesh (then nothing is changed).      */
+{
+  const int nel = M->nel, next = M->next, ns = nel * next;
+  int *to, *from, i, e, ok = 1;
+  if (nglobal != M->nglobal || nsolve != M->nsolve) return 1;
+  to = xcalloc (nglobal, sizeof (int));
+  from = xcalloc (nglobal, sizeof (int));
+  for (i = 0; i < nglobal; i++) to[i] = from[i] = -1;
+  for (i = 0; i < ns && ok; i++) {
+    const int g = btog[i], p = M->btog[i];
+    if (g < 0 || g >= nglobal || bmask[i] != M->bmask[i] || (bmask[i] != 0) != (g >= nsolve))
+      ok = 0;
+    else if ((to[p] >= 0 && to[p] != g) || (from[g] >= 0 && from[g] != p))
+      ok = 0;
+    else { to[p] = g; from[g] = p; }
+  }
+  free (to); free (from);
+  if (!ok) return 1;
+  memcpy (M->btog, btog, ns * sizeof (int));
+  for (e = 0; e < nel; e++) {
+    M->emask[e] = 0;
+    for (i = 0; i < next; i++) if (M->bmask[e*next + i]) M->emask[e] = 1;
+  }
+  M->nband = element_band (M, M->btog, M->bmask, nel, next);
+  for (i = 0; i < nglobal; i++) M->imass[i] = 0.0;
+  for (e = 0; e < nel; e++)
+    for (i = 0; i < next; i++)
+      M->imass[M->btog[e*next + i]] += M->Q4[e*M->npnp + M->emap[i]];
+  for (i = 0; i < nglobal; i++) M->imass[i] = 1.0 / M->imass[i];
+  return 0;
+}
+
+/* -- enumerate's numbering ------------------------------------------------
+ * The node-numbering utility semtex runs for session.num (utility/
+ * enumerate.cpp:540-943): the buildMap traverse numbering (src/mesh.cpp:
+ * 889-1015, the provisional numbers here), unknowns moved before the
+ * essential nodes keeping their order (Nsys::sortGid), then for
+ * optimisation level 1 the SPARSPAK pseudo-peripheral root search from
+ * node 1 and reverse Cuthill-McKee (George & Liu 1981; femlib/netlib.f:
+ * 1543-1900) on the adjacency lists of Nsys::connectivSC.               */
+
+static void sp_rootls (int root, const int* xadj, const int* adj, int* mask, int* nlvl,
+                       int* xls, int* ls)
+/* ROOTLS: level structure rooted at root over nodes with mask != 0 (all
+ * arrays 1-based as in the reference)                                   */
+{
+  int lbegin, lvlend = 0, ccsize = 1, i, j;
+  mask[root] = 0;
+  ls[1] = root;
+  *nlvl = 0;
+  for (;;) {
+    lbegin = lvlend + 1;
+    lvlend = ccsize;
+    *nlvl += 1;
+    xls[*nlvl] = lbegin;
+    for (i = lbegin; i <= lvlend; i++) {
+      const int node = ls[i];
+      for (j = xadj[node]; j <= xadj[node + 1] - 1; j++) {
+        const int nbr = adj[j];
+        if (mask[nbr] == 0) continue;
+        ccsize++;
+        ls[ccsize] = nbr;
+        mask[nbr] = 0;
+      }
+    }
+    if (ccsize - lvlend <= 0) break;
+  }
+  xls[*nlvl + 1] = lvlend + 1;
+  for (i = 1; i <= ccsize; i++) mask[ls[i]] = 1;
+}
+
+static void sp_fnroot (int* root, const int* xadj, const int* adj, int* mask, int* nlvl,
+                       int* xls, int* ls)
+/* FNROOT: pseudo-peripheral node of the component of root (modified
+ * Gibbs-Poole-Stockmeyer)                                               */
+{
+  int ccsize, nunlvl, j, k;
+  sp_rootls (*root, xadj, adj, mask, nlvl, xls, ls);
+  ccsize = xls[*nlvl + 1] - 1;
+  if (*nlvl == 1 || *nlvl == ccsize) return;
+  for (;;) {
+    const int jstrt = xls[*nlvl];
+    int mindeg = ccsize;
+    *root = ls[jstrt];
+    if (ccsize != jstrt)
+      for (j = jstrt; j <= ccsize; j++) {
+        const int node = ls[j];
+        int ndeg = 0;
+        for (k = xadj[node]; k <= xadj[node + 1] - 1; k++)
+          if (mask[adj[k]] > 0) ndeg++;
+        if (ndeg < mindeg) { *root = node; mindeg = ndeg; }
+      }
+    sp_rootls (*root, xadj, adj, mask, &nunlvl, xls, ls);
+    if (nunlvl <= *nlvl) return;
+    *nlvl = nunlvl;
+    if (*nlvl >= ccsize) return;
+  }
+}
+
+static void sp_degree (int root, int* xadj, const int* adj, const int* mask, int* deg,
+                       int* ccsize, int* ls)
+/* DEGREE: degrees within the masked component of root (xadj signs mark
+ * visited nodes and are restored)                                       */
+{
+  int lbegin, lvlend = 0, i, j;
+  ls[1] = root;
+  xadj[root] = -xadj[root];
+  *ccsize = 1;
+  for (;;) {
+    lbegin = lvlend + 1;
+    lvlend = *ccsize;
+    for (i = lbegin; i <= lvlend; i++) {
+      const int node = ls[i];
+      const int jstrt = -xadj[node], jstop = abs (xadj[node + 1]) - 1;
+      int ideg = 0;
+      for (j = jstrt; j <= jstop; j++) {
+        const int nbr = adj[j];
+        if (mask[nbr] == 0) continue;
+        ideg++;
+        if (xadj[nbr] < 0) continue;
+        xadj[nbr] = -xadj[nbr];
+        (*ccsize)++;
+        ls[*ccsize] = nbr;
+      }
+      deg[node] = ideg;
+    }
+    if (*ccsize - lvlend <= 0) break;
+  }
+  for (i = 1; i <= *ccsize; i++) xadj[ls[i]] = -xadj[ls[i]];
+}
+
+static void sp_rcm (int root, int* xadj, const int* adj, int* mask, int* perm, int* ccsize,
+                    int* deg)
+/* RCM: reverse Cuthill-McKee numbering of the component of root from root,
+ * neighbours of each node taken by increasing degree (linear insertion)  */
+{
+  int lbegin, lvlend = 0, lnbr = 1, i, j, k, l;
+  sp_degree (root, xadj, adj, mask, deg, ccsize, perm);
+  mask[root] = 0;
+  if (*ccsize <= 1) return;
+  do {
+    lbegin = lvlend + 1;
+    lvlend = lnbr;
+    for (i = lbegin; i <= lvlend; i++) {
+      const int node = perm[i];
+      const int fnbr = lnbr + 1;
+      for (j = xadj[node]; j <= xadj[node + 1] - 1; j++) {
+        const int nbr = adj[j];
+        if (mask[nbr] == 0) continue;
+        lnbr++;
+        mask[nbr] = 0;
+        perm[lnbr] = nbr;
+      }
+      if (fnbr >= lnbr) continue;
+      k = fnbr;
+      do {
+        int nbr;
+        l = k;
+        k++;
+        nbr = perm[k];
+        while (l >= fnbr && deg[perm[l]] > deg[nbr]) { perm[l + 1] = perm[l]; l--; }
+        perm[l + 1] = nbr;
+      } while (k < lnbr);
+    }
+  } while (lnbr > lvlend);
+  for (i = 1, l = *ccsize; i <= *ccsize / 2; i++, l--) {
+    const int t = perm[l]; perm[l] = perm[i]; perm[i] = t;
+  }
+}
+
+int orc_mesh_enumerate (orc_mesh* M, int optlevel)
+/* Renumber M as `enumerate -O<optlevel>` numbers it (levels 0-3; the
+ * utility's default, and the regression sessions', is 3).
+ * Returns 0, or 1 for an unsupported level.                             */
+{
+  const int nel = M->nel, next = M->next, ns = nel * next, ng = M->nglobal;
+  int *ess, *newid, *bt, i, e, k;
+  int nunk = 0, ne = 0;
+  if (optlevel < 0 || optlevel > 3) return 1;
+  ess = xcalloc (ng, sizeof (int));
+  newid = xcalloc (ng, sizeof (int));
+  bt = xcalloc (ns, sizeof (int));
+  for (i = 0; i < ns; i++) ess[M->prov[i]] = M->bmask[i];
+  for (i = 0; i < ng; i++) nunk += !ess[i];
+  for (i = 0, k = 0; i < ng; i++) newid[i] = ess[i] ? nunk + ne++ : k++;   /* sortGid */
+  for (i = 0; i < ns; i++) bt[i] = newid[M->prov[i]];
+  if (optlevel >= 1 && nunk > 0) {
+    /* adjacency lists (connectivSC): per element, for each unknown slot i,
+     * the other unknown slots' nodes in slot order, first appearance only */
+    int* cnt = xcalloc (nunk, sizeof (int));
+    int* cap = xcalloc (nunk, sizeof (int));
+    int** lst = xcalloc (nunk, sizeof (int*));
+    int *xadj, *adj, *mask, *xls, *perm, *invperm, tab = 0, root = 1, nlvl, ccs;
+    for (e = 0; e < nel; e++) {
+      const int* bm = bt + e * next;
+      const int* mk = M->bmask + e * next;
+      for (i = 0; i < next; i++) {
+        int j;
+        if (mk[i]) continue;
+        for (j = 0; j < next; j++) {
+          const int g = bm[i], h = bm[j];
+          int q, found = 0;
+          if (i == j || mk[j]) continue;
+          for (q = 0; q < cnt[g] && !found; q++) found = lst[g][q] == h;
+          if (found) continue;
+          if (cnt[g] == cap[g]) {
+            cap[g] = cap[g] ? 2 * cap[g] : 16;
+            lst[g] = realloc (lst[g], cap[g] * sizeof (int));
+          }
+          lst[g][cnt[g]++] = h;
+        }
+      }
+    }
+    for (i = 0; i < nunk; i++) tab += cnt[i];
+    xadj = xcalloc (nunk + 2, sizeof (int));
+    adj = xcalloc (tab + 2, sizeof (int));
+    mask = xcalloc (nunk + 2, sizeof (int));
+    xls = xcalloc (nunk + 2, sizeof (int));
+    perm = xcalloc (nunk + 2, sizeof (int));
+    invperm = xcalloc (ng, sizeof (int));
+    for (i = 0, k = 1; i < nunk; i++) {            /* fillAdjncy, 1-based */
+      int q;
+      xadj[i + 1] = k;
+      for (q = 0; q < cnt[i]; q++, k++) adj[k] = lst[i][q] + 1;
+      free (lst[i]);
+    }
+    xadj[nunk + 1] = k;
+    if (optlevel == 1) {
+      for (i = 1; i <= nunk; i++) mask[i] = 1;
+      sp_fnroot (&root, xadj, adj, mask, &nlvl, xls, perm);
+    } else {
+      /* level 2: fnroot from trial roots 1, 11, 21, ...; level 3: every
+       * unknown as the RCM root.  Keep the first root of least global
+       * bandwidth (Nsys::globalBandwidth: max over elements of the
+       * unknowns' index span, plus one).                                 */
+      int best = 1, bwmin = INT_MAX, r;
+      for (r = 1; r <= nunk; r += (optlevel == 2) ? 10 : 1) {
+        int rt = r, bw = 0;
+        for (i = 1; i <= nunk; i++) mask[i] = 1;
+        if (optlevel == 2) {
+          sp_fnroot (&rt, xadj, adj, mask, &nlvl, xls, perm);
+          for (i = 1; i <= nunk; i++) mask[i] = 1;  /* as left by fnroot */
+        }
+        sp_rcm (rt, xadj, adj, mask, perm, &ccs, xls);
+        for (i = 0; i < nunk; i++) invperm[perm[i + 1] - 1] = i;
+        for (e = 0; e < nel; e++) {
+          int lo = INT_MAX, hi = INT_MIN;
+          for (k = 0; k < next; k++) {
+            int g;
+            if (M->bmask[e*next + k]) continue;
+            g = invperm[bt[e*next + k]];
+            lo = g < lo ? g : lo; hi = g > hi ? g : hi;
+          }
+          if (hi >= lo && hi - lo > bw) bw = hi - lo;
+        }
+        if (bw + 1 < bwmin) { bwmin = bw + 1; best = rt; }
+      }
+      root = best;
+    }
+    for (i = 1; i <= nunk; i++) mask[i] = 1;
+    sp_rcm (root, xadj, adj, mask, perm, &ccs, xls);
+    for (i = nunk; i < ng; i++) invperm[i] = i;
+    for (i = 0; i < nunk; i++) invperm[perm[i + 1] - 1] = i;
+    for (i = 0; i < ns; i++) bt[i] = invperm[bt[i]];
+    free (cnt); free (cap); free (lst); free (xadj); free (adj); free (mask);
+    free (xls); free (perm); free (invperm);
+  }
+  memcpy (M->btog, bt, ns * sizeof (int));
+  free (ess); free (newid); free (bt);
+  for (e = 0; e < nel; e++) {
+    M->emask[e] = 0;
+    for (i = 0; i < next; i++) if (M->bmask[e*next + i]) M->emask[e] = 1;
+  }
+  M->nband = element_band (M, M->btog, M->bmask, nel, next);
+  for (i = 0; i < ng; i++) M->imass[i] = 0.0;
+  for (e = 0; e < nel; e++)
+    for (i = 0; i < next; i++)
+      M->imass[M->btog[e*next + i]] += M->Q4[e*M->npnp + M->emap[i]];
+  for (i = 0; i < ng; i++) M->imass[i] = 1.0 / M->imass[i];
+  return 0;
 }
 
 void orc_mesh_numbering (const orc_mesh* M, int* btog, int* bmask, int* emask)
@@ -665,6 +958,36 @@ void orc_mesh_barea (const orc_mesh* M, double* barea)
 /* ======================================================================== *
  * Element operators.
  * ======================================================================== */
+
+/* -- Fourier-direction SVV weights (SVV::coeffs_z, src/svv.cpp:82-123):
+ *    S[k - base] = 1 + eps/nu exp(-((N-k)/(mz-k))^2) for modes k > mz of
+ *    base..base+nummodes-1, N = nmode - 1; 1 elsewhere, and all 1 unless
+ *    0 <= mz < N and eps > EPSDP.  The solve settings below are the tokens
+ *    SVV_MZ, SVV_EPSZ, KINVIS and Geometry::nMode (0: base + nummodes).   */
+static int    svvz_mz = -1, svvz_nmode = 0;
+static double svvz_eps = 0.0, svvz_nu = 1.0;
+
+void orc_svv_coeffs_z (int nummodes, int nmode, int base, int mz, double eps, double nu,
+                       double* S)
+{
+  const int N = nmode - 1;
+  int i;
+  for (i = 0; i < nummodes; i++) S[i] = 0.0;
+  if (mz >= 0 && mz < N && eps > EPSDP) {
+    for (i = base; i < base + nummodes; i++)
+      if (i > mz) {
+        const double q = (N - i) / (1.0 * mz - i);
+        S[i - base] = exp (-q * q);
+      }
+    for (i = 0; i < nummodes; i++) S[i] = 1.0 + eps / nu * S[i];
+  } else
+    for (i = 0; i < nummodes; i++) S[i] = 1.0;
+}
+
+void orc_set_svv_z (int mz, double eps, double nu, int nmode)
+{
+  svvz_mz = mz; svvz_eps = eps; svvz_nu = nu; svvz_nmode = nmode;
+}
 
 static void pick_ops (const orc_mesh* M, double lambda2, const double** dv,
                       const double** dt)
@@ -1195,8 +1518,8 @@ static int field_solve_planes (const orc_mesh* M, int nz, double lambda2, double
                         double tol_rel, int step_max, int nthreads, int nrep,
                         double* t_solve, int* iters)
 /* Field::solve over planes (field.cpp:564-723) with one MatrixSys per
- * mode base..base+nz/2-1 (matrix.cpp:33-110; SVV z-weights are 1 with SVV
- * off).  The Nyquist plane 1 is skipped only on the planes that start at
+ * mode base..base+nz/2-1 (matrix.cpp:33-110; betak2 with the z SVV
+ * weights of orc_set_svv_z, 1 unless set).  The Nyquist plane 1 is skipped only on the planes that start at
  * mode 0 (the ROOTONLY skip of field.cpp:566).  iters (may be NULL)
  * receives each plane's PCG iteration count, -1 at the limit.            */
 {
@@ -1206,14 +1529,21 @@ static int field_solve_planes (const orc_mesh* M, int nz, double lambda2, double
   int m, k, warn = 0;
   double t0;
 
+  double* Sz = xcalloc (nmodes, sizeof (double));
+  orc_svv_coeffs_z (nmodes, svvz_nmode ? svvz_nmode : base + nmodes, base, svvz_mz, svvz_eps,
+                    svvz_nu, Sz);
   if (nthreads > 0) omp_set_num_threads (nthreads);
 #pragma omp parallel for schedule(dynamic, 1)
   for (m = 0; m < nmodes; m++) {
     /* MIXED: DIRECT for mode 0, JACPCG otherwise (matrix.cpp:93-96) */
     const int kind = (method == 2) ? (base + m == 0 ? ORC_DIRECT : ORC_JACPCG) : method;
     const double bk = beta * (base + m);
-    sys[m] = orc_sys_create (M, lambda2, bk * bk, kind);
+    /* betak2 weighted by the z SVV coefficient for lambda2 > EPSDP
+     * (src/matrix.cpp:78-83)                                            */
+    const double bk2 = (lambda2 > EPSDP) ? bk * bk * Sz[m] : bk * bk;
+    sys[m] = orc_sys_create (M, lambda2, bk2, kind);
   }
+  free (Sz);
   {
     /* repeated solves restore the (destroyed) forcing from a copy, outside
      * the timed loop body's plane solves                                  */
@@ -1444,13 +1774,23 @@ void orc_smooth (const orc_mesh* M, int nz, double* data)
  * ======================================================================== */
 
 struct orc_dns {
-  const orc_mesh* M;
+  const orc_mesh* M;        /* velocity mesh (its essential sides: walls)  */
+  const orc_mesh* Mp;       /* pressure mesh (its natural sides: computed
+                             * high-order BCs); M when there are none      */
   int nz, ncom, nord, advection, step, nthreads, step_max, toggle;
   double beta, dt, kinvis, tol_rel, time;
   size_t ntot;
   double **Us, **Uf;        /* [nord][ncom] level pointers                 */
   double *store;
   orc_sys **vsys, **psys;   /* per mode                                   */
+  /* boundaries (M->nbound > 0): velocity BC lines (Fourier space), the
+   * pressure BC lines and the computed-BC history of BCmgr
+   * (src/bcmgr.cpp:920-996): u.n and the high-order pressure flux per
+   * time level                                                           */
+  int nline;
+  double *bcu[3], *bcp;
+  double **un, **hop;       /* [nord] level pointers, nz*nline each        */
+  double *bstore;
 };
 
 static void stiffly_stable (int n, double* c)
@@ -1481,7 +1821,7 @@ orc_dns* orc_dns_create (const orc_mesh* M, int nz, double beta, double dt,
   const int nmodes = (nz > 1) ? nz / 2 : 1;
   double alpha[4];
   int q, i, m;
-  D->M = M; D->nz = nz; D->nord = norder; D->advection = advection;
+  D->M = M; D->Mp = M; D->nz = nz; D->nord = norder; D->advection = advection;
   D->beta = beta; D->dt = dt; D->kinvis = kinvis; D->tol_rel = tol_rel;
   D->step_max = step_max; D->nthreads = nthreads; D->toggle = 1;
   D->ncom = (nz > 1) ? 3 : 2;
@@ -1507,8 +1847,272 @@ orc_dns* orc_dns_create (const orc_mesh* M, int nz, double beta, double dt,
   return D;
 }
 
+/* -- Computed pressure boundary conditions (KIO91) ---------------------- */
+
+static int side_node (int np, int s, int q)
+/* element-storage index of node q (CCW) of side s (Edge offsets and
+ * skips, src/edge.cpp:43-50)                                            */
+{
+  switch (s) {
+  case 0:  return q;
+  case 1:  return q*np + np - 1;
+  case 2:  return (np - 1)*np + (np - 1 - q);
+  default: return (np - 1 - q)*np;
+  }
+}
+
+static void side_normal (const orc_mesh* M, int e, int s, double* nx, double* ny)
+/* unit outward normal at the side's nodes, CCW (Element::sideGeom,
+ * src/element.cpp:1653-1744)                                            */
+{
+  const int np = M->np;
+  int q;
+  for (q = 0; q < np; q++) {
+    const int i = e*M->npnp + side_node (np, s, q);
+    double a, b, len;
+    switch (s) {
+    case 0:  a = -M->dsdx[i]; b = -M->dsdy[i]; break;
+    case 1:  a =  M->drdx[i]; b =  M->drdy[i]; break;
+    case 2:  a =  M->dsdx[i]; b =  M->dsdy[i]; break;
+    default: a = -M->drdx[i]; b = -M->drdy[i]; break;
+    }
+    len = hypot (a, b);
+    nx[q] = a / len; ny[q] = b / len;
+  }
+}
+
+static void side_grad (const orc_mesh* M, int e, int s, const double* src, double* gx,
+                       double* gy)
+/* x and y derivatives of element data src at the side's nodes, CCW
+ * (Element::sideGrad, src/element.cpp:652-714)                           */
+{
+  const int np = M->np;
+  int q, k;
+  for (q = 0; q < np; q++) {
+    const int idx = side_node (np, s, q), i = idx / np, j = idx % np;
+    const int g = e*M->npnp + idx;
+    double ddr = 0.0, dds = 0.0;
+    for (k = 0; k < np; k++) {
+      ddr += M->DV[j*np + k] * src[i*np + k];
+      dds += M->DV[i*np + k] * src[k*np + j];
+    }
+    if (gx) gx[q] = ddr * M->drdx[g] + dds * M->dsdx[g];
+    if (gy) gy[q] = ddr * M->drdy[g] + dds * M->dsdy[g];
+  }
+}
+
+static void curl_curl (const orc_mesh* M, int e, int s, double betaK, const double* Ur,
+                       const double* Ui, const double* Vr, const double* Vi,
+                       const double* Wr, const double* Wi, double* xr, double* xi,
+                       double* yr, double* yi)
+/* Fourier-mode curl curl u along a side, first two components
+ * (Edge::curlCurl, src/edge.cpp:98-258, Cartesian): for betaK == 0 the
+ * 2D / mode-0 form from the real planes only.                          */
+{
+  const int np = M->np, npnp = M->npnp;
+  double w[32*32], a[32*32], b[32*32], t[32];
+  int i, q;
+  /* real parts: z vorticity w = dV/dx - dU/dy, then (dw/dy, -dw/dx) */
+  memcpy (a, Vr, npnp * sizeof (double)); elmt_grad (M, e, a, 0);
+  memcpy (b, Ur, npnp * sizeof (double)); elmt_grad (M, e, b, 1);
+  for (i = 0; i < npnp; i++) w[i] = a[i] - b[i];
+  side_grad (M, e, s, w, yr, xr);
+  for (q = 0; q < np; q++) yr[q] = -yr[q];
+  if (betaK == 0.0) return;
+  memcpy (a, Vi, npnp * sizeof (double)); elmt_grad (M, e, a, 0);
+  memcpy (b, Ui, npnp * sizeof (double)); elmt_grad (M, e, b, 1);
+  for (i = 0; i < npnp; i++) w[i] = a[i] - b[i];
+  side_grad (M, e, s, w, yi, xi);
+  for (q = 0; q < np; q++) yi[q] = -yi[q];
+  /* semi-Fourier terms of W */
+  memcpy (a, Wr, npnp * sizeof (double)); elmt_grad (M, e, a, 0);
+  memcpy (b, Wr, npnp * sizeof (double)); elmt_grad (M, e, b, 1);
+  for (q = 0; q < np; q++) {
+    const int n = side_node (np, s, q);
+    xi[q] += betaK * a[n];
+    yi[q] += betaK * b[n];
+  }
+  memcpy (a, Wi, npnp * sizeof (double)); elmt_grad (M, e, a, 0);
+  memcpy (b, Wi, npnp * sizeof (double)); elmt_grad (M, e, b, 1);
+  for (q = 0; q < np; q++) {
+    const int n = side_node (np, s, q);
+    xr[q] -= betaK * a[n];
+    yr[q] -= betaK * b[n];
+  }
+  /* second derivatives in z */
+  for (q = 0; q < np; q++) {
+    const int n = side_node (np, s, q);
+    xr[q] += betaK * betaK * Ur[n];
+    xi[q] += betaK * betaK * Ui[n];
+    yr[q] += betaK * betaK * Vr[n];
+    yi[q] += betaK * betaK * Vi[n];
+  }
+  (void) t;
+}
+
+static void rollv (double** lev, int n)
+/* rollv (src/sem.h:90-103)                                              */
+{
+  double* t;
+  int q;
+  if (n < 2) return;
+  t = lev[n - 1];
+  for (q = n - 1; q; q--) lev[q] = lev[q - 1];
+  lev[0] = t;
+}
+
+static void maintain_fourier (orc_dns* D, double** Us, double** Uf)
+/* BCmgr::maintainFourier (src/bcmgr.cpp:1039-1298), the terms of the
+ * high-order pressure BC: at each boundary node of every plane,
+ *   hop = -(1/dt)(a0 u.n + sum_q a_{q+1} (u.n)_q)   (Je = min(step-1,
+ *         N_TIME) > 0, the velocity on the boundary may vary in time)
+ *       + n.N(u) - nu n.curlCurl(u),
+ * with u.n of past levels kept; velocity in Fourier space (Us[0]),
+ * nonlinear terms N (Uf[0]).  Plane 1 (Nyquist) is skipped.  The
+ * div(u) and grad(u).n stores of the reference serve other BC kinds and
+ * are not formed.                                                       */
+{
+  const orc_mesh* M = D->Mp;
+  const int np = M->np, npnp = M->npnp, nz = D->nz, nline = D->nline, ps = M->planeSize;
+  const int Je = (D->step - 1 < D->nord) ? D->step - 1 : D->nord;
+  const double nu = D->kinvis, invDt = 1.0 / D->dt;
+  double alpha[4], nx[32], ny[32], xr[32], xi[32], yr[32], yi[32];
+  double* unew = xcalloc ((size_t) nz * nline, sizeof (double));
+  int b, k, q, m;
+  if (Je) stiffly_stable (Je, alpha);
+  rollv (D->hop, D->nord);
+  memset (D->hop[0], 0, (size_t) nz * nline * sizeof (double));
+  for (b = 0; b < M->nbound; b++) {
+    const int e = M->belmt[b], s = M->bside[b], j = b * np;
+    side_normal (M, e, s, nx, ny);
+    for (k = 0; k < nz; k++) {
+      const double *ux, *uy, *fx, *fy;
+      double* h = D->hop[0] + (size_t) k * nline + j;
+      double* un = unew + (size_t) k * nline + j;
+      if (nz > 1 && k == 1) continue;
+      ux = Us[0] + (size_t) k * ps + e * npnp;
+      uy = Us[1] + (size_t) k * ps + e * npnp;
+      fx = Uf[0] + (size_t) k * ps + e * npnp;
+      fy = Uf[1] + (size_t) k * ps + e * npnp;
+      for (q = 0; q < np; q++) {
+        const int n = side_node (np, s, q);
+        un[q] = ux[n] * nx[q] + uy[n] * ny[q];
+      }
+      if (Je) {
+        int l;
+        for (q = 0; q < np; q++) {
+          double wk = alpha[0] * un[q];
+          for (l = 0; l < Je; l++) wk += alpha[l + 1] * D->un[l][(size_t) k * nline + j + q];
+          h[q] = -invDt * wk;
+        }
+      }
+      for (q = 0; q < np; q++) {
+        const int n = side_node (np, s, q);
+        h[q] += fx[n] * nx[q] + fy[n] * ny[q];
+      }
+    }
+  }
+  rollv (D->un, D->nord);
+  memcpy (D->un[0], unew, (size_t) nz * nline * sizeof (double));
+  free (unew);
+  for (b = 0; b < M->nbound; b++) {
+    const int e = M->belmt[b], s = M->bside[b], j = b * np;
+    side_normal (M, e, s, nx, ny);
+    {
+      const double* ur = Us[0] + e * npnp;
+      const double* vr = Us[1] + e * npnp;
+      double* h = D->hop[0] + j;
+      curl_curl (M, e, s, 0.0, ur, NULL, vr, NULL, NULL, NULL, xr, NULL, yr, NULL);
+      for (q = 0; q < np; q++) h[q] += -nu * xr[q] * nx[q] + -nu * yr[q] * ny[q];
+    }
+    for (m = 1; m < (nz > 1 ? nz / 2 : 1); m++) {
+      const size_t kr = 2 * (size_t) m, ki = kr + 1;
+      const double* Ur = Us[0] + kr * ps + e * npnp; const double* Ui = Us[0] + ki * ps + e * npnp;
+      const double* Vr = Us[1] + kr * ps + e * npnp; const double* Vi = Us[1] + ki * ps + e * npnp;
+      const double* Wr = Us[2] + kr * ps + e * npnp; const double* Wi = Us[2] + ki * ps + e * npnp;
+      double* hr = D->hop[0] + kr * nline + j;
+      double* hi = D->hop[0] + ki * nline + j;
+      curl_curl (M, e, s, m * D->beta, Ur, Ui, Vr, Vi, Wr, Wi, xr, xi, yr, yi);
+      for (q = 0; q < np; q++) {
+        hr[q] += -nu * xr[q] * nx[q];
+        hi[q] += -nu * xi[q] * nx[q];
+        hr[q] += -nu * yr[q] * ny[q];
+        hi[q] += -nu * yi[q] * ny[q];
+      }
+    }
+  }
+}
+
+static void evaluate_pressure_bcs (orc_dns* D)
+/* Field::evaluateBoundaries of the pressure with BCmgr::evaluateCNBCp
+ * (src/bcmgr.cpp:1301-1337) on every natural side of the pressure mesh:
+ * the flux at the end of the step extrapolated from the stored levels,
+ * sum_{q < Je} beta_q hop_q, Je = min(step, N_TIME); essential sides of
+ * the pressure (p = 0) and plane 1 zero.                                */
+{
+  const orc_mesh* M = D->Mp;
+  const int np = M->np, nz = D->nz, nline = D->nline;
+  const int Je = (D->step < D->nord) ? D->step : D->nord;
+  double beta[3];
+  int b, k, q, l;
+  extrapolation (Je, beta);
+  memset (D->bcp, 0, (size_t) nz * nline * sizeof (double));
+  for (b = 0; b < M->nbound; b++) {
+    if (M->bkind[b] != 2) continue;
+    for (k = 0; k < nz; k++) {
+      if (nz > 1 && k == 1) continue;
+      for (q = 0; q < np; q++) {
+        const size_t i = (size_t) k * nline + b * np + q;
+        double v = 0.0;
+        for (l = 0; l < Je; l++) v += beta[l] * D->hop[l][i];
+        D->bcp[i] = v;
+      }
+    }
+  }
+}
+
+orc_dns* orc_dns_create_bc (const orc_mesh* M, const orc_mesh* Mp, int nz, double beta,
+                            double dt, double kinvis, int norder, int advection,
+                            double tol_rel, int step_max, int nthreads, const double* bcu,
+                            const double* bcv, const double* bcw)
+/* A DNS on a mesh with boundaries: M numbers the velocity (its essential
+ * sides carry the given Fourier-space BC lines bcu/bcv/bcw, nz*nline
+ * each, static in time), Mp the pressure (same elements and boundary
+ * list; its natural sides take the computed high-order flux, its
+ * essential sides p = 0).                                               */
+{
+  orc_dns* D = orc_dns_create (M, nz, beta, dt, kinvis, norder, advection, tol_rel,
+                               step_max, nthreads);
+  const int nmodes = (nz > 1) ? nz / 2 : 1;
+  const size_t nl = (size_t) nz * M->nbound * M->np;
+  const double* src[3] = {bcu, bcv, bcw};
+  int m, i, q;
+  D->Mp = Mp;
+  D->nline = M->nbound * M->np;
+  D->bstore = xcalloc ((3 + 1 + 2 * (size_t) norder) * nl, sizeof (double));
+  for (i = 0; i < 3; i++) {
+    D->bcu[i] = D->bstore + i * nl;
+    if (src[i]) memcpy (D->bcu[i], src[i], nl * sizeof (double));
+  }
+  D->bcp = D->bstore + 3 * nl;
+  D->un = xcalloc (norder, sizeof (double*));
+  D->hop = xcalloc (norder, sizeof (double*));
+  for (q = 0; q < norder; q++) {
+    D->un[q] = D->bstore + (4 + 2 * (size_t) q) * nl;
+    D->hop[q] = D->bstore + (5 + 2 * (size_t) q) * nl;
+  }
+#pragma omp parallel for schedule(dynamic, 1)
+  for (m = 0; m < nmodes; m++) {
+    const double bk = beta * m;
+    orc_sys_destroy (D->psys[m]);
+    D->psys[m] = orc_sys_create (Mp, 0.0, bk * bk, ORC_DIRECT);
+  }
+  return D;
+}
+
 void orc_dns_destroy (orc_dns* D)
 {
+  free (D->bstore); free (D->un); free (D->hop);
   const int nmodes = (D->nz > 1) ? D->nz / 2 : 1;
   int m;
   for (m = 0; m < nmodes; m++) { orc_sys_destroy (D->vsys[m]); orc_sys_destroy (D->psys[m]); }
@@ -1529,9 +2133,10 @@ static void transform (const orc_dns* D, double* a, int sign)
 }
 
 static void field_solve (orc_dns* D, orc_sys** sys, double lambda2_tmp,
-                         int use_tmp, double* f, double* u)
+                         int use_tmp, double* f, double* u, const double* bc)
+/* bc: nz BC lines of the field's mesh, or NULL without boundaries       */
 {
-  const orc_mesh* M = D->M;
+  const orc_mesh* M = sys ? sys[0]->M : D->M;
   const int nz = D->nz, nmodes = (nz > 1) ? nz / 2 : 1;
   orc_sys** s = sys;
   int k, m;
@@ -1546,7 +2151,8 @@ static void field_solve (orc_dns* D, orc_sys** sys, double lambda2_tmp,
 #pragma omp parallel for schedule(dynamic, 1)
   for (k = 0; k < nz; k++) {
     if (nz > 1 && k == 1) continue;
-    orc_solve_plane (s[k >> 1], f + (size_t) k * M->planeSize, NULL,
+    orc_solve_plane (s[k >> 1], f + (size_t) k * M->planeSize,
+                     bc ? bc + (size_t) k * M->nbound * M->np : NULL,
                      u + (size_t) k * M->planeSize, D->tol_rel, D->step_max);
   }
   if (use_tmp) {
@@ -1639,6 +2245,12 @@ int orc_dns_run (orc_dns* D, int nstep, double* u0, double* v0, double* w0,
     nonlinear (D, u, p);            /* pressure storage is scratch here  */
     D->step += 1;
     D->time += D->dt;
+    /* high-order pressure BC storage, then the pressure BCs
+     * (dns/integrate.cpp:168-172)                                       */
+    if (D->bstore) {
+      maintain_fourier (D, D->Us, D->Uf);
+      evaluate_pressure_bcs (D);
+    }
     Je = (D->step < nord) ? D->step : nord;
     stiffly_stable (Je, alpha);
     extrapolation (Je, beta);
@@ -1669,7 +2281,7 @@ int orc_dns_run (orc_dns* D, int nstep, double* u0, double* v0, double* w0,
     for (i = 1; i < ndim; i++) { size_t k; for (k = 0; k < n; k++) D->Uf[0][k] += D->Uf[i][k]; }
     { size_t k; for (k = 0; k < n; k++) D->Uf[0][k] /= D->dt; }
     /* pressure solve */
-    field_solve (D, D->psys, 0.0, 0, D->Uf[0], p);
+    field_solve (D, D->psys, 0.0, 0, D->Uf[0], p, D->bstore ? D->bcp : NULL);
     /* project */
     {
       const double a = -1.0 / (D->dt * D->kinvis), b = 1.0 / D->kinvis;
@@ -1691,9 +2303,10 @@ int orc_dns_run (orc_dns* D, int nstep, double* u0, double* v0, double* w0,
       if (D->step < nord) {
         double a2[4];
         stiffly_stable (D->step, a2);
-        field_solve (D, NULL, a2[0] / (D->dt * D->kinvis), 1, D->Uf[i], u[i]);
+        field_solve (D, NULL, a2[0] / (D->dt * D->kinvis), 1, D->Uf[i], u[i],
+                     D->bstore ? D->bcu[i] : NULL);
       } else
-        field_solve (D, D->vsys, 0.0, 0, D->Uf[i], u[i]);
+        field_solve (D, D->vsys, 0.0, 0, D->Uf[i], u[i], D->bstore ? D->bcu[i] : NULL);
     }
   }
   if (t_step) *t_step = (omp_get_wtime () - t0) / (nstep ? nstep : 1);

@@ -30,6 +30,9 @@ void orc_dglj  (double* D, double* DT, const double* z, int np); /* polylib.c:51
 void orc_legtran (int np, double* FW, double* BW);              /* operators.c:391-480 */
 void orc_svv_operators (int np, int svv_mn, double svv_eps, double kinvis,
                         double* SDV, double* SDT);              /* svv.cpp:41-79,126-185 */
+void orc_svv_coeffs_z (int nummodes, int nmode, int base, int mz, double eps, double nu,
+                       double* S);                            /* svv.cpp:82-123 */
+void orc_set_svv_z (int mz, double eps, double nu, int nmode); /* SVV_MZ, SVV_EPSZ */
 void orc_edgemaps (int np, int* emap, int* pmap);               /* mapping.c:46-132 */
 
 /* --- mesh, geometry, numbering ------------------------------------------ */
@@ -51,6 +54,9 @@ void orc_mesh_info (const orc_mesh* M, int* info);
 void orc_mesh_nodes (const orc_mesh* M, double* x, double* y);
 void orc_mesh_bsides (const orc_mesh* M, int* belmt, int* bside);
 void orc_mesh_numbering (const orc_mesh* M, int* btog, int* bmask, int* emask);
+int  orc_mesh_renumber (orc_mesh* M, const int* btog, const int* bmask, int nglobal,
+                        int nsolve);                /* src/bcmgr.cpp:586-824 */
+int  orc_mesh_enumerate (orc_mesh* M, int optlevel);   /* utility/enumerate.cpp */
 void orc_mesh_factors (const orc_mesh* M, double* Q1, double* Q2, double* Q3,
                        double* Q4, double* drdx, double* dsdx, double* drdy,
                        double* dsdy, int* null_flags /* nel*5 */);
@@ -134,6 +140,14 @@ typedef struct orc_dns orc_dns;
 orc_dns* orc_dns_create (const orc_mesh* M, int nz, double beta, double dt,
                          double kinvis, int norder, int advection,
                          double tol_rel, int step_max, int nthreads);
+/* the same on a mesh with boundaries: M numbers the velocity (given
+ * Fourier-space BC lines on its essential sides), Mp the pressure
+ * (computed high-order flux on its natural sides, BCmgr::maintainFourier
+ * and evaluateCNBCp, src/bcmgr.cpp:1039-1337)                          */
+orc_dns* orc_dns_create_bc (const orc_mesh* M, const orc_mesh* Mp, int nz, double beta,
+                            double dt, double kinvis, int norder, int advection,
+                            double tol_rel, int step_max, int nthreads, const double* bcu,
+                            const double* bcv, const double* bcw);
 void orc_dns_destroy (orc_dns* D);
 /* Advance nstep steps; fields are read from/written to u,v,w,p (ncom=3)
  * or u,v,p (ncom=2, pass w = NULL).                                       */

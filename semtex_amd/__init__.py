@@ -47,15 +47,27 @@ def _ip(a):
 class Mesh:
     """Elements + numbering of a quad mesh (see semtex_amd.mesh)."""
 
-    def __init__(self, np_, m, svv_mn=-1, svv_eps=0.0, kinvis=1.0):
+    def __init__(self, np_, m, svv_mn=-1, svv_eps=0.0, kinvis=1.0, numbering=None):
+        """numbering: None for the library's own, or semtex's numbering as
+        read from a session.num file (semtex_amd.numfile.read_num: a dict
+        with btog, bmask, nglobal, nsolve)."""
         L = load()
         vx = np.ascontiguousarray(m["vx"], dtype=np.float64)
         vy = np.ascontiguousarray(m["vy"], dtype=np.float64)
         nbr = np.ascontiguousarray(m["nbr"], dtype=np.int32)
         sk = np.ascontiguousarray(m["skind"], dtype=np.int32)
         h = C.c_void_p()
-        check(L.semx_mesh_create(np_, int(m["nel"]), _dp(vx), _dp(vy), _ip(nbr), _ip(sk),
-                                 svv_mn, svv_eps, kinvis, C.byref(h)), "semx_mesh_create")
+        if numbering is None:
+            check(L.semx_mesh_create(np_, int(m["nel"]), _dp(vx), _dp(vy), _ip(nbr), _ip(sk),
+                                     svv_mn, svv_eps, kinvis, C.byref(h)), "semx_mesh_create")
+        else:
+            bt = np.ascontiguousarray(numbering["btog"], dtype=np.int32)
+            bm = np.ascontiguousarray(numbering["bmask"], dtype=np.int32)
+            check(L.semx_mesh_create_numbered(np_, int(m["nel"]), _dp(vx), _dp(vy), _ip(nbr),
+                                              _ip(sk), svv_mn, svv_eps, kinvis, _ip(bt),
+                                              _ip(bm), int(numbering["nglobal"]),
+                                              int(numbering["nsolve"]), C.byref(h)),
+                  "semx_mesh_create_numbered")
         self.h = h
         info = np.zeros(10, dtype=np.int32)
         check(L.semx_mesh_info(h, _ip(info)))
@@ -95,12 +107,23 @@ class ModalMatrixSys:
     """Helmholtz systems for modes base..base+nmodes-1 of a field of nz
     planes (src/matrix.cpp:33-110)."""
 
-    def __init__(self, mesh, lambda2, beta, base_mode, nmodes, nz, method=DIRECT):
+    def __init__(self, mesh, lambda2, beta, base_mode, nmodes, nz, method=DIRECT,
+                 svv_mz=-1, svv_epsz=0.0, kinvis=1.0, nmode_total=None):
+        """svv_mz >= 0: semtex's Fourier-direction SVV (SVV_MZ, SVV_EPSZ,
+        KINVIS) on the betak2 of modes above svv_mz; nmode_total is the
+        field's Fourier mode count (default base_mode + nmodes)."""
         L = load()
         h = C.c_void_p()
-        check(L.semx_helm_create(mesh.h, float(lambda2), float(beta), int(base_mode),
-                                 int(nmodes), int(nz), int(method), C.byref(h)),
-              "semx_helm_create")
+        if svv_mz >= 0:
+            nmt = base_mode + nmodes if nmode_total is None else nmode_total
+            check(L.semx_helm_create_svv(mesh.h, float(lambda2), float(beta), int(base_mode),
+                                         int(nmodes), int(nz), int(method), int(svv_mz),
+                                         float(svv_epsz), float(kinvis), int(nmt),
+                                         C.byref(h)), "semx_helm_create_svv")
+        else:
+            check(L.semx_helm_create(mesh.h, float(lambda2), float(beta), int(base_mode),
+                                     int(nmodes), int(nz), int(method), C.byref(h)),
+                  "semx_helm_create")
         self.h = h
         self.mesh = mesh
         self.nz = nz
@@ -238,6 +261,17 @@ class Batch:
             check(st, "semx_batch_solve")
         return it.value
 
+    OPERATORS = ("block", "wave", "mfma")
+
+    def info(self):
+        """(element operator per problem -- 'block', 'wave' or 'mfma', None
+        before the first solve -- and the tuner's host time in ms)."""
+        n = len(self.systems)
+        opk = np.zeros(n, dtype=np.int32)
+        ms = C.c_double(0.0)
+        check(load().semx_batch_info(self.h, _ip(opk), C.byref(ms)), "semx_batch_info")
+        return [self.OPERATORS[k] if k >= 0 else None for k in opk], ms.value
+
     def __del__(self):
         try:
             if getattr(self, "h", None):
@@ -269,6 +303,35 @@ def dft_r_noalloc(data, tlen, ntrn, sign, stream=None):
                                     _stream(stream)), "semx_dft_r_noalloc")
 
 
+def write_field(path, mesh, nz, fields, session="session", step=0, time=0.0, dt=0.0,
+                kinvis=0.0, beta=1.0, append=False, stream=None):
+    """Write device fields (Fourier space) as one dump of a semtex field
+    file; fields: {name letter: tensor of nz*planeSize}."""
+    names = "".join(fields)
+    arr = (C.c_void_p * len(fields))(*[_ptr(t) for t in fields.values()])
+    for t in fields.values():
+        assert t.numel() == nz * mesh.planeSize
+    check(load().semx_field_write(path.encode(), int(bool(append)), mesh.h, int(nz), len(fields),
+                                  names.encode(), arr, session.encode(), int(step), float(time),
+                                  float(dt), float(kinvis), float(beta), _stream(stream)),
+          "semx_field_write")
+
+
+def read_field(path, mesh, nz, fields, dump=0, stream=None):
+    """Read dump `dump` of a semtex field file into device fields (Fourier
+    space), {name letter: tensor}; returns (step, time, field names in the
+    file)."""
+    names = "".join(fields)
+    arr = (C.c_void_p * max(len(fields), 1))(*[_ptr(t) for t in fields.values()])
+    step = C.c_int(0)
+    t = C.c_double(0.0)
+    buf = C.create_string_buffer(64)
+    check(load().semx_field_read(path.encode(), int(dump), mesh.h, int(nz), len(fields),
+                                 names.encode(), arr, C.byref(step), C.byref(t), buf,
+                                 _stream(stream)), "semx_field_read")
+    return step.value, t.value, buf.value.decode()
+
+
 def gradient(mesh, nz, beta, data, direction, stream=None):
     """AuxField::gradient in place (direction 0: x, 1: y, 2: z in Fourier space)."""
     assert data.numel() == nz * mesh.planeSize
@@ -283,17 +346,40 @@ def smooth(mesh, nz, data, stream=None):
 
 
 class DNS:
-    """Device time stepping of semtex's dns solver on a periodic mesh
-    (integrate, dns/integrate.cpp:79-221); fields in Fourier space."""
+    """Device time stepping of semtex's dns solver (integrate,
+    dns/integrate.cpp:79-221); fields in Fourier space.
+
+    On a periodic mesh, pressure_mesh and bc are left out.  On a mesh with
+    boundaries, mesh numbers the velocity and pressure_mesh the pressure
+    (two numberings of one element mesh, e.g. from session.num), and bc is
+    the velocity BC lines (u, v[, w]): device tensors of nz*nbound*np in
+    Fourier space, fixed in time; the pressure takes the computed
+    high-order flux on its natural sides."""
 
     def __init__(self, mesh, nz, beta, dt, kinvis, norder=2, advection=0, tol_rel=1e-12,
-                 step_max=500):
+                 step_max=500, pressure_mesh=None, bc=None):
         h = C.c_void_p()
-        check(load().semx_dns_create(mesh.h, int(nz), float(beta), float(dt), float(kinvis),
-                                     int(norder), int(advection), float(tol_rel),
-                                     int(step_max), C.byref(h)), "semx_dns_create")
+        if pressure_mesh is None:
+            check(load().semx_dns_create(mesh.h, int(nz), float(beta), float(dt),
+                                         float(kinvis), int(norder), int(advection),
+                                         float(tol_rel), int(step_max), C.byref(h)),
+                  "semx_dns_create")
+        else:
+            ncom = 3 if nz > 1 else 2
+            if bc is None or len(bc) < ncom:
+                raise ValueError(f"bc needs {ncom} velocity BC line sets")
+            nl = nz * mesh.nbound * mesh.np
+            for t in bc[:ncom]:
+                assert t.numel() == nl
+            check(load().semx_dns_create_bc(mesh.h, pressure_mesh.h, int(nz), float(beta),
+                                            float(dt), float(kinvis), int(norder),
+                                            int(advection), float(tol_rel), int(step_max),
+                                            _ptr(bc[0]), _ptr(bc[1]),
+                                            _ptr(bc[2]) if ncom == 3 else None, C.byref(h)),
+                  "semx_dns_create_bc")
         self.h = h
         self.mesh = mesh
+        self.pressure_mesh = pressure_mesh
         self.nz = nz
 
     def run(self, nstep, u, v, w, p, stream=None):

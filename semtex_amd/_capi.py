@@ -21,6 +21,9 @@ PROTOTYPES = {
     "semx_device_init": (C.c_int, [C.c_int]),
     "semx_mesh_create": (C.c_int, [C.c_int, C.c_int, D, D, I, I, C.c_int, C.c_double,
                                    C.c_double, C.POINTER(VP)]),
+    "semx_mesh_create_numbered": (C.c_int, [C.c_int, C.c_int, D, D, I, I, C.c_int, C.c_double,
+                                          C.c_double, I, I, C.c_int, C.c_int,
+                                          C.POINTER(VP)]),
     "semx_mesh_destroy": (C.c_int, [VP]),
     "semx_mesh_info": (C.c_int, [VP, I]),
     "semx_mesh_nodes": (C.c_int, [VP, D, D]),
@@ -29,6 +32,9 @@ PROTOTYPES = {
     "semx_mesh_nsets": (C.c_int, [VP, I]),
     "semx_helm_create": (C.c_int, [VP, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
                                    C.c_int, C.POINTER(VP)]),
+    "semx_helm_create_svv": (C.c_int, [VP, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                       C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
+                                       C.POINTER(VP)]),
     "semx_helm_destroy": (C.c_int, [VP]),
     "semx_helm_info": (C.c_int, [VP, I]),
     "semx_helm_element": (C.c_int, [VP, C.c_int, C.c_int, D, D, D]),
@@ -40,14 +46,23 @@ PROTOTYPES = {
                                          C.POINTER(VP), C.c_double, C.c_int, I, VP]),
     "semx_batch_create": (C.c_int, [C.c_int, C.POINTER(VP), C.POINTER(VP)]),
     "semx_batch_destroy": (C.c_int, [VP]),
+    "semx_batch_info": (C.c_int, [VP, I, C.POINTER(C.c_double)]),
     "semx_batch_solve": (C.c_int, [VP, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP),
                                    C.c_double, C.c_int, I, VP]),
     "semx_dft_r": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, VP]),
     "semx_dft_r_noalloc": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, VP]),
+    "semx_field_write": (C.c_int, [C.c_char_p, C.c_int, VP, C.c_int, C.c_int, C.c_char_p,
+                                   C.POINTER(VP), C.c_char_p, C.c_int, C.c_double, C.c_double,
+                                   C.c_double, C.c_double, VP]),
+    "semx_field_read": (C.c_int, [C.c_char_p, C.c_int, VP, C.c_int, C.c_int, C.c_char_p,
+                                  C.POINTER(VP), I, C.POINTER(C.c_double), C.c_char_p, VP]),
     "semx_gradient": (C.c_int, [VP, C.c_int, C.c_double, VP, C.c_int, VP]),
     "semx_smooth": (C.c_int, [VP, C.c_int, VP, VP]),
     "semx_dns_create": (C.c_int, [VP, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
                                   C.c_int, C.c_double, C.c_int, C.POINTER(VP)]),
+    "semx_dns_create_bc": (C.c_int, [VP, VP, C.c_int, C.c_double, C.c_double, C.c_double,
+                                     C.c_int, C.c_int, C.c_double, C.c_int, VP, VP, VP,
+                                     C.POINTER(VP)]),
     "semx_dns_destroy": (C.c_int, [VP]),
     "semx_dns_run": (C.c_int, [VP, C.c_int, VP, VP, VP, VP, VP]),
     "semx_dns_info": (C.c_int, [VP, I]),

@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libsemx.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["semx_setup.cpp", "semx_solve.hip", "semx_dns.hip"]
+SOURCES = ["semx_setup.cpp", "semx_io.cpp", "semx_solve.hip", "semx_dns.hip"]
 
 # Packed FP32 instructions are switched off for all device code.
 DEVICE_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",

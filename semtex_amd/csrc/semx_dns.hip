@@ -11,9 +11,14 @@
 //   k_smooth     Field::smooth, src/field.cpp:190-236 (mass-weighted DSS)
 //   elementwise  AuxField times / timesMinus / axpy / scaling passes
 //
+//   k_hopbc      BCmgr::maintainFourier and evaluateCNBCp (src/bcmgr.cpp:
+//                1039-1337): the computed high-order pressure flux on the
+//                boundary sides of a bounded mesh
+//
 // and the host driver of integrate (dns/integrate.cpp:79-221) with
 // waveProp, setPForce, project and the velocity/pressure solves of
-// Field::solve (semx_solve.hip).  Periodic (boundary-free) meshes only.
+// Field::solve (semx_solve.hip), on a periodic mesh or on a bounded one
+// (velocity BC lines given, pressure flux computed).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -273,6 +278,131 @@ __global__ __launch_bounds__(OP_THREADS) void k_grad_nl(double* __restrict__ N,
   else if (MODE == 1) N[off] = (sc == 0.0 ? 0.0 : N[off]) - d;
   else if (MODE == 2) N[off] = d;
   else N[off] = sc * N[off] + c * d;
+}
+
+// Computed pressure boundary conditions (KIO91; BCmgr::maintainFourier,
+// src/bcmgr.cpp:1039-1298, and evaluateCNBCp, :1301-1337), Cartesian.  One
+// workgroup per (boundary side, Fourier mode); at each node of the side and
+// each plane of the mode (the Nyquist plane is left alone):
+//   un   = u.n
+//   hop  = -(1/dt) (a0 un + sum_{l<Jd} a_{l+1} un_l)      (Jd > 0 only)
+//          + n.N(u) - nu n.curlCurl(u)
+//   un_l, hop_l shift down one level (level 0 the new value)
+//   bcp  = sum_{l<Je} b_l hop_l on the pressure's natural sides, else 0
+// with u the Fourier velocity at the start of the step and N(u) its
+// nonlinear terms.  curlCurl (Edge::curlCurl, src/edge.cpp:98-258) takes
+// the z vorticity w = dv/dx - du/dy on the whole element and its gradient
+// at the side: (dw/dy, -dw/dx), plus for mode k > 0 (beta k) the
+// semi-Fourier terms of w-velocity and (beta k)^2 (u, v).  Each thread owns
+// one (plane, side node) entry of every level array, so the in-place level
+// shift has no hazards, and the level arrays never move (one HIP graph of
+// the step stays valid).
+struct HopArgs {
+  const double* U[3];     // Fourier velocity, nz*planeSize each (W unused in 2D)
+  const double* N[2];     // Fourier nonlinear terms, x and y
+  double* un;             // nord levels of u.n, nz*nline each
+  double* hop;            // nord levels of the flux terms
+  double* bcp;            // pressure BC lines, nz*nline
+  const int* bel;         // nbound: element
+  const int* bsd;         // nbound: side 0-3
+  const int* pnat;        // nbound: pressure side natural
+  const double* nrm;      // 2*nline: unit outward normal x, then y
+  const double* G;
+  const int* eset;
+  const double* ops;      // DV first
+  int np, nz, nline, planeSize, nord, w3, Jd, Je;
+  double beta, nu, invdt;
+  double a[4], b[3];
+};
+
+__device__ __forceinline__ int side_node(int np, int s, int q) {
+  return s == 0 ? q : s == 1 ? q * np + np - 1 : s == 2 ? (np - 1) * np + (np - 1 - q)
+                                                        : (np - 1 - q) * np;
+}
+
+// x and y derivatives at element node n of element data f (LDS, np x np).
+__device__ __forceinline__ void node_grad(const double* f, const double* sD, int np, int n,
+                                          const double* g, int npnp, double& gx, double& gy) {
+  const int i = n / np, j = n - (n / np) * np;
+  double ddr = 0.0, dds = 0.0;
+  for (int k = 0; k < np; ++k) {
+    ddr += sD[j * np + k] * f[i * np + k];
+    dds += sD[i * np + k] * f[k * np + j];
+  }
+  gx = ddr * g[G_DRDX * npnp + n] + dds * g[G_DSDX * npnp + n];
+  gy = ddr * g[G_DRDY * npnp + n] + dds * g[G_DSDY * npnp + n];
+}
+
+__global__ __launch_bounds__(256) void k_hopbc(HopArgs A) {
+  __shared__ double sD[256], sf[6][256], sw[2][256];
+  const int np = A.np, npnp = np * np, t = threadIdx.x;
+  const int b = blockIdx.x, m = blockIdx.y;
+  const int e = A.bel[b], sd = A.bsd[b];
+  const int npl = (m == 0) ? 1 : 2;               // planes of this mode
+  const int k0 = 2 * m;                           // mode 0: plane 0 only
+  const bool zt = m > 0 && A.w3;                  // semi-Fourier terms of w
+  const double* g = A.G + (size_t)A.eset[e] * G_N * npnp;
+  if (t < npnp) {
+    sD[t] = A.ops[t];
+    for (int p = 0; p < npl; ++p) {
+      const size_t off = (size_t)(k0 + p) * A.planeSize + (size_t)e * npnp + t;
+      sf[2 * p][t] = A.U[0][off];
+      sf[2 * p + 1][t] = A.U[1][off];
+      if (zt) sf[4 + p][t] = A.U[2][off];
+    }
+  }
+  __syncthreads();
+  // z vorticity of each plane on the whole element
+  if (t < npnp)
+    for (int p = 0; p < npl; ++p) {
+      double ux, uy, vx, vy;
+      node_grad(sf[2 * p], sD, np, t, g, npnp, ux, uy);
+      node_grad(sf[2 * p + 1], sD, np, t, g, npnp, vx, vy);
+      sw[p][t] = vx - uy;
+    }
+  __syncthreads();
+  if (t >= npl * np) return;
+  const int p = t / np, q = t - (t / np) * np;    // p 0 real, 1 imaginary
+  const int n = side_node(np, sd, q);
+  const double bk = A.beta * m;
+  double wx, wy;
+  node_grad(sw[p], sD, np, n, g, npnp, wx, wy);
+  double cx = wy, cy = -wx;                       // curl curl (u, v) of the plane
+  if (zt) {   // real: -bk d(Wi)/dx_j, imaginary: +bk d(Wr)/dx_j
+    double zx, zy;
+    node_grad(sf[4 + (1 - p)], sD, np, n, g, npnp, zx, zy);
+    const double sg = p ? bk : -bk;
+    cx += sg * zx;
+    cy += sg * zy;
+  }
+  if (m > 0) {
+    cx += bk * bk * sf[2 * p][n];
+    cy += bk * bk * sf[2 * p + 1][n];
+  }
+  const int j = b * np + q;
+  const size_t i = (size_t)(k0 + p) * A.nline + j;
+  const size_t lv = (size_t)A.nz * A.nline;
+  const double nx = A.nrm[j], ny = A.nrm[A.nline + j];
+  const size_t off = (size_t)(k0 + p) * A.planeSize + (size_t)e * npnp + n;
+  const double un = sf[2 * p][n] * nx + sf[2 * p + 1][n] * ny;
+  double h = 0.0;
+  if (A.Jd) {
+    double wk = A.a[0] * un;
+    for (int l = 0; l < A.Jd; ++l) wk += A.a[l + 1] * A.un[l * lv + i];
+    h = -A.invdt * wk;
+  }
+  h += A.N[0][off] * nx + A.N[1][off] * ny;
+  h += -A.nu * cx * nx + -A.nu * cy * ny;
+  for (int l = A.nord - 1; l > 0; --l) {
+    A.un[l * lv + i] = A.un[(l - 1) * lv + i];
+    A.hop[l * lv + i] = A.hop[(l - 1) * lv + i];
+  }
+  A.un[i] = un;
+  A.hop[i] = h;
+  double v = 0.0;
+  if (A.pnat[b])
+    for (int l = 0; l < A.Je; ++l) v += A.b[l] * A.hop[l * lv + i];
+  A.bcp[i] = v;
 }
 
 // z-derivative of Fourier planes: mode k Re <- -beta k Im, Im <- beta k Re;
@@ -677,6 +807,17 @@ struct semx_dns_s {
   Helm* vel = nullptr;                  // DIRECT, lambda2 = alpha0/(dt nu)
   Helm* pre = nullptr;                  // DIRECT, lambda2 = 0
   Helm* start[2] = {nullptr, nullptr};  // JACPCG start-up velocity systems
+  // bounded mesh: the pressure's numbering and side kinds (same elements
+  // and boundary list as the velocity mesh), BC lines and flux history
+  Mesh* Mp = nullptr;
+  int nline = 0;
+  double* bstore = nullptr;             // the arrays below in one allocation
+  double* bcu[3] = {nullptr, nullptr, nullptr};   // velocity BC lines, nz*nline each
+  double* bcp = nullptr;                // pressure BC lines
+  double* un = nullptr;                 // nord levels of u.n
+  double* hop = nullptr;                // nord levels of the flux terms
+  double* nrm = nullptr;                // side normals, 2*nline
+  int* bint = nullptr;                  // element, side, pressure-natural per boundary
   bool graphs_on = false;               // replay steady steps from HIP graphs
   hipStream_t capture = nullptr;        // stream the steps are captured on
   std::vector<DnsGraph> graphs;         // one per role assignment met
@@ -787,6 +928,31 @@ int dns_nonlinear(semx_dns_s& D, hipStream_t s) {
   return SEMX_OK;
 }
 
+// BCmgr::maintainFourier + the pressure's evaluateBoundaries
+// (dns/integrate.cpp:168-172) for step D.step (already advanced): Us[0]
+// and Uf[0] hold the Fourier velocity and nonlinear terms of the step.
+int dns_hopbc(semx_dns_s& D, hipStream_t s) {
+  Mesh& M = *D.M;
+  HopArgs A{};
+  for (int i = 0; i < 3; ++i) A.U[i] = D.Us[0][i];
+  A.N[0] = D.Uf[0][0]; A.N[1] = D.Uf[0][1];
+  A.un = D.un; A.hop = D.hop; A.bcp = D.bcp;
+  A.bel = D.bint; A.bsd = D.bint + M.nbound; A.pnat = D.bint + 2 * M.nbound;
+  A.nrm = D.nrm;
+  A.G = M.d.G; A.eset = M.d.eset; A.ops = M.d.ops;
+  A.np = M.np; A.nz = D.nz; A.nline = D.nline; A.planeSize = M.planeSize;
+  A.nord = D.nord; A.w3 = D.ncom == 3;
+  A.Jd = std::min(D.step - 1, D.nord);
+  A.Je = std::min(D.step, D.nord);
+  if (A.Jd) stiffly_stable(A.Jd, A.a);
+  extrapolation(A.Je, A.b);
+  A.beta = D.beta; A.nu = D.kinvis; A.invdt = 1.0 / D.dt;
+  const int nmodes = D.nz > 1 ? D.nz / 2 : 1;
+  hipLaunchKernelGGL(k_hopbc, dim3(M.nbound, nmodes), dim3(256), 0, s, A);
+  HIPCHK(hipGetLastError());
+  return SEMX_OK;
+}
+
 // rollm (src/sem.h:106-120): level q takes level q-1, level 0 the last.
 void roll(double* lev[3][3], int nord, int ncom) {
   if (nord < 2) return;
@@ -804,6 +970,7 @@ int dns_step(semx_dns_s& D, hipStream_t s) {
   const int ncom = D.ncom, nord = D.nord;
   CHK(dns_nonlinear(D, s));
   D.step += 1;
+  if (D.Mp) CHK(dns_hopbc(D, s));
   const int Je = std::min(D.step, nord);
   double alpha[4], beta[3];
   stiffly_stable(Je, alpha);
@@ -837,7 +1004,7 @@ int dns_step(semx_dns_s& D, hipStream_t s) {
   }
   EW(k_divergence_sum, D.Uf[0][0], D.Uf[0][1], D.ndim > 2 ? D.Uf[0][2] : nullptr, D.dt);
   // pressure solve
-  CHK(helm_solve(*D.pre, D.Uf[0][0], nullptr, D.P, D.tol_rel, D.step_max, nullptr, s));
+  CHK(helm_solve(*D.pre, D.Uf[0][0], D.bcp, D.P, D.tol_rel, D.step_max, nullptr, s));
   // project
   {
     const double a = -1.0 / (D.dt * D.kinvis), b = 1.0 / D.kinvis;
@@ -863,7 +1030,9 @@ int dns_step(semx_dns_s& D, hipStream_t s) {
   Helm* V = (D.step < nord) ? D.start[D.step - 1] : D.vel;
   double* fs[3] = {D.Uf[0][0], D.Uf[0][1], D.Uf[0][2]};
   double* us[3] = {D.U[0], D.U[1], D.U[2]};
-  CHK(helm_solve_fields(*V, ncom, fs, nullptr, us, D.tol_rel, D.step_max, nullptr, s));
+  const double* bcs[3] = {D.bcu[0], D.bcu[1], D.bcu[2]};
+  CHK(helm_solve_fields(*V, ncom, fs, D.Mp ? bcs : nullptr, us, D.tol_rel, D.step_max,
+                        nullptr, s));
   return SEMX_OK;
 }
 
@@ -946,6 +1115,8 @@ void dns_free(semx_dns_s* D) {
     delete h;
   }
   dev_free(D->store);
+  dev_free(D->bstore);
+  dev_free(D->bint);
   delete D;
 }
 
@@ -1084,15 +1255,71 @@ int semx_smooth(semx_mesh mesh, int nz, double* data, void* stream) {
   return smooth_launch(M, nz, data, (hipStream_t)stream);
 }
 
-int semx_dns_create(semx_mesh mesh, int nz, double beta, double dt, double kinvis, int norder,
-                    int advection, double tol_rel, int step_max, semx_dns* dns) {
-  if (!mesh || !dns || nz < 1 || (nz > 1 && (nz & 1)) || norder < 1 || norder > 3 ||
+}  // extern "C"
+
+namespace {
+
+// The outward unit normals of every boundary side at its nodes, CCW
+// (Element::sideGeom, src/element.cpp:1653-1744): side 0 -grad s, side 1
+// grad r, side 2 grad s, side 3 -grad r, normalised.
+std::vector<double> side_normals(const Mesh& M) {
+  const int np = M.np, npnp = M.npnp, nline = M.nbound * np;
+  std::vector<double> nrm(2 * (size_t)nline);
+  for (int b = 0; b < M.nbound; ++b) {
+    const int e = M.belmt[b], sd = M.bside[b];
+    const double* g = M.G.data() + (size_t)M.eset[e] * G_N * npnp;
+    for (int q = 0; q < np; ++q) {
+      const int n = sd == 0 ? q : sd == 1 ? q * np + np - 1
+                  : sd == 2 ? (np - 1) * np + (np - 1 - q) : (np - 1 - q) * np;
+      const int rs = (sd == 0 || sd == 2) ? 1 : 0;     // s or r gradient
+      const double sg = (sd == 0 || sd == 3) ? -1.0 : 1.0;
+      const double a = sg * g[(rs ? G_DSDX : G_DRDX) * npnp + n];
+      const double c = sg * g[(rs ? G_DSDY : G_DRDY) * npnp + n];
+      const double len = std::hypot(a, c);
+      nrm[b * np + q] = a / len;
+      nrm[nline + b * np + q] = c / len;
+    }
+  }
+  return nrm;
+}
+
+// Device storage of a bounded DNS: velocity BC lines copied from the
+// caller's (device) lines, zeroed pressure lines and flux history, normals
+// and the boundary tables.
+int dns_bounded(semx_dns_s& D, Mesh& Mp, const double* const* bc) {
+  Mesh& M = *D.M;
+  D.Mp = &Mp;
+  D.nline = M.nbound * M.np;
+  const size_t nl = (size_t)D.nz * D.nline;
+  const size_t total = (3 + 1 + 2 * (size_t)D.nord) * nl + 2 * (size_t)D.nline;
+  CHK(dev_upload<double>(&D.bstore, nullptr, total));
+  double* f = D.bstore;
+  for (int i = 0; i < 3; ++i) { D.bcu[i] = f; f += nl; }
+  D.bcp = f; f += nl;
+  D.un = f; f += D.nord * nl;
+  D.hop = f; f += D.nord * nl;
+  D.nrm = f;
+  for (int i = 0; i < D.ncom; ++i)
+    HIPCHK(hipMemcpy(D.bcu[i], bc[i], nl * sizeof(double), hipMemcpyDeviceToDevice));
+  const std::vector<double> nrm = side_normals(M);
+  HIPCHK(hipMemcpy(D.nrm, nrm.data(), nrm.size() * sizeof(double), hipMemcpyHostToDevice));
+  std::vector<int> bint(3 * (size_t)M.nbound);
+  for (int b = 0; b < M.nbound; ++b) {
+    bint[b] = M.belmt[b];
+    bint[M.nbound + b] = M.bside[b];
+    bint[2 * M.nbound + b] = Mp.bkind[b] == 2;
+  }
+  return dev_upload<int>(&D.bint, bint.data(), bint.size());
+}
+
+int dns_create(Mesh& M, Mesh* Mp, const double* const* bc, int nz, double beta, double dt,
+               double kinvis, int norder, int advection, double tol_rel, int step_max,
+               semx_dns* dns) {
+  if (!dns || nz < 1 || (nz > 1 && (nz & 1)) || norder < 1 || norder > 3 ||
       advection < 0 || advection > 1 || !(dt > 0.0) || !(kinvis > 0.0) || step_max < 1)
     return SEMX_E_ARG;
   if (nz > 2 && !fft_length_ok(nz)) return SEMX_E_ARG;
   *dns = nullptr;
-  Mesh& M = mesh->M;
-  if (M.nbound) return SEMX_E_UNSUPPORTED;   // boundary conditions are not part of the DNS path
   semx_dns_s* D = new (std::nothrow) semx_dns_s();
   if (!D) return SEMX_E_NOMEM;
   D->M = &M; D->nz = nz; D->nord = norder; D->advection = advection; D->beta = beta;
@@ -1117,7 +1344,9 @@ int semx_dns_create(semx_mesh mesh, int nz, double beta, double dt, double kinvi
   double alpha[4];
   stiffly_stable(norder, alpha);
   st = make_system(M, alpha[0] / (dt * kinvis), beta, nz, SEMX_DIRECT, &D->vel);
-  if (!st) st = make_system(M, 0.0, beta, nz, SEMX_DIRECT, &D->pre);
+  if (!st && Mp) st = mesh_upload(*Mp);
+  if (!st && Mp) st = dns_bounded(*D, *Mp, bc);
+  if (!st) st = make_system(Mp ? *Mp : M, 0.0, beta, nz, SEMX_DIRECT, &D->pre);
   for (int q = 1; q < norder && !st; ++q) {
     stiffly_stable(q, alpha);
     st = make_system(M, alpha[0] / (dt * kinvis), beta, nz, SEMX_JACPCG, &D->start[q - 1]);
@@ -1125,6 +1354,34 @@ int semx_dns_create(semx_mesh mesh, int nz, double beta, double dt, double kinvi
   if (st) { dns_free(D); return st; }
   *dns = D;
   return SEMX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int semx_dns_create(semx_mesh mesh, int nz, double beta, double dt, double kinvis, int norder,
+                    int advection, double tol_rel, int step_max, semx_dns* dns) {
+  if (!mesh) return SEMX_E_ARG;
+  // a bounded mesh needs its BC lines and pressure numbering: semx_dns_create_bc
+  if (mesh->M.nbound) return SEMX_E_UNSUPPORTED;
+  return dns_create(mesh->M, nullptr, nullptr, nz, beta, dt, kinvis, norder, advection, tol_rel,
+                    step_max, dns);
+}
+
+int semx_dns_create_bc(semx_mesh vmesh, semx_mesh pmesh, int nz, double beta, double dt,
+                       double kinvis, int norder, int advection, double tol_rel, int step_max,
+                       const double* bcu, const double* bcv, const double* bcw, semx_dns* dns) {
+  if (!vmesh || !pmesh || !bcu || !bcv || (nz > 1 && !bcw)) return SEMX_E_ARG;
+  const Mesh& M = vmesh->M;
+  const Mesh& P = pmesh->M;
+  // the two numberings must be of one element mesh with one boundary list
+  if (!M.nbound || P.np != M.np || P.nel != M.nel || P.nbound != M.nbound ||
+      P.belmt != M.belmt || P.bside != M.bside || P.x != M.x || P.y != M.y)
+    return SEMX_E_MESH;
+  const double* bc[3] = {bcu, bcv, bcw};
+  return dns_create(vmesh->M, &pmesh->M, bc, nz, beta, dt, kinvis, norder, advection, tol_rel,
+                    step_max, dns);
 }
 
 int semx_dns_destroy(semx_dns dns) {

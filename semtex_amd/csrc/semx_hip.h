@@ -59,5 +59,4 @@ int helm_check(Helm& H, hipStream_t s);
 
 }  // namespace semx
 
-struct semx_mesh_s { semx::Mesh M; };
-struct semx_helm_s { semx::Helm H; };
+

@@ -106,12 +106,18 @@ struct HelmDev {
   double* g = nullptr;        // nz*nzero essential values
   double* sol = nullptr;      // nplanes*solstride
   int* plane_of = nullptr;    // nplanes -> field plane
+  double* hcon = nullptr;     // per mode: lambda2 + betak2 (betak2 with its SVV weight)
 };
 
 struct Helm {
   Mesh* M = nullptr;
   double lambda2 = 0.0, beta = 0.0;
   int base = 0, nmodes = 0, nz = 0, method = SEMX_DIRECT;
+  // Fourier-direction SVV (SVV_MZ, SVV_EPSZ, KINVIS; src/svv.cpp:82-123):
+  // betak2 of mode k > svv_mz weighted by 1 + epsz/nu exp(-((N-k)/(mz-k))^2)
+  // when lambda2 > EPSDP (src/matrix.cpp:78-83); svv_mz < 0: off
+  int svv_mz = -1, svv_nmode = 0;   // svv_nmode: Fourier modes of the whole field
+  double svv_epsz = 0.0, svv_nu = 1.0;
   int nb = 32;                // band-solve block size (columns per step)
   int npass = 1, bt = 1024;   // band-solve row passes, threads per block
   int band_d = 1;             // band-solve panel prefetch depth (steps); depth 2
@@ -141,7 +147,10 @@ struct Helm {
   std::vector<int> plane_of;
   std::vector<int> dmode, mode_plane;
   std::vector<int> pcg_plane;         // plane-list indices of JACPCG modes
-  int pcg_prec = 0;                   // SEMX_PCG_FP64 or SEMX_PCG_FP32_REFINED
+  int pcg_prec = 0;                   // SEMX_PCG_FP64, _FP32_REFINED or _BF16_REFINED
+  int pcg_opk = -1;                   // element operator of the PCG planes (block-wide
+                                      // or wavefront-level scalar, BF16 matrix cores),
+                                      // -1 until chosen on the first solve
   double inner_tol = 1e-5;            // inner relative tolerance (refined)
   int max_outer = 12;                 // refinement sweeps (refined)
   size_t workstride = 0;              // doubles per PCG plane
@@ -157,9 +166,12 @@ struct Helm {
 };
 
 // setup (semx_setup.cpp)
+// btog/bmask (nel*next each, or null for the library's own numbering),
+// nglobal, nsolve: a caller-supplied numbering (semtex's session.num)
 int mesh_build(Mesh& M, int np, int nel, const double* vx, const double* vy,
                const int* nbr, const int* skind, int svv_mn, double svv_eps,
-               double kinvis);
+               double kinvis, const int* btog = nullptr, const int* bmask = nullptr,
+               int nglobal = 0, int nsolve = 0);
 int helm_build(Helm& H);
 void gll(int np, std::vector<double>& z, std::vector<double>& w);
 void gll_deriv(int np, const std::vector<double>& z, std::vector<double>& D);
@@ -173,3 +185,7 @@ int helm_upload(Helm& H);
 void helm_release(Helm& H);
 
 }  // namespace semx
+
+// the C ABI's opaque handles
+struct semx_mesh_s { semx::Mesh M; };
+struct semx_helm_s { semx::Helm H; };

@@ -297,7 +297,17 @@ static int find_root(std::vector<int>& par, int a) {
   return a;
 }
 
-static int number_mesh(Mesh& M, const int* nbr, const int* skind) {
+// A numbering supplied by the caller (semtex's session.num, read by
+// BCmgr::buildnum, src/bcmgr.cpp:586-824): it must join exactly the slots
+// the connectivity joins, mask exactly the essential nodes and number the
+// nsolve unknowns before the essential nodes.
+struct GivenNumbering {
+  const int* btog;
+  const int* bmask;
+  int nglobal, nsolve;
+};
+
+static int number_mesh(Mesh& M, const int* nbr, const int* skind, const GivenNumbering* given) {
   const int np = M.np, nel = M.nel, next = M.next, nm = np - 1, ni = np - 2;
   const int nside = 4 * nel;
   // -- validate the side connectivity: joined sides pair up symmetrically,
@@ -392,10 +402,32 @@ static int number_mesh(Mesh& M, const int* nbr, const int* skind) {
     G.adj.reserve(G.ptr[nunk]);
     for (int a = 0; a < nunk; ++a) G.adj.insert(G.adj.end(), nbr[a].begin(), nbr[a].end());
   }
-  // -- candidate orderings; keep the narrowest band (the role of enumerate's
-  //    -O2/-O3 root searches, utility/enumerate.cpp:680-809).
   std::vector<int> bmask(nel * next);
   for (int i = 0; i < nel * next; ++i) bmask[i] = ess[prov[i]];
+  if (given) {
+    const GivenNumbering& gv = *given;
+    if (gv.nglobal != ng || gv.nsolve != nunk) return SEMX_E_MESH;
+    std::vector<int> to(ng, -1), from(ng, -1);
+    for (int i = 0; i < nel * next; ++i) {
+      const int g = gv.btog[i], p = prov[i];
+      if (g < 0 || g >= ng || gv.bmask[i] != bmask[i]) return SEMX_E_MESH;
+      if ((bmask[i] != 0) != (g >= nunk)) return SEMX_E_MESH;
+      if ((to[p] >= 0 && to[p] != g) || (from[g] >= 0 && from[g] != p)) return SEMX_E_MESH;
+      to[p] = g;
+      from[g] = p;
+    }
+    M.btog.assign(gv.btog, gv.btog + nel * next);
+    M.bmask = bmask;
+    M.nglobal = ng;
+    M.nsolve = nunk;
+    M.nband = band_of(M.btog, M.bmask, nel, next);
+    M.emask.assign(nel, 0);
+    for (int e = 0; e < nel; ++e)
+      for (int i = 0; i < next; ++i) if (M.bmask[e * next + i]) M.emask[e] = 1;
+    return SEMX_OK;
+  }
+  // -- candidate orderings; keep the narrowest band (the role of enumerate's
+  //    -O2/-O3 root searches, utility/enumerate.cpp:680-809).
   auto apply = [&](const std::vector<int>& perm, std::vector<int>& btog) {
     std::vector<int> newid(ng, -1);
     for (int k = 0; k < nunk; ++k) newid[uold[perm[k]]] = k;
@@ -460,7 +492,7 @@ static int number_mesh(Mesh& M, const int* nbr, const int* skind) {
 
 int mesh_build(Mesh& M, int np, int nel, const double* vx, const double* vy,
                const int* nbr, const int* skind, int svv_mn, double svv_eps,
-               double kinvis) {
+               double kinvis, const int* btog, const int* bmask, int nglobal, int nsolve) {
   if (np < 3 || np > 16 || nel < 1) return SEMX_E_ARG;
   M.np = np; M.nel = nel; M.npnp = np * np;
   M.next = 4 * (np - 1); M.nint = M.npnp - M.next;
@@ -512,7 +544,8 @@ int mesh_build(Mesh& M, int np, int nel, const double* vx, const double* vy,
     }
     M.eset[e] = found;
   }
-  int st = number_mesh(M, nbr, skind);
+  const GivenNumbering given{btog, bmask, nglobal, nsolve};
+  int st = number_mesh(M, nbr, skind, (btog && bmask) ? &given : nullptr);
   if (st) return st;
   const int next = M.next, nm = np - 1;
   // -- inverse global mass (src/bcmgr.cpp:806-822)
@@ -773,10 +806,23 @@ int helm_build(Helm& H) {
   }
   size_t offH = 0, offL = 0;
   const int nb = H.nb;
+  // SVV weights of the Fourier modes (SVV::coeffs_z, src/svv.cpp:82-123:
+  // N = nMode - 1, nMode the field's Fourier mode count, src/geometry.h:42)
+  std::vector<double> Sz(nmodes, 1.0);
+  {
+    const int N = H.svv_nmode - 1, mz = H.svv_mz;
+    if (mz >= 0 && mz < N && H.svv_epsz > EPSDP)
+      for (int m = 0; m < nmodes; ++m) {
+        const int i = H.base + m;
+        const double q = (i > mz) ? std::exp(-std::pow((N - i) / (1.0 * mz - i), 2)) : 0.0;
+        Sz[m] = 1.0 + H.svv_epsz / H.svv_nu * q;
+      }
+  }
   for (int m = 0; m < nmodes; ++m) {
     ModeSys& S = H.modes[m];
     const double bk = H.beta * (H.base + m);
     S.betak2 = bk * bk;
+    if (H.lambda2 > EPSDP) S.betak2 *= Sz[m];   // src/matrix.cpp:82-83
     S.kind = (H.method == SEMX_MIXED) ? ((H.base + m == 0) ? SEMX_DIRECT : SEMX_JACPCG)
                                       : H.method;
     // Singular: no essential BCs and lambda2 + betak2 ~ 0 (matrix.cpp:167-169).

@@ -24,6 +24,9 @@
 #include <cstring>
 #include <new>
 #include <type_traits>
+#include <chrono>
+#include <map>
+#include <string>
 
 #include "semx_hip.h"
 
@@ -37,6 +40,7 @@ struct ElemArgs {
   int np, npnp, next, nint, nel, planeSize, nline, nsolve, nzero, nset;
   double lambda2, beta;
   int base;
+  const double* hcon;     // per mode: lambda2 + betak2
   const double* G;
   const int* eset;
   const int* emap;
@@ -88,8 +92,7 @@ __device__ inline void condense_chunk(const ElemArgs& a, int p, int e0, int e_en
   const int e = e0 + el;
   const bool act = el < ce && e < e_end;
   const int m = k >> 1;
-  const double bk = a.beta * (a.base + m);
-  const double hcon = a.lambda2 + bk * bk;
+  const double hcon = a.hcon[m];
   const bool svv = a.lambda2 > EPSDP;
   double* sD = esh;                       // npnp
   double* sT = sD + npnp;                 // npnp
@@ -222,8 +225,7 @@ __global__ __launch_bounds__(ELT_THREADS, 8) void k_condense_staged(ElemArgs a,
   const int np = a.np, next = a.next, nint = a.nint, npnp = a.npnp;
   const int m = blockIdx.y;
   const bool direct = a.mode_kind[m] == SEMX_DIRECT;
-  const double bk = a.beta * (a.base + m);
-  const double hcon = a.lambda2 + bk * bk;
+  const double hcon = a.hcon[m];
   const bool svv = a.lambda2 > EPSDP;
   double* sHbiT = esh;                          // nint*next, [c][b]
   double* sD = sHbiT + (size_t)next * nint;     // npnp
@@ -702,11 +704,10 @@ __device__ inline double lane_swap1(double x) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-template <int BT, int NR, int D, bool FWD>
+template <int NB, int BT, int NR, int D, bool FWD>
 __device__ inline void band_res_sweep(const double* __restrict__ P, double* const (&vec)[NR],
                                       const int* __restrict__ xrows, int n, int nblk, int kd,
                                       int ldx) {
-  constexpr int NB = XNB;
   constexpr int NH = NB / 2;            // values per thread and step
   const int tid = threadIdx.x;
   const int row = tid >> 1, par = tid & 1;
@@ -864,7 +865,7 @@ __global__ __launch_bounds__(BT) void k_band_ext(BandArgs a) {
                                               a.ring - 1);
 }
 
-template <int BT, int NR, int D>
+template <int NB, int BT, int NR, int D>
 __global__ __launch_bounds__(BT) void k_band_res(BandArgs a) {
   extern __shared__ double lds[];
   int md, fsel;
@@ -886,8 +887,8 @@ __global__ __launch_bounds__(BT) void k_band_res(BandArgs a) {
   }
   const int n = uni(a.mode_n[m]);
   const long long offp = uni64(a.offP[m]);
-  const int nblk = (n + XNB - 1) / XNB;
-  const int npad = nblk * XNB;
+  const int nblk = (n + NB - 1) / NB;
+  const int npad = nblk * NB;
   double* vec[NR];
   double* gy[NR];
 #pragma unroll
@@ -907,8 +908,8 @@ __global__ __launch_bounds__(BT) void k_band_res(BandArgs a) {
     for (int c = 0; c < NR; ++c) vec[c][i] = (i < n) ? gy[c][i] : 0.0;
   }
   __syncthreads();   // the row table is read by the first prefetches
-  band_res_sweep<BT, NR, D, true>(a.Pf + offp, vec, xr, n, nblk, a.kd, a.kdp);
-  band_res_sweep<BT, NR, D, false>(a.Pb + offp, vec, xr, n, nblk, a.kd, a.kdp);
+  band_res_sweep<NB, BT, NR, D, true>(a.Pf + offp, vec, xr, n, nblk, a.kd, a.kdp);
+  band_res_sweep<NB, BT, NR, D, false>(a.Pb + offp, vec, xr, n, nblk, a.kd, a.kdp);
   for (int i = threadIdx.x; i < n; i += BT) {
 #pragma unroll
     for (int c = 0; c < NR; ++c) gy[c][i] = vec[c][i];
@@ -1603,6 +1604,7 @@ struct PcgArgs {
   int np, npnp, next, nint, nel, nglobal, nsolve, nzero, npts, planeSize, nset;
   double lambda2, beta;
   int base;
+  const double* hcon;     // per mode: lambda2 + betak2
   const double* G;
   const int* eset;
   const int* emap;
@@ -1630,6 +1632,8 @@ struct PcgArgs {
 };
 
 constexpr int PCG_THREADS = 1024;
+constexpr int PCG_VU = 4;     // vector-pass entries per thread per load group
+constexpr int PCG_TUNE_ITERS = 30;   // iterations per timed operator candidate
 
 __device__ inline double block_sum(double v, double* red) {
 #pragma unroll
@@ -1656,7 +1660,55 @@ struct PcgOp {
   const double* ops;
   const int* asm_ptr;
   const int* asm_idx;
+  int nset;               // geometric-factor sets (1: the same for all elements)
 };
+
+// Owner-gather assembly of the element-boundary results lb into rows
+// [0, nglobal) of y (Field::HelmholtzOperator's local2globalSum,
+// src/field.cpp:771-856, without atomics), returning this thread's share
+// of w . y.  PCG_VU rows per thread at a time, the index loads of all of
+// them (up to four contributions each; more are summed after, in order)
+// issued before the value loads: the sums are the same as one row at a time.
+template <typename T, bool MASK>
+__device__ inline double pcg_assemble(const PcgOp& a, int n, const T* __restrict__ lb,
+                                      T* __restrict__ y, const T* __restrict__ w) {
+  const int tid = threadIdx.x, ng = a.nglobal;
+  double wy = 0.0;
+  for (int g0 = tid; g0 < ng; g0 += PCG_THREADS * PCG_VU) {
+    int c0[PCG_VU], c1[PCG_VU], ix[PCG_VU][4];
+#pragma unroll
+    for (int u = 0; u < PCG_VU; ++u) {
+      const int gI = g0 + u * PCG_THREADS;
+      const bool live = gI < ng && (!MASK || gI < n);
+      const int gc = min(gI, ng - 1);
+      c0[u] = a.asm_ptr[gc];
+      c1[u] = live ? a.asm_ptr[gc + 1] : c0[u];
+    }
+#pragma unroll
+    for (int u = 0; u < PCG_VU; ++u)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) ix[u][k] = (c0[u] + k < c1[u]) ? a.asm_idx[c0[u] + k] : -1;
+    T lv[PCG_VU][4];
+#pragma unroll
+    for (int u = 0; u < PCG_VU; ++u)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) lv[u][k] = (ix[u][k] >= 0) ? lb[ix[u][k]] : (T)0;
+#pragma unroll
+    for (int u = 0; u < PCG_VU; ++u) {
+      const int gI = g0 + u * PCG_THREADS;
+      T v = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (ix[u][k] >= 0) v += lv[u][k];
+      for (int c = c0[u] + 4; c < c1[u]; ++c) v += lb[a.asm_idx[c]];
+      if (gI < ng) {
+        y[gI] = v;
+        wy += (double)w[gI] * (double)v;
+      }
+    }
+  }
+  return wy;
+}
 
 // y = A x on the globally numbered vector; returns the dot product w . y
 // (w may be x itself).  MASK: unknowns >= n are treated as zero and rows
@@ -1682,22 +1734,31 @@ __device__ double pcg_matvec(const PcgOp& a, int n, const T* __restrict__ x,
   const int el = tid / npnp, pt = tid - (tid / npnp) * npnp;
   const int i = pt / np, j = pt - (pt / np) * np;
   const int pm = a.pmap[min(pt, npnp - 1)];
+  // x was just written by other threads (the caller's vector pass): every
+  // gather below comes after this barrier
+  __syncthreads();
+  // gather pipeline: the global index of this thread's node two passes
+  // ahead, its value one pass ahead (-1: masked, value zero)
+  auto gidx = [&](int e0) {
+    const int e = min(e0 + el, a.nel - 1);
+    if (pm < next) {
+      const int gid = a.btog[e * next + pm];
+      return (!MASK || gid < n) ? gid : -1;
+    }
+    return a.nglobal + e * nint + (pm - next);
+  };
+  int ia = gidx(0), ib = gidx(ce);
+  T vnext = ia >= 0 ? x[ia] : (T)0;
   double wy = 0.0;
   for (int e0 = 0; e0 < a.nel; e0 += ce) {
     const int e = e0 + el;
     const bool act = el < ce && e < a.nel;
     const int base = el * npnp;
+    const T v = vnext;
     __syncthreads();
-    if (act) {
-      T v;
-      if (pm < next) {
-        const int gid = a.btog[e * next + pm];
-        v = (!MASK || gid < n) ? x[gid] : (T)0;
-      } else {
-        v = x[a.nglobal + e * nint + (pm - next)];
-      }
-      sU[base + pt] = v;
-    }
+    if (act) sU[base + pt] = v;
+    vnext = ib >= 0 ? x[ib] : (T)0;
+    ib = gidx(e0 + 2 * ce);
     __syncthreads();
     const double* g = a.G + (size_t)(act ? a.eset[e] : 0) * G_N * npnp;
     if (act) {
@@ -1731,13 +1792,7 @@ __device__ double pcg_matvec(const PcgOp& a, int n, const T* __restrict__ x,
     }
   }
   __syncthreads();
-  for (int gI = tid; gI < a.nglobal; gI += PCG_THREADS) {
-    T v = 0;
-    if (!MASK || gI < n)
-      for (int c = a.asm_ptr[gI]; c < a.asm_ptr[gI + 1]; ++c) v += lb[a.asm_idx[c]];
-    y[gI] = v;
-    wy += (double)w[gI] * (double)v;
-  }
+  wy += pcg_assemble<T, MASK>(a, n, lb, y, w);
   return block_sum(wy, red);
 }
 
@@ -1773,26 +1828,36 @@ __device__ double pcg_matvec_wave(const PcgOp& a, int n, const T* __restrict__ x
     sT[tid] = (T)a.ops[(svv ? 3 : 1) * npnp + tid];
   }
   __syncthreads();
-  // this lane's nodes: pt = lane + 64 k
-  auto gather = [&](int e, T (&v)[PCG_NPT]) {
+  // this lane's nodes: pt = lane + 64 k; gather pipeline: global indices
+  // two elements ahead, values one ahead (-1: masked, value zero)
+  int pmv[PCG_NPT];
+#pragma unroll
+  for (int k = 0; k < PCG_NPT; ++k) pmv[k] = a.pmap[min(lane + 64 * k, npnp - 1)];
+  auto gidx = [&](int e, int (&g)[PCG_NPT]) {
     const int ec = min(e, a.nel - 1);
 #pragma unroll
     for (int k = 0; k < PCG_NPT; ++k) {
-      const int pmk = a.pmap[min(lane + 64 * k, npnp - 1)];
-      if (pmk < next) {
-        const int gid = a.btog[ec * next + pmk];
-        const T xv = x[gid];
-        v[k] = (!MASK || gid < n) ? xv : (T)0;
+      if (pmv[k] < next) {
+        const int gid = a.btog[ec * next + pmv[k]];
+        g[k] = (!MASK || gid < n) ? gid : -1;
       } else {
-        v[k] = x[a.nglobal + ec * nint + (pmk - next)];
+        g[k] = a.nglobal + ec * nint + (pmv[k] - next);
       }
     }
   };
+  auto gval = [&](const int (&g)[PCG_NPT], T (&v)[PCG_NPT]) {
+#pragma unroll
+    for (int k = 0; k < PCG_NPT; ++k) v[k] = g[k] >= 0 ? x[g[k]] : (T)0;
+  };
   double wy = 0.0;
   T cur[PCG_NPT], nxt[PCG_NPT];
-  gather(wave, cur);
+  int gi[PCG_NPT];
+  gidx(wave, gi);
+  gval(gi, cur);
+  gidx(wave + PCG_WAVES, gi);
   for (int e = wave; e < a.nel; e += PCG_WAVES) {
-    gather(e + PCG_WAVES, nxt);
+    gval(gi, nxt);
+    gidx(e + 2 * PCG_WAVES, gi);
 #pragma unroll
     for (int k = 0; k < PCG_NPT; ++k)
       if (lane + 64 * k < npnp) sU[lane + 64 * k] = cur[k];
@@ -1843,13 +1908,7 @@ __device__ double pcg_matvec_wave(const PcgOp& a, int n, const T* __restrict__ x
     for (int k = 0; k < PCG_NPT; ++k) cur[k] = nxt[k];
   }
   __syncthreads();
-  for (int gI = tid; gI < a.nglobal; gI += PCG_THREADS) {
-    T v = 0;
-    if (!MASK || gI < n)
-      for (int c = a.asm_ptr[gI]; c < a.asm_ptr[gI + 1]; ++c) v += lb[a.asm_idx[c]];
-    y[gI] = v;
-    wy += (double)w[gI] * (double)v;
-  }
+  wy += pcg_assemble<T, MASK>(a, n, lb, y, w);
   return block_sum(wy, red);
 }
 
@@ -1913,30 +1972,42 @@ __device__ double pcg_matvec_mfma(const PcgOp& a, int n, const float* __restrict
     pm[v] = in ? a.pmap[pt[v]] : -1;
   }
   const float hcon = (float)a.hcon;
-  auto gather = [&](int e, float (&u)[4]) {
+  // gather pipeline: global indices two elements ahead, values one ahead
+  // (-1: padding or masked, value zero)
+  auto gidx = [&](int e, int (&g)[4]) {
     const int ec = min(e, a.nel - 1);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      float xv = 0.0f;
+      int gid = -1;
       if (pm[v] >= 0) {
         if (pm[v] < next) {
-          const int gid = a.btog[ec * next + pm[v]];
-          const float t = x[gid];
-          xv = (!MASK || gid < n) ? t : 0.0f;
+          gid = a.btog[ec * next + pm[v]];
+          if (MASK && gid >= n) gid = -1;
         } else {
-          xv = x[a.nglobal + ec * nint + (pm[v] - next)];
+          gid = a.nglobal + ec * nint + (pm[v] - next);
         }
       }
-      u[v] = xv;
+      g[v] = gid;
     }
+  };
+  auto gval = [&](const int (&g)[4], float (&u)[4]) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) u[v] = g[v] >= 0 ? x[g[v]] : 0.0f;
   };
   double wy = 0.0;
   float cur[4], nxt[4];
-  gather(wave, cur);
-  for (int e = wave; e < a.nel; e += PCG_WAVES) {
-    gather(e + PCG_WAVES, nxt);
+  int gi[4];
+  // x was just written by other threads (the caller's vector pass): every
+  // gather comes after this barrier
+  __syncthreads();
+  gidx(wave, gi);
+  gval(gi, cur);
+  gidx(wave + PCG_WAVES, gi);
+  // geometric factors of this lane's nodes: loaded once when every element
+  // shares one set, per element otherwise
+  float q1[4], q2[4], q3[4], q4[4];
+  auto qload = [&](int e) {
     const double* g = a.G + (size_t)a.eset[e] * G_N * npnp;
-    float q1[4], q2[4], q3[4], q4[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const bool in = pm[v] >= 0;
@@ -1945,6 +2016,13 @@ __device__ double pcg_matvec_mfma(const PcgOp& a, int n, const float* __restrict
       q3[v] = in ? (float)g[G_Q3 * npnp + pt[v]] : 0.0f;
       q4[v] = in ? (float)g[G_Q4 * npnp + pt[v]] : 0.0f;
     }
+  };
+  const bool one_set = a.nset == 1;
+  if (one_set) qload(0);
+  for (int e = wave; e < a.nel; e += PCG_WAVES) {
+    gval(gi, nxt);
+    gidx(e + 2 * PCG_WAVES, gi);
+    if (!one_set) qload(e);
 #pragma unroll
     for (int v = 0; v < 4; ++v) tU[(rb + v) * MF_LD + c] = cur[v];
     wave_sync();
@@ -1983,13 +2061,7 @@ __device__ double pcg_matvec_mfma(const PcgOp& a, int n, const float* __restrict
     for (int v = 0; v < 4; ++v) cur[v] = nxt[v];
   }
   __syncthreads();
-  for (int gI = tid; gI < a.nglobal; gI += PCG_THREADS) {
-    float v = 0;
-    if (!MASK || gI < n)
-      for (int cc = a.asm_ptr[gI]; cc < a.asm_ptr[gI + 1]; ++cc) v += lb[a.asm_idx[cc]];
-    y[gI] = v;
-    wy += (double)w[gI] * (double)v;
-  }
+  wy += pcg_assemble<float, MASK>(a, n, lb, y, w);
   return block_sum(wy, red);
 }
 
@@ -2023,21 +2095,53 @@ __device__ int pcg_core(const PcgOp& a, int n, T* x, T* r, T* p, T* q, T* lb, co
   r2 = block_sum(r2, red);
   rho1 = block_sum(rho1, red);
   int it = 0;
+  // The vector passes take PCG_VU entries per thread at a time (i = tid +
+  // (g PCG_VU + u) PCG_THREADS, the same order as one at a time, so the
+  // sums are unchanged) with every load issued before the first use: one
+  // memory latency per group instead of one per entry.
   while (r2 > epsb2 && ++it < step_max) {
     const T beta = (it == 1) ? (T)0 : (T)(rho1 / rho2);
-    for (int i = tid; i < npts; i += PCG_THREADS) {
-      const T z = pc[i] * r[i];
-      p[i] = (it == 1) ? z : z + beta * p[i];
+    for (int i0 = tid; i0 < npts; i0 += PCG_THREADS * PCG_VU) {
+      T cv[PCG_VU], rv[PCG_VU], pv[PCG_VU];
+#pragma unroll
+      for (int u = 0; u < PCG_VU; ++u) {
+        const int i = min(i0 + u * PCG_THREADS, npts - 1);
+        cv[u] = pc[i];
+        rv[u] = r[i];
+        pv[u] = p[i];
+      }
+#pragma unroll
+      for (int u = 0; u < PCG_VU; ++u) {
+        const int i = i0 + u * PCG_THREADS;
+        const T z = cv[u] * rv[u];
+        if (i < npts) p[i] = (it == 1) ? z : z + beta * pv[u];
+      }
     }
     const double pq = pcg_op<T, true, OPK>(a, n, p, q, lb, p, sh, red);
     const T alpha = (T)(rho1 / pq);
     double nr2 = 0.0, nrho = 0.0;
-    for (int i = tid; i < npts; i += PCG_THREADS) {
-      x[i] += alpha * p[i];
-      const T ri = r[i] - alpha * q[i];
-      r[i] = ri;
-      nr2 += (double)ri * (double)ri;
-      nrho += (double)ri * (double)pc[i] * (double)ri;
+    for (int i0 = tid; i0 < npts; i0 += PCG_THREADS * PCG_VU) {
+      T xv[PCG_VU], pv[PCG_VU], rv[PCG_VU], qv[PCG_VU], cv[PCG_VU];
+#pragma unroll
+      for (int u = 0; u < PCG_VU; ++u) {
+        const int i = min(i0 + u * PCG_THREADS, npts - 1);
+        xv[u] = x[i];
+        pv[u] = p[i];
+        rv[u] = r[i];
+        qv[u] = q[i];
+        cv[u] = pc[i];
+      }
+#pragma unroll
+      for (int u = 0; u < PCG_VU; ++u) {
+        const int i = i0 + u * PCG_THREADS;
+        if (i < npts) {
+          x[i] = xv[u] + alpha * pv[u];
+          const T ri = rv[u] - alpha * qv[u];
+          r[i] = ri;
+          nr2 += (double)ri * (double)ri;
+          nrho += (double)ri * (double)cv[u] * (double)ri;
+        }
+      }
     }
     rho2 = rho1;
     r2 = block_sum(nr2, red);
@@ -2150,9 +2254,9 @@ __global__ __launch_bounds__(PCG_THREADS) void k_pcg(PcgArgs a) {
   const int m = k >> 1;
   const int n = a.mode_n[m];
   const int npts = a.npts, nglobal = a.nglobal, next = a.next;
-  const double bk = a.beta * (a.base + m);
-  const PcgOp op{a.np, a.npnp, next, a.nint, a.nel, nglobal, npts, a.lambda2 + bk * bk,
-                 a.lambda2 > EPSDP, a.G, a.eset, a.pmap, a.btog, a.ops, a.asm_ptr, a.asm_idx};
+  const PcgOp op{a.np, a.npnp, next, a.nint, a.nel, nglobal, npts, a.hcon[m],
+                 a.lambda2 > EPSDP, a.G, a.eset, a.pmap, a.btog, a.ops, a.asm_ptr, a.asm_idx,
+                 a.nset};
   double* wk = a.work + (size_t)blockIdx.x * a.workstride;
   double* X = wk;                 // FP64 solution
   double* R = X + npts;           // FP64 residual
@@ -2179,7 +2283,8 @@ __global__ __launch_bounds__(PCG_THREADS) void k_pcg(PcgArgs a) {
   }
   const double epsb = a.tol * sqrt(block_sum(bb, red));
   pcg_x0(op, a.emap, n, uk, X);
-  const int its = pcg_run<T, REFINE, OPK>(op, n, X, R, B, P, Q, LB, a.pc + (size_t)m * npts,
+  constexpr int OP64 = (OPK == OPK_WAVE) ? OPK_WAVE : OPK_BLOCK;
+  const int its = pcg_run<T, REFINE, OPK, OP64>(op, n, X, R, B, P, Q, LB, a.pc + (size_t)m * npts,
                                      a.pc32 + (size_t)m * npts, epsb * epsb, a.step_max,
                                      a.inner_tol, a.max_outer, sh, red);
   const double* gp = a.g + (size_t)pidx * a.nzero;
@@ -2200,6 +2305,8 @@ struct PcgInst {
   const double* pc;
   const float* pc32;
   double* work;             // 5*npts + nel*next doubles
+  int opk;                  // element operator of the inner iteration (OPK_*),
+                            // chosen per (np, element count) by the batch tuner
 };
 
 struct PcgBatchArgs {
@@ -2217,7 +2324,7 @@ struct PcgBatchArgs {
 // (Field::constrain and buildRHS, src/field.cpp:728-768, 859-925, with the
 // lifting A g of the essential values g as one unmasked operator
 // application), x0, the iteration and the essential re-imposition.
-template <typename T, bool REFINE, int OPK = OPK_WAVE>
+template <typename T, bool REFINE>
 __global__ __launch_bounds__(PCG_THREADS) void k_pcg_batch(PcgBatchArgs ba) {
   extern __shared__ double pcg_lds[];
   double* red = pcg_lds;
@@ -2271,9 +2378,28 @@ __global__ __launch_bounds__(PCG_THREADS) void k_pcg_batch(PcgBatchArgs ba) {
   }
   const double epsb = ba.tol * sqrt(block_sum(bb, red));
   pcg_x0(op, I.emap, n, uk, X);
-  const int its = pcg_run<T, REFINE, OPK, OPK_WAVE>(op, n, X, R, B, P, Q, LB, I.pc, I.pc32,
+  // the problem's element operator (uniform over the workgroup): the BF16
+  // matrix-core one for FP32 inner iterations, or the scalar block-wide or
+  // wavefront-level one
+  int its = 0;
+  bool done = false;
+  if constexpr (std::is_same<T, float>::value) {
+    if (I.opk == OPK_MFMA) {
+      its = pcg_run<float, REFINE, OPK_MFMA, OPK_WAVE>(op, n, X, R, B, P, Q, LB, I.pc, I.pc32,
+                                                       epsb * epsb, ba.step_max, ba.inner_tol,
+                                                       ba.max_outer, sh, red);
+      done = true;
+    }
+  }
+  if (done) {
+  } else if (I.opk == OPK_BLOCK)
+    its = pcg_run<T, REFINE, OPK_BLOCK, OPK_BLOCK>(op, n, X, R, B, P, Q, LB, I.pc, I.pc32,
                                                    epsb * epsb, ba.step_max, ba.inner_tol,
                                                    ba.max_outer, sh, red);
+  else
+    its = pcg_run<T, REFINE, OPK_WAVE, OPK_WAVE>(op, n, X, R, B, P, Q, LB, I.pc, I.pc32,
+                                                 epsb * epsb, ba.step_max, ba.inner_tol,
+                                                 ba.max_outer, sh, red);
   pcg_store(op, n, nsolve, X, uk, [&](int gid) { return bc[I.ess_src[gid - nsolve]]; });
   if (tid == 0) ba.iters[blockIdx.x] = its;
 }
@@ -2418,11 +2544,14 @@ int helm_upload(Helm& H) {
   const int nm = H.nmodes;
   std::vector<int> mn(nm);
   std::vector<long long> oh(nm), ol(nm);
+  std::vector<double> hc(nm);
   for (int m = 0; m < nm; ++m) {
     mn[m] = H.modes[m].n;
     oh[m] = (long long)H.modes[m].off_H;
     ol[m] = (long long)H.modes[m].off_L;
+    hc[m] = H.lambda2 + H.modes[m].betak2;
   }
+  if ((st = dev_upload(&H.d.hcon, hc.data(), hc.size()))) return st;
   if ((st = dev_upload(&H.d.hbi, H.hbi.data(), H.hbi.size()))) return st;
   if ((st = dev_upload(&H.d.hii, H.hii.data(), H.hii.size()))) return st;
   if ((st = dev_upload(&H.d.Pf, H.Pf.data(), H.Pf.size()))) return st;
@@ -2462,6 +2591,7 @@ void helm_release(Helm& H) {
   dev_free(H.d.mode_offL); dev_free(H.d.pc); dev_free(H.d.Fb); dev_free(H.d.Fi);
   dev_free(H.d.g); dev_free(H.d.sol); dev_free(H.d.plane_of);
   dev_free(H.d.pc32); dev_free(H.d.mode_kind); dev_free(H.d.pcg_plane); dev_free(H.d.work);
+  dev_free(H.d.hcon);
   dev_free(H.d.iters);
   dev_free(H.d.inbox); dev_free(H.d.part); dev_free(H.d.clo);
   dev_free(H.d.err);
@@ -2526,6 +2656,7 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
   ea.np = M.np; ea.npnp = M.npnp; ea.next = M.next; ea.nint = M.nint; ea.nel = M.nel;
   ea.planeSize = M.planeSize; ea.nline = nline; ea.nsolve = M.nsolve; ea.nzero = nzero;
   ea.nset = M.nset; ea.lambda2 = H.lambda2; ea.beta = H.beta; ea.base = H.base;
+  ea.hcon = H.d.hcon;
   ea.G = M.d.G; ea.eset = M.d.eset; ea.emap = M.d.emap; ea.pmap = M.d.pmap; ea.btog = M.d.btog;
   ea.emask = M.d.emask; ea.ops = M.d.ops; ea.nat_ptr = M.d.nat_ptr; ea.nat_b = M.d.nat_b;
   ea.bside = M.d.bside; ea.barea = M.d.barea; ea.hbi = H.d.hbi; ea.hii = H.d.hii;
@@ -2642,16 +2773,17 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
         return f;
       };
       int npadmax = 0;
-      for (int m : H.dmode) npadmax = std::max(npadmax, H.modes[m].nblk * XNB);
+      for (int m : H.dmode) npadmax = std::max(npadmax, H.modes[m].nblk * H.nb);
       const int rbt = H.nb == XNB ? (2 * H.ldx + 63) / 64 * 64 : 0;
       const void* rfn = nullptr;
-#define SEMX_RPICK(BT_)                                                          \
-      if (rbt == BT_)                                                            \
-        rfn = (npl == 2) ? (const void*)k_band_res<BT_, 2, 1>                    \
-                         : (const void*)k_band_res<BT_, 1, 1>;
-      SEMX_RPICK(128) SEMX_RPICK(192) SEMX_RPICK(256) SEMX_RPICK(320) SEMX_RPICK(384)
-      SEMX_RPICK(448) SEMX_RPICK(512) SEMX_RPICK(576) SEMX_RPICK(640) SEMX_RPICK(704)
-      SEMX_RPICK(768) SEMX_RPICK(832) SEMX_RPICK(896) SEMX_RPICK(960) SEMX_RPICK(1024)
+#define SEMX_RPICK(NB_, BT_)                                                     \
+      if (H.nb == NB_ && rbt == BT_)                                             \
+        rfn = (npl == 2) ? (const void*)k_band_res<NB_, BT_, 2, 1>               \
+                         : (const void*)k_band_res<NB_, BT_, 1, 1>;
+      SEMX_RPICK(32, 128) SEMX_RPICK(32, 192) SEMX_RPICK(32, 256) SEMX_RPICK(32, 320)
+      SEMX_RPICK(32, 384) SEMX_RPICK(32, 448) SEMX_RPICK(32, 512) SEMX_RPICK(32, 576)
+      SEMX_RPICK(32, 640) SEMX_RPICK(32, 704) SEMX_RPICK(32, 768) SEMX_RPICK(32, 832)
+      SEMX_RPICK(32, 896) SEMX_RPICK(32, 960) SEMX_RPICK(32, 1024)
 #undef SEMX_RPICK
       struct Cand { const void* fn; size_t lds; unsigned bt; int F; int ring; };
       constexpr int NCAND = 7;
@@ -2671,11 +2803,13 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
           cand[7] = {(const void*)k_band_ext<16, 1, 960, 2, 1>, l3, 960u, nf, ba.ring};
         if (npl == 2) {
           const void* r1 = nullptr;
-#define SEMX_RPICK1(BT_) if (rbt == BT_) r1 = (const void*)k_band_res<BT_, 1, 1>;
-          SEMX_RPICK1(128) SEMX_RPICK1(192) SEMX_RPICK1(256) SEMX_RPICK1(320)
-          SEMX_RPICK1(384) SEMX_RPICK1(448) SEMX_RPICK1(512) SEMX_RPICK1(576)
-          SEMX_RPICK1(640) SEMX_RPICK1(704) SEMX_RPICK1(768) SEMX_RPICK1(832)
-          SEMX_RPICK1(896) SEMX_RPICK1(960) SEMX_RPICK1(1024)
+#define SEMX_RPICK1(NB_, BT_)                                                    \
+          if (H.nb == NB_ && rbt == BT_) r1 = (const void*)k_band_res<NB_, BT_, 1, 1>;
+          SEMX_RPICK1(32, 128) SEMX_RPICK1(32, 192) SEMX_RPICK1(32, 256)
+          SEMX_RPICK1(32, 320) SEMX_RPICK1(32, 384) SEMX_RPICK1(32, 448)
+          SEMX_RPICK1(32, 512) SEMX_RPICK1(32, 576) SEMX_RPICK1(32, 640)
+          SEMX_RPICK1(32, 704) SEMX_RPICK1(32, 768) SEMX_RPICK1(32, 832)
+          SEMX_RPICK1(32, 896) SEMX_RPICK1(32, 960) SEMX_RPICK1(32, 1024)
 #undef SEMX_RPICK1
           const size_t l4 = (size_t)npadmax * sizeof(double) + xr_b;
           if (r1 && l4 <= 160 * 1024) cand[4] = {r1, l4, (unsigned)rbt, 2 * nf, npadmax};
@@ -2845,7 +2979,7 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
     pa.np = M.np; pa.npnp = M.npnp; pa.next = M.next; pa.nint = M.nint; pa.nel = M.nel;
     pa.nglobal = M.nglobal; pa.nsolve = M.nsolve; pa.nzero = nzero; pa.npts = H.npts;
     pa.planeSize = M.planeSize; pa.nset = M.nset;
-    pa.lambda2 = H.lambda2; pa.beta = H.beta; pa.base = H.base;
+    pa.lambda2 = H.lambda2; pa.beta = H.beta; pa.base = H.base; pa.hcon = H.d.hcon;
     pa.G = M.d.G; pa.eset = M.d.eset; pa.emap = M.d.emap; pa.pmap = M.d.pmap;
     pa.btog = M.d.btog; pa.ops = M.d.ops; pa.asm_ptr = M.d.asm_ptr; pa.asm_idx = M.d.asm_idx;
     pa.plane_of = H.d.plane_of; pa.pcg_plane = H.d.pcg_plane; pa.mode_n = H.d.mode_n;
@@ -2854,18 +2988,78 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
     pa.tol = tol_rel; pa.step_max = step_max; pa.inner_tol = H.inner_tol;
     pa.max_outer = H.max_outer; pa.iters = H.d.iters;
     const int cep = PCG_THREADS / M.npnp;
+    // LDS of every element operator (block-wide, wavefront-level, BF16)
     size_t lds = 32 * sizeof(double) +
         (2 * (size_t)M.npnp + 3 * (size_t)cep * M.npnp) * sizeof(double);
-    const void* kfn = (const void*)k_pcg<double, false>;
-    if (H.pcg_prec == SEMX_PCG_FP32_REFINED) {
-      kfn = (const void*)k_pcg<float, true>;
-    } else if (H.pcg_prec == SEMX_PCG_BF16_REFINED) {
-      kfn = (const void*)k_pcg<float, true, OPK_MFMA>;
+    lds = std::max(lds, 32 * sizeof(double) +
+                        (2 + 3 * (size_t)PCG_WAVES) * M.npnp * sizeof(double));
+    if (H.pcg_prec == SEMX_PCG_BF16_REFINED)
       lds = std::max(lds, 32 * sizeof(double) + (size_t)PCG_WAVES * 2 * MF_TILE * sizeof(float));
+    auto pcg_fn = [&](int opk) -> const void* {
+      if (H.pcg_prec == SEMX_PCG_FP64)
+        return opk == OPK_WAVE ? (const void*)k_pcg<double, false, OPK_WAVE>
+                               : (const void*)k_pcg<double, false, OPK_BLOCK>;
+      if (H.pcg_prec == SEMX_PCG_BF16_REFINED && opk == OPK_MFMA)
+        return (const void*)k_pcg<float, true, OPK_MFMA>;
+      return opk == OPK_WAVE ? (const void*)k_pcg<float, true, OPK_WAVE>
+                             : (const void*)k_pcg<float, true, OPK_BLOCK>;
+    };
+    const size_t npcg = H.pcg_plane.size();
+    if (H.pcg_opk < 0) {
+      // The element operator of the system's PCG planes: the BF16 one for
+      // SEMX_PCG_BF16_REFINED; otherwise the block-wide or the wavefront-
+      // level scalar operator, whichever runs an iteration faster on this
+      // system (timed once: PCG_TUNE_ITERS iterations of every plane of the
+      // first field into a scratch copy of its solution), or the one
+      // SEMX_PCG_OP=block|wave forces.
+      const char* env = getenv("SEMX_PCG_OP");
+      if (H.pcg_prec == SEMX_PCG_BF16_REFINED) {
+        H.pcg_opk = OPK_MFMA;
+      } else if (env) {
+        const std::string v(env);
+        if (v != "block" && v != "wave") return SEMX_E_ARG;
+        H.pcg_opk = v == "block" ? OPK_BLOCK : OPK_WAVE;
+      } else {
+        double* scr = nullptr;
+        const size_t nu = (size_t)H.nz * M.planeSize;
+        int st2 = dev_upload<double>(&scr, nullptr, nu);
+        if (st2) return st2;
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        double best = 0.0;
+        for (int c : {OPK_BLOCK, OPK_WAVE}) {
+          PcgArgs pt = pa;
+          pt.u = scr;
+          pt.tol = 0.0;
+          pt.step_max = PCG_TUNE_ITERS;
+          pt.max_outer = 1;
+          const void* fn = pcg_fn(c);
+          if (lds > 64 * 1024)
+            HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+          HIPCHK(hipMemcpyAsync(scr, us[0], nu * sizeof(double), hipMemcpyDeviceToDevice, s));
+          void* kt[] = {&pt};
+          HIPCHK(hipEventRecord(e0, s));
+          HIPCHK(hipLaunchKernel(fn, dim3((unsigned)npcg), dim3(PCG_THREADS), kt, lds, s));
+          HIPCHK(hipEventRecord(e1, s));
+          HIPCHK(hipEventSynchronize(e1));
+          float ms = 0.f;
+          HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+          std::vector<int> it(npcg);
+          HIPCHK(hipMemcpy(it.data(), H.d.iters, npcg * sizeof(int), hipMemcpyDeviceToHost));
+          int mx = 1;
+          for (int v : it) mx = std::max(mx, std::abs(v));
+          const double per = ms / mx;
+          if (c == OPK_BLOCK || per < best) { best = per; H.pcg_opk = c; }
+        }
+        HIPCHK(hipEventDestroy(e0));
+        HIPCHK(hipEventDestroy(e1));
+        dev_free(scr);
+      }
     }
+    const void* kfn = pcg_fn(H.pcg_opk);
     if (lds > 64 * 1024)
       HIPCHK(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    const size_t npcg = H.pcg_plane.size();
     for (int i = 0; i < nf; ++i) {
       PcgArgs pf = pa;
       pf.Fb = H.d.Fb + i * pFb; pf.Fi = H.d.Fi + i * pFi; pf.g = H.d.g + i * pg;
@@ -2971,6 +3165,22 @@ int semx_mesh_create(int np, int nel, const double* vx, const double* vy, const 
   return SEMX_OK;
 }
 
+int semx_mesh_create_numbered(int np, int nel, const double* vx, const double* vy,
+                              const int* nbr, const int* skind, int svv_mn, double svv_eps,
+                              double kinvis, const int* btog, const int* bmask, int nglobal,
+                              int nsolve, semx_mesh* mesh) {
+  if (!mesh || !vx || !vy || !nbr || !skind || !btog || !bmask || nglobal < 1 || nsolve < 0)
+    return SEMX_E_ARG;
+  *mesh = nullptr;
+  semx_mesh_s* h = new (std::nothrow) semx_mesh_s();
+  if (!h) return SEMX_E_NOMEM;
+  const int st = mesh_build(h->M, np, nel, vx, vy, nbr, skind, svv_mn, svv_eps, kinvis, btog,
+                            bmask, nglobal, nsolve);
+  if (st) { delete h; return st; }
+  *mesh = h;
+  return SEMX_OK;
+}
+
 int semx_mesh_destroy(semx_mesh mesh) {
   if (!mesh) return SEMX_E_ARG;
   mesh_release(mesh->M);
@@ -3018,8 +3228,11 @@ int semx_mesh_nsets(semx_mesh mesh, int* nsets) {
   return SEMX_OK;
 }
 
-int semx_helm_create(semx_mesh mesh, double lambda2, double beta, int base_mode, int nmodes,
-                     int nz, int method, semx_helm* helm) {
+}  // extern "C"
+
+static int helm_create_impl(semx_mesh mesh, double lambda2, double beta, int base_mode,
+                            int nmodes, int nz, int method, int svv_mz, double svv_epsz,
+                            double kinvis, int nmode_total, semx_helm* helm) {
   if (!mesh || !helm || nmodes < 1 || nz < 1 || base_mode < 0) return SEMX_E_ARG;
   if (method != SEMX_DIRECT && method != SEMX_JACPCG && method != SEMX_MIXED) return SEMX_E_ARG;
   if (!(nz == 1 && nmodes == 1) && nz != 2 * nmodes) return SEMX_E_ARG;
@@ -3030,10 +3243,28 @@ int semx_helm_create(semx_mesh mesh, double lambda2, double beta, int base_mode,
   H.M = &mesh->M;
   H.lambda2 = lambda2; H.beta = beta; H.base = base_mode; H.nmodes = nmodes;
   H.nz = nz; H.method = method;
+  H.svv_mz = svv_mz; H.svv_epsz = svv_epsz; H.svv_nu = kinvis;
+  H.svv_nmode = nmode_total;
   const int st = helm_build(H);
   if (st) { delete h; return st; }
   *helm = h;
   return SEMX_OK;
+}
+
+extern "C" {
+
+int semx_helm_create(semx_mesh mesh, double lambda2, double beta, int base_mode, int nmodes,
+                     int nz, int method, semx_helm* helm) {
+  return helm_create_impl(mesh, lambda2, beta, base_mode, nmodes, nz, method, -1, 0.0, 1.0, 0,
+                          helm);
+}
+
+int semx_helm_create_svv(semx_mesh mesh, double lambda2, double beta, int base_mode,
+                         int nmodes, int nz, int method, int svv_mz, double svv_epsz,
+                         double kinvis, int nmode_total, semx_helm* helm) {
+  if (!(kinvis > 0.0) || nmode_total < base_mode + nmodes) return SEMX_E_ARG;
+  return helm_create_impl(mesh, lambda2, beta, base_mode, nmodes, nz, method, svv_mz, svv_epsz,
+                          kinvis, nmode_total, helm);
 }
 
 int semx_helm_destroy(semx_helm helm) {
@@ -3106,6 +3337,9 @@ int semx_helm_solve_fields(semx_helm helm, int nfield, double* const* f, const d
 
 struct semx_batch_s {
   std::vector<semx_helm> helms;
+  std::vector<PcgInst> hinst;      // host copy of inst
+  bool tuned = false;              // element operators chosen
+  double tune_ms = 0.0;            // host time the choice took
   PcgInst* inst = nullptr;         // device, one per problem
   double* work = nullptr;
   int* iters = nullptr;
@@ -3154,22 +3388,119 @@ int semx_batch_create(int count, const semx_helm* helms, semx_batch* batch) {
     PcgInst& I = inst[i];
     I.op = PcgOp{M.np, M.npnp, M.next, M.nint, M.nel, M.nglobal, H.npts,
                  H.lambda2 + H.modes[0].betak2, H.lambda2 > EPSDP, M.d.G, M.d.eset, M.d.pmap,
-                 M.d.btog, M.d.ops, M.d.asm_ptr, M.d.asm_idx};
+                 M.d.btog, M.d.ops, M.d.asm_ptr, M.d.asm_idx, M.nset};
     I.n = H.modes[0].n; I.nsolve = M.nsolve; I.nbound = M.nbound;
     I.emap = M.d.emap; I.ess_src = M.d.ess_src; I.nat_ptr = M.d.nat_ptr; I.nat_b = M.d.nat_b;
     I.bside = M.d.bside; I.barea = M.d.barea; I.pc = H.d.pc; I.pc32 = H.d.pc32;
     I.work = b->work + off[i];
+    I.opk = (b->prec == SEMX_PCG_BF16_REFINED) ? OPK_MFMA : OPK_WAVE;
     // wavefront-level element operator: a private slice per wavefront
     if (M.np > 13) { batch_free(b); return SEMX_E_UNSUPPORTED; }
     b->lds = std::max(b->lds, 32 * sizeof(double) +
                       (2 * (size_t)M.npnp + 3 * (size_t)PCG_WAVES * M.npnp) * sizeof(double));
+    // the block-wide element operator (a tuner candidate)
+    b->lds = std::max(b->lds, 32 * sizeof(double) +
+                      (2 + 3 * (size_t)(PCG_THREADS / M.npnp)) * M.npnp * sizeof(double));
     if (b->prec == SEMX_PCG_BF16_REFINED)
       b->lds = std::max(b->lds, 32 * sizeof(double) +
                         (size_t)PCG_WAVES * 2 * MF_TILE * sizeof(float));
   }
   st = dev_upload(&b->inst, inst.data(), inst.size());
   if (st) { batch_free(b); return st; }
+  b->hinst = inst;
   *batch = b;
+  return SEMX_OK;
+}
+
+// The batch tuner: for every class of problems with the same np and element
+// count, time each element operator that the precision allows (scalar
+// block-wide and wavefront-level; with SEMX_PCG_BF16_REFINED also the BF16
+// matrix-core operator) on the class's first problem -- a one-workgroup
+// launch of BATCH_TUNE_ITERS iterations from the caller's data into a
+// scratch solution -- and keep the fastest per iteration for the class.
+// SEMX_BATCH_OP=block|wave|mfma forces one operator for every problem
+// (mfma only with BF16 precision).  Runs once, on the first solve, and
+// synchronises the stream.
+constexpr int BATCH_TUNE_ITERS = 40;
+
+static int batch_tune(semx_batch_s& b, const double* const* f, const double* const* bc,
+                      double* const* u, const void* kfn, hipStream_t s) {
+  b.tuned = true;
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<int> cands = {OPK_BLOCK, OPK_WAVE};
+  if (b.prec == SEMX_PCG_BF16_REFINED) cands.push_back(OPK_MFMA);
+  if (const char* e = getenv("SEMX_BATCH_OP")) {
+    const std::string v(e);
+    const int force = v == "block" ? OPK_BLOCK : v == "wave" ? OPK_WAVE
+                      : v == "mfma" ? OPK_MFMA : -1;
+    if (force < 0 || (force == OPK_MFMA && b.prec != SEMX_PCG_BF16_REFINED)) return SEMX_E_ARG;
+    for (PcgInst& I : b.hinst) I.opk = force;
+    HIPCHK(hipMemcpy(b.inst, b.hinst.data(), b.hinst.size() * sizeof(PcgInst),
+                     hipMemcpyHostToDevice));
+    return SEMX_OK;
+  }
+  // classes: (np, nel) -> problems
+  std::map<std::pair<int, int>, std::vector<int>> cls;
+  for (int i = 0; i < b.count; ++i) cls[{b.hinst[i].op.np, b.hinst[i].op.nel}].push_back(i);
+  size_t umax = 0;
+  for (int i = 0; i < b.count; ++i)
+    umax = std::max(umax, (size_t)b.helms[i]->H.M->planeSize);
+  PcgInst* dinst = nullptr;
+  const double** dptr = nullptr;
+  double* du = nullptr;
+  int* dit = nullptr;
+  int st = dev_upload<PcgInst>(&dinst, nullptr, 1);
+  if (!st) st = dev_upload<const double*>(&dptr, nullptr, 3);
+  if (!st) st = dev_upload<double>(&du, nullptr, umax);
+  if (!st) st = dev_upload<int>(&dit, nullptr, 1);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!st && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    st = SEMX_E_HIP;
+  for (auto& kv : cls) {
+    if (st) break;
+    const int r = kv.second[0];
+    const size_t nu = (size_t)b.helms[r]->H.M->planeSize;
+    const double* ptr3[3] = {f[r], bc ? bc[r] : nullptr, du};
+    if (hipMemcpy(dptr, ptr3, sizeof(ptr3), hipMemcpyHostToDevice) != hipSuccess) {
+      st = SEMX_E_HIP;
+      break;
+    }
+    PcgBatchArgs a;
+    a.inst = dinst; a.f = dptr; a.bc = dptr + 1; a.u = (double* const*)(dptr + 2);
+    a.iters = dit; a.tol = 0.0; a.inner_tol = b.inner_tol;
+    a.step_max = BATCH_TUNE_ITERS; a.max_outer = 1;
+    double best = 0.0;
+    int pick = b.hinst[r].opk;
+    for (int c : cands) {
+      PcgInst I = b.hinst[r];
+      I.opk = c;
+      int its = 0;
+      float ms = 0.f;
+      void* kargs[] = {&a};
+      if (hipMemcpy(dinst, &I, sizeof(I), hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpyAsync(du, u[r], nu * sizeof(double), hipMemcpyDeviceToDevice, s) !=
+              hipSuccess ||
+          hipEventRecord(e0, s) != hipSuccess ||
+          hipLaunchKernel(kfn, dim3(1), dim3(PCG_THREADS), kargs, b.lds, s) != hipSuccess ||
+          hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess ||
+          hipMemcpy(&its, dit, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+        st = SEMX_E_HIP;
+        break;
+      }
+      const double per = ms / std::max(1, std::abs(its));
+      if (c == cands[0] || per < best) { best = per; pick = c; }
+    }
+    for (int i : kv.second) b.hinst[i].opk = pick;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  dev_free(dinst); dev_free(dptr); dev_free(du); dev_free(dit);
+  if (st) return st;
+  HIPCHK(hipMemcpy(b.inst, b.hinst.data(), b.hinst.size() * sizeof(PcgInst),
+                   hipMemcpyHostToDevice));
+  b.tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                  .count();
   return SEMX_OK;
 }
 
@@ -3202,10 +3533,13 @@ int semx_batch_solve(semx_batch batch, const double* const* f, const double* con
   a.iters = b.iters; a.tol = tol_rel; a.inner_tol = b.inner_tol;
   a.step_max = step_max; a.max_outer = b.max_outer;
   const void* kfn = (const void*)k_pcg_batch<double, false>;
-  if (b.prec == SEMX_PCG_FP32_REFINED) kfn = (const void*)k_pcg_batch<float, true>;
-  else if (b.prec == SEMX_PCG_BF16_REFINED) kfn = (const void*)k_pcg_batch<float, true, OPK_MFMA>;
+  if (b.prec != SEMX_PCG_FP64) kfn = (const void*)k_pcg_batch<float, true>;
   if (b.lds > 64 * 1024)
     HIPCHK(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b.lds));
+  if (!b.tuned) {
+    const int st = batch_tune(b, f, bc, u, kfn, s);
+    if (st) return st;
+  }
   void* kargs[] = {&a};
   hipEvent_t ev;
   prof_begin(s, &ev);
@@ -3225,6 +3559,14 @@ int semx_batch_solve(semx_batch batch, const double* const* f, const double* con
   return SEMX_OK;
 }
 
+int semx_batch_info(semx_batch batch, int* opk, double* tune_ms) {
+  if (!batch) return SEMX_E_ARG;
+  if (opk)
+    for (int i = 0; i < batch->count; ++i) opk[i] = batch->tuned ? batch->hinst[i].opk : -1;
+  if (tune_ms) *tune_ms = batch->tune_ms;
+  return SEMX_OK;
+}
+
 int semx_helm_set_pcg(semx_helm helm, int precision, double inner_tol, int max_outer) {
   if (!helm) return SEMX_E_ARG;
   if (precision != SEMX_PCG_FP64 && precision != SEMX_PCG_FP32_REFINED &&
@@ -3235,6 +3577,7 @@ int semx_helm_set_pcg(semx_helm helm, int precision, double inner_tol, int max_o
   Helm& H = helm->H;
   // the matrix-core operator pads elements to 16 x 16
   if (precision == SEMX_PCG_BF16_REFINED && H.M->np > 16) return SEMX_E_UNSUPPORTED;
+  if (H.pcg_prec != precision) H.pcg_opk = -1;
   H.pcg_prec = precision;
   if (refine) { H.inner_tol = inner_tol; H.max_outer = max_outer; }
   return SEMX_OK;

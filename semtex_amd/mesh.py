@@ -98,3 +98,33 @@ def bc_lines(field, belmt, bside, np_):
         idx = [e * npnp + k for k in side_nodes(np_, s)]
         out[:, b * np_:(b + 1) * np_] = field[:, idx]
     return out
+
+
+def node_mesh(xy, elements, side_kind=None, default_kind="D"):
+    """A mesh from semtex-style NODES and ELEMENTS (mesh/kovas1: <NODES>
+    coordinates, <ELEMENTS> four CCW node numbers each, 0-based here).
+    Sides shared by two elements are joined; every other side is a boundary
+    of kind side_kind.get((e, s), default_kind) ('D' or 'N'), the role of
+    the session's <SURFACES> and <BCS>."""
+    xy = np.asarray(xy, dtype=np.float64)
+    elements = np.asarray(elements, dtype=np.int64)
+    nel = len(elements)
+    vx = xy[elements, 0].copy()
+    vy = xy[elements, 1].copy()
+    nbr = -np.ones((nel, 4), dtype=np.int32)
+    skind = np.zeros((nel, 4), dtype=np.int32)
+    owner = {}
+    for e in range(nel):
+        for s in range(4):
+            a, b = int(elements[e, s]), int(elements[e, (s + 1) % 4])
+            key = (min(a, b), max(a, b))
+            if key in owner:
+                e2, s2 = owner.pop(key)
+                nbr[e, s] = e2 * 4 + s2
+                nbr[e2, s2] = e * 4 + s
+            else:
+                owner[key] = (e, s)
+    for (e, s) in owner.values():
+        kind = (side_kind or {}).get((e, s), default_kind)
+        skind[e, s] = KIND[kind]
+    return dict(vx=vx, vy=vy, nbr=nbr, skind=skind, nel=nel)

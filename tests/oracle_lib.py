@@ -51,11 +51,22 @@ def lib():
         L.orc_helm_sc.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, _D, _D, _D]
         L.orc_helm_diag.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, _D]
         L.orc_helm_global.argtypes = [C.c_void_p, _D, _D]
+        L.orc_svv_coeffs_z.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                       C.c_double, _D]
+        L.orc_set_svv_z.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int]
+        L.orc_mesh_enumerate.restype = C.c_int
+        L.orc_mesh_enumerate.argtypes = [C.c_void_p, C.c_int]
+        L.orc_mesh_renumber.restype = C.c_int
+        L.orc_mesh_renumber.argtypes = [C.c_void_p, _I, _I, C.c_int, C.c_int]
         L.orc_dns_create.restype = C.c_void_p
         L.orc_dns_create.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double,
                                      C.c_double, C.c_int, C.c_int, C.c_double,
                                      C.c_int, C.c_int]
         L.orc_dns_destroy.argtypes = [C.c_void_p]
+        L.orc_dns_create_bc.restype = C.c_void_p
+        L.orc_dns_create_bc.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double,
+                                        C.c_double, C.c_double, C.c_int, C.c_int,
+                                        C.c_double, C.c_int, C.c_int, _D, _D, _D]
         L.orc_dns_run.argtypes = [C.c_void_p, C.c_int, _D, _D, _D, _D, _D]
         _lib = L
     return _lib
@@ -84,6 +95,18 @@ def svv_operators(np_, mn=-1, eps=0.0, nu=1.0):
     lib().orc_svv_operators(C.c_int(np_), C.c_int(mn), C.c_double(eps),
                             C.c_double(nu), dp(S), dp(ST))
     return S, ST
+
+
+def svv_coeffs_z(nummodes, nmode, base, mz, eps, nu):
+    S = np.zeros(nummodes)
+    lib().orc_svv_coeffs_z(nummodes, nmode, base, mz, eps, nu, dp(S))
+    return S
+
+
+def set_svv_z(mz=-1, eps=0.0, nu=1.0, nmode=0):
+    """Fourier-direction SVV settings of the oracle's field solves (tokens
+    SVV_MZ, SVV_EPSZ, KINVIS; nmode = Geometry::nMode, 0: base + nmodes)."""
+    lib().orc_set_svv_z(mz, eps, nu, nmode)
 
 
 def edgemaps(np_):
@@ -117,6 +140,29 @@ class Mesh:
         self.btog = np.zeros(self.nel * self.next, dtype=np.int32)
         self.bmask = np.zeros_like(self.btog)
         self.emask = np.zeros(self.nel, dtype=np.int32)
+        lib().orc_mesh_numbering(C.c_void_p(self.h), ip(self.btog), ip(self.bmask),
+                                 ip(self.emask))
+
+    def renumber(self, numbering):
+        """Adopt a given numbering (dict with btog, bmask, nglobal, nsolve)."""
+        bt = np.ascontiguousarray(numbering["btog"], dtype=np.int32)
+        bm = np.ascontiguousarray(numbering["bmask"], dtype=np.int32)
+        if lib().orc_mesh_renumber(C.c_void_p(self.h), ip(bt), ip(bm),
+                                   int(numbering["nglobal"]), int(numbering["nsolve"])):
+            raise ValueError("numbering does not fit the mesh")
+        info = np.zeros(10, dtype=np.int32)
+        lib().orc_mesh_info(C.c_void_p(self.h), ip(info))
+        self.nband = int(info[6])
+        lib().orc_mesh_numbering(C.c_void_p(self.h), ip(self.btog), ip(self.bmask),
+                                 ip(self.emask))
+
+    def enumerate(self, optlevel=1):
+        """Renumber as semtex's `enumerate -O<optlevel>` (levels 0-3)."""
+        if lib().orc_mesh_enumerate(C.c_void_p(self.h), int(optlevel)):
+            raise ValueError("unsupported optimisation level")
+        info = np.zeros(10, dtype=np.int32)
+        lib().orc_mesh_info(C.c_void_p(self.h), ip(info))
+        self.nband = int(info[6])
         lib().orc_mesh_numbering(C.c_void_p(self.h), ip(self.btog), ip(self.bmask),
                                  ip(self.emask))
 

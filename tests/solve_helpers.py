@@ -36,17 +36,25 @@ def natural_lines(P, nz, bcl):
     return np.ascontiguousarray(out)
 
 
-def make_case(nx, np_, nz, lam2, beta, dist, bc, seed=11):
+def oracle_numbering(Q):
+    """The oracle mesh's numbering in the form semx_mesh_create_numbered
+    takes (as semtex's session.num would give it)."""
+    return dict(btog=Q.btog, bmask=Q.bmask, nglobal=Q.nglobal, nsolve=Q.nsolve)
+
+
+def make_case(nx, np_, nz, lam2, beta, dist, bc, seed=11, same_numbering=False):
     """Mesh pair (product, oracle) and seeded forcing / BC lines / exact
-    solution for lap u - (lambda2 + (beta k)^2) u = f on every plane."""
+    solution for lap u - (lambda2 + (beta k)^2) u = f on every plane.
+    same_numbering: the product mesh takes the oracle's numbering (so a
+    singular mode 0 is pinned at the same node)."""
     per = bc == PERIODIC
     # periodic meshes span [0, 2 pi]^2 so that the manufactured forcing is
     # periodic and the singular k = 0 system consistent (zero-mean f)
     x1 = 2 * np.pi if (nx >= 16 or per) else 1.0
     m = rect_mesh(nx, nx, 0, x1, 0, x1, bc=("D",) * 4 if per else bc, distort=dist,
                   periodic_x=per, periodic_y=per)
-    P = sx.Mesh(np_, m)
     Q = O.Mesh(np_, m)
+    P = sx.Mesh(np_, m, numbering=oracle_numbering(Q) if same_numbering else None)
     f, bcl, ue = helmholtz_batch(P, nz, lam2, beta, seed=seed)
     if not per and "N" in bc:
         bcl = natural_lines(P, nz, bcl)

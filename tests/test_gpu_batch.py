@@ -29,11 +29,26 @@ def problems(count, seed, bcs=("D", "D", "D", "D"), distort=0.0):
     return out
 
 
-@pytest.mark.parametrize("prec", [sx.PCG_FP64, sx.PCG_FP32_REFINED, sx.PCG_BF16_REFINED],
-                         ids=["fp64", "fp32r", "bf16r"])
-def test_batch_matches_oracle(cuda, prec):
+BATCH_CASES = [(sx.PCG_FP64, None), (sx.PCG_FP64, "block"), (sx.PCG_FP64, "wave"),
+               (sx.PCG_FP32_REFINED, None), (sx.PCG_FP32_REFINED, "block"),
+               (sx.PCG_FP32_REFINED, "wave"),
+               (sx.PCG_BF16_REFINED, None), (sx.PCG_BF16_REFINED, "block"),
+               (sx.PCG_BF16_REFINED, "wave"), (sx.PCG_BF16_REFINED, "mfma")]
+
+
+@pytest.mark.parametrize("prec,op", BATCH_CASES,
+                         ids=[f"{['fp64', 'fp32r', 'bf16r'][p]}-{o or 'tuned'}"
+                              for p, o in BATCH_CASES])
+def test_batch_matches_oracle(cuda, prec, op, monkeypatch):
+    """Every element operator the batch tuner may pick (forced through
+    SEMX_BATCH_OP), and the tuner's own choice, which is one operator per
+    (np, element count) class."""
     import torch
-    print(f"[batch] prec {prec}", flush=True)
+    if op is None:
+        monkeypatch.delenv("SEMX_BATCH_OP", raising=False)
+    else:
+        monkeypatch.setenv("SEMX_BATCH_OP", op)
+    print(f"[batch] prec {prec} operator {op or 'tuned'}", flush=True)
     probs = problems(7, seed=4) + problems(3, seed=9, bcs=("D", "N", "D", "N"), distort=0.2)
     systems = []
     for (P, Q, lam, f, bc, ue) in probs:
@@ -57,8 +72,17 @@ def test_batch_matches_oracle(cuda, prec):
         r = rel(g[:, :P.nPlane], ud[:, :P.nPlane])
         worst = max(worst, r)
         assert r <= 1e-10
-    print(f"  {len(probs)} problems, max iterations {it}, worst rel L2 vs DIRECT {worst:.3e}",
-          flush=True)
+    ops, tms = B.info()
+    print(f"  {len(probs)} problems, max iterations {it}, worst rel L2 vs DIRECT {worst:.3e}; "
+          f"operators {ops}, tuning {tms:.1f} ms", flush=True)
+    if op is not None:
+        assert ops == [op] * len(probs)
+    else:
+        allowed = {"block", "wave"} | ({"mfma"} if prec == sx.PCG_BF16_REFINED else set())
+        assert set(ops) <= allowed
+        cls = {}
+        for (P, *_), o in zip(probs, ops):
+            assert cls.setdefault((P.np, P.nel), o) == o
 
 
 def test_batch_iterations_match_single_solves(cuda):

@@ -53,6 +53,36 @@ def test_direct_parity(cuda, case):
         assert err < 1e-5
 
 
+@pytest.mark.parametrize("method", [sx.DIRECT, sx.MIXED], ids=["direct", "mixed"])
+def test_singular_mode0_same_numbering(cuda, method):
+    """With semtex's numbering handed over (semx_mesh_create_numbered, as
+    BCmgr::buildnum reads it from session.num) the singular mode-0 system is
+    pinned at the same node as the reference's (src/matrix.cpp:167-169), so
+    the solution equals the oracle's, given the same numbering, with no
+    constant removed: rel L2 <= 1e-12 (DIRECT) / 1e-10 (MIXED, mode 0
+    direct, the others PCG to 1e-13).  The library's own numbering pins
+    another node, and its mode 0 then differs by a visible constant."""
+    import torch
+    nx, np_, nz = 8, 7, 4
+    print(f"[same numbering] method {method}", flush=True)
+    P, Q, f, bcl, ue = make_case(nx, np_, nz, 0.0, 1.0, 0.0, PERIODIC, same_numbering=True)
+    assert np.array_equal(P.btog, Q.btog) and P.nband == Q.nband
+    S = sx.ModalMatrixSys(P, 0.0, 1.0, 0, nz // 2, nz, method)
+    assert S.info["singular0"]
+    ug, _ = gpu_solve(torch, P, S, f, bcl, tol=1e-13)
+    uo, _ = oracle_solve(Q, nz, 0.0, 1.0, f, bcl)
+    r = rel(ug[:, :P.nPlane], uo[:, :P.nPlane])
+    print(f"  rel L2 vs oracle, no constant removed: {r:.3e}", flush=True)
+    assert r <= (TOL if method == sx.DIRECT else 1e-10)
+    P2, _, _, _, _ = make_case(nx, np_, nz, 0.0, 1.0, 0.0, PERIODIC)
+    assert not np.array_equal(P2.btog, Q.btog)
+    S2 = sx.ModalMatrixSys(P2, 0.0, 1.0, 0, nz // 2, nz, sx.DIRECT)
+    u2, _ = gpu_solve(torch, P2, S2, f, bcl)
+    off = abs((u2[0, :P.nPlane] - uo[0, :P.nPlane]).mean())
+    print(f"  own numbering: mode-0 offset {off:.3e}", flush=True)
+    assert off > 1e-8
+
+
 def test_regress_helmholtz1(cuda):
     """regress/helmholtz1.ok: noise-level (< 1e-12) error vs the analytic
     solution -sin(pi x) sin(pi y)/(2 pi^2 + 16), mesh/helmholtz1."""

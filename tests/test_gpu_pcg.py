@@ -43,10 +43,11 @@ def initial_guess(ue, seed=3):
 def test_pcg_same_iterates_at_step_limit(cuda, case):
     import torch
     name, nx, np_, nz, lam2, beta, dist, bc = case
-    if bc == PERIODIC:
-        pytest.skip("pinned node differs between numberings; iterates are not comparable")
     print(f"[pcg step-limit] {name}", flush=True)
-    P, Q, f, bcl, ue = make_case(nx, np_, nz, lam2, beta, dist, bc)
+    # a singular mode 0 is pinned at the last unknown: with the oracle's
+    # numbering (as semtex's session.num) the pinned node is the same
+    P, Q, f, bcl, ue = make_case(nx, np_, nz, lam2, beta, dist, bc,
+                                 same_numbering=bc == PERIODIC)
     u0 = initial_guess(ue)
     S = sx.ModalMatrixSys(P, lam2, beta, 0, max(nz // 2, 1), nz, sx.JACPCG)
     with pytest.warns(RuntimeWarning, match="step limit"):
@@ -110,6 +111,27 @@ def test_pcg_tight_vs_direct(cuda, case, method, prec):
     ud, _ = oracle_solve(Q, nz, lam2, beta, f, bcl, method=sx.DIRECT)
     if S.info["singular0"]:
         drop_mode0_constant(P, ug, ud)
+    e = rel(solved(P, ug), solved(P, ud))
+    print(f"  iterations {it}, rel L2 vs DIRECT {e:.3e}", flush=True)
+    assert e <= 1e-10
+
+
+@pytest.mark.parametrize("op", ["block", "wave"])
+@pytest.mark.parametrize("prec", [sx.PCG_FP64, sx.PCG_FP32_REFINED], ids=["fp64", "fp32r"])
+@pytest.mark.parametrize("case", [CASES[1], CASES[4]], ids=[CASES[1][0], CASES[4][0]])
+def test_pcg_element_operators(cuda, case, prec, op, monkeypatch):
+    """Both scalar element operators the first solve chooses between
+    (block-wide, wavefront-level; forced through SEMX_PCG_OP), converged at
+    TOL_REL 1e-13: rel L2 <= 1e-10 vs the oracle DIRECT solution."""
+    import torch
+    monkeypatch.setenv("SEMX_PCG_OP", op)
+    name, nx, np_, nz, lam2, beta, dist, bc = case
+    print(f"[pcg operator] {name} prec {prec} {op}", flush=True)
+    P, Q, f, bcl, ue = make_case(nx, np_, nz, lam2, beta, dist, bc)
+    S = sx.ModalMatrixSys(P, lam2, beta, 0, max(nz // 2, 1), nz, sx.JACPCG)
+    set_precision(S, prec)
+    ug, it = gpu_solve(torch, P, S, f, bcl, tol=1e-13)
+    ud, _ = oracle_solve(Q, nz, lam2, beta, f, bcl, method=sx.DIRECT)
     e = rel(solved(P, ug), solved(P, ud))
     print(f"  iterations {it}, rel L2 vs DIRECT {e:.3e}", flush=True)
     assert e <= 1e-10

@@ -79,6 +79,47 @@ def test_svv_operator():
     assert not np.allclose(S2 @ PN, D @ PN, atol=1e-6)
 
 
+@pytest.mark.parametrize("np_,mn,eps,nu", [(9, 4, 0.5, 0.01), (11, 6, 0.1, 0.002), (7, 0, 1.0, 1.0)])
+def test_svv_operator_closed_form(np_, mn, eps, nu):
+    """SDV = BW diag(sqrt S) FW DV (src/svv.cpp:126-185) on Legendre
+    polynomials: P_m' = sum_{j = m-1, m-3, ... >= 0} (2j+1) P_j, so
+    SDV P_m = sum_j (2j+1) sqrt(S_j) P_j at the GLL points, with the filter
+    S_j = 1 for j <= mN and 1 + eps/nu exp(-((N-j)/(mN-j))^2) above
+    (src/svv.cpp:41-79).  A wrong power of S, a shifted filter index or a
+    transposed operator fails this."""
+    N = np_ - 1
+    z, _, D, _ = O.gll(np_)
+    SDV, SDT = O.svv_operators(np_, mn=mn, eps=eps, nu=nu)
+    assert np.array_equal(SDT, SDV.T)
+    S = np.ones(np_)
+    for j in range(mn + 1, np_):
+        S[j] = 1.0 + eps / nu * np.exp(-((N - j) / (mn - j)) ** 2)
+    L = np.polynomial.legendre.Legendre.basis
+    for m in range(np_):
+        ref = sum((2 * j + 1) * np.sqrt(S[j]) * L(j)(z) for j in range(m - 1, -1, -2))
+        ref = ref if m else np.zeros_like(z)
+        assert np.allclose(SDV @ L(m)(z), ref, rtol=1e-12, atol=1e-11 * max(1, np.abs(ref).max()))
+    # S_N = 1 + eps/nu exactly: the top mode's derivative coefficient ratio
+    assert np.isclose(S[N], 1.0 + eps / nu)
+
+
+def test_svv_coeffs_z_closed_form():
+    """SVV::coeffs_z (src/svv.cpp:82-123): 1 for modes k <= SVV_MZ, exactly
+    1 + eps/nu at the last mode k = N (exp(0)), increasing between;
+    a process's share (base > 0) takes the same values; all 1 when SVV_MZ
+    is negative or not below N, or eps <= EPSDP."""
+    nmode, mz, eps, nu = 16, 5, 0.3, 0.01
+    S = O.svv_coeffs_z(nmode, nmode, 0, mz, eps, nu)
+    assert np.all(S[:mz + 1] == 1.0)
+    assert S[nmode - 1] == 1.0 + eps / nu
+    assert np.all(np.diff(S[mz:]) >= 0) and np.all(np.diff(S[mz + 1:]) > 0)
+    share = O.svv_coeffs_z(4, nmode, 8, mz, eps, nu)
+    assert np.array_equal(share, S[8:12])
+    for args in ((nmode, nmode, 0, -1, eps, nu), (nmode, nmode, 0, nmode - 1, eps, nu),
+                 (nmode, nmode, 0, mz, 0.0, nu)):
+        assert np.all(O.svv_coeffs_z(*args) == 1.0)
+
+
 # --------------------------------------------------------------------------
 # a4: edge maps (femlib/mapping.c:46-132) -- brute-force construction
 # --------------------------------------------------------------------------
