@@ -112,6 +112,20 @@ int semx_mesh_create_numbered (int np, int nel, const double* vx,
                                double kinvis, const int* btog,
                                const int* bmask, int nglobal, int nsolve,
                                semx_mesh* mesh);
+/* The same mesh numbered as semtex's `enumerate -O<optlevel>` numbers it
+ * when it writes session.num (utility/enumerate.cpp:540-943; BCmgr runs
+ * `enumerate -O1` itself when the file is missing, src/bcmgr.cpp:620-632):
+ * the traverse numbering of Mesh::buildMap (src/mesh.cpp:889-1015), the
+ * unknowns before the essential nodes (Nsys::sortGid), and for optlevel
+ * 1-3 a reverse Cuthill-McKee order of the unknowns (SPARSPAK FNROOT/RCM,
+ * femlib/netlib.f) from the root that level picks -- 3, enumerate's
+ * default and the regression scripts', tries every unknown as the root
+ * (one RCM per unknown).  SEMX_E_ARG for optlevel outside 0-3,
+ * SEMX_E_UNSUPPORTED if the unknowns' graph is disconnected.            */
+int semx_mesh_create_enumerated (int np, int nel, const double* vx,
+                                 const double* vy, const int* nbr,
+                                 const int* skind, int svv_mn, double svv_eps,
+                                 double kinvis, int optlevel, semx_mesh* mesh);
 int semx_mesh_destroy (semx_mesh mesh);
 /* info[10] = np nel next nint nglobal nsolve nband nbound nPlane planeSize,
  * nband including the diagonal (utility/enumerate.cpp:903-943).           */

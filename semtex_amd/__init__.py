@@ -48,9 +48,10 @@ class Mesh:
     """Elements + numbering of a quad mesh (see semtex_amd.mesh)."""
 
     def __init__(self, np_, m, svv_mn=-1, svv_eps=0.0, kinvis=1.0, numbering=None):
-        """numbering: None for the library's own, or semtex's numbering as
-        read from a session.num file (semtex_amd.numfile.read_num: a dict
-        with btog, bmask, nglobal, nsolve)."""
+        """numbering: None for the library's own; semtex's numbering as read
+        from a session.num file (semtex_amd.numfile.read_num: a dict with
+        btog, bmask, nglobal, nsolve); or ("enumerate", level) to number as
+        semtex's `enumerate -O<level>` would (levels 0-3)."""
         L = load()
         vx = np.ascontiguousarray(m["vx"], dtype=np.float64)
         vy = np.ascontiguousarray(m["vy"], dtype=np.float64)
@@ -60,6 +61,11 @@ class Mesh:
         if numbering is None:
             check(L.semx_mesh_create(np_, int(m["nel"]), _dp(vx), _dp(vy), _ip(nbr), _ip(sk),
                                      svv_mn, svv_eps, kinvis, C.byref(h)), "semx_mesh_create")
+        elif isinstance(numbering, tuple) and numbering[0] == "enumerate":
+            check(L.semx_mesh_create_enumerated(np_, int(m["nel"]), _dp(vx), _dp(vy), _ip(nbr),
+                                                _ip(sk), svv_mn, svv_eps, kinvis,
+                                                int(numbering[1]), C.byref(h)),
+                  "semx_mesh_create_enumerated")
         else:
             bt = np.ascontiguousarray(numbering["btog"], dtype=np.int32)
             bm = np.ascontiguousarray(numbering["bmask"], dtype=np.int32)

@@ -167,11 +167,12 @@ struct Helm {
 
 // setup (semx_setup.cpp)
 // btog/bmask (nel*next each, or null for the library's own numbering),
-// nglobal, nsolve: a caller-supplied numbering (semtex's session.num)
+// nglobal, nsolve: a caller-supplied numbering (semtex's session.num);
+// enum_level 0-3: without one, number as `enumerate -O<enum_level>` does
 int mesh_build(Mesh& M, int np, int nel, const double* vx, const double* vy,
                const int* nbr, const int* skind, int svv_mn, double svv_eps,
                double kinvis, const int* btog = nullptr, const int* bmask = nullptr,
-               int nglobal = 0, int nsolve = 0);
+               int nglobal = 0, int nsolve = 0, int enum_level = -1);
 int helm_build(Helm& H);
 void gll(int np, std::vector<double>& z, std::vector<double>& w);
 void gll_deriv(int np, const std::vector<double>& z, std::vector<double>& D);

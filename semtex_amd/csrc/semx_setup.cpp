@@ -307,7 +307,202 @@ struct GivenNumbering {
   int nglobal, nsolve;
 };
 
-static int number_mesh(Mesh& M, const int* nbr, const int* skind, const GivenNumbering* given) {
+// ---------------------------------------------------------------------------
+// The numbering of semtex's `enumerate` utility (utility/enumerate.cpp:
+// 540-943), which writes session.num: the traverse numbering of
+// Mesh::buildMap (src/mesh.cpp:889-1015; the provisional ids below), the
+// unknowns moved before the essential nodes in their traverse order
+// (Nsys::sortGid), then for levels 1-3 a reverse Cuthill-McKee order of the
+// unknowns (George & Liu 1981, SPARSPAK ROOTLS/FNROOT/DEGREE/RCM as in
+// femlib/netlib.f) over the adjacency lists of Nsys::connectivSC:
+//   1: FNROOT from the first unknown, then RCM from the root it finds;
+//   2: FNROOT from every tenth unknown, RCM from the root of least
+//      global bandwidth;
+//   3: RCM from every unknown, the root of least global bandwidth
+//      (enumerate's default, and the regression scripts').
+// The first root of least bandwidth wins.  Levels 2 and 3 cost one RCM per
+// trial root.  Returns false when the unknowns' graph is disconnected (RCM
+// from one root then numbers only its component).
+namespace {
+
+struct SparGraph {
+  std::vector<int> xadj, adj;   // 0-based CSR, neighbours in insertion order
+};
+
+// ROOTLS: the level structure rooted at root over nodes with mask != 0;
+// xls[k] is the start of level k in ls, xls[nlvl] the end.
+void sp_rootls(const SparGraph& G, int root, std::vector<char>& mask, int& nlvl,
+               std::vector<int>& xls, std::vector<int>& ls) {
+  mask[root] = 0;
+  ls[0] = root;
+  nlvl = 0;
+  int lvlend = 0, ccsize = 1;
+  for (;;) {
+    const int lbegin = lvlend;
+    lvlend = ccsize;
+    xls[nlvl++] = lbegin;
+    for (int i = lbegin; i < lvlend; ++i)
+      for (int j = G.xadj[ls[i]]; j < G.xadj[ls[i] + 1]; ++j) {
+        const int v = G.adj[j];
+        if (!mask[v]) continue;
+        mask[v] = 0;
+        ls[ccsize++] = v;
+      }
+    if (ccsize == lvlend) break;
+  }
+  xls[nlvl] = lvlend;
+  for (int i = 0; i < ccsize; ++i) mask[ls[i]] = 1;
+}
+
+// FNROOT: a pseudo-peripheral node of root's component.
+void sp_fnroot(const SparGraph& G, int& root, std::vector<char>& mask, std::vector<int>& xls,
+               std::vector<int>& ls) {
+  int nlvl = 0, nunlvl = 0;
+  sp_rootls(G, root, mask, nlvl, xls, ls);
+  const int ccsize = xls[nlvl];
+  if (nlvl == 1 || nlvl == ccsize) return;
+  for (;;) {
+    const int jstrt = xls[nlvl - 1];
+    int mindeg = ccsize;
+    root = ls[jstrt];
+    if (ccsize != jstrt + 1)
+      for (int j = jstrt; j < ccsize; ++j) {
+        const int v = ls[j];
+        int ndeg = 0;
+        for (int k = G.xadj[v]; k < G.xadj[v + 1]; ++k) ndeg += mask[G.adj[k]] != 0;
+        if (ndeg < mindeg) { root = v; mindeg = ndeg; }
+      }
+    sp_rootls(G, root, mask, nunlvl, xls, ls);
+    if (nunlvl <= nlvl) return;
+    nlvl = nunlvl;
+    if (nlvl >= ccsize) return;
+  }
+}
+
+// RCM from root over the masked component: breadth-first, each node's new
+// neighbours in order of increasing degree (insertion, ties keep adjacency
+// order), the whole order reversed.  Returns the component size.
+int sp_rcm(const SparGraph& G, int root, std::vector<char>& mask, std::vector<int>& perm,
+           std::vector<int>& deg) {
+  // DEGREE: degree within the masked graph, for root's component
+  {
+    std::vector<int>& ls = perm;
+    std::vector<char> seen(mask.size(), 0);
+    int n = 1;
+    ls[0] = root;
+    seen[root] = 1;
+    for (int h = 0; h < n; ++h) {
+      const int v = ls[h];
+      int d = 0;
+      for (int j = G.xadj[v]; j < G.xadj[v + 1]; ++j) {
+        const int w = G.adj[j];
+        if (!mask[w]) continue;
+        ++d;
+        if (!seen[w]) { seen[w] = 1; ls[n++] = w; }
+      }
+      deg[v] = d;
+    }
+  }
+  mask[root] = 0;
+  perm[0] = root;
+  int lnbr = 1, lvlend = 0;
+  do {
+    const int lbegin = lvlend;
+    lvlend = lnbr;
+    for (int i = lbegin; i < lvlend; ++i) {
+      const int v = perm[i];
+      const int fnbr = lnbr;
+      for (int j = G.xadj[v]; j < G.xadj[v + 1]; ++j) {
+        const int w = G.adj[j];
+        if (!mask[w]) continue;
+        mask[w] = 0;
+        perm[lnbr++] = w;
+      }
+      for (int k = fnbr + 1; k < lnbr; ++k) {
+        const int w = perm[k];
+        int l = k - 1;
+        while (l >= fnbr && deg[perm[l]] > deg[w]) { perm[l + 1] = perm[l]; --l; }
+        perm[l + 1] = w;
+      }
+    }
+  } while (lnbr > lvlend);
+  std::reverse(perm.begin(), perm.begin() + lnbr);
+  return lnbr;
+}
+
+}  // namespace
+
+static bool enumerate_numbering(const std::vector<int>& prov, const std::vector<int>& bmask,
+                                int ng, int nunk, int nel, int next, int level,
+                                std::vector<int>& btog) {
+  // sortGid: unknowns first, each class in traverse order
+  std::vector<char> ess(ng, 0);
+  for (size_t i = 0; i < prov.size(); ++i) ess[prov[i]] = (char)bmask[i];
+  std::vector<int> newid(ng);
+  for (int v = 0, ku = 0, ke = nunk; v < ng; ++v) newid[v] = ess[v] ? ke++ : ku++;
+  btog.resize(prov.size());
+  for (size_t i = 0; i < prov.size(); ++i) btog[i] = newid[prov[i]];
+  if (level < 1 || nunk == 0) return true;
+  // connectivSC adjacency: element by element, slot by slot, first
+  // appearance only
+  std::vector<std::vector<int>> lst(nunk);
+  for (int e = 0; e < nel; ++e) {
+    const int* bt = &btog[(size_t)e * next];
+    const int* mk = &bmask[(size_t)e * next];
+    for (int i = 0; i < next; ++i) {
+      if (mk[i]) continue;
+      std::vector<int>& L = lst[bt[i]];
+      for (int j = 0; j < next; ++j)
+        if (j != i && !mk[j] && std::find(L.begin(), L.end(), bt[j]) == L.end())
+          L.push_back(bt[j]);
+    }
+  }
+  SparGraph G;
+  G.xadj.assign(nunk + 1, 0);
+  for (int v = 0; v < nunk; ++v) G.xadj[v + 1] = G.xadj[v] + (int)lst[v].size();
+  G.adj.reserve(G.xadj[nunk]);
+  for (int v = 0; v < nunk; ++v) G.adj.insert(G.adj.end(), lst[v].begin(), lst[v].end());
+  std::vector<char> mask(nunk, 1);
+  std::vector<int> perm(nunk), xls(nunk + 1), deg(nunk), inv(nunk);
+  auto bandwidth = [&]() {   // Nsys::globalBandwidth of the order in perm
+    for (int k = 0; k < nunk; ++k) inv[perm[k]] = k;
+    int bw = 0;
+    for (int e = 0; e < nel; ++e) {
+      int lo = INT32_MAX, hi = -1;
+      for (int i = 0; i < next; ++i) {
+        if (bmask[(size_t)e * next + i]) continue;
+        const int g = inv[btog[(size_t)e * next + i]];
+        lo = std::min(lo, g);
+        hi = std::max(hi, g);
+      }
+      if (hi >= 0) bw = std::max(bw, hi - lo);
+    }
+    return bw + 1;
+  };
+  int root = 0;
+  if (level == 1) {
+    sp_fnroot(G, root, mask, xls, perm);
+  } else {
+    int bwmin = INT32_MAX;
+    for (int r = 0; r < nunk; r += (level == 2) ? 10 : 1) {
+      int rt = r;
+      std::fill(mask.begin(), mask.end(), 1);
+      if (level == 2) sp_fnroot(G, rt, mask, xls, perm);
+      if (sp_rcm(G, rt, mask, perm, deg) != nunk) return false;
+      const int bw = bandwidth();
+      if (bw < bwmin) { bwmin = bw; root = rt; }
+    }
+    std::fill(mask.begin(), mask.end(), 1);
+  }
+  if (sp_rcm(G, root, mask, perm, deg) != nunk) return false;
+  for (int k = 0; k < nunk; ++k) inv[perm[k]] = k;
+  for (size_t i = 0; i < btog.size(); ++i)
+    if (btog[i] < nunk) btog[i] = inv[btog[i]];
+  return true;
+}
+
+static int number_mesh(Mesh& M, const int* nbr, const int* skind, const GivenNumbering* given,
+                       int enum_level) {
   const int np = M.np, nel = M.nel, next = M.next, nm = np - 1, ni = np - 2;
   const int nside = 4 * nel;
   // -- validate the side connectivity: joined sides pair up symmetrically,
@@ -426,6 +621,18 @@ static int number_mesh(Mesh& M, const int* nbr, const int* skind, const GivenNum
       for (int i = 0; i < next; ++i) if (M.bmask[e * next + i]) M.emask[e] = 1;
     return SEMX_OK;
   }
+  if (enum_level >= 0) {
+    if (!enumerate_numbering(prov, bmask, ng, nunk, nel, next, enum_level, M.btog))
+      return SEMX_E_UNSUPPORTED;
+    M.bmask = bmask;
+    M.nglobal = ng;
+    M.nsolve = nunk;
+    M.nband = band_of(M.btog, M.bmask, nel, next);
+    M.emask.assign(nel, 0);
+    for (int e = 0; e < nel; ++e)
+      for (int i = 0; i < next; ++i) if (M.bmask[e * next + i]) M.emask[e] = 1;
+    return SEMX_OK;
+  }
   // -- candidate orderings; keep the narrowest band (the role of enumerate's
   //    -O2/-O3 root searches, utility/enumerate.cpp:680-809).
   auto apply = [&](const std::vector<int>& perm, std::vector<int>& btog) {
@@ -492,7 +699,8 @@ static int number_mesh(Mesh& M, const int* nbr, const int* skind, const GivenNum
 
 int mesh_build(Mesh& M, int np, int nel, const double* vx, const double* vy,
                const int* nbr, const int* skind, int svv_mn, double svv_eps,
-               double kinvis, const int* btog, const int* bmask, int nglobal, int nsolve) {
+               double kinvis, const int* btog, const int* bmask, int nglobal, int nsolve,
+               int enum_level) {
   if (np < 3 || np > 16 || nel < 1) return SEMX_E_ARG;
   M.np = np; M.nel = nel; M.npnp = np * np;
   M.next = 4 * (np - 1); M.nint = M.npnp - M.next;
@@ -545,7 +753,7 @@ int mesh_build(Mesh& M, int np, int nel, const double* vx, const double* vy,
     M.eset[e] = found;
   }
   const GivenNumbering given{btog, bmask, nglobal, nsolve};
-  int st = number_mesh(M, nbr, skind, (btog && bmask) ? &given : nullptr);
+  int st = number_mesh(M, nbr, skind, (btog && bmask) ? &given : nullptr, enum_level);
   if (st) return st;
   const int next = M.next, nm = np - 1;
   // -- inverse global mass (src/bcmgr.cpp:806-822)

@@ -3181,6 +3181,20 @@ int semx_mesh_create_numbered(int np, int nel, const double* vx, const double* v
   return SEMX_OK;
 }
 
+int semx_mesh_create_enumerated(int np, int nel, const double* vx, const double* vy,
+                                const int* nbr, const int* skind, int svv_mn, double svv_eps,
+                                double kinvis, int optlevel, semx_mesh* mesh) {
+  if (!mesh || !vx || !vy || !nbr || !skind || optlevel < 0 || optlevel > 3) return SEMX_E_ARG;
+  *mesh = nullptr;
+  semx_mesh_s* h = new (std::nothrow) semx_mesh_s();
+  if (!h) return SEMX_E_NOMEM;
+  const int st = mesh_build(h->M, np, nel, vx, vy, nbr, skind, svv_mn, svv_eps, kinvis, nullptr,
+                            nullptr, 0, 0, optlevel);
+  if (st) { delete h; return st; }
+  *mesh = h;
+  return SEMX_OK;
+}
+
 int semx_mesh_destroy(semx_mesh mesh) {
   if (!mesh) return SEMX_E_ARG;
   mesh_release(mesh->M);

@@ -154,3 +154,38 @@ def test_error_paths():
     assert L.semx_helm_create(P.h, 1.0, 1.0, 0, 3, 4, 0, C.byref(h)) == 1
     with pytest.raises(sx.SemxError):
         sx.ModalMatrixSys(P, 1.0, 1.0, 0, 2, 4, 7)
+
+
+ENUM_MESHES = [
+    ("dirichlet", 5, dict(nx=5, ny=4)),
+    ("mixed-distorted", 6, dict(nx=4, ny=3, bc=("D", "N", "D", "N"), distort=0.3)),
+    ("periodic-x", 5, dict(nx=5, ny=3, periodic_x=True, bc=("D", "D", "N", "D"))),
+    ("all-natural", 4, dict(nx=3, ny=5, bc=("N", "N", "N", "N"))),
+]
+
+
+@pytest.mark.parametrize("level", [0, 1, 2, 3])
+@pytest.mark.parametrize("name,np_,kw", ENUM_MESHES, ids=[m[0] for m in ENUM_MESHES])
+def test_enumerate_numbering_matches_oracle(name, np_, kw, level):
+    """semx_mesh_create_enumerated numbers as the oracle's restatement of
+    enumerate does, node for node (the restatement is pinned by
+    regress/kovas1.ok: the -O3 numbering fixes the node at which kovas1's
+    singular pressure is pinned, and with it the printed pressure norm)."""
+    m = rect_mesh(**kw)
+    P = sx.Mesh(np_, m, numbering=("enumerate", level))
+    Q = O.Mesh(np_, m)
+    Q.enumerate(level)
+    assert (P.nglobal, P.nsolve, P.nband) == (Q.nglobal, Q.nsolve, Q.nband)
+    assert np.array_equal(P.btog, Q.btog)
+    assert np.array_equal(P.bmask, Q.bmask)
+
+
+def test_enumerate_kovas1_numbering():
+    from bounded_cases import KOVAS, kovas_meshes
+    for m in kovas_meshes():
+        P = sx.Mesh(KOVAS["np"], m, numbering=("enumerate", 3))
+        Q = O.Mesh(KOVAS["np"], m)
+        Q.enumerate(3)
+        assert np.array_equal(P.btog, Q.btog)
+    with pytest.raises(sx.SemxError):
+        sx.Mesh(5, rect_mesh(2, 2), numbering=("enumerate", 4))

@@ -101,3 +101,50 @@ def test_batch_iterations_match_single_solves(cuda):
     it = B.solve(fs, bcs, us, tol_rel=1e-12)
     print(f"[batch] iterations batch {it}, single {single}", flush=True)
     assert abs(it - max(single)) <= 1
+
+
+@pytest.mark.parametrize("prec", [sx.PCG_FP32_REFINED, sx.PCG_BF16_REFINED],
+                         ids=["fp32r", "bf16r"])
+def test_full_batch_sampled_against_oracle(cuda, prec):
+    """The measured batch itself: 512 problems, K x K quads with K ~
+    U{4..16}, np ~ U{5..11}, lambda2 ~ U[0.1,100], seed 4, Dirichlet, in one
+    launch (element operator by the tuner); every 16th problem (32 of them,
+    every np and a spread of K) against the oracle's DIRECT solution,
+    rel L2 <= 1e-10 at TOL_REL 1e-13."""
+    import torch
+    from semtex_amd.workloads import batch_problems
+    print(f"[batch 512] prec {prec}", flush=True)
+    dev = torch.device("cuda:0")
+    spec = batch_problems(512, seed=4)
+    systems, fs, bcs, keep = [], [], [], []
+    for i, (np_, K, lam, cs) in enumerate(spec):
+        m = rect_mesh(K, K, 0, 2 * np.pi, 0, 2 * np.pi)
+        P = sx.Mesh(np_, m)
+        f, bc, _ = single_plane_problem(P, lam, cs)
+        S = sx.ModalMatrixSys(P, lam, 1.0, 0, 1, 1, sx.JACPCG)
+        if prec == sx.PCG_BF16_REFINED:
+            S.set_pcg(prec, inner_tol=1e-2, max_outer=120)
+        else:
+            S.set_pcg(prec, inner_tol=1e-4, max_outer=40)
+        systems.append(S)
+        fs.append(torch.from_numpy(f.ravel().copy()).to(dev))
+        bcs.append(torch.from_numpy(bc.ravel().copy()).to(dev))
+        if i % 16 == 0:
+            keep.append((i, P, m, np_, lam, f, bc))
+    us = [torch.zeros_like(f) for f in fs]
+    B = sx.Batch(systems)
+    it = B.solve(fs, bcs, us, tol_rel=1e-13)
+    torch.cuda.synchronize()
+    worst, nps, ks = 0.0, set(), set()
+    for (i, P, m, np_, lam, f, bc) in keep:
+        Q = O.Mesh(np_, m)
+        ud, _ = oracle_solve(Q, 1, lam, 1.0, f, bc, method=sx.DIRECT)
+        g = us[i].cpu().numpy().reshape(f.shape)
+        r = rel(g[:, :P.nPlane], ud[:, :P.nPlane])
+        worst = max(worst, r)
+        nps.add(np_)
+        ks.add(int(round(P.nel ** 0.5)))
+        assert r <= 1e-10, (i, np_, P.nel, r)
+    print(f"  max iterations {it}; {len(keep)} sampled problems (np {sorted(nps)}, "
+          f"K {sorted(ks)}), worst rel L2 vs DIRECT {worst:.3e}", flush=True)
+    assert len(nps) >= 5 and max(ks) >= 12

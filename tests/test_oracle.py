@@ -546,3 +546,78 @@ def test_regress_taylor_z(case):
         O.dftr(f, nz, ps, -1)
         got.append(norm_text(np.abs(f[:, :M.nPlane] - e[:, :M.nPlane]).max()))
     assert tuple(got) == TAYLOR_Z[case][1]
+
+
+# --------------------------------------------------------------------------
+# f3: computed high-order pressure BCs (BCmgr::maintainFourier and
+# evaluateCNBCp, src/bcmgr.cpp:1039-1337) against regress/kovas1.ok, and
+# enumerate's numbering (utility/enumerate.cpp:680-806), which fixes the
+# node at which kovas1's singular pressure is pinned: every printed digit
+# of the three norms is reproduced only with the -O3 numbering of the
+# regression script (test/testregress) and the exact-solution start.
+# --------------------------------------------------------------------------
+
+def run_kovas(Mu, Mp, nstep=None):
+    from bounded_cases import KOVAS, kovas_fields
+    (u, v, p), (bcu, bcv) = kovas_fields(Mu)
+    nu = 1.0 / KOVAS["Re"]
+    D = O.lib().orc_dns_create_bc(C.c_void_p(Mu.h), C.c_void_p(Mp.h), 1, 1.0, KOVAS["dt"], nu,
+                                  KOVAS["norder"], KOVAS["advection"], 1e-12, 500, 0, O.dp(bcu),
+                                  O.dp(bcv), None)
+    t = C.c_double()
+    O.lib().orc_dns_run(C.c_void_p(D), nstep or KOVAS["nstep"], O.dp(u), O.dp(v), None,
+                        O.dp(p), C.byref(t))
+    O.lib().orc_dns_destroy(C.c_void_p(D))
+    return u, v, p
+
+
+def test_regress_kovas1():
+    from bounded_cases import KOVAS, KOVAS_OK, error_norms, kovas_meshes
+    mu, mp = kovas_meshes()
+    Mu, Mp = O.Mesh(KOVAS["np"], mu), O.Mesh(KOVAS["np"], mp)
+    Mu.enumerate(3)
+    Mp.enumerate(3)
+    assert error_norms(Mu, *run_kovas(Mu, Mp)) == KOVAS_OK
+    # a plausible slip -- the pressure pinned at another node (the -O1
+    # numbering) -- moves the printed pressure norm
+    Mp1 = O.Mesh(KOVAS["np"], mp)
+    Mp1.enumerate(1)
+    assert error_norms(Mu, *run_kovas(Mu, Mp1))[2] != KOVAS_OK[2]
+
+
+def bandwidth(Q):
+    bw = 0
+    for e in range(Q.nel):
+        g = [Q.btog[e * Q.next + i] for i in range(Q.next) if not Q.bmask[e * Q.next + i]]
+        if g:
+            bw = max(bw, max(g) - min(g))
+    return bw + 1
+
+
+@pytest.mark.parametrize("opt", [0, 1, 2, 3])
+def test_enumerate_numbering(opt):
+    """Each level gives a bijection of the global nodes with the unknowns
+    first (sortGid keeps essential nodes last, in traverse order); level 0
+    is the traverse order itself; RCM levels never widen the band of the
+    traverse order on this mesh, and level 3 (least bandwidth over every
+    RCM root) is no wider than levels 1 and 2."""
+    from bounded_cases import sheared_meshes
+    m = sheared_meshes(5, 4)[0]
+    bws = {}
+    for lev in (0, 1, 2, 3):
+        Q = O.Mesh(5, m)
+        Q.enumerate(lev)
+        ids = np.unique(Q.btog)
+        assert np.array_equal(ids, np.arange(Q.nglobal))
+        unk = Q.btog[Q.bmask == 0]
+        ess = Q.btog[Q.bmask == 1]
+        assert unk.max() < Q.nsolve <= ess.min()
+        assert Q.nband == bandwidth(Q)
+        bws[lev] = Q.nband
+    assert bws[3] <= min(bws[1], bws[2]) and max(bws[1], bws[2], bws[3]) <= bws[0]
+    # the traverse order (level 0) agrees with the library's provisional
+    # numbering up to the unknowns-first shuffle: same node partition
+    Q0, Ql = O.Mesh(5, m), O.Mesh(5, m)
+    Q0.enumerate(0)
+    pairs = set(zip(Q0.btog.tolist(), Ql.btog.tolist()))
+    assert len(pairs) == Q0.nglobal
