@@ -1635,7 +1635,7 @@ constexpr int PCG_THREADS = 1024;
 constexpr int PCG_VU = 4;     // vector-pass entries per thread per load group
 constexpr int PCG_TUNE_ITERS = 30;   // iterations per timed operator candidate
 
-__device__ inline double block_sum(double v, double* red) {
+__device__ __forceinline__ double block_sum(double v, double* red) {
 #pragma unroll
   for (int off = 32; off >= 1; off /= 2) v += __shfl_xor(v, off);
   const int wave = threadIdx.x >> 6;
@@ -1670,7 +1670,7 @@ struct PcgOp {
 // them (up to four contributions each; more are summed after, in order)
 // issued before the value loads: the sums are the same as one row at a time.
 template <typename T, bool MASK>
-__device__ inline double pcg_assemble(const PcgOp& a, int n, const T* __restrict__ lb,
+__device__ __forceinline__ double pcg_assemble(const PcgOp& a, int n, const T* __restrict__ lb,
                                       T* __restrict__ y, const T* __restrict__ w) {
   const int tid = threadIdx.x, ng = a.nglobal;
   double wy = 0.0;
@@ -1715,7 +1715,7 @@ __device__ inline double pcg_assemble(const PcgOp& a, int n, const T* __restrict
 // [n, nglobal) of y are zero (the solved system); otherwise every node takes
 // part (the lifting of essential values).
 template <typename T, bool MASK = true>
-__device__ double pcg_matvec(const PcgOp& a, int n, const T* __restrict__ x,
+__device__ __forceinline__ double pcg_matvec(const PcgOp& a, int n, const T* __restrict__ x,
                              T* __restrict__ y, T* __restrict__ lb, const T* __restrict__ w,
                              T* sh, double* red) {
   const int tid = threadIdx.x, np = a.np, npnp = a.npnp, next = a.next, nint = a.nint;
@@ -1810,7 +1810,7 @@ constexpr int PCG_WAVES = PCG_THREADS / 64;
 constexpr int PCG_NPT = 3;                    // nodes per lane (np <= 13)
 
 template <typename T, bool MASK = true>
-__device__ double pcg_matvec_wave(const PcgOp& a, int n, const T* __restrict__ x,
+__device__ __forceinline__ double pcg_matvec_wave(const PcgOp& a, int n, const T* __restrict__ x,
                                   T* __restrict__ y, T* __restrict__ lb,
                                   const T* __restrict__ w, T* sh, double* red) {
   const int tid = threadIdx.x, np = a.np, npnp = a.npnp, next = a.next, nint = a.nint;
@@ -1943,7 +1943,7 @@ __device__ inline mf_bf4 bf16x4(float a, float b, float c, float d) {
 }
 
 template <bool MASK = true>
-__device__ double pcg_matvec_mfma(const PcgOp& a, int n, const float* __restrict__ x,
+__device__ __forceinline__ double pcg_matvec_mfma(const PcgOp& a, int n, const float* __restrict__ x,
                                   float* __restrict__ y, float* __restrict__ lb,
                                   const float* __restrict__ w, float* sh, double* red) {
   const int tid = threadIdx.x, np = a.np, npnp = a.npnp, next = a.next, nint = a.nint;
@@ -2070,7 +2070,7 @@ __device__ double pcg_matvec_mfma(const PcgOp& a, int n, const float* __restrict
 enum { OPK_BLOCK = 0, OPK_WAVE = 1, OPK_MFMA = 2 };
 
 template <typename T, bool MASK, int OPK>
-__device__ inline double pcg_op(const PcgOp& a, int n, const T* x, T* y, T* lb, const T* w,
+__device__ __forceinline__ double pcg_op(const PcgOp& a, int n, const T* x, T* y, T* lb, const T* w,
                                 T* sh, double* red) {
   if constexpr (OPK == OPK_MFMA) {
     static_assert(std::is_same<T, float>::value, "the BF16 operator works on FP32 vectors");
@@ -2083,7 +2083,7 @@ __device__ inline double pcg_op(const PcgOp& a, int n, const T* x, T* y, T* lb, 
 }
 
 template <typename T, int OPK = OPK_BLOCK>
-__device__ int pcg_core(const PcgOp& a, int n, T* x, T* r, T* p, T* q, T* lb, const T* pc,
+__device__ __forceinline__ int pcg_core(const PcgOp& a, int n, T* x, T* r, T* p, T* q, T* lb, const T* pc,
                         double epsb2, int step_max, T* sh, double* red) {
   const int tid = threadIdx.x, npts = a.npts;
   double r2 = 0.0, rho1 = 0.0, rho2 = 1.0;
@@ -2159,7 +2159,7 @@ __device__ int pcg_core(const PcgOp& a, int n, T* x, T* r, T* p, T* q, T* lb, co
 // OPK: the operator of the inner T-precision iteration; OP64: the one of
 // the FP64 residuals.
 template <typename T, bool REFINE, int OPK = OPK_BLOCK, int OP64 = OPK_BLOCK>
-__device__ int pcg_run(const PcgOp& op, int n, double* X, double* R, const double* B,
+__device__ __forceinline__ int pcg_run(const PcgOp& op, int n, double* X, double* R, const double* B,
                        double* P, double* Q, double* LB, const double* pc, const float* pc32,
                        double epsb2, int step_max, double inner_tol, int max_outer, T* sh,
                        double* red) {
@@ -2208,7 +2208,7 @@ __device__ int pcg_run(const PcgOp& op, int n, double* X, double* R, const doubl
 // shared node differ, the copy written last wins (Veclib::gathr_scatr over
 // elements in order, src/element.cpp:323): the last entry of the node's
 // assembly list, which is in increasing element-storage order.
-__device__ inline void pcg_x0(const PcgOp& op, const int* emap, int n, const double* uk,
+__device__ __forceinline__ void pcg_x0(const PcgOp& op, const int* emap, int n, const double* uk,
                               double* X) {
   const int tid = threadIdx.x, next = op.next, nint = op.nint;
   for (int gI = tid; gI < op.nglobal; gI += PCG_THREADS) {
@@ -2226,7 +2226,7 @@ __device__ inline void pcg_x0(const PcgOp& op, const int* emap, int n, const dou
 // (Field::global2local + setEssential, src/field.cpp:950-969, 1022-1040);
 // ess(gid) gives the value of essential node gid >= nsolve.
 template <class Ess>
-__device__ inline void pcg_store(const PcgOp& op, int n, int nsolve, const double* X,
+__device__ __forceinline__ void pcg_store(const PcgOp& op, int n, int nsolve, const double* X,
                                  double* uk, Ess ess) {
   const int next = op.next, npnp = op.npnp;
   for (int idx = threadIdx.x; idx < op.nel * npnp; idx += PCG_THREADS) {
@@ -2324,7 +2324,7 @@ struct PcgBatchArgs {
 // (Field::constrain and buildRHS, src/field.cpp:728-768, 859-925, with the
 // lifting A g of the essential values g as one unmasked operator
 // application), x0, the iteration and the essential re-imposition.
-template <typename T, bool REFINE>
+template <typename T, bool REFINE, bool MF>
 __global__ __launch_bounds__(PCG_THREADS) void k_pcg_batch(PcgBatchArgs ba) {
   extern __shared__ double pcg_lds[];
   double* red = pcg_lds;
@@ -2383,7 +2383,7 @@ __global__ __launch_bounds__(PCG_THREADS) void k_pcg_batch(PcgBatchArgs ba) {
   // wavefront-level one
   int its = 0;
   bool done = false;
-  if constexpr (std::is_same<T, float>::value) {
+  if constexpr (MF) {
     if (I.opk == OPK_MFMA) {
       its = pcg_run<float, REFINE, OPK_MFMA, OPK_WAVE>(op, n, X, R, B, P, Q, LB, I.pc, I.pc32,
                                                        epsb * epsb, ba.step_max, ba.inner_tol,
@@ -3546,8 +3546,11 @@ int semx_batch_solve(semx_batch batch, const double* const* f, const double* con
   a.u = (double* const*)(b.ptrs + 2 * b.count);
   a.iters = b.iters; a.tol = tol_rel; a.inner_tol = b.inner_tol;
   a.step_max = step_max; a.max_outer = b.max_outer;
-  const void* kfn = (const void*)k_pcg_batch<double, false>;
-  if (b.prec != SEMX_PCG_FP64) kfn = (const void*)k_pcg_batch<float, true>;
+  // the matrix-core operator's code (and its registers) only in the BF16
+  // batch's kernel
+  const void* kfn = (const void*)k_pcg_batch<double, false, false>;
+  if (b.prec == SEMX_PCG_FP32_REFINED) kfn = (const void*)k_pcg_batch<float, true, false>;
+  if (b.prec == SEMX_PCG_BF16_REFINED) kfn = (const void*)k_pcg_batch<float, true, true>;
   if (b.lds > 64 * 1024)
     HIPCHK(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b.lds));
   if (!b.tuned) {
