@@ -38,6 +38,7 @@ PROTOTYPES = {
                                        C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
                                        C.POINTER(VP)]),
     "semx_helm_destroy": (C.c_int, [VP]),
+    "semx_helm_sweep_info": (C.c_int, [VP, I]),
     "semx_helm_info": (C.c_int, [VP, I]),
     "semx_helm_element": (C.c_int, [VP, C.c_int, C.c_int, D, D, D]),
     "semx_helm_solve": (C.c_int, [VP, VP, VP, VP, C.c_double, C.c_int, I, VP]),

@@ -107,6 +107,12 @@ struct HelmDev {
   double* sol = nullptr;      // nplanes*solstride
   int* plane_of = nullptr;    // nplanes -> field plane
   double* hcon = nullptr;     // per mode: lambda2 + betak2 (betak2 with its SVV weight)
+  // partitioned band sweep: spikes, their per-mode offsets, partition row
+  // starts (npart+1 per mode) and the tail/head hand-off slots
+  double* spk = nullptr;
+  long long* spk_off = nullptr;
+  int* prow = nullptr;
+  double* pslot = nullptr;
 };
 
 struct Helm {
@@ -161,6 +167,17 @@ struct Helm {
   int nblkmax = 0;
   size_t coop_xstride = 0;            // hand-off slots per direction
   std::vector<int> coop_clo;          // chunk ranges per lane
+  // partitioned band sweep (32-column extended panels): each mode's band
+  // cut into npart row ranges at block boundaries, swept concurrently and
+  // joined through spikes computed at setup; npart = 1: not available.
+  // Per mode: prow (npart+1 row starts), spk_off into spk, which holds the
+  // forward spikes V_p = L_p^-1 B_p of partitions 1..npart-1 and then the
+  // backward spikes W_p = U_p^-1 C_p of partitions 0..npart-2, each n_p x
+  // kd column-major (L = U^T the factor, B_p / C_p its coupling blocks)
+  int npart = 1;
+  std::vector<int> prow;
+  std::vector<long long> spk_off;
+  std::vector<double> spk;            // host, released after upload
   bool on_device = false;
   HelmDev d;
 };

@@ -975,6 +975,70 @@ static int band_chol_rows(int n, int kd, int ldab, double* A) {
   return SEMX_OK;
 }
 
+// Spikes of the partitioned band sweep for mode m from its factor
+// (row band storage A[j*ldab + d] = U(j, j+d)).  With L = U^T, partition p
+// holding rows [s, e):
+//   forward  V_p = L_p^-1 [B_p; 0], B_p(r, c) = L(s+r, s-kd+c), r < kd:
+//            the solution of L y = b is y_p = L_p^-1 b_p - V_p y[s-kd, s);
+//   backward W_p = U_p^-1 [0; C_p], C_p(r, c) = U(s+r, e+c):
+//            the solution of U x = y is x_p = U_p^-1 y_p - W_p x[e, e+kd).
+// Each by substitution over the partition's rows with all kd columns at
+// once; stored column-major, each column n_p rows padded to an even count.
+// (cloned for AVX2+FMA hosts: the kd-wide row updates are most of the
+// setup time of a partitioned system)
+__attribute__((target_clones("arch=haswell", "default")))
+static void band_spikes(Helm& H, int m, const double* A) {
+  const int P = H.npart, kd = H.kd, ldab = H.ldab;
+  const int* pr = &H.prow[(size_t)m * (P + 1)];
+  double* out = &H.spk[H.spk_off[m]];
+  std::vector<double> T;
+  auto store = [&](int np_) {   // columns of an even number of rows (16-byte aligned)
+    const int ld = (np_ + 1) & ~1;
+    for (int r = 0; r < np_; ++r)
+      for (int c = 0; c < kd; ++c) out[(size_t)c * ld + r] = T[(size_t)r * kd + c];
+    out += (size_t)ld * kd;
+  };
+  for (int p = 1; p < P; ++p) {
+    const int s = pr[p], np_ = pr[p + 1] - s;
+    T.assign((size_t)np_ * kd, 0.0);
+    for (int r = 0; r < np_; ++r) {
+      const int g = s + r;
+      double* row = &T[(size_t)r * kd];
+      for (int c = r; c < kd; ++c) {
+        const int h = s - kd + c;
+        row[c] = A[(size_t)h * ldab + (g - h)];
+      }
+      for (int k = std::max(0, r - kd); k < r; ++k) {
+        const double l = A[(size_t)(s + k) * ldab + (r - k)];
+        if (l == 0.0) continue;
+        const double* vk = &T[(size_t)k * kd];
+        for (int c = 0; c < kd; ++c) row[c] -= l * vk[c];
+      }
+      const double d = A[(size_t)g * ldab];
+      for (int c = 0; c < kd; ++c) row[c] /= d;
+    }
+    store(np_);
+  }
+  for (int p = 0; p + 1 < P; ++p) {
+    const int s = pr[p], e = pr[p + 1], np_ = e - s;
+    T.assign((size_t)np_ * kd, 0.0);
+    for (int r = np_ - 1; r >= 0; --r) {
+      const int g = s + r;
+      double* row = &T[(size_t)r * kd];
+      for (int c = 0; c < kd && e + c - g <= kd; ++c) row[c] = A[(size_t)g * ldab + (e + c - g)];
+      for (int k = r + 1; k <= std::min(np_ - 1, r + kd); ++k) {
+        const double u = A[(size_t)g * ldab + (k - r)];
+        if (u == 0.0) continue;
+        const double* wk = &T[(size_t)k * kd];
+        for (int c = 0; c < kd; ++c) row[c] -= u * wk[c];
+      }
+      const double d = A[(size_t)g * ldab];
+      for (int c = 0; c < kd; ++c) row[c] /= d;
+    }
+    store(np_);
+  }
+}
+
 int helm_build(Helm& H) {
   Mesh& M = *H.M;
   const int next = M.next, nint = M.nint, nset = M.nset, npnp = M.npnp;
@@ -1047,6 +1111,45 @@ int helm_build(Helm& H) {
   }
   H.band_doubles = offH;
   H.inv_doubles = offL;
+  // Partitioned sweep: enough partitions that one workgroup per direct
+  // mode, plane and partition fills the chip's 256 CUs (at most 8), each
+  // partition at least kd + one block of rows (its coupling reaches only
+  // the neighbouring partition).  SEMX_BAND_PARTS sets the count (1: off).
+  H.npart = 1;
+  H.prow.clear(); H.spk_off.clear(); H.spk.clear();
+  if (H.band_ext && nb == 32 && H.kd >= 1) {
+    int ndirect = 0, nmin = INT32_MAX;
+    for (int m = 0; m < nmodes; ++m)
+      if (H.modes[m].kind == SEMX_DIRECT) { ++ndirect; nmin = std::min(nmin, H.modes[m].n); }
+    int P = ndirect ? std::max(1, std::min(8, 256 / (2 * ndirect))) : 1;
+    if (const char* ep = getenv("SEMX_BAND_PARTS")) P = std::max(1, std::min(8, atoi(ep)));
+    if (ndirect) P = std::min(P, nmin / (H.kd + nb));
+    if (P >= 2) {
+      H.npart = P;
+      H.prow.assign((size_t)nmodes * (P + 1), 0);
+      H.spk_off.assign(nmodes, 0);
+      size_t off = 0;
+      bool fits = true;
+      for (int m = 0; m < nmodes; ++m) {
+        const ModeSys& S = H.modes[m];
+        if (S.kind != SEMX_DIRECT) continue;
+        int* pr = &H.prow[(size_t)m * (P + 1)];
+        for (int q = 0; q < P; ++q) pr[q] = (int)((long long)q * S.nblk / P) * nb;
+        pr[P] = S.n;
+        for (int q = 0; q < P; ++q) fits &= pr[q + 1] - pr[q] >= H.kd + 1;
+        H.spk_off[m] = (long long)off;
+        auto ldq = [&](int q) { return (size_t)((pr[q + 1] - pr[q] + 1) & ~1); };
+        for (int q = 1; q < P; ++q) off += ldq(q) * H.kd;
+        for (int q = 0; q + 1 < P; ++q) off += ldq(q) * H.kd;
+      }
+      if (!fits) {
+        H.npart = 1;
+        H.prow.clear(); H.spk_off.clear();
+      } else {
+        H.spk.assign(off, 0.0);
+      }
+    }
+  }
   if (H.band_ext) {
     H.xblk = 0;
     for (int m = 0; m < nmodes; ++m)
@@ -1159,6 +1262,7 @@ int helm_build(Helm& H) {
         H.xrows[((size_t)m * H.xblk + J) * 2] = nb + hf;
         H.xrows[((size_t)m * H.xblk + J) * 2 + 1] = nb + hb;
       }
+      if (H.npart > 1) band_spikes(H, m, A);
       // diagonal-block inverses: Linv_J = (U_JJ^T)^-1, lower triangular
       for (int J = 0; !ext && J < S.nblk; ++J) {
         block_inverse(A, n, kd, ldab, nb, J * nb, &H.Linv[S.off_L + (size_t)J * nb * nb],

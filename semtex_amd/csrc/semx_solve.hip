@@ -705,9 +705,14 @@ __device__ inline double lane_swap1(double x) {
 }
 
 template <int NB, int BT, int NR, int D, bool FWD>
-__device__ inline void band_res_sweep(const double* __restrict__ P, double* const (&vec)[NR],
+__device__ inline void band_res_sweep(const double* __restrict__ P, double* const (&vec0)[NR],
                                       const int* __restrict__ xrows, int n, int nblk, int kd,
-                                      int ldx) {
+                                      int ldx, int Jlo = 0, int Jhi = -1, int base = 0) {
+  // blocks [Jlo, Jhi) only, updates to rows outside them dropped (a
+  // partition's local solve); vec holds rows from `base` on
+  if (Jhi < 0) Jhi = nblk;
+  const int nst = Jhi - Jlo;
+  double* const (&vec)[NR] = vec0;
   constexpr int NH = NB / 2;            // values per thread and step
   const int tid = threadIdx.x;
   const int row = tid >> 1, par = tid & 1;
@@ -717,7 +722,7 @@ __device__ inline void band_res_sweep(const double* __restrict__ P, double* cons
   // rows past a block's live rows load the last live row's address (the
   // same cache lines, no extra memory traffic) and skip the product
   auto load = [&](int s, double (&pp)[NH]) {
-    const int J = FWD ? s : nblk - 1 - s;
+    const int J = FWD ? Jlo + s : Jhi - 1 - s;
     const double2* pj = reinterpret_cast<const double2*>(P + (size_t)J * NB * ldx);
     const int r = min(row, xrows[2 * J + (FWD ? 0 : 1)] - 1);
 #pragma unroll
@@ -728,7 +733,7 @@ __device__ inline void band_res_sweep(const double* __restrict__ P, double* cons
     }
   };
   auto step = [&](int s, double (&pp)[NH]) {
-    const int J = FWD ? s : nblk - 1 - s;
+    const int J = FWD ? Jlo + s : Jhi - 1 - s;
     const int j0 = J * NB;
     // Every thread takes delivery of this step's registers here, on every
     // path: consumed only inside the branch below, the loads would count
@@ -740,13 +745,13 @@ __device__ inline void band_res_sweep(const double* __restrict__ P, double* cons
     // the previous block's solution (its window is no longer read)
     if (held_i >= 0 && par == 0) {
 #pragma unroll
-      for (int c = 0; c < NR; ++c) vec[c][held_i] = held[c];
+      for (int c = 0; c < NR; ++c) vec[c][held_i - base] = held[c];
     }
     held_i = -1;
     const int rr = row - NB;
     const int i = row < NB ? j0 + row : (FWD ? j0 + NB + rr : j0 - 1 - rr);
     const bool live = row < NB || (row < xrows[2 * J + (FWD ? 0 : 1)] && rr < kd &&
-                                   (FWD ? i < nblk * NB : i >= 0));
+                                   (FWD ? i < Jhi * NB : i >= Jlo * NB));
     // column pair 2k+par of the window: values 4k + 2 par, 4k + 2 par + 1
     double acc[NR][2];
 #pragma unroll
@@ -754,7 +759,7 @@ __device__ inline void band_res_sweep(const double* __restrict__ P, double* cons
     if (live) {
 #pragma unroll
       for (int c = 0; c < NR; ++c) {
-        const double2* wv = reinterpret_cast<const double2*>(vec[c] + j0) + par;
+        const double2* wv = reinterpret_cast<const double2*>(vec[c] + (j0 - base)) + par;
 #pragma unroll
         for (int k0 = 0; k0 < NH / 2; k0 += 4) {
           double2 w[4];
@@ -778,34 +783,34 @@ __device__ inline void band_res_sweep(const double* __restrict__ P, double* cons
 #pragma unroll
       for (int c = 0; c < NR; ++c) {
         if (row < NB) held[c] = v[c];
-        else vec[c][i] -= v[c];
+        else vec[c][i - base] -= v[c];
       }
       if (row < NB) held_i = i;
     }
   };
 
-  if (nblk <= 0) return;
+  if (nst <= 0) return;
 #pragma unroll
-  for (int d = 0; d < D; ++d) load(min(d, nblk - 1), pbuf[d]);
+  for (int d = 0; d < D; ++d) load(min(d, nst - 1), pbuf[d]);
   // whole groups of D+1 steps with no exit inside a group (an exit between
   // the steps of a group would leave its prefetches unconsumed on a path
   // back to the loop head), then the remaining steps
   int s = 0;
-  for (; s + D < nblk; s += D + 1) {
+  for (; s + D < nst; s += D + 1) {
 #pragma unroll
     for (int u = 0; u <= D; ++u) {
       const int nx = (u + D) % (D + 1);
-      load(min(s + u + D, nblk - 1), pbuf[nx]);
+      load(min(s + u + D, nst - 1), pbuf[nx]);
       step(s + u, pbuf[u]);
     }
   }
 #pragma unroll
   for (int u = 0; u < D; ++u)
-    if (s + u < nblk) step(s + u, pbuf[u]);
+    if (s + u < nst) step(s + u, pbuf[u]);
   __syncthreads();
   if (held_i >= 0 && par == 0) {
 #pragma unroll
-    for (int c = 0; c < NR; ++c) vec[c][held_i] = held[c];
+    for (int c = 0; c < NR; ++c) vec[c][held_i - base] = held[c];
   }
   __syncthreads();
 }
@@ -993,6 +998,217 @@ __device__ inline double coop_take(const double* p, double v, int* abort) {
 __global__ void k_coop_fill(double* p, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = __longlong_as_double((long long)COOP_SENT);
+}
+
+// ===========================================================================
+// Partitioned band solve: each mode's band cut into P row ranges at block
+// boundaries, one workgroup per (mode, plane, partition), the workgroups of
+// a mode on one XCD.  With L = U^T and partition p holding rows [s, e):
+//   forward   y_p = z_p - V_p t_{p-1},  z_p = L_p^-1 b_p (the local sweep of
+//             the extended panels over the partition's blocks, updates
+//             beyond e dropped), t_{p-1} = y[s-kd, s) from partition p-1
+//   backward  x_p = w_p - W_p h_{p+1},  w_p = U_p^-1 y_p (local sweep,
+//             updates before s dropped), h_{p+1} = x[e, e+kd)
+// with the spikes V_p, W_p (n_p x kd, column-major) formed at setup.  The
+// tails t and heads h pass along the partitions through hand-off slots
+// (the sentinel slots of the cooperative sweep): each workgroup corrects
+// its kd boundary rows first and publishes them before correcting the
+// rest, so a chain link costs one kd x kd product.  Every wait is bounded;
+// a timeout sets the error word.
+// ===========================================================================
+
+struct PartArgs {
+  BandArgs b;
+  const double* spk;          // spikes, all modes
+  const long long* spk_off;   // per mode
+  const int* prow;            // per mode: P+1 partition row starts
+  double* slot;               // [md*F + fsel][part][2][kd]
+  int* err;
+  int P;
+};
+
+// rows [ra, rb) of vec -= S t: S a spike (column-major, ld rows per
+// column, ld even, columns 16-byte aligned), t in LDS (kd values).  Row
+// pairs with 16-byte loads, eight columns in flight per thread; when the
+// rows leave threads idle (the kd boundary rows, on the critical path of
+// the chain) the columns are split over up to 8 thread groups whose
+// partial sums meet in LDS (red: BT * 2 doubles).
+template <int BT>
+__device__ inline void spike_apply(double* vec, const double* __restrict__ S, int ld,
+                                   const double* t, int kd, int ra, int rb, double* red) {
+  const int pa = ra >> 1, pb = (rb + 1) >> 1;      // row pairs [pa, pb)
+  const int npair = pb - pa;
+  if (npair <= 0) return;
+  const int G = max(1, min(8, BT / npair));
+  const int tid = threadIdx.x;
+  if (G == 1) {
+    for (int q = pa + tid; q < pb; q += BT) {
+      const double2* col = reinterpret_cast<const double2*>(S + 2 * q);
+      const int st = ld >> 1;
+      double2 acc[2] = {{0.0, 0.0}, {0.0, 0.0}};
+      int c = 0;
+      for (; c + 8 <= kd; c += 8) {
+        double2 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = col[(size_t)(c + u) * st];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          acc[u & 1].x += v[u].x * t[c + u];
+          acc[u & 1].y += v[u].y * t[c + u];
+        }
+      }
+      for (; c < kd; ++c) {
+        const double2 v = col[(size_t)c * st];
+        acc[0].x += v.x * t[c];
+        acc[0].y += v.y * t[c];
+      }
+      const int r = 2 * q;
+      if (r >= ra && r < rb) vec[r] -= acc[0].x + acc[1].x;
+      if (r + 1 >= ra && r + 1 < rb) vec[r + 1] -= acc[0].y + acc[1].y;
+    }
+    return;
+  }
+  const int q = pa + tid % npair, g = tid / npair;
+  if (g < G) {
+    const int c0 = (int)((long)g * kd / G), c1 = (int)((long)(g + 1) * kd / G);
+    const double2* col = reinterpret_cast<const double2*>(S + 2 * q);
+    const int st = ld >> 1;
+    double2 acc[2] = {{0.0, 0.0}, {0.0, 0.0}};
+    int c = c0;
+    for (; c + 8 <= c1; c += 8) {
+      double2 v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = col[(size_t)(c + u) * st];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        acc[u & 1].x += v[u].x * t[c + u];
+        acc[u & 1].y += v[u].y * t[c + u];
+      }
+    }
+    for (; c < c1; ++c) {
+      const double2 v = col[(size_t)c * st];
+      acc[0].x += v.x * t[c];
+      acc[0].y += v.y * t[c];
+    }
+    red[2 * tid] = acc[0].x + acc[1].x;
+    red[2 * tid + 1] = acc[0].y + acc[1].y;
+  }
+  __syncthreads();
+  if (tid < npair) {
+    double sx = 0.0, sy = 0.0;
+    for (int gg = 0; gg < G; ++gg) {
+      sx += red[2 * (gg * npair + tid)];
+      sy += red[2 * (gg * npair + tid) + 1];
+    }
+    const int r = 2 * (pa + tid);
+    if (r >= ra && r < rb) vec[r] -= sx;
+    if (r + 1 >= ra && r + 1 < rb) vec[r + 1] -= sy;
+  }
+  __syncthreads();
+}
+
+// kd values from a hand-off slot into LDS; false (agreed by the workgroup)
+// if the wait timed out
+template <int BT>
+__device__ inline bool slot_take(const double* slot, double* t, int kd, int* flag) {
+  if (threadIdx.x == 0) *flag = 0;
+  __syncthreads();
+  for (int a = threadIdx.x; a < kd; a += BT) {
+    int ab = 0;
+    const double v = coop_take(slot + a, ld_dev(slot + a), &ab);
+    t[a] = v;
+    if (ab) *flag = 1;
+  }
+  __syncthreads();
+  return *flag == 0;
+}
+
+template <int NB, int BT>
+__global__ __launch_bounds__(BT) void k_band_part(PartArgs pa) {
+  extern __shared__ double lds[];
+  const BandArgs& a = pa.b;
+  const int P = pa.P, F = a.fsplit, G = F * P;
+  const int bx = blockIdx.x;
+  const int md = (bx / (8 * G)) * 8 + (bx & 7);
+  const int rq = (bx >> 3) % G;
+  const int fsel = rq / P, part = rq - (rq / P) * P;
+  if (md >= a.ndm) return;
+  const int m = uni(a.dmode[md]);
+  const int fld = fsel / a.npl, kk = fsel - (fsel / a.npl) * a.npl;
+  const int pl = a.mode_plane[2 * m + kk];
+  if (pl < 0) return;        // every partition of this plane agrees
+  double* gy = a.sol + fld * a.fstride + (size_t)uni(pl) * a.solstride;
+  const int n = uni(a.mode_n[m]);
+  const int nblk = (n + NB - 1) / NB;
+  const long long offp = uni64(a.offP[m]);
+  const int kd = a.kd;
+  const int* pr = pa.prow + (size_t)m * (P + 1);
+  const int r0 = uni(pr[part]), r1 = uni(pr[part + 1]);
+  const int np_ = r1 - r0;
+  const int J0 = r0 / NB, J1 = (part == P - 1) ? nblk : r1 / NB;
+  const int nloc = (J1 - J0) * NB;
+  double* vec = lds;                                  // a.ring rows
+  double* tv = lds + a.ring;                          // kd values (even count)
+  double* red = tv + ((kd + 1) / 2) * 2;              // 2 * BT partial sums
+  int* flag = reinterpret_cast<int*>(red + 2 * BT);
+  int* xr = flag + 2;
+  // spikes of this partition (columns of an even number of rows)
+  auto ldv = [&](int q) { return ((uni(pr[q + 1]) - uni(pr[q])) + 1) & ~1; };
+  const int ld = ldv(part);
+  const double* spk = pa.spk + uni64(pa.spk_off[m]);
+  const double* V = spk;
+  const double* W = spk;
+  for (int q = 1; q < P; ++q) {
+    const long long sz = (long long)ldv(q) * kd;
+    if (q < part) V += sz;
+    W += sz;
+  }
+  for (int q = 0; q < part; ++q) W += (long long)ldv(q) * kd;
+  double* sl = pa.slot + (size_t)(md * F + fsel) * P * 2 * kd;
+  for (int i = threadIdx.x; i < 2 * nblk; i += BT) xr[i] = a.xrows[(size_t)m * 2 * a.xblk + i];
+  for (int i = threadIdx.x; i < nloc; i += BT) vec[i] = (i < np_) ? gy[r0 + i] : 0.0;
+  __syncthreads();
+  double* const vv[1] = {vec};
+  band_res_sweep<NB, BT, 1, 1, true>(a.Pf + offp, vv, xr, n, nblk, kd, a.kdp, J0, J1, r0);
+  // forward join: the tail first, published, then the other rows
+  bool ok = true;
+  if (part > 0) {
+    ok = slot_take<BT>(sl + (size_t)(part - 1) * 2 * kd, tv, kd, flag);
+    if (ok) {
+      spike_apply<BT>(vec, V, ld, tv, kd, np_ - kd, np_, red);
+      __syncthreads();
+    }
+  }
+  if (ok && part < P - 1)
+    for (int i = threadIdx.x; i < kd; i += BT) st_dev(sl + (size_t)part * 2 * kd + i, vec[np_ - kd + i]);
+  if (ok && part > 0) {
+    spike_apply<BT>(vec, V, ld, tv, kd, 0, np_ - kd, red);
+    __syncthreads();
+  }
+  if (!ok) {
+    if (threadIdx.x == 0) __hip_atomic_store(pa.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  band_res_sweep<NB, BT, 1, 1, false>(a.Pb + offp, vv, xr, n, nblk, kd, a.kdp, J0, J1, r0);
+  // backward join: the head first, published, then the other rows
+  if (part < P - 1) {
+    ok = slot_take<BT>(sl + (size_t)(part + 1) * 2 * kd + kd, tv, kd, flag);
+    if (ok) {
+      spike_apply<BT>(vec, W, ld, tv, kd, 0, kd, red);
+      __syncthreads();
+    }
+  }
+  if (ok && part > 0)
+    for (int i = threadIdx.x; i < kd; i += BT) st_dev(sl + (size_t)part * 2 * kd + kd + i, vec[i]);
+  if (ok && part < P - 1) {
+    spike_apply<BT>(vec, W, ld, tv, kd, kd, np_, red);
+    __syncthreads();
+  }
+  if (!ok) {
+    if (threadIdx.x == 0) __hip_atomic_store(pa.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  for (int i = threadIdx.x; i < np_; i += BT) gy[r0 + i] = vec[i];
 }
 
 // Threads per workgroup: ceil(NR*NB/64) hand-off waves (one value per
@@ -2575,6 +2791,19 @@ int helm_upload(Helm& H) {
   }
   if ((st = dev_upload(&H.d.plane_of, H.plane_of.data(), H.plane_of.size()))) return st;
   if ((st = coop_setup(H))) return st;
+  if (H.npart > 1) {
+    int nd = 0;
+    for (int m = 0; m < nm; ++m) nd += H.modes[m].kind == SEMX_DIRECT;
+    if ((st = dev_upload(&H.d.spk, H.spk.data(), H.spk.size()))) return st;
+    if ((st = dev_upload(&H.d.spk_off, H.spk_off.data(), H.spk_off.size()))) return st;
+    if ((st = dev_upload(&H.d.prow, H.prow.data(), H.prow.size()))) return st;
+    // tail/head slots for one field of two planes (rounded up to whole
+    // groups of 8 modes)
+    const size_t slots = (size_t)((nd + 7) / 8 * 8) * 2 * H.npart * 2 * H.kd;
+    if ((st = dev_upload<double>(&H.d.pslot, nullptr, slots))) return st;
+    if (!H.d.err && (st = dev_upload<int>(&H.d.err, nullptr, 1))) return st;
+    std::vector<double>().swap(H.spk);
+  }
   // host band copies are no longer needed
   std::vector<double>().swap(H.Pf);
   std::vector<double>().swap(H.Pb);
@@ -2595,6 +2824,7 @@ void helm_release(Helm& H) {
   dev_free(H.d.iters);
   dev_free(H.d.inbox); dev_free(H.d.part); dev_free(H.d.clo);
   dev_free(H.d.err);
+  dev_free(H.d.spk); dev_free(H.d.spk_off); dev_free(H.d.prow); dev_free(H.d.pslot);
   H.nf_cap = 0;
   H.on_device = false;
 }
@@ -2738,6 +2968,7 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
     const size_t lds = (size_t)nr * (ba.ring + H.nb) * sizeof(double);
     if (lds > 160 * 1024 && H.coop_w == 0) return SEMX_E_UNSUPPORTED;
     const void* kfn = nullptr;
+    bool part_done = false;
     unsigned bt = (unsigned)H.bt;
     size_t lds_x = lds;
     unsigned grid = (unsigned)H.dmode.size();
@@ -2749,9 +2980,12 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
       //   3  LDS ring window, one workgroup per mode and field (split)
       //   4, 5  as 1 and 3 with one workgroup per mode and plane
       //   6  as 5 with one panel row per thread (16-column blocks)
+      //   7  as 3 with one panel row per thread (16-column blocks)
+      //   8  partitioned sweep: one workgroup per mode, plane and partition,
+      //      the partitions joined through spikes (one field)
       // A split puts a mode's workgroups on one XCD, so they read the same
       // panels through its L2.  The first call for a field count times the
-      // candidates and keeps the fastest (SEMX_AUTOTUNE=0 / 2..7 force 1 / 2..7).
+      // candidates and keeps the fastest (SEMX_AUTOTUNE=0 / 2..8 force 1 / 2..8).
       ba.kdp = H.ldx;
       ba.ndm = (int)H.dmode.size();
       const int xbt = std::min(512, (H.ldx + 63) / 64 * 64);
@@ -2786,7 +3020,7 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
       SEMX_RPICK(32, 896) SEMX_RPICK(32, 960) SEMX_RPICK(32, 1024)
 #undef SEMX_RPICK
       struct Cand { const void* fn; size_t lds; unsigned bt; int F; int ring; };
-      constexpr int NCAND = 7;
+      constexpr int NCAND = 8;
       Cand cand[NCAND + 1] = {};
       const int gpad = (ba.ndm + 7) / 8 * 8;
       {
@@ -2820,7 +3054,56 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
           if (H.nb == 16 && H.ldx > 896 && H.ldx <= 960 && l5 <= 160 * 1024)
             cand[6] = {(const void*)k_band_ext<16, 1, 960, 1, 1>, l5, 960u, 2 * nf, ba.ring};
         }
+        // partitioned sweep, one workgroup per mode, plane and partition (one
+        // field), if the whole grid can be resident at once (not when the
+        // cooperative sweep is configured: that one then runs instead)
+        if (nf == 1 && H.npart > 1 && H.d.spk && H.coop_w == 0) {
+          const void* pfn = nullptr;
+#define SEMX_PPICK(BT_) if (H.nb == 32 && rbt == BT_) pfn = (const void*)k_band_part<32, BT_>;
+          SEMX_PPICK(128) SEMX_PPICK(192) SEMX_PPICK(256) SEMX_PPICK(320) SEMX_PPICK(384)
+          SEMX_PPICK(448) SEMX_PPICK(512) SEMX_PPICK(576) SEMX_PPICK(640) SEMX_PPICK(704)
+          SEMX_PPICK(768) SEMX_PPICK(832) SEMX_PPICK(896) SEMX_PPICK(960) SEMX_PPICK(1024)
+#undef SEMX_PPICK
+          const int P = H.npart;
+          int maxloc = 0;
+          for (int m : H.dmode) {
+            const int* pr = &H.prow[(size_t)m * (P + 1)];
+            const int nbk = H.modes[m].nblk;
+            for (int q = 0; q < P; ++q) {
+              const int J1 = (q == P - 1) ? nbk : pr[q + 1] / H.nb;
+              maxloc = std::max(maxloc, (J1 - pr[q] / H.nb) * H.nb);
+            }
+          }
+          const size_t l8 = ((size_t)maxloc + (size_t)(H.kd + 1) / 2 * 2 + 2 * (size_t)rbt) *
+                                sizeof(double) + 8 + xr_b;
+          const unsigned g8 = (unsigned)(gpad * npl * P);
+          int per_cu = 0, cus = 0, dev = 0;
+          if (pfn && l8 <= 160 * 1024 && hipGetDevice(&dev) == hipSuccess &&
+              hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+              hipFuncSetAttribute(pfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l8) ==
+                  hipSuccess &&
+              hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pfn, rbt, l8) == hipSuccess &&
+              (long)per_cu * cus >= (long)g8)
+            cand[8] = {pfn, l8, (unsigned)rbt, npl, maxloc};
+        }
       }
+      // the partitioned sweep's launch: hand-off slots back to the sentinel,
+      // then one workgroup per mode, plane and partition
+      auto launch_part = [&](const BandArgs& bc, const Cand& cd, bool prof) -> int {
+        PartArgs pa;
+        pa.b = bc;
+        pa.spk = H.d.spk; pa.spk_off = H.d.spk_off; pa.prow = H.d.prow;
+        pa.slot = H.d.pslot; pa.err = H.d.err; pa.P = H.npart;
+        const size_t used = (size_t)gpad * cd.F * H.npart * 2 * H.kd;
+        hipLaunchKernelGGL(k_coop_fill, dim3((unsigned)((used + 255) / 256)), dim3(256), 0, s,
+                           H.d.pslot, used);
+        void* kargs[] = {&pa};
+        if (prof) prof_begin(s, &ev);
+        HIPCHK(hipLaunchKernel(cd.fn, dim3((unsigned)(gpad * cd.F * H.npart)), dim3(cd.bt),
+                               kargs, cd.lds, s));
+        if (prof) prof_end(s, "k_band_solve", ev);
+        return SEMX_OK;
+      };
       int navail = 0, only = 0;
       for (int c = 1; c <= NCAND; ++c)
         if (cand[c].fn) { ++navail; only = c; }
@@ -2854,6 +3137,11 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
                                        (int)cand[c].lds));
             for (int rep = 0; rep < 3; ++rep) {
               if (rep == 1) HIPCHK(hipEventRecord(e0, s));
+              if (c == 8) {
+                const int st3 = launch_part(bc, cand[c], false);
+                if (st3) return st3;
+                continue;
+              }
               void* kargs[] = {&bc};
               HIPCHK(hipLaunchKernel(cand[c].fn, dim3(gc), dim3(cand[c].bt), kargs,
                                      cand[c].lds, s));
@@ -2878,6 +3166,12 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
       grid = cc.F > 1 ? (unsigned)(gpad * cc.F) : (unsigned)ba.ndm;
       nr = pick == 2 ? npl * nf : (pick >= 4 && pick <= 6) ? 1 : npl;
       launches = 1;
+      if (pick == 8) {
+        HIPCHK(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_x));
+        const int st3 = launch_part(ba, cc, true);
+        if (st3) return st3;
+        part_done = true;
+      }
     }
     if (!H.band_ext) {
 #define SEMX_PICK(NB_, NP_, BT_, D_)                                        \
@@ -2895,7 +3189,7 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
 #undef SEMX_PICK
     }
     if (!kfn && H.coop_w == 0) return SEMX_E_UNSUPPORTED;
-    bool done = false;
+    bool done = part_done;
     if (H.coop_w > 0) {
       // cooperative sweep: all fields' planes of a mode in one launch
       const int nrc = npl * nf;
@@ -3093,20 +3387,20 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
   return SEMX_OK;
 }
 
-// The cooperative band sweeps set a sticky error word when a bounded
+// The cooperative and partitioned band sweeps set a sticky error word when a bounded
 // hand-off wait expires (their result is then wrong).  Every host
 // synchronisation point of the solve path reads it here, after the stream
 // has drained, and clears it, so a timeout is reported exactly once, by the
 // first check after it happened, whatever the caller passed for iters.
 int helm_check(Helm& H, hipStream_t s) {
   HIPCHK(hipStreamSynchronize(s));
-  if (H.coop_w <= 0 || !H.d.err) return SEMX_OK;
+  if ((H.coop_w <= 0 && H.npart <= 1) || !H.d.err) return SEMX_OK;
   int e = 0;
   HIPCHK(hipMemcpy(&e, H.d.err, sizeof(int), hipMemcpyDeviceToHost));
   if (!e) return SEMX_OK;
   const int zero = 0;
   HIPCHK(hipMemcpy(H.d.err, &zero, sizeof(int), hipMemcpyHostToDevice));
-  fprintf(stderr, "semx: cooperative band solve: hand-off wait timed out\n");
+  fprintf(stderr, "semx: cooperative or partitioned band solve: hand-off wait timed out\n");
   return SEMX_E_TIMEOUT;
 }
 
@@ -3299,6 +3593,15 @@ int semx_helm_info(semx_helm helm, int* info) {
   info[4] = H.ldab; info[5] = H.nb;
   info[6] = (int)(bytes / 1000);
   info[7] = H.modes.empty() ? 0 : H.modes[0].singular;
+  return SEMX_OK;
+}
+
+int semx_helm_sweep_info(semx_helm helm, int* info) {
+  if (!helm || !info) return SEMX_E_ARG;
+  const Helm& H = helm->H;
+  info[0] = H.npart;
+  for (int f = 1; f <= SEMX_MAX_FIELDS; ++f) info[f] = H.band_pick[f];
+  info[4] = H.coop_w;
   return SEMX_OK;
 }
 

@@ -201,3 +201,43 @@ def test_config0_square8_np8(cuda):
     print(f"[config0] rel L2 vs oracle {r:.3e}, |u-u*|_inf {err:.3e}", flush=True)
     assert r <= TOL
     assert err < 1e-6
+
+
+PART_CASES = [
+    # name, nx, np, nz, lambda2, beta, distort, bc, partitions (None: the default count)
+    ("natural-2d-p2", 8, 8, 1, 2.5, 1.0, 0.0, ("D", "N", "D", "N"), 2),
+    ("natural-2d-p5", 8, 8, 1, 2.5, 1.0, 0.0, ("D", "N", "D", "N"), 5),
+    ("distorted-3d-p3", 8, 6, 8, 1.0, 1.3, 0.3, ("D", "N", "D", "D"), 3),
+    ("uniform16-np9-nz64-default", 16, 9, 64, 20000.0, 1.0, 0.0, ("D", "D", "D", "D"), None),
+    ("periodic12-np7-singular-p3", 12, 7, 4, 0.0, 1.0, 0.0, PERIODIC, 3),
+    ("odd-plane-9x9-p2", 9, 7, 8, 2.0, 1.0, 0.2, ("D", "N", "D", "D"), 2),
+]
+
+
+@pytest.mark.parametrize("case", PART_CASES, ids=[c[0] for c in PART_CASES])
+def test_partitioned_band_sweep(cuda, case, monkeypatch):
+    """The partitioned sweep (each mode's band cut into row ranges solved
+    concurrently and joined through spikes) forced with SEMX_AUTOTUNE=8:
+    relative L2 <= 1e-12 vs the oracle, as the whole-band sweeps; the
+    singular periodic mode 0 up to its pinned constant."""
+    import torch
+    name, nx, np_, nz, lam2, beta, dist, bc, parts = case
+    monkeypatch.setenv("SEMX_AUTOTUNE", "8")
+    if parts is None:
+        monkeypatch.delenv("SEMX_BAND_PARTS", raising=False)
+    else:
+        monkeypatch.setenv("SEMX_BAND_PARTS", str(parts))
+    print(f"[partitioned sweep] {name}", flush=True)
+    P, Q, f, bcl, ue = make_case(nx, np_, nz, lam2, beta, dist, bc)
+    S = sx.ModalMatrixSys(P, lam2, beta, 0, max(nz // 2, 1), nz, sx.DIRECT)
+    info = S.sweep_info()
+    assert info["npart"] == (parts or 4), info
+    ug, _ = gpu_solve(torch, P, S, f, bcl)
+    assert S.sweep_info()["pick"][1] == 8
+    uo, _ = oracle_solve(Q, nz, lam2, beta, f, bcl)
+    if S.info["singular0"]:
+        drop_mode0_constant(P, ug, uo)
+    r = rel(ug[:, :P.nPlane], uo[:, :P.nPlane])
+    print(f"  {info['npart']} partitions, nsolve {P.nsolve}, nband {P.nband}: "
+          f"rel L2 gpu vs oracle = {r:.3e}", flush=True)
+    assert r <= TOL
