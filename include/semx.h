@@ -168,11 +168,12 @@ int semx_helm_destroy (semx_helm helm);
  *           bytes_per_solve_factor(MB*1000, int) singular(mode0)           */
 int semx_helm_info (semx_helm helm, int* info);
 /* How the direct planes' band sweeps run; no counterpart in semtex (whose
- * dpbtrs is one call).  info[5] = partitions per mode of the partitioned
- * sweep (1: not available), the sweep variant the first call picked for 1,
- * 2 and 3 fields (0 before that call; 1-7 one workgroup per mode, field or
- * plane over the whole band, 8 the partitioned sweep), workgroups per mode
- * of the cooperative sweep (0: not used).                                 */
+ * dpbtrs is one call) -- and how the PCG planes run.  info[6] = partitions
+ * per mode of the partitioned sweep (1: not available), the sweep variant
+ * the first call picked for 1, 2 and 3 fields (0 before that call; 1-7 one
+ * workgroup per mode, field or plane over the whole band, 8 the
+ * partitioned sweep), workgroups per mode of the cooperative sweep (0: not
+ * used), workgroups per PCG plane in the last solve.                      */
 int semx_helm_sweep_info (semx_helm helm, int* info);
 /* Host copies of one element's condensed matrices for mode index m
  * (HelmholtzSC, src/element.cpp:131-229): hbb next^2, hbi next*nint,

@@ -163,11 +163,12 @@ class ModalMatrixSys:
         return hbb, hbi[:, :M.nint], hii[:M.nint, :M.nint]
 
     def sweep_info(self):
-        """Band-sweep layout: dict(npart, pick={1: , 2: , 3: }, coop_w)."""
-        info = np.zeros(5, dtype=np.int32)
+        """Band-sweep and PCG layout: dict(npart, pick={1: , 2: , 3: },
+        coop_w, pcg_w)."""
+        info = np.zeros(6, dtype=np.int32)
         check(load().semx_helm_sweep_info(self.h, _ip(info)), "semx_helm_sweep_info")
         return dict(npart=int(info[0]), pick={f: int(info[f]) for f in (1, 2, 3)},
-                    coop_w=int(info[4]))
+                    coop_w=int(info[4]), pcg_w=int(info[5]))
 
     def __del__(self):
         try:

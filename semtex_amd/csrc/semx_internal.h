@@ -113,6 +113,10 @@ struct HelmDev {
   long long* spk_off = nullptr;
   int* prow = nullptr;
   double* pslot = nullptr;
+  // PCG planes with several workgroups each: barrier counters (16 per
+  // plane, one used) and partial-sum slots (2 x 8 x 2 per plane)
+  unsigned* pctr = nullptr;
+  double* pslot2 = nullptr;
 };
 
 struct Helm {
@@ -154,6 +158,8 @@ struct Helm {
   std::vector<int> dmode, mode_plane;
   std::vector<int> pcg_plane;         // plane-list indices of JACPCG modes
   int pcg_prec = 0;                   // SEMX_PCG_FP64, _FP32_REFINED or _BF16_REFINED
+  int pcg_w = 1;                      // workgroups per PCG plane (last solve)
+  size_t pcg_ws = 0;                  // plane-workgroup slots allocated
   int pcg_opk = -1;                   // element operator of the PCG planes (block-wide
                                       // or wavefront-level scalar, BF16 matrix cores),
                                       // -1 until chosen on the first solve

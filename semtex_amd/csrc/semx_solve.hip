@@ -995,6 +995,11 @@ __device__ inline double coop_take(const double* p, double v, int* abort) {
   return v;
 }
 
+__global__ void k_zero_u32(unsigned* p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
 __global__ void k_coop_fill(double* p, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = __longlong_as_double((long long)COOP_SENT);
@@ -1845,6 +1850,10 @@ struct PcgArgs {
   double inner_tol;       // inner relative tolerance with refinement
   int max_outer;
   int* iters;             // per PCG plane: iterations done (negative: limit hit)
+  int W;                  // workgroups per plane
+  unsigned* pctr;         // per plane: barrier arrivals (zero at launch)
+  double* pslot;          // per plane: 2 x W x 2 partial sums
+  int* err;               // set when a barrier wait timed out
 };
 
 constexpr int PCG_THREADS = 1024;
@@ -1862,6 +1871,92 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 #pragma unroll
   for (int w = 0; w < PCG_THREADS / 64; ++w) t += red[w];
   return t;
+}
+
+// Workgroups sharing one plane's iteration.  W = 1: the whole plane in one
+// workgroup (every range below is the full one, part_sync is a block
+// barrier).  W > 1: workgroup w owns the vector rows [vlo, vhi), the
+// elements [elo, ehi) of the operator and the global rows [glo, ghi) of its
+// assembly; the W workgroups meet at a grid barrier (an arrival counter in
+// device memory, agent-scope fences so that their vector writes are seen
+// across XCDs) before every gather of another workgroup's data, and dot
+// products add the W workgroups' partial sums in workgroup order, so all
+// of them take the same value.  Every wait is bounded: a timeout sets the
+// error word and the workgroup stops waiting (its results are then wrong
+// and the error is reported by the next check).
+struct PcgPart {
+  int W = 1, w = 0;
+  int vlo = 0, vhi = 0;       // vector rows
+  int elo = 0, ehi = 0;       // elements of the operator
+  int glo = 0, ghi = 0;       // assembled global rows
+  unsigned* ctr = nullptr;    // arrivals at this plane's barriers
+  double* slot = nullptr;     // [2 parities][W][2] partial sums
+  int* err = nullptr;
+  unsigned epoch = 0;         // barriers passed
+};
+
+__device__ inline PcgPart part_of(int W, int w, int npts, int nel, int nglobal, unsigned* ctr,
+                                  double* slot, int* err) {
+  PcgPart P;
+  P.W = W; P.w = w;
+  P.vlo = (int)((long long)npts * w / W); P.vhi = (int)((long long)npts * (w + 1) / W);
+  P.elo = (int)((long long)nel * w / W); P.ehi = (int)((long long)nel * (w + 1) / W);
+  P.glo = (int)((long long)nglobal * w / W); P.ghi = (int)((long long)nglobal * (w + 1) / W);
+  P.ctr = ctr; P.slot = slot; P.err = err;
+  return P;
+}
+
+constexpr long PCG_SPIN = 1L << 22;    // polls before a barrier wait gives up
+
+// A barrier of the plane's workgroups (a block barrier when W = 1).  red:
+// the workgroup's reduction scratch (entry 24 carries a timeout to every
+// thread).
+__device__ __forceinline__ void part_sync(PcgPart& P, double* red) {
+  __syncthreads();
+  if (P.W == 1) return;
+  if (threadIdx.x == 0) {
+    __threadfence();
+    atomicAdd(P.ctr, 1u);
+    const unsigned target = (P.epoch + 1) * (unsigned)P.W;
+    long it = 0;
+    bool late = false;
+    while (__hip_atomic_load(P.ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (++it > PCG_SPIN) { late = true; break; }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __threadfence();
+    if (late) __hip_atomic_store(P.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    red[24] = late ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  P.epoch += 1;
+  if (red[24] != 0.0) P.W = 1;    // stop waiting (the error word is set)
+}
+
+// sums of v and u over the threads of every workgroup of the plane
+__device__ __forceinline__ double2 psum2(double v, double u, double* red, PcgPart& P) {
+  const double a = block_sum(v, red);
+  const double b = block_sum(u, red);
+  if (P.W == 1) return make_double2(a, b);
+  double* sl = P.slot + (size_t)(P.epoch & 1) * P.W * 2;
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(sl + 2 * P.w, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(sl + 2 * P.w + 1, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  const int W = P.W;
+  part_sync(P, red);
+  if (P.W == 1) return make_double2(a, b);   // timed out
+  double x = 0.0, y = 0.0;
+  for (int k = 0; k < W; ++k) {
+    x += __hip_atomic_load(sl + 2 * k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    y += __hip_atomic_load(sl + 2 * k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return make_double2(x, y);
+}
+
+__device__ __forceinline__ double psum(double v, double* red, PcgPart& P) {
+  if (P.W == 1) return block_sum(v, red);
+  return psum2(v, 0.0, red, P).x;
 }
 
 // What the matrix-free operator of one plane needs.
@@ -1887,15 +1982,16 @@ struct PcgOp {
 // issued before the value loads: the sums are the same as one row at a time.
 template <typename T, bool MASK>
 __device__ __forceinline__ double pcg_assemble(const PcgOp& a, int n, const T* __restrict__ lb,
-                                      T* __restrict__ y, const T* __restrict__ w) {
+                                      T* __restrict__ y, const T* __restrict__ w,
+                                      const PcgPart& P) {
   const int tid = threadIdx.x, ng = a.nglobal;
   double wy = 0.0;
-  for (int g0 = tid; g0 < ng; g0 += PCG_THREADS * PCG_VU) {
+  for (int g0 = P.glo + tid; g0 < P.ghi; g0 += PCG_THREADS * PCG_VU) {
     int c0[PCG_VU], c1[PCG_VU], ix[PCG_VU][4];
 #pragma unroll
     for (int u = 0; u < PCG_VU; ++u) {
       const int gI = g0 + u * PCG_THREADS;
-      const bool live = gI < ng && (!MASK || gI < n);
+      const bool live = gI < P.ghi && (!MASK || gI < n);
       const int gc = min(gI, ng - 1);
       c0[u] = a.asm_ptr[gc];
       c1[u] = live ? a.asm_ptr[gc + 1] : c0[u];
@@ -1917,7 +2013,7 @@ __device__ __forceinline__ double pcg_assemble(const PcgOp& a, int n, const T* _
       for (int k = 0; k < 4; ++k)
         if (ix[u][k] >= 0) v += lv[u][k];
       for (int c = c0[u] + 4; c < c1[u]; ++c) v += lb[a.asm_idx[c]];
-      if (gI < ng) {
+      if (gI < P.ghi) {
         y[gI] = v;
         wy += (double)w[gI] * (double)v;
       }
@@ -1933,7 +2029,7 @@ __device__ __forceinline__ double pcg_assemble(const PcgOp& a, int n, const T* _
 template <typename T, bool MASK = true>
 __device__ __forceinline__ double pcg_matvec(const PcgOp& a, int n, const T* __restrict__ x,
                              T* __restrict__ y, T* __restrict__ lb, const T* __restrict__ w,
-                             T* sh, double* red) {
+                             T* sh, double* red, PcgPart& P) {
   const int tid = threadIdx.x, np = a.np, npnp = a.npnp, next = a.next, nint = a.nint;
   const int ce = PCG_THREADS / npnp;
   const T hcon = (T)a.hcon;
@@ -1952,7 +2048,7 @@ __device__ __forceinline__ double pcg_matvec(const PcgOp& a, int n, const T* __r
   const int pm = a.pmap[min(pt, npnp - 1)];
   // x was just written by other threads (the caller's vector pass): every
   // gather below comes after this barrier
-  __syncthreads();
+  part_sync(P, red);
   // gather pipeline: the global index of this thread's node two passes
   // ahead, its value one pass ahead (-1: masked, value zero)
   auto gidx = [&](int e0) {
@@ -1963,12 +2059,12 @@ __device__ __forceinline__ double pcg_matvec(const PcgOp& a, int n, const T* __r
     }
     return a.nglobal + e * nint + (pm - next);
   };
-  int ia = gidx(0), ib = gidx(ce);
+  int ia = gidx(P.elo), ib = gidx(P.elo + ce);
   T vnext = ia >= 0 ? x[ia] : (T)0;
   double wy = 0.0;
-  for (int e0 = 0; e0 < a.nel; e0 += ce) {
+  for (int e0 = P.elo; e0 < P.ehi; e0 += ce) {
     const int e = e0 + el;
-    const bool act = el < ce && e < a.nel;
+    const bool act = el < ce && e < P.ehi;
     const int base = el * npnp;
     const T v = vnext;
     __syncthreads();
@@ -2007,9 +2103,9 @@ __device__ __forceinline__ double pcg_matvec(const PcgOp& a, int n, const T* __r
       }
     }
   }
-  __syncthreads();
-  wy += pcg_assemble<T, MASK>(a, n, lb, y, w);
-  return block_sum(wy, red);
+  part_sync(P, red);
+  wy += pcg_assemble<T, MASK>(a, n, lb, y, w, P);
+  return psum(wy, red, P);
 }
 
 // PCG iterations on A x = r (r holds the residual of the initial x and is
@@ -2028,7 +2124,7 @@ constexpr int PCG_NPT = 3;                    // nodes per lane (np <= 13)
 template <typename T, bool MASK = true>
 __device__ __forceinline__ double pcg_matvec_wave(const PcgOp& a, int n, const T* __restrict__ x,
                                   T* __restrict__ y, T* __restrict__ lb,
-                                  const T* __restrict__ w, T* sh, double* red) {
+                                  const T* __restrict__ w, T* sh, double* red, PcgPart& P) {
   const int tid = threadIdx.x, np = a.np, npnp = a.npnp, next = a.next, nint = a.nint;
   const int lane = tid & 63, wave = tid >> 6;
   const T hcon = (T)a.hcon;
@@ -2038,7 +2134,7 @@ __device__ __forceinline__ double pcg_matvec_wave(const PcgOp& a, int n, const T
   T* sU = sT + npnp + wave * 3 * npnp;
   T* sR = sU + npnp;
   T* sS = sR + npnp;
-  __syncthreads();   // the previous use of the shared operators is complete
+  part_sync(P, red);   // x is complete, the previous use of the operators too
   if (tid < npnp) {
     sD[tid] = (T)a.ops[(svv ? 2 : 0) * npnp + tid];
     sT[tid] = (T)a.ops[(svv ? 3 : 1) * npnp + tid];
@@ -2068,10 +2164,10 @@ __device__ __forceinline__ double pcg_matvec_wave(const PcgOp& a, int n, const T
   double wy = 0.0;
   T cur[PCG_NPT], nxt[PCG_NPT];
   int gi[PCG_NPT];
-  gidx(wave, gi);
+  gidx(P.elo + wave, gi);
   gval(gi, cur);
-  gidx(wave + PCG_WAVES, gi);
-  for (int e = wave; e < a.nel; e += PCG_WAVES) {
+  gidx(P.elo + wave + PCG_WAVES, gi);
+  for (int e = P.elo + wave; e < P.ehi; e += PCG_WAVES) {
     gval(gi, nxt);
     gidx(e + 2 * PCG_WAVES, gi);
 #pragma unroll
@@ -2123,9 +2219,9 @@ __device__ __forceinline__ double pcg_matvec_wave(const PcgOp& a, int n, const T
 #pragma unroll
     for (int k = 0; k < PCG_NPT; ++k) cur[k] = nxt[k];
   }
-  __syncthreads();
-  wy += pcg_assemble<T, MASK>(a, n, lb, y, w);
-  return block_sum(wy, red);
+  part_sync(P, red);
+  wy += pcg_assemble<T, MASK>(a, n, lb, y, w, P);
+  return psum(wy, red, P);
 }
 
 // The operator on the BF16 matrix cores (SEMX_PCG_BF16_REFINED): each
@@ -2161,7 +2257,7 @@ __device__ inline mf_bf4 bf16x4(float a, float b, float c, float d) {
 template <bool MASK = true>
 __device__ __forceinline__ double pcg_matvec_mfma(const PcgOp& a, int n, const float* __restrict__ x,
                                   float* __restrict__ y, float* __restrict__ lb,
-                                  const float* __restrict__ w, float* sh, double* red) {
+                                  const float* __restrict__ w, float* sh, double* red, PcgPart& P) {
   const int tid = threadIdx.x, np = a.np, npnp = a.npnp, next = a.next, nint = a.nint;
   const int lane = tid & 63, wave = tid >> 6;
   const int c = lane & 15, rb = 4 * (lane >> 4);
@@ -2215,10 +2311,10 @@ __device__ __forceinline__ double pcg_matvec_mfma(const PcgOp& a, int n, const f
   int gi[4];
   // x was just written by other threads (the caller's vector pass): every
   // gather comes after this barrier
-  __syncthreads();
-  gidx(wave, gi);
+  part_sync(P, red);
+  gidx(P.elo + wave, gi);
   gval(gi, cur);
-  gidx(wave + PCG_WAVES, gi);
+  gidx(P.elo + wave + PCG_WAVES, gi);
   // geometric factors of this lane's nodes: loaded once when every element
   // shares one set, per element otherwise
   float q1[4], q2[4], q3[4], q4[4];
@@ -2235,7 +2331,7 @@ __device__ __forceinline__ double pcg_matvec_mfma(const PcgOp& a, int n, const f
   };
   const bool one_set = a.nset == 1;
   if (one_set) qload(0);
-  for (int e = wave; e < a.nel; e += PCG_WAVES) {
+  for (int e = P.elo + wave; e < P.ehi; e += PCG_WAVES) {
     gval(gi, nxt);
     gidx(e + 2 * PCG_WAVES, gi);
     if (!one_set) qload(e);
@@ -2276,9 +2372,9 @@ __device__ __forceinline__ double pcg_matvec_mfma(const PcgOp& a, int n, const f
 #pragma unroll
     for (int v = 0; v < 4; ++v) cur[v] = nxt[v];
   }
-  __syncthreads();
-  wy += pcg_assemble<float, MASK>(a, n, lb, y, w);
-  return block_sum(wy, red);
+  part_sync(P, red);
+  wy += pcg_assemble<float, MASK>(a, n, lb, y, w, P);
+  return psum(wy, red, P);
 }
 
 // Element-operator variants: the block-wide and wavefront-level scalar
@@ -2287,29 +2383,32 @@ enum { OPK_BLOCK = 0, OPK_WAVE = 1, OPK_MFMA = 2 };
 
 template <typename T, bool MASK, int OPK>
 __device__ __forceinline__ double pcg_op(const PcgOp& a, int n, const T* x, T* y, T* lb, const T* w,
-                                T* sh, double* red) {
+                                T* sh, double* red, PcgPart& P) {
   if constexpr (OPK == OPK_MFMA) {
     static_assert(std::is_same<T, float>::value, "the BF16 operator works on FP32 vectors");
-    return pcg_matvec_mfma<MASK>(a, n, x, y, lb, w, sh, red);
+    return pcg_matvec_mfma<MASK>(a, n, x, y, lb, w, sh, red, P);
   } else if constexpr (OPK == OPK_WAVE) {
-    return pcg_matvec_wave<T, MASK>(a, n, x, y, lb, w, sh, red);
+    return pcg_matvec_wave<T, MASK>(a, n, x, y, lb, w, sh, red, P);
   } else {
-    return pcg_matvec<T, MASK>(a, n, x, y, lb, w, sh, red);
+    return pcg_matvec<T, MASK>(a, n, x, y, lb, w, sh, red, P);
   }
 }
 
 template <typename T, int OPK = OPK_BLOCK>
 __device__ __forceinline__ int pcg_core(const PcgOp& a, int n, T* x, T* r, T* p, T* q, T* lb, const T* pc,
-                        double epsb2, int step_max, T* sh, double* red) {
+                        double epsb2, int step_max, T* sh, double* red, PcgPart& P) {
   const int tid = threadIdx.x, npts = a.npts;
   double r2 = 0.0, rho1 = 0.0, rho2 = 1.0;
-  for (int i = tid; i < npts; i += PCG_THREADS) {
+  for (int i = P.vlo + tid; i < P.vhi; i += PCG_THREADS) {
     const double ri = (double)r[i];
     r2 += ri * ri;
     rho1 += ri * (double)pc[i] * ri;
   }
-  r2 = block_sum(r2, red);
-  rho1 = block_sum(rho1, red);
+  {
+    const double2 t = psum2(r2, rho1, red, P);
+    r2 = t.x;
+    rho1 = t.y;
+  }
   int it = 0;
   // The vector passes take PCG_VU entries per thread at a time (i = tid +
   // (g PCG_VU + u) PCG_THREADS, the same order as one at a time, so the
@@ -2317,11 +2416,11 @@ __device__ __forceinline__ int pcg_core(const PcgOp& a, int n, T* x, T* r, T* p,
   // memory latency per group instead of one per entry.
   while (r2 > epsb2 && ++it < step_max) {
     const T beta = (it == 1) ? (T)0 : (T)(rho1 / rho2);
-    for (int i0 = tid; i0 < npts; i0 += PCG_THREADS * PCG_VU) {
+    for (int i0 = P.vlo + tid; i0 < P.vhi; i0 += PCG_THREADS * PCG_VU) {
       T cv[PCG_VU], rv[PCG_VU], pv[PCG_VU];
 #pragma unroll
       for (int u = 0; u < PCG_VU; ++u) {
-        const int i = min(i0 + u * PCG_THREADS, npts - 1);
+        const int i = min(i0 + u * PCG_THREADS, P.vhi - 1);
         cv[u] = pc[i];
         rv[u] = r[i];
         pv[u] = p[i];
@@ -2330,17 +2429,17 @@ __device__ __forceinline__ int pcg_core(const PcgOp& a, int n, T* x, T* r, T* p,
       for (int u = 0; u < PCG_VU; ++u) {
         const int i = i0 + u * PCG_THREADS;
         const T z = cv[u] * rv[u];
-        if (i < npts) p[i] = (it == 1) ? z : z + beta * pv[u];
+        if (i < P.vhi) p[i] = (it == 1) ? z : z + beta * pv[u];
       }
     }
-    const double pq = pcg_op<T, true, OPK>(a, n, p, q, lb, p, sh, red);
+    const double pq = pcg_op<T, true, OPK>(a, n, p, q, lb, p, sh, red, P);
     const T alpha = (T)(rho1 / pq);
     double nr2 = 0.0, nrho = 0.0;
-    for (int i0 = tid; i0 < npts; i0 += PCG_THREADS * PCG_VU) {
+    for (int i0 = P.vlo + tid; i0 < P.vhi; i0 += PCG_THREADS * PCG_VU) {
       T xv[PCG_VU], pv[PCG_VU], rv[PCG_VU], qv[PCG_VU], cv[PCG_VU];
 #pragma unroll
       for (int u = 0; u < PCG_VU; ++u) {
-        const int i = min(i0 + u * PCG_THREADS, npts - 1);
+        const int i = min(i0 + u * PCG_THREADS, P.vhi - 1);
         xv[u] = x[i];
         pv[u] = p[i];
         rv[u] = r[i];
@@ -2350,7 +2449,7 @@ __device__ __forceinline__ int pcg_core(const PcgOp& a, int n, T* x, T* r, T* p,
 #pragma unroll
       for (int u = 0; u < PCG_VU; ++u) {
         const int i = i0 + u * PCG_THREADS;
-        if (i < npts) {
+        if (i < P.vhi) {
           x[i] = xv[u] + alpha * pv[u];
           const T ri = rv[u] - alpha * qv[u];
           r[i] = ri;
@@ -2360,8 +2459,9 @@ __device__ __forceinline__ int pcg_core(const PcgOp& a, int n, T* x, T* r, T* p,
       }
     }
     rho2 = rho1;
-    r2 = block_sum(nr2, red);
-    rho1 = block_sum(nrho, red);
+    const double2 t = psum2(nr2, nrho, red, P);
+    r2 = t.x;
+    rho1 = t.y;
   }
   __syncthreads();
   return (it == step_max) ? -it : it;
@@ -2378,15 +2478,16 @@ template <typename T, bool REFINE, int OPK = OPK_BLOCK, int OP64 = OPK_BLOCK>
 __device__ __forceinline__ int pcg_run(const PcgOp& op, int n, double* X, double* R, const double* B,
                        double* P, double* Q, double* LB, const double* pc, const float* pc32,
                        double epsb2, int step_max, double inner_tol, int max_outer, T* sh,
-                       double* red) {
+                       double* red, PcgPart& pt) {
   const int tid = threadIdx.x, npts = op.npts;
   __syncthreads();
-  pcg_op<double, true, OP64>(op, n, X, Q, LB, X, (double*)sh, red);
-  for (int i = tid; i < npts; i += PCG_THREADS) R[i] = B[i] - Q[i];
+  pcg_op<double, true, OP64>(op, n, X, Q, LB, X, (double*)sh, red, pt);
+  for (int i = pt.vlo + tid; i < pt.vhi; i += PCG_THREADS) R[i] = B[i] - Q[i];
   __syncthreads();
   int its;
   if constexpr (!REFINE) {
-    its = pcg_core<double, OP64>(op, n, X, R, P, Q, LB, pc, epsb2, step_max, (double*)sh, red);
+    its = pcg_core<double, OP64>(op, n, X, R, P, Q, LB, pc, epsb2, step_max, (double*)sh, red,
+                                  pt);
   } else {
     T* d = (T*)P;
     T* rr = d + npts;
@@ -2396,23 +2497,23 @@ __device__ __forceinline__ int pcg_run(const PcgOp& op, int n, double* X, double
     its = 0;
     for (int outer = 0;; ++outer) {
       double rn = 0.0;
-      for (int i = tid; i < npts; i += PCG_THREADS) rn += R[i] * R[i];
-      rn = block_sum(rn, red);
+      for (int i = pt.vlo + tid; i < pt.vhi; i += PCG_THREADS) rn += R[i] * R[i];
+      rn = psum(rn, red, pt);
       if (rn <= epsb2) break;
       if (outer == max_outer) { its = -its; break; }
-      for (int i = tid; i < npts; i += PCG_THREADS) {
+      for (int i = pt.vlo + tid; i < pt.vhi; i += PCG_THREADS) {
         d[i] = (T)0;
         rr[i] = (T)R[i];
       }
       __syncthreads();
       const double ein = inner_tol * inner_tol * rn;
       const int it = pcg_core<T, OPK>(op, n, d, rr, pp, qq, lb, (const T*)pc32, ein, step_max,
-                                      sh, red);
+                                      sh, red, pt);
       its += (it < 0) ? -it : it;
-      for (int i = tid; i < npts; i += PCG_THREADS) X[i] += (double)d[i];
+      for (int i = pt.vlo + tid; i < pt.vhi; i += PCG_THREADS) X[i] += (double)d[i];
       __syncthreads();
-      pcg_op<double, true, OP64>(op, n, X, Q, LB, X, (double*)sh, red);
-      for (int i = tid; i < npts; i += PCG_THREADS) R[i] = B[i] - Q[i];
+      pcg_op<double, true, OP64>(op, n, X, Q, LB, X, (double*)sh, red, pt);
+      for (int i = pt.vlo + tid; i < pt.vhi; i += PCG_THREADS) R[i] = B[i] - Q[i];
       __syncthreads();
     }
   }
@@ -2425,14 +2526,14 @@ __device__ __forceinline__ int pcg_run(const PcgOp& op, int n, double* X, double
 // elements in order, src/element.cpp:323): the last entry of the node's
 // assembly list, which is in increasing element-storage order.
 __device__ __forceinline__ void pcg_x0(const PcgOp& op, const int* emap, int n, const double* uk,
-                              double* X) {
+                              double* X, const PcgPart& P) {
   const int tid = threadIdx.x, next = op.next, nint = op.nint;
-  for (int gI = tid; gI < op.nglobal; gI += PCG_THREADS) {
+  for (int gI = P.vlo + tid; gI < min(P.vhi, op.nglobal); gI += PCG_THREADS) {
     const int c = op.asm_idx[op.asm_ptr[gI + 1] - 1];
     const int e = c / next, b = c - (c / next) * next;
     X[gI] = (gI < n) ? uk[(size_t)e * op.npnp + emap[b]] : 0.0;
   }
-  for (int i = tid; i < op.nel * nint; i += PCG_THREADS) {
+  for (int i = max(P.vlo - op.nglobal, 0) + tid; i < P.vhi - op.nglobal; i += PCG_THREADS) {
     const int e = i / nint, q = i - (i / nint) * nint;
     X[op.nglobal + i] = uk[(size_t)e * op.npnp + emap[next + q]];
   }
@@ -2443,9 +2544,9 @@ __device__ __forceinline__ void pcg_x0(const PcgOp& op, const int* emap, int n, 
 // ess(gid) gives the value of essential node gid >= nsolve.
 template <class Ess>
 __device__ __forceinline__ void pcg_store(const PcgOp& op, int n, int nsolve, const double* X,
-                                 double* uk, Ess ess) {
+                                 double* uk, Ess ess, const PcgPart& P) {
   const int next = op.next, npnp = op.npnp;
-  for (int idx = threadIdx.x; idx < op.nel * npnp; idx += PCG_THREADS) {
+  for (int idx = P.elo * npnp + threadIdx.x; idx < P.ehi * npnp; idx += PCG_THREADS) {
     const int e = idx / npnp, q = idx - (idx / npnp) * npnp;
     const int pm = op.pmap[q];
     double v;
@@ -2465,7 +2566,8 @@ __global__ __launch_bounds__(PCG_THREADS) void k_pcg(PcgArgs a) {
   double* red = pcg_lds;               // 32 doubles
   T* sh = (T*)(pcg_lds + 32);
   const int tid = threadIdx.x;
-  const int pidx = a.pcg_plane[blockIdx.x];
+  const int slot = blockIdx.x / a.W, wg = blockIdx.x - (blockIdx.x / a.W) * a.W;
+  const int pidx = a.pcg_plane[slot];
   const int k = a.plane_of[pidx];
   const int m = k >> 1;
   const int n = a.mode_n[m];
@@ -2473,7 +2575,9 @@ __global__ __launch_bounds__(PCG_THREADS) void k_pcg(PcgArgs a) {
   const PcgOp op{a.np, a.npnp, next, a.nint, a.nel, nglobal, npts, a.hcon[m],
                  a.lambda2 > EPSDP, a.G, a.eset, a.pmap, a.btog, a.ops, a.asm_ptr, a.asm_idx,
                  a.nset};
-  double* wk = a.work + (size_t)blockIdx.x * a.workstride;
+  double* wk = a.work + (size_t)slot * a.workstride;
+  PcgPart PP = part_of(a.W, wg, npts, a.nel, nglobal, a.pctr + (size_t)slot * 16,
+                      a.pslot + (size_t)slot * 4 * a.W, a.err);
   double* X = wk;                 // FP64 solution
   double* R = X + npts;           // FP64 residual
   double* B = R + npts;           // FP64 right-hand side
@@ -2486,26 +2590,26 @@ __global__ __launch_bounds__(PCG_THREADS) void k_pcg(PcgArgs a) {
   // b: assembled -(M f + H g) + <h,w>, essential and pinned rows zero
   // (Field::buildRHS, src/field.cpp:859-925)
   double bb = 0.0;
-  for (int gI = tid; gI < nglobal; gI += PCG_THREADS) {
+  for (int gI = PP.vlo + tid; gI < min(PP.vhi, nglobal); gI += PCG_THREADS) {
     double v = 0.0;
     if (gI < n)
       for (int c = a.asm_ptr[gI]; c < a.asm_ptr[gI + 1]; ++c) v += fb[a.asm_idx[c]];
     B[gI] = v;
     bb += v * v;
   }
-  for (int i = tid; i < a.nel * a.nint; i += PCG_THREADS) {
+  for (int i = max(PP.vlo - nglobal, 0) + tid; i < PP.vhi - nglobal; i += PCG_THREADS) {
     B[nglobal + i] = fi[i];
     bb += fi[i] * fi[i];
   }
-  const double epsb = a.tol * sqrt(block_sum(bb, red));
-  pcg_x0(op, a.emap, n, uk, X);
+  const double epsb = a.tol * sqrt(psum(bb, red, PP));
+  pcg_x0(op, a.emap, n, uk, X, PP);
   constexpr int OP64 = (OPK == OPK_WAVE) ? OPK_WAVE : OPK_BLOCK;
   const int its = pcg_run<T, REFINE, OPK, OP64>(op, n, X, R, B, P, Q, LB, a.pc + (size_t)m * npts,
                                      a.pc32 + (size_t)m * npts, epsb * epsb, a.step_max,
-                                     a.inner_tol, a.max_outer, sh, red);
+                                     a.inner_tol, a.max_outer, sh, red, PP);
   const double* gp = a.g + (size_t)pidx * a.nzero;
-  pcg_store(op, n, a.nsolve, X, uk, [&](int gid) { return gp[gid - a.nsolve]; });
-  if (tid == 0) a.iters[blockIdx.x] = its;
+  pcg_store(op, n, a.nsolve, X, uk, [&](int gid) { return gp[gid - a.nsolve]; }, PP);
+  if (tid == 0 && wg == 0) a.iters[slot] = its;
 }
 
 // --- independent 2D problems, one workgroup each, in one launch ----------
@@ -2559,11 +2663,12 @@ __global__ __launch_bounds__(PCG_THREADS) void k_pcg_batch(PcgBatchArgs ba) {
   double* P = B + npts;
   double* Q = P + npts;
   double* LB = Q + npts;
+  PcgPart PP = part_of(1, 0, npts, op.nel, nglobal, nullptr, nullptr, nullptr);
   // lifting vector: essential values on essential nodes, zero elsewhere
   for (int i = tid; i < npts; i += PCG_THREADS)
     P[i] = (i >= nsolve && i < nglobal) ? bc[I.ess_src[i - nsolve]] : 0.0;
   __syncthreads();
-  pcg_op<double, false, OPK_WAVE>(op, n, P, Q, LB, P, (double*)sh, red);
+  pcg_op<double, false, OPK_WAVE>(op, n, P, Q, LB, P, (double*)sh, red, PP);
   // b = -(M f) - A g + <h,w>, rows of essential and pinned nodes zero
   double bb = 0.0;
   for (int gI = tid; gI < nglobal; gI += PCG_THREADS) {
@@ -2593,7 +2698,7 @@ __global__ __launch_bounds__(PCG_THREADS) void k_pcg_batch(PcgBatchArgs ba) {
     bb += v * v;
   }
   const double epsb = ba.tol * sqrt(block_sum(bb, red));
-  pcg_x0(op, I.emap, n, uk, X);
+  pcg_x0(op, I.emap, n, uk, X, PP);
   // the problem's element operator (uniform over the workgroup): the BF16
   // matrix-core one for FP32 inner iterations, or the scalar block-wide or
   // wavefront-level one
@@ -2603,7 +2708,7 @@ __global__ __launch_bounds__(PCG_THREADS) void k_pcg_batch(PcgBatchArgs ba) {
     if (I.opk == OPK_MFMA) {
       its = pcg_run<float, REFINE, OPK_MFMA, OPK_WAVE>(op, n, X, R, B, P, Q, LB, I.pc, I.pc32,
                                                        epsb * epsb, ba.step_max, ba.inner_tol,
-                                                       ba.max_outer, sh, red);
+                                                       ba.max_outer, sh, red, PP);
       done = true;
     }
   }
@@ -2611,12 +2716,12 @@ __global__ __launch_bounds__(PCG_THREADS) void k_pcg_batch(PcgBatchArgs ba) {
   } else if (I.opk == OPK_BLOCK)
     its = pcg_run<T, REFINE, OPK_BLOCK, OPK_BLOCK>(op, n, X, R, B, P, Q, LB, I.pc, I.pc32,
                                                    epsb * epsb, ba.step_max, ba.inner_tol,
-                                                   ba.max_outer, sh, red);
+                                                   ba.max_outer, sh, red, PP);
   else
     its = pcg_run<T, REFINE, OPK_WAVE, OPK_WAVE>(op, n, X, R, B, P, Q, LB, I.pc, I.pc32,
                                                  epsb * epsb, ba.step_max, ba.inner_tol,
-                                                 ba.max_outer, sh, red);
-  pcg_store(op, n, nsolve, X, uk, [&](int gid) { return bc[I.ess_src[gid - nsolve]]; });
+                                                 ba.max_outer, sh, red, PP);
+  pcg_store(op, n, nsolve, X, uk, [&](int gid) { return bc[I.ess_src[gid - nsolve]]; }, PP);
   if (tid == 0) ba.iters[blockIdx.x] = its;
 }
 
@@ -2825,6 +2930,8 @@ void helm_release(Helm& H) {
   dev_free(H.d.inbox); dev_free(H.d.part); dev_free(H.d.clo);
   dev_free(H.d.err);
   dev_free(H.d.spk); dev_free(H.d.spk_off); dev_free(H.d.prow); dev_free(H.d.pslot);
+  dev_free(H.d.pctr); dev_free(H.d.pslot2);
+  H.pcg_ws = 0;
   H.nf_cap = 0;
   H.on_device = false;
 }
@@ -3299,6 +3406,51 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
                              : (const void*)k_pcg<float, true, OPK_BLOCK>;
     };
     const size_t npcg = H.pcg_plane.size();
+    // Workgroups per plane: enough that the planes of one field fill the
+    // chip's CUs (each 1024-thread workgroup holds a CU), at most 8, each
+    // with at least 8192 vector rows, and only if the whole grid can be
+    // resident (the plane's workgroups wait for each other); SEMX_PCG_WG
+    // sets the count (1: one workgroup per plane).
+    int W = 1;
+    {
+      int cus = 256, dev = 0, per_cu = 0;
+      if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+      W = std::max(1, std::min(8, cus / (int)std::max<size_t>(npcg, 1)));
+      W = std::max(1, std::min(W, H.npts / 8192));
+      if (const char* e = getenv("SEMX_PCG_WG")) W = std::max(1, std::min(8, atoi(e)));
+      if (W > 1) {
+        const void* probe = pcg_fn(H.pcg_opk < 0 ? OPK_BLOCK : H.pcg_opk);
+        if (lds > 64 * 1024)
+          HIPCHK(hipFuncSetAttribute(probe, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)lds));
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, probe, PCG_THREADS, lds) !=
+                hipSuccess ||
+            (long)per_cu * cus < (long)npcg * W)
+          W = 1;
+      }
+      if (W > 1 && (H.pcg_ws < npcg * nf * (size_t)W || !H.d.pctr)) {
+        dev_free(H.d.pctr);
+        dev_free(H.d.pslot2);
+        int st2 = dev_upload<unsigned>(&H.d.pctr, nullptr, npcg * nf * 16);
+        if (!st2) st2 = dev_upload<double>(&H.d.pslot2, nullptr, npcg * nf * 4 * 8);
+        if (!st2 && !H.d.err) st2 = dev_upload<int>(&H.d.err, nullptr, 1);
+        if (st2) return st2;
+        H.pcg_ws = npcg * nf * (size_t)W;
+      }
+      H.pcg_w = W;
+    }
+    pa.W = W;
+    pa.pctr = H.d.pctr;
+    pa.pslot = H.d.pslot2;
+    pa.err = H.d.err;
+    // barrier counters back to zero before a launch (a fill kernel: the
+    // launch may be captured into a graph)
+    auto zero_ctr = [&](const PcgArgs& pp) {
+      if (W > 1)
+        hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((npcg * 16 + 255) / 256)), dim3(256), 0,
+                           s, pp.pctr, npcg * 16);
+    };
     if (H.pcg_opk < 0) {
       // The element operator of the system's PCG planes: the BF16 one for
       // SEMX_PCG_BF16_REFINED; otherwise the block-wide or the wavefront-
@@ -3333,8 +3485,9 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
             HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
           HIPCHK(hipMemcpyAsync(scr, us[0], nu * sizeof(double), hipMemcpyDeviceToDevice, s));
           void* kt[] = {&pt};
+          zero_ctr(pt);
           HIPCHK(hipEventRecord(e0, s));
-          HIPCHK(hipLaunchKernel(fn, dim3((unsigned)npcg), dim3(PCG_THREADS), kt, lds, s));
+          HIPCHK(hipLaunchKernel(fn, dim3((unsigned)(npcg * W)), dim3(PCG_THREADS), kt, lds, s));
           HIPCHK(hipEventRecord(e1, s));
           HIPCHK(hipEventSynchronize(e1));
           float ms = 0.f;
@@ -3360,9 +3513,14 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
       pf.u = us[i];
       pf.work = H.d.work + i * npcg * H.workstride;
       pf.iters = H.d.iters + i * npcg;
+      if (W > 1) {
+        pf.pctr = H.d.pctr + (size_t)i * npcg * 16;
+        pf.pslot = H.d.pslot2 + (size_t)i * npcg * 4 * 8;
+      }
       void* kargs[] = {&pf};
+      zero_ctr(pf);
       prof_begin(s, &ev);
-      HIPCHK(hipLaunchKernel(kfn, dim3((unsigned)npcg), dim3(PCG_THREADS), kargs, lds, s));
+      HIPCHK(hipLaunchKernel(kfn, dim3((unsigned)(npcg * W)), dim3(PCG_THREADS), kargs, lds, s));
       prof_end(s, "k_pcg", ev);
     }
     HIPCHK(hipGetLastError());
@@ -3394,7 +3552,7 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
 // first check after it happened, whatever the caller passed for iters.
 int helm_check(Helm& H, hipStream_t s) {
   HIPCHK(hipStreamSynchronize(s));
-  if ((H.coop_w <= 0 && H.npart <= 1) || !H.d.err) return SEMX_OK;
+  if ((H.coop_w <= 0 && H.npart <= 1 && H.pcg_w <= 1) || !H.d.err) return SEMX_OK;
   int e = 0;
   HIPCHK(hipMemcpy(&e, H.d.err, sizeof(int), hipMemcpyDeviceToHost));
   if (!e) return SEMX_OK;
@@ -3602,6 +3760,7 @@ int semx_helm_sweep_info(semx_helm helm, int* info) {
   info[0] = H.npart;
   for (int f = 1; f <= SEMX_MAX_FIELDS; ++f) info[f] = H.band_pick[f];
   info[4] = H.coop_w;
+  info[5] = H.pcg_w;
   return SEMX_OK;
 }
 

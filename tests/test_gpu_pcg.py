@@ -164,3 +164,41 @@ def test_base_mode_share(cuda, method):
     r = rel(ug[:, :P.nPlane], uo[:, :P.nPlane])
     print(f"  rel L2 vs oracle {r:.3e}", flush=True)
     assert r <= (1e-12 if method == sx.DIRECT else 1e-10)
+
+
+WG_CASES = [(c, W, prec) for c in (CASES[0], CASES[1], CASES[3]) for W in (2, 3, 4)
+            for prec in (sx.PCG_FP64, sx.PCG_FP32_REFINED, sx.PCG_BF16_REFINED)]
+
+
+@pytest.mark.parametrize("case,W,prec", WG_CASES,
+                         ids=[f"{c[0]}-w{W}-{['fp64', 'fp32r', 'bf16r'][p]}"
+                              for c, W, p in WG_CASES])
+def test_pcg_workgroups_per_plane(cuda, case, W, prec, monkeypatch):
+    """Each plane's iteration shared by W workgroups (rows, elements and
+    assembly rows split, grid barriers, partial sums added in workgroup
+    order), forced through SEMX_PCG_WG; converged at TOL_REL 1e-13: rel L2
+    <= 1e-10 vs the oracle DIRECT solution, iterations within 2 (+2%) of
+    one workgroup per plane, and (FP64) a step-limited solve reported as
+    such."""
+    import torch
+    name, nx, np_, nz, lam2, beta, dist, bc = case
+    print(f"[pcg workgroups] {name} W={W} prec {prec}", flush=True)
+    P, Q, f, bcl, ue = make_case(nx, np_, nz, lam2, beta, dist, bc)
+    its = {}
+    for w in (1, W):
+        monkeypatch.setenv("SEMX_PCG_WG", str(w))
+        S = sx.ModalMatrixSys(P, lam2, beta, 0, max(nz // 2, 1), nz, sx.JACPCG)
+        set_precision(S, prec)
+        ug, its[w] = gpu_solve(torch, P, S, f, bcl, tol=1e-13)
+        assert S.sweep_info()["pcg_w"] == w
+    ud, _ = oracle_solve(Q, nz, lam2, beta, f, bcl, method=sx.DIRECT)
+    if S.info["singular0"]:
+        drop_mode0_constant(P, ug, ud)
+    e = rel(solved(P, ug), solved(P, ud))
+    print(f"  iterations {its}, rel L2 vs DIRECT {e:.3e}", flush=True)
+    assert e <= 1e-10
+    assert abs(its[W] - its[1]) <= 2 + its[1] // 50
+    if prec == sx.PCG_FP64:     # the step limit is reported by every workgroup count
+        with pytest.warns(RuntimeWarning, match="step limit"):
+            gpu_solve(torch, P, S, f, bcl, tol=1e-13, step_max=3)
+    sx.helm_check(S)
