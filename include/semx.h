@@ -242,9 +242,12 @@ int semx_helm_set_pcg (semx_helm helm, int precision, double inner_tol,
  * No counterpart in semtex, which solves one problem per process: count
  * systems created with nz = 1, nmodes = 1 and method SEMX_JACPCG (each on
  * its own mesh, np, lambda2), all with the same semx_helm_set_pcg setting,
- * are solved together, one workgroup per problem; each workgroup does all
+ * are solved together in one launch; the workgroups of a problem do all
  * of Field::solve's JACPCG branch (src/field.cpp:564-725) for its plane:
  * -(M f + H g) + <h,w>, x0 from u, the iteration, essential re-imposition.
+ * A problem whose expected cost exceeds the batch's per-CU share gets
+ * several workgroups (up to 8; SEMX_BATCH_WG=1: one each), launched first;
+ * the rest one workgroup each, largest first.
  * f, bc, u: host arrays of count device pointers (bc[i] may be NULL when
  * mesh i has no boundary); iters as in semx_helm_solve.  The systems must
  * outlive the batch.                                                      */
@@ -257,7 +260,8 @@ int semx_batch_destroy (semx_batch batch);
  * and element count it times each operator the precision allows on the
  * class's first problem (40 iterations, into a scratch solution) and keeps
  * the fastest per iteration; it synchronises the stream, so make it before
- * capturing a graph.  SEMX_BATCH_OP=block|wave|mfma forces one operator.  */
+ * capturing a graph.  SEMX_BATCH_OP=block|wave|mfma forces one operator
+ * (the BF16 batch offers wave and mfma).                                 */
 int semx_batch_info (semx_batch batch, int* opk, double* tune_ms);
 int semx_batch_solve (semx_batch batch, const double* const* f,
                       const double* const* bc, double* const* u, double tol_rel,

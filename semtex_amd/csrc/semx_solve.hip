@@ -1860,13 +1860,25 @@ constexpr int PCG_THREADS = 1024;
 constexpr int PCG_VU = 4;     // vector-pass entries per thread per load group
 constexpr int PCG_TUNE_ITERS = 30;   // iterations per timed operator candidate
 
+// Block barrier of the PCG kernels, whose vectors live in global memory and
+// are written and read by different threads of the workgroup: this
+// wavefront's outstanding vector-memory operations are waited for first.
+// The workgroup barrier as compiled for gfx950 waits for LDS operations
+// only, and a global store still in flight when it releases can be missed
+// by a wavefront that loads the same address after it (seen as results
+// that differ from run to run).
+__device__ __forceinline__ void pcg_barrier() {
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+}
+
 __device__ __forceinline__ double block_sum(double v, double* red) {
 #pragma unroll
   for (int off = 32; off >= 1; off /= 2) v += __shfl_xor(v, off);
   const int wave = threadIdx.x >> 6;
-  __syncthreads();
+  pcg_barrier();
   if ((threadIdx.x & 63) == 0) red[wave] = v;
-  __syncthreads();
+  pcg_barrier();
   double t = 0.0;
 #pragma unroll
   for (int w = 0; w < PCG_THREADS / 64; ++w) t += red[w];
@@ -1903,6 +1915,10 @@ __device__ inline PcgPart part_of(int W, int w, int npts, int nel, int nglobal, 
   P.elo = (int)((long long)nel * w / W); P.ehi = (int)((long long)nel * (w + 1) / W);
   P.glo = (int)((long long)nglobal * w / W); P.ghi = (int)((long long)nglobal * (w + 1) / W);
   P.ctr = ctr; P.slot = slot; P.err = err;
+  // workgroup-uniform: kept in scalar registers
+  P.W = uni(P.W); P.w = uni(P.w);
+  P.vlo = uni(P.vlo); P.vhi = uni(P.vhi); P.elo = uni(P.elo); P.ehi = uni(P.ehi);
+  P.glo = uni(P.glo); P.ghi = uni(P.ghi);
   return P;
 }
 
@@ -1912,7 +1928,7 @@ constexpr long PCG_SPIN = 1L << 22;    // polls before a barrier wait gives up
 // the workgroup's reduction scratch (entry 24 carries a timeout to every
 // thread).
 __device__ __forceinline__ void part_sync(PcgPart& P, double* red) {
-  __syncthreads();
+  pcg_barrier();
   if (P.W == 1) return;
   if (threadIdx.x == 0) {
     __threadfence();
@@ -1928,7 +1944,7 @@ __device__ __forceinline__ void part_sync(PcgPart& P, double* red) {
     if (late) __hip_atomic_store(P.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     red[24] = late ? 1.0 : 0.0;
   }
-  __syncthreads();
+  pcg_barrier();
   P.epoch += 1;
   if (red[24] != 0.0) P.W = 1;    // stop waiting (the error word is set)
 }
@@ -2067,11 +2083,11 @@ __device__ __forceinline__ double pcg_matvec(const PcgOp& a, int n, const T* __r
     const bool act = el < ce && e < P.ehi;
     const int base = el * npnp;
     const T v = vnext;
-    __syncthreads();
+    pcg_barrier();
     if (act) sU[base + pt] = v;
     vnext = ib >= 0 ? x[ib] : (T)0;
     ib = gidx(e0 + 2 * ce);
-    __syncthreads();
+    pcg_barrier();
     const double* g = a.G + (size_t)(act ? a.eset[e] : 0) * G_N * npnp;
     if (act) {
       T r = 0, s2 = 0;
@@ -2084,7 +2100,7 @@ __device__ __forceinline__ double pcg_matvec(const PcgOp& a, int n, const T* __r
       sR[base + pt] = q1 * r + q3 * s2;
       sS[base + pt] = q2 * s2 + q3 * r;
     }
-    __syncthreads();
+    pcg_barrier();
     if (act) {
       T out = (T)g[G_Q4 * npnp + pt] * sU[base + pt] * hcon;
       T s1 = 0, s3 = 0;
@@ -2139,7 +2155,7 @@ __device__ __forceinline__ double pcg_matvec_wave(const PcgOp& a, int n, const T
     sD[tid] = (T)a.ops[(svv ? 2 : 0) * npnp + tid];
     sT[tid] = (T)a.ops[(svv ? 3 : 1) * npnp + tid];
   }
-  __syncthreads();
+  pcg_barrier();
   // this lane's nodes: pt = lane + 64 k; gather pipeline: global indices
   // two elements ahead, values one ahead (-1: masked, value zero)
   int pmv[PCG_NPT];
@@ -2463,7 +2479,7 @@ __device__ __forceinline__ int pcg_core(const PcgOp& a, int n, T* x, T* r, T* p,
     r2 = t.x;
     rho1 = t.y;
   }
-  __syncthreads();
+  pcg_barrier();
   return (it == step_max) ? -it : it;
 }
 
@@ -2480,10 +2496,10 @@ __device__ __forceinline__ int pcg_run(const PcgOp& op, int n, double* X, double
                        double epsb2, int step_max, double inner_tol, int max_outer, T* sh,
                        double* red, PcgPart& pt) {
   const int tid = threadIdx.x, npts = op.npts;
-  __syncthreads();
+  pcg_barrier();
   pcg_op<double, true, OP64>(op, n, X, Q, LB, X, (double*)sh, red, pt);
   for (int i = pt.vlo + tid; i < pt.vhi; i += PCG_THREADS) R[i] = B[i] - Q[i];
-  __syncthreads();
+  pcg_barrier();
   int its;
   if constexpr (!REFINE) {
     its = pcg_core<double, OP64>(op, n, X, R, P, Q, LB, pc, epsb2, step_max, (double*)sh, red,
@@ -2505,16 +2521,16 @@ __device__ __forceinline__ int pcg_run(const PcgOp& op, int n, double* X, double
         d[i] = (T)0;
         rr[i] = (T)R[i];
       }
-      __syncthreads();
+      pcg_barrier();
       const double ein = inner_tol * inner_tol * rn;
       const int it = pcg_core<T, OPK>(op, n, d, rr, pp, qq, lb, (const T*)pc32, ein, step_max,
                                       sh, red, pt);
       its += (it < 0) ? -it : it;
       for (int i = pt.vlo + tid; i < pt.vhi; i += PCG_THREADS) X[i] += (double)d[i];
-      __syncthreads();
+      pcg_barrier();
       pcg_op<double, true, OP64>(op, n, X, Q, LB, X, (double*)sh, red, pt);
       for (int i = pt.vlo + tid; i < pt.vhi; i += PCG_THREADS) R[i] = B[i] - Q[i];
-      __syncthreads();
+      pcg_barrier();
     }
   }
   return its;
@@ -2630,6 +2646,12 @@ struct PcgInst {
 };
 
 struct PcgBatchArgs {
+  // workgroup -> (problem, workgroup of the problem, workgroups of the
+  // problem, barrier slot); null: one workgroup per problem, in order
+  const int* wmap;
+  unsigned* pctr;           // per multi-workgroup problem: barrier arrivals
+  double* pslot;            // per multi-workgroup problem: 2 x 8 x 2 partial sums
+  int* err;
   const PcgInst* inst;
   const double* const* f;   // per instance: forcing plane
   const double* const* bc;  // per instance: BC lines (null without boundaries)
@@ -2650,28 +2672,32 @@ __global__ __launch_bounds__(PCG_THREADS) void k_pcg_batch(PcgBatchArgs ba) {
   double* red = pcg_lds;
   T* sh = (T*)(pcg_lds + 32);
   const int tid = threadIdx.x;
-  const PcgInst& I = ba.inst[blockIdx.x];
+  const int* wm = ba.wmap ? ba.wmap + 4 * blockIdx.x : nullptr;
+  const int pid = wm ? wm[0] : (int)blockIdx.x;
+  const int wg = wm ? wm[1] : 0, W = wm ? wm[2] : 1, sl = wm ? wm[3] : 0;
+  const PcgInst& I = ba.inst[pid];
   const PcgOp& op = I.op;
   const int npts = op.npts, nglobal = op.nglobal, next = op.next, nint = op.nint;
   const int npnp = op.npnp, np = op.np, n = I.n, nsolve = I.nsolve;
-  const double* f = ba.f[blockIdx.x];
-  const double* bc = ba.bc[blockIdx.x];
-  double* uk = ba.u[blockIdx.x];
+  const double* f = ba.f[pid];
+  const double* bc = ba.bc[pid];
+  double* uk = ba.u[pid];
   double* X = I.work;
   double* R = X + npts;
   double* B = R + npts;
   double* P = B + npts;
   double* Q = P + npts;
   double* LB = Q + npts;
-  PcgPart PP = part_of(1, 0, npts, op.nel, nglobal, nullptr, nullptr, nullptr);
+  PcgPart PP = part_of(W, wg, npts, op.nel, nglobal, ba.pctr + (size_t)sl * 16,
+                       ba.pslot + (size_t)sl * 32, ba.err);
   // lifting vector: essential values on essential nodes, zero elsewhere
-  for (int i = tid; i < npts; i += PCG_THREADS)
+  for (int i = PP.vlo + tid; i < PP.vhi; i += PCG_THREADS)
     P[i] = (i >= nsolve && i < nglobal) ? bc[I.ess_src[i - nsolve]] : 0.0;
-  __syncthreads();
+  pcg_barrier();
   pcg_op<double, false, OPK_WAVE>(op, n, P, Q, LB, P, (double*)sh, red, PP);
   // b = -(M f) - A g + <h,w>, rows of essential and pinned nodes zero
   double bb = 0.0;
-  for (int gI = tid; gI < nglobal; gI += PCG_THREADS) {
+  for (int gI = PP.vlo + tid; gI < min(PP.vhi, nglobal); gI += PCG_THREADS) {
     double v = 0.0;
     if (gI < n) {
       for (int c = op.asm_ptr[gI]; c < op.asm_ptr[gI + 1]; ++c) {
@@ -2690,14 +2716,14 @@ __global__ __launch_bounds__(PCG_THREADS) void k_pcg_batch(PcgBatchArgs ba) {
     B[gI] = v;
     bb += v * v;
   }
-  for (int i = tid; i < op.nel * nint; i += PCG_THREADS) {
+  for (int i = max(PP.vlo - nglobal, 0) + tid; i < PP.vhi - nglobal; i += PCG_THREADS) {
     const int e = i / nint, q = I.emap[next + (i - (i / nint) * nint)];
     const double v = -op.G[((size_t)op.eset[e] * G_N + G_Q4) * npnp + q] *
                          f[(size_t)e * npnp + q] - Q[nglobal + i];
     B[nglobal + i] = v;
     bb += v * v;
   }
-  const double epsb = ba.tol * sqrt(block_sum(bb, red));
+  const double epsb = ba.tol * sqrt(psum(bb, red, PP));
   pcg_x0(op, I.emap, n, uk, X, PP);
   // the problem's element operator (uniform over the workgroup): the BF16
   // matrix-core one for FP32 inner iterations, or the scalar block-wide or
@@ -2713,7 +2739,7 @@ __global__ __launch_bounds__(PCG_THREADS) void k_pcg_batch(PcgBatchArgs ba) {
     }
   }
   if (done) {
-  } else if (I.opk == OPK_BLOCK)
+  } else if (!MF && I.opk == OPK_BLOCK)
     its = pcg_run<T, REFINE, OPK_BLOCK, OPK_BLOCK>(op, n, X, R, B, P, Q, LB, I.pc, I.pc32,
                                                    epsb * epsb, ba.step_max, ba.inner_tol,
                                                    ba.max_outer, sh, red, PP);
@@ -2722,7 +2748,7 @@ __global__ __launch_bounds__(PCG_THREADS) void k_pcg_batch(PcgBatchArgs ba) {
                                                  epsb * epsb, ba.step_max, ba.inner_tol,
                                                  ba.max_outer, sh, red, PP);
   pcg_store(op, n, nsolve, X, uk, [&](int gid) { return bc[I.ess_src[gid - nsolve]]; }, PP);
-  if (tid == 0) ba.iters[blockIdx.x] = its;
+  if (tid == 0 && wg == 0) ba.iters[pid] = its;
 }
 
 }  // namespace
@@ -3818,6 +3844,11 @@ struct semx_batch_s {
   double tune_ms = 0.0;            // host time the choice took
   PcgInst* inst = nullptr;         // device, one per problem
   double* work = nullptr;
+  int* wmap = nullptr;             // device: workgroup -> problem, part, parts, slot
+  unsigned* pctr = nullptr;        // barrier arrivals of multi-workgroup problems
+  double* pslot = nullptr;
+  int* err = nullptr;
+  int nwg = 0, nmulti = 0;         // workgroups of a launch, multi-workgroup problems
   int* iters = nullptr;
   const double** ptrs = nullptr;   // device: f, bc, u pointer arrays
   int count = 0, prec = SEMX_PCG_FP64, max_outer = 12;
@@ -3828,6 +3859,7 @@ struct semx_batch_s {
 static void batch_free(semx_batch_s* b) {
   if (!b) return;
   dev_free(b->inst); dev_free(b->work); dev_free(b->iters); dev_free(b->ptrs);
+  dev_free(b->wmap); dev_free(b->pctr); dev_free(b->pslot); dev_free(b->err);
   delete b;
 }
 
@@ -3884,32 +3916,94 @@ int semx_batch_create(int count, const semx_helm* helms, semx_batch* batch) {
   st = dev_upload(&b->inst, inst.data(), inst.size());
   if (st) { batch_free(b); return st; }
   b->hinst = inst;
+  // Launch layout.  A launch lasts as long as its longest problem, so the
+  // problems whose expected cost (unknowns x elements per side x np, the
+  // iteration count growing with the resolution) exceeds the per-CU share
+  // of the whole batch get several workgroups (up to 8, at least 4096 rows
+  // each; their workgroups wait for each other, so they come first in the
+  // grid and together take at most one workgroup per CU); the rest follow,
+  // one workgroup each, largest first.  SEMX_BATCH_WG=1: one workgroup per
+  // problem.
+  {
+    int cus = 256, dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess)
+      (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    int wmax = 8;
+    if (const char* e = getenv("SEMX_BATCH_WG")) wmax = std::max(1, std::min(8, atoi(e)));
+    std::vector<double> cost(count);
+    double tot = 0.0, cmax = 0.0;
+    for (int i = 0; i < count; ++i) {
+      const Helm& H = helms[i]->H;
+      cost[i] = (double)H.npts * std::sqrt((double)H.M->nel) * H.M->np;
+      tot += cost[i];
+      cmax = std::max(cmax, cost[i]);
+    }
+    const double target = std::max(tot / cus, cmax / wmax);
+    std::vector<int> W(count, 1), ord(count);
+    for (int i = 0; i < count; ++i) {
+      ord[i] = i;
+      const int byrows = std::max(1, helms[i]->H.npts / 4096);
+      W[i] = std::max(1, std::min({wmax, byrows, (int)std::ceil(cost[i] / target - 1e-9)}));
+    }
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+    // the multi-workgroup problems must all be resident together
+    auto multi_wgs = [&]() {
+      long t = 0;
+      for (int i = 0; i < count; ++i) if (W[i] > 1) t += W[i];
+      return t;
+    };
+    for (int k = 0; multi_wgs() > cus && k < count; ++k)
+      for (int i = 0; i < count && multi_wgs() > cus; ++i)
+        if (W[ord[count - 1 - i]] > 1) W[ord[count - 1 - i]] = 1;
+    std::vector<int> wm;
+    int nm = 0;
+    for (int pass = 0; pass < 2; ++pass)
+      for (int i : ord) {
+        if ((W[i] > 1) != (pass == 0)) continue;
+        for (int w = 0; w < W[i]; ++w) {
+          wm.push_back(i); wm.push_back(w); wm.push_back(W[i]);
+          wm.push_back(W[i] > 1 ? nm : 0);
+        }
+        if (W[i] > 1) ++nm;
+      }
+    b->nwg = (int)(wm.size() / 4);
+    b->nmulti = nm;
+    st = dev_upload(&b->wmap, wm.data(), wm.size());
+    if (!st) st = dev_upload<unsigned>(&b->pctr, nullptr, (size_t)std::max(nm, 1) * 16);
+    if (!st) st = dev_upload<double>(&b->pslot, nullptr, (size_t)std::max(nm, 1) * 32);
+    if (!st) st = dev_upload<int>(&b->err, nullptr, 1);
+    if (st) { batch_free(b); return st; }
+  }
   *batch = b;
   return SEMX_OK;
 }
 
 // The batch tuner: for every class of problems with the same np and element
-// count, time each element operator that the precision allows (scalar
-// block-wide and wavefront-level; with SEMX_PCG_BF16_REFINED also the BF16
-// matrix-core operator) on the class's first problem -- a one-workgroup
+// count, time each element operator that the precision's kernel holds
+// (scalar block-wide and wavefront-level; with SEMX_PCG_BF16_REFINED the
+// wavefront-level and the BF16 matrix-core one) on the class's first
+// problem -- a one-workgroup
 // launch of BATCH_TUNE_ITERS iterations from the caller's data into a
 // scratch solution -- and keep the fastest per iteration for the class.
 // SEMX_BATCH_OP=block|wave|mfma forces one operator for every problem
-// (mfma only with BF16 precision).  Runs once, on the first solve, and
-// synchronises the stream.
+// (mfma only with BF16 precision, block not with it).  Runs once, on the
+// first solve, and synchronises the stream.
 constexpr int BATCH_TUNE_ITERS = 40;
 
 static int batch_tune(semx_batch_s& b, const double* const* f, const double* const* bc,
                       double* const* u, const void* kfn, hipStream_t s) {
   b.tuned = true;
   const auto t0 = std::chrono::steady_clock::now();
+  // the BF16 batch kernel holds the wavefront-level and the matrix-core
+  // operators only (the block-wide one would cost it registers)
   std::vector<int> cands = {OPK_BLOCK, OPK_WAVE};
-  if (b.prec == SEMX_PCG_BF16_REFINED) cands.push_back(OPK_MFMA);
+  if (b.prec == SEMX_PCG_BF16_REFINED) cands = {OPK_WAVE, OPK_MFMA};
   if (const char* e = getenv("SEMX_BATCH_OP")) {
     const std::string v(e);
     const int force = v == "block" ? OPK_BLOCK : v == "wave" ? OPK_WAVE
                       : v == "mfma" ? OPK_MFMA : -1;
     if (force < 0 || (force == OPK_MFMA && b.prec != SEMX_PCG_BF16_REFINED)) return SEMX_E_ARG;
+    if (force == OPK_BLOCK && b.prec == SEMX_PCG_BF16_REFINED) return SEMX_E_ARG;
     for (PcgInst& I : b.hinst) I.opk = force;
     HIPCHK(hipMemcpy(b.inst, b.hinst.data(), b.hinst.size() * sizeof(PcgInst),
                      hipMemcpyHostToDevice));
@@ -3942,6 +4036,7 @@ static int batch_tune(semx_batch_s& b, const double* const* f, const double* con
       break;
     }
     PcgBatchArgs a;
+    a.wmap = nullptr; a.pctr = nullptr; a.pslot = nullptr; a.err = nullptr;
     a.inst = dinst; a.f = dptr; a.bc = dptr + 1; a.u = (double* const*)(dptr + 2);
     a.iters = dit; a.tol = 0.0; a.inner_tol = b.inner_tol;
     a.step_max = BATCH_TUNE_ITERS; a.max_outer = 1;
@@ -4004,6 +4099,7 @@ int semx_batch_solve(semx_batch batch, const double* const* f, const double* con
   HIPCHK(hipMemcpyAsync(b.ptrs, ptrs.data(), ptrs.size() * sizeof(double*),
                         hipMemcpyHostToDevice, s));
   PcgBatchArgs a;
+  a.wmap = b.wmap; a.pctr = b.pctr; a.pslot = b.pslot; a.err = b.err;
   a.inst = b.inst; a.f = b.ptrs; a.bc = b.ptrs + b.count;
   a.u = (double* const*)(b.ptrs + 2 * b.count);
   a.iters = b.iters; a.tol = tol_rel; a.inner_tol = b.inner_tol;
@@ -4021,8 +4117,21 @@ int semx_batch_solve(semx_batch batch, const double* const* f, const double* con
   }
   void* kargs[] = {&a};
   hipEvent_t ev;
+  // the multi-workgroup problems' barrier counters back to zero (a fill
+  // kernel: the launch may be captured into a graph); they are resident
+  // together only if the kernel takes one CU per workgroup or less
+  if (b.nmulti > 0) {
+    int cus = 256, dev = 0, per_cu = 0;
+    if (hipGetDevice(&dev) == hipSuccess)
+      (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, PCG_THREADS, b.lds) !=
+            hipSuccess || per_cu < 1)
+      return SEMX_E_HIP;
+    hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((b.nmulti * 16 + 255) / 256)), dim3(256), 0,
+                       s, b.pctr, (size_t)b.nmulti * 16);
+  }
   prof_begin(s, &ev);
-  HIPCHK(hipLaunchKernel(kfn, dim3((unsigned)b.count), dim3(PCG_THREADS), kargs, b.lds, s));
+  HIPCHK(hipLaunchKernel(kfn, dim3((unsigned)b.nwg), dim3(PCG_THREADS), kargs, b.lds, s));
   prof_end(s, "k_pcg_batch", ev);
   HIPCHK(hipGetLastError());
   if (iters) {
@@ -4033,6 +4142,15 @@ int semx_batch_solve(semx_batch batch, const double* const* f, const double* con
     bool limit = false;
     for (int v : it) { limit |= v < 0; mx = std::max(mx, v < 0 ? -v : v); }
     *iters = mx;
+    if (b.nmulti > 0) {   // a multi-workgroup problem's barrier wait timed out
+      int e = 0;
+      HIPCHK(hipMemcpy(&e, b.err, sizeof(int), hipMemcpyDeviceToHost));
+      if (e) {
+        const int zero = 0;
+        HIPCHK(hipMemcpy(b.err, &zero, sizeof(int), hipMemcpyHostToDevice));
+        return SEMX_E_TIMEOUT;
+      }
+    }
     if (limit) return SEMX_E_MAXITER;
   }
   return SEMX_OK;

@@ -202,3 +202,26 @@ def test_pcg_workgroups_per_plane(cuda, case, W, prec, monkeypatch):
         with pytest.warns(RuntimeWarning, match="step limit"):
             gpu_solve(torch, P, S, f, bcl, tol=1e-13, step_max=3)
     sx.helm_check(S)
+
+
+@pytest.mark.parametrize("W,prec", [(1, sx.PCG_BF16_REFINED), (1, sx.PCG_FP64),
+                                    (3, sx.PCG_BF16_REFINED), (3, sx.PCG_FP64)],
+                         ids=["w1-bf16r", "w1-fp64", "w3-bf16r", "w3-fp64"])
+def test_pcg_repeated_solves_bitwise_equal(cuda, W, prec, monkeypatch):
+    """The same JACPCG solve repeated from the same data gives the same bits,
+    with one and with several workgroups per plane (the vectors are shared
+    through global memory inside a workgroup and across its plane's
+    workgroups; every barrier first waits for the wavefront's outstanding
+    global stores)."""
+    import torch
+    name, nx, np_, nz, lam2, beta, dist, bc = CASES[1]
+    print(f"[pcg repeat] {name} W={W} prec {prec}", flush=True)
+    monkeypatch.setenv("SEMX_PCG_WG", str(W))
+    P, Q, f, bcl, ue = make_case(nx, np_, nz, lam2, beta, dist, bc)
+    S = sx.ModalMatrixSys(P, lam2, beta, 0, max(nz // 2, 1), nz, sx.JACPCG)
+    set_precision(S, prec)
+    first, its0 = gpu_solve(torch, P, S, f, bcl, tol=1e-12)
+    for r in range(4):
+        u, its = gpu_solve(torch, P, S, f, bcl, tol=1e-12)
+        assert its == its0
+        assert np.array_equal(u, first), r
