@@ -497,7 +497,8 @@ def batch_bench(args, torch, sx, L):
         "dtype": "f32 inner PCG, f64 refinement",
         "workload": f"{args.batch} independent 2D problems, K x K quads K~U{{4..16}}, "
                     "np~U{5..11}, lambda2~U[0.1,100], seed 4, Dirichlet, Jacobi-PCG to 1e-10, "
-                    "one workgroup per problem, one launch",
+                    "one launch (the largest problems on up to 8 workgroups each, the "
+                    "rest on one)",
         "dof_total": dofs,
         "iterations_max": it32,
         "fp64_pcg_ms_per_step": ms64,
