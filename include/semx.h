@@ -126,6 +126,23 @@ int semx_mesh_create_enumerated (int np, int nel, const double* vx,
                                  const double* vy, const int* nbr,
                                  const int* skind, int svv_mn, double svv_eps,
                                  double kinvis, int optlevel, semx_mesh* mesh);
+/* Cylindrical coordinates for the mesh (the CYLINDRICAL token; x axial,
+ * y radial, every node at y >= 0): the geometric factors of the stiffness
+ * terms and the side weights of natural BCs are multiplied by y, and every
+ * Helmholtz system built on the mesh afterwards is the premultiplied one of
+ * src/element.cpp:1316-1426 and 1464-1650 -- mass term Q4 y (lambda2 +
+ * betak^2 / y^2), zero on the axis -- so the forcing handed to a solve is
+ * y f (AuxField::mulY, elliptic/helmholtz.cpp:45-63), weighted by the
+ * unweighted mass (Element::weight, src/element.cpp:831-837).  betak is
+ * beta k as for the scalar fields c, p and u (Field::modeConstant,
+ * src/field.cpp:1121-1146).  Axis sides (semtex's <A> BCs,
+ * src/bcmgr.cpp:133-184) are given by the caller per mode range, as
+ * semtex's per-mode numberings 'c' / 'C' hold them: natural with zero
+ * values for mode 0, essential zero for modes >= 1 (a second mesh of the
+ * same elements, and a system with base_mode 1 on it).  The DNS step does
+ * not take cylindrical meshes (SEMX_E_UNSUPPORTED).  SEMX_E_MESH if a node
+ * has y < 0; SEMX_E_ARG once a system has been built on the mesh.         */
+int semx_mesh_set_cylindrical (semx_mesh mesh);
 int semx_mesh_destroy (semx_mesh mesh);
 /* info[10] = np nel next nint nglobal nsolve nband nbound nPlane planeSize,
  * nband including the diagonal (utility/enumerate.cpp:903-943).           */

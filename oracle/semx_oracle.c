@@ -200,7 +200,8 @@ struct orc_mesh {
   double *z, *w, *DV, *DT, *SDV, *SDT;
   int    *emap, *pmap;
   double *x, *y;
-  double *Q1, *Q2, *Q3, *Q4, *drdx, *dsdx, *drdy, *dsdy;
+  double *Q1, *Q2, *Q3, *Q4, *Q8, *drdx, *dsdx, *drdy, *dsdy;
+  int     cyl;              /* cylindrical: Q1-Q4 premultiplied by y, Q8 not */
   int    *hasQ3, *hasdrdx, *hasdsdx, *hasdrdy, *hasdsdy;
   int     nglobal, nsolve, nband;
   int    *btog, *bmask, *emask;
@@ -291,8 +292,9 @@ static double nrm2 (int n, const double* x)
 }
 
 static void mesh_mapping (orc_mesh* M, int e)
-/* Geometric factors, src/element.cpp:1281-1436 (Element::mapping),
- * Cartesian branch, with the null-mapping flags of :1431-1435.           */
+/* Geometric factors, src/element.cpp:1281-1436 (Element::mapping), with
+ * the cylindrical premultiplication of Q1-Q4 by y (:1417-1426; Q8 keeps
+ * the unweighted mass) and the null-mapping flags of :1431-1435.         */
 {
   const int np = M->np, npnp = M->npnp;
   const double EPS = 4 * EPSDP;
@@ -302,6 +304,7 @@ static void mesh_mapping (orc_mesh* M, int e)
   double *jac = xcalloc (npnp, sizeof (double));
   double *Q1 = M->Q1 + e*npnp, *Q2 = M->Q2 + e*npnp, *Q3 = M->Q3 + e*npnp;
   double *Q4 = M->Q4 + e*npnp, *drdx = M->drdx + e*npnp, *dsdx = M->dsdx + e*npnp;
+  double *Q8 = M->Q8 + e*npnp;
   double *drdy = M->drdy + e*npnp, *dsdy = M->dsdy + e*npnp;
   int i, j;
 
@@ -328,6 +331,11 @@ static void mesh_mapping (orc_mesh* M, int e)
       drdy[k] = -dxds[k] / jac[k];
       dsdx[k] = -dydr[k] / jac[k];
       dsdy[k] =  dxdr[k] / jac[k];
+    }
+  memcpy (Q8, Q4, npnp * sizeof (double));
+  if (M->cyl)
+    for (i = 0; i < npnp; i++) {
+      Q1[i] *= y[i]; Q2[i] *= y[i]; Q3[i] *= y[i]; Q4[i] *= y[i];
     }
   M->hasdrdx[e] = nrm2 (npnp, drdx) >= EPS;
   M->hasdrdy[e] = nrm2 (npnp, drdy) >= EPS;
@@ -526,49 +534,19 @@ static void mesh_number (orc_mesh* M, const int* nbr, const int* skind)
   free (prov); free (ess); free (perm);
 }
 
-orc_mesh* orc_mesh_create (int np, int nel, const double* vx, const double* vy,
-                           const int* nbr, const int* skind,
-                           int svv_mn, double svv_eps, double kinvis)
+static int side_node (int np, int s, int q);
+
+static void mesh_weights (orc_mesh* M)
+/* Inverse global mass (unweighted Q8) and side quadrature weights.       */
 {
-  orc_mesh* M = xcalloc (1, sizeof (orc_mesh));
-  const int npnp = np * np;
+  const int np = M->np, npnp = M->npnp, nel = M->nel;
   int e, b, i;
-
-  M->np = np; M->nel = nel; M->npnp = npnp;
-  M->next = 4 * (np - 1); M->nint = npnp - M->next;
-  M->nPlane = nel * npnp;
-  M->planeSize = M->nPlane + (M->nPlane % 2);      /* src/geometry.cpp:107 */
-
-  M->z = xcalloc (np, sizeof (double)); M->w = xcalloc (np, sizeof (double));
-  M->DV = xcalloc (npnp, sizeof (double)); M->DT = xcalloc (npnp, sizeof (double));
-  M->SDV = xcalloc (npnp, sizeof (double)); M->SDT = xcalloc (npnp, sizeof (double));
-  M->emap = xcalloc (npnp, sizeof (int)); M->pmap = xcalloc (npnp, sizeof (int));
-  orc_zwglj (M->z, M->w, np);
-  orc_dglj (M->DV, M->DT, M->z, np);
-  orc_svv_operators (np, svv_mn, svv_eps, kinvis, M->SDV, M->SDT);
-  orc_edgemaps (np, M->emap, M->pmap);
-
-  M->x = xcalloc (nel * npnp, sizeof (double));
-  M->y = xcalloc (nel * npnp, sizeof (double));
-  M->Q1 = xcalloc (nel * npnp, sizeof (double)); M->Q2 = xcalloc (nel * npnp, sizeof (double));
-  M->Q3 = xcalloc (nel * npnp, sizeof (double)); M->Q4 = xcalloc (nel * npnp, sizeof (double));
-  M->drdx = xcalloc (nel * npnp, sizeof (double)); M->dsdx = xcalloc (nel * npnp, sizeof (double));
-  M->drdy = xcalloc (nel * npnp, sizeof (double)); M->dsdy = xcalloc (nel * npnp, sizeof (double));
-  M->hasQ3 = xcalloc (nel, sizeof (int)); M->hasdrdx = xcalloc (nel, sizeof (int));
-  M->hasdsdx = xcalloc (nel, sizeof (int)); M->hasdrdy = xcalloc (nel, sizeof (int));
-  M->hasdsdy = xcalloc (nel, sizeof (int));
-
-  for (e = 0; e < nel; e++) {
-    mesh_elmt (M, vx + 4*e, vy + 4*e, M->x + e*npnp, M->y + e*npnp);
-    mesh_mapping (M, e);
-  }
-  mesh_number (M, nbr, skind);
-
+  free (M->imass); free (M->barea);
   /* -- Inverse global mass (src/bcmgr.cpp:806-822).                      */
   M->imass = xcalloc (M->nglobal, sizeof (double));
   for (e = 0; e < nel; e++)
     for (i = 0; i < M->next; i++)
-      M->imass[M->btog[e*M->next + i]] += M->Q4[e*npnp + M->emap[i]];
+      M->imass[M->btog[e*M->next + i]] += M->Q8[e*npnp + M->emap[i]];
   for (i = 0; i < M->nglobal; i++) M->imass[i] = 1.0 / M->imass[i];
 
   /* -- Side quadrature weights (src/element.cpp:1653-1744, sideGeom).   */
@@ -590,8 +568,65 @@ orc_mesh* orc_mesh_create (int np, int nel, const double* vx, const double* vy,
         ty += M->DV[i*np + k] * y[idx];
       }
       M->barea[b*np + i] = sqrt (tx*tx + ty*ty) * M->w[i];
+      /* cylindrical: times y at the side's i-th node, sides run CCW
+         (src/edge.cpp:75 with Element::sideGeom's node order)          */
+      if (M->cyl) M->barea[b*np + i] *= y[side_node (np, s, i)];
     }
   }
+}
+
+int orc_mesh_set_cylindrical (orc_mesh* M)
+/* The CYLINDRICAL token: factors and side weights from y (x axial, y
+ * radial, no node at y < 0).                                            */
+{
+  int e, i;
+  for (i = 0; i < M->nPlane; i++) if (M->y[i] < 0.0) return 1;
+  M->cyl = 1;
+  for (e = 0; e < M->nel; e++) mesh_mapping (M, e);
+  mesh_weights (M);
+  return 0;
+}
+
+orc_mesh* orc_mesh_create (int np, int nel, const double* vx, const double* vy,
+                           const int* nbr, const int* skind,
+                           int svv_mn, double svv_eps, double kinvis)
+{
+  orc_mesh* M = xcalloc (1, sizeof (orc_mesh));
+  const int npnp = np * np;
+  int e;
+
+  M->np = np; M->nel = nel; M->npnp = npnp;
+  M->next = 4 * (np - 1); M->nint = npnp - M->next;
+  M->nPlane = nel * npnp;
+  M->planeSize = M->nPlane + (M->nPlane % 2);      /* src/geometry.cpp:107 */
+
+  M->z = xcalloc (np, sizeof (double)); M->w = xcalloc (np, sizeof (double));
+  M->DV = xcalloc (npnp, sizeof (double)); M->DT = xcalloc (npnp, sizeof (double));
+  M->SDV = xcalloc (npnp, sizeof (double)); M->SDT = xcalloc (npnp, sizeof (double));
+  M->emap = xcalloc (npnp, sizeof (int)); M->pmap = xcalloc (npnp, sizeof (int));
+  orc_zwglj (M->z, M->w, np);
+  orc_dglj (M->DV, M->DT, M->z, np);
+  orc_svv_operators (np, svv_mn, svv_eps, kinvis, M->SDV, M->SDT);
+  orc_edgemaps (np, M->emap, M->pmap);
+
+  M->x = xcalloc (nel * npnp, sizeof (double));
+  M->y = xcalloc (nel * npnp, sizeof (double));
+  M->Q1 = xcalloc (nel * npnp, sizeof (double)); M->Q2 = xcalloc (nel * npnp, sizeof (double));
+  M->Q3 = xcalloc (nel * npnp, sizeof (double)); M->Q4 = xcalloc (nel * npnp, sizeof (double));
+  M->Q8 = xcalloc (nel * npnp, sizeof (double));
+  M->drdx = xcalloc (nel * npnp, sizeof (double)); M->dsdx = xcalloc (nel * npnp, sizeof (double));
+  M->drdy = xcalloc (nel * npnp, sizeof (double)); M->dsdy = xcalloc (nel * npnp, sizeof (double));
+  M->hasQ3 = xcalloc (nel, sizeof (int)); M->hasdrdx = xcalloc (nel, sizeof (int));
+  M->hasdsdx = xcalloc (nel, sizeof (int)); M->hasdrdy = xcalloc (nel, sizeof (int));
+  M->hasdsdy = xcalloc (nel, sizeof (int));
+
+  for (e = 0; e < nel; e++) {
+    mesh_elmt (M, vx + 4*e, vy + 4*e, M->x + e*npnp, M->y + e*npnp);
+    mesh_mapping (M, e);
+  }
+  mesh_number (M, nbr, skind);
+
+  mesh_weights (M);
   return M;
 }
 
@@ -600,7 +635,7 @@ void orc_mesh_destroy (orc_mesh* M)
   if (!M) return;
   free (M->z); free (M->w); free (M->DV); free (M->DT); free (M->SDV); free (M->SDT);
   free (M->emap); free (M->pmap); free (M->x); free (M->y);
-  free (M->Q1); free (M->Q2); free (M->Q3); free (M->Q4);
+  free (M->Q1); free (M->Q2); free (M->Q3); free (M->Q4); free (M->Q8);
   free (M->drdx); free (M->dsdx); free (M->drdy); free (M->dsdy);
   free (M->hasQ3); free (M->hasdrdx); free (M->hasdsdx); free (M->hasdrdy); free (M->hasdsdy);
   free (M->btog); free (M->bmask); free (M->emask); free (M->prov);
@@ -661,7 +696,7 @@ int orc_mesh_renumber (orc_mesh* M, const int* btog, const int* bmask, int nglob
   for (i = 0; i < nglobal; i++) M->imass[i] = 0.0;
   for (e = 0; e < nel; e++)
     for (i = 0; i < next; i++)
-      M->imass[M->btog[e*next + i]] += M->Q4[e*M->npnp + M->emap[i]];
+      M->imass[M->btog[e*next + i]] += M->Q8[e*M->npnp + M->emap[i]];
   for (i = 0; i < nglobal; i++) M->imass[i] = 1.0 / M->imass[i];
   return 0;
 }
@@ -911,7 +946,7 @@ int orc_mesh_enumerate (orc_mesh* M, int optlevel)
   for (i = 0; i < ng; i++) M->imass[i] = 0.0;
   for (e = 0; e < nel; e++)
     for (i = 0; i < next; i++)
-      M->imass[M->btog[e*next + i]] += M->Q4[e*M->npnp + M->emap[i]];
+      M->imass[M->btog[e*next + i]] += M->Q8[e*M->npnp + M->emap[i]];
   for (i = 0; i < ng; i++) M->imass[i] = 1.0 / M->imass[i];
   return 0;
 }
@@ -999,16 +1034,21 @@ static void pick_ops (const orc_mesh* M, double lambda2, const double** dv,
 
 void orc_helm_op (const orc_mesh* M, int e, double lambda2, double betak2,
                   const double* src, double* tgt)
-/* src/element.cpp:1621-1650 (HelmholtzOp) with the Cartesian kernel of
- * :1564-1618 (HelmholtzKern).  src and tgt may alias.                    */
+/* src/element.cpp:1621-1650 (HelmholtzOp) with the kernel of :1564-1618
+ * (HelmholtzKern; cylindrical: hCon = betak2/r^2 + lambda2 per node, 0
+ * where r^2 <= EPSDP).  src and tgt may alias.                           */
 {
   const int np = M->np, npnp = M->npnp;
   const double *dv, *dt;
   const double *g1 = M->Q1 + e*npnp, *g2 = M->Q2 + e*npnp, *g3 = M->Q3 + e*npnp;
-  const double *g4 = M->Q4 + e*npnp;
-  const double hCon = betak2 + lambda2;
-  double R[32*32], S[32*32], T[32*32];
+  const double *g4 = M->Q4 + e*npnp, *r = M->y + e*npnp;
+  double R[32*32], S[32*32], T[32*32], hc[32*32];
   int ij;
+
+  for (ij = 0; ij < npnp; ij++) {
+    const double r2 = r[ij] * r[ij];
+    hc[ij] = M->cyl ? ((r2 > EPSDP) ? (betak2 / r2 + lambda2) : 0.0) : betak2 + lambda2;
+  }
 
   pick_ops (M, lambda2, &dv, &dt);
   mxm (src, np, dt, np, R, np);
@@ -1018,13 +1058,13 @@ void orc_helm_op (const orc_mesh* M, int e, double lambda2, double betak2,
       const double tmp = R[ij];
       R[ij] = g1[ij] * R[ij] + g3[ij] * S[ij];
       S[ij] = g2[ij] * S[ij] + g3[ij] * tmp;
-      T[ij] = g4[ij] * src[ij] * hCon;
+      T[ij] = g4[ij] * src[ij] * hc[ij];
     }
   else
     for (ij = 0; ij < npnp; ij++) {
       R[ij] *= g1[ij];
       S[ij] *= g2[ij];
-      T[ij] = g4[ij] * src[ij] * hCon;
+      T[ij] = g4[ij] * src[ij] * hc[ij];
     }
   mxma (dt, np, S, np, T, np);
   mxma (R, np, dv, np, T, np);
@@ -1033,13 +1073,15 @@ void orc_helm_op (const orc_mesh* M, int e, double lambda2, double betak2,
 
 static void helm_row (const orc_mesh* M, int e, double lambda2, double betak2,
                       int i, int j, double* hij)
-/* src/element.cpp:1439-1509 (HelmholtzRow), Cartesian.                   */
+/* src/element.cpp:1439-1509 (HelmholtzRow; cylindrical: hCon = betak2/r^2
+ * + lambda2 at node (i, j) where r^2 > EPSDP).                           */
 {
   const int np = M->np, npnp = M->npnp;
   const double *dvr, *dtr;
   const double *Q1 = M->Q1 + e*npnp, *Q2 = M->Q2 + e*npnp, *Q3 = M->Q3 + e*npnp;
   const double *Q4 = M->Q4 + e*npnp;
-  const double hCon = betak2 + lambda2;
+  const double r2 = M->y[e*npnp + RM (i, j, np)] * M->y[e*npnp + RM (i, j, np)];
+  const double hCon = (M->cyl && r2 > EPSDP) ? (betak2 / r2 + lambda2) : betak2 + lambda2;
   int m, n, q;
 
   pick_ops (M, lambda2, &dvr, &dtr);
@@ -1144,7 +1186,8 @@ void orc_helm_sc (const orc_mesh* M, int e, double lambda2, double betak2,
 
 void orc_helm_diag (const orc_mesh* M, int e, double lambda2, double betak2,
                     double* diag)
-/* src/element.cpp:1512-1561 (HelmholtzDiag), Cartesian branch.          */
+/* src/element.cpp:1512-1561 (HelmholtzDiag; cylindrical: HCon =
+ * betak2/r^2 + lambda2 per node, 0 where r^2 <= EPSDP).                  */
 {
   const int np = M->np, npnp = M->npnp;
   const double *dvr, *dtr;
@@ -1164,7 +1207,11 @@ void orc_helm_diag (const orc_mesh* M, int e, double lambda2, double betak2,
       for (q = 0; q < np; q++) s += Q2[q*np + j] * (dtr[i*np + q] * dtr[i*np + q]);
       dg[ij] += s;
       if (M->hasQ3[e]) dg[ij] += 2.0 * Q3[ij] * dvr[j*np + j] * dvr[i*np + i];
-      dg[ij] += HCon * Q4[ij];
+      if (M->cyl) {
+        const double r2 = M->y[e*npnp + ij] * M->y[e*npnp + ij];
+        dg[ij] += ((r2 > EPSDP) ? (betak2 / r2 + lambda2) : 0.0) * Q4[ij];
+      } else
+        dg[ij] += HCon * Q4[ij];
     }
   for (i = 0; i < npnp; i++) diag[i] = dg[M->emap[i]];
 }
@@ -1335,7 +1382,7 @@ static void constrain (const orc_sys* S, double* force, const double* esstlbc)
   int e, i;
   for (e = 0; e < nel; e++) {
     double* f = force + e*npnp;
-    const double* Q8 = M->Q4 + e*npnp;
+    const double* Q8 = M->Q8 + e*npnp;    /* Element::weight: not times y */
     for (i = 0; i < npnp; i++) f[i] *= Q8[i];
     if (M->emask[e]) {
       const int* btog = M->btog + e*next;

@@ -57,6 +57,8 @@ void orc_mesh_numbering (const orc_mesh* M, int* btog, int* bmask, int* emask);
 int  orc_mesh_renumber (orc_mesh* M, const int* btog, const int* bmask, int nglobal,
                         int nsolve);                /* src/bcmgr.cpp:586-824 */
 int  orc_mesh_enumerate (orc_mesh* M, int optlevel);   /* utility/enumerate.cpp */
+/* CYLINDRICAL = 1: x axial, y radial; 1 if a node has y < 0 */
+int  orc_mesh_set_cylindrical (orc_mesh* M);
 void orc_mesh_factors (const orc_mesh* M, double* Q1, double* Q2, double* Q3,
                        double* Q4, double* drdx, double* dsdx, double* drdy,
                        double* dsdy, int* null_flags /* nel*5 */);

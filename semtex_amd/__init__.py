@@ -47,11 +47,16 @@ def _ip(a):
 class Mesh:
     """Elements + numbering of a quad mesh (see semtex_amd.mesh)."""
 
-    def __init__(self, np_, m, svv_mn=-1, svv_eps=0.0, kinvis=1.0, numbering=None):
+    def __init__(self, np_, m, svv_mn=-1, svv_eps=0.0, kinvis=1.0, numbering=None,
+                 cylindrical=False):
         """numbering: None for the library's own; semtex's numbering as read
         from a session.num file (semtex_amd.numfile.read_num: a dict with
         btog, bmask, nglobal, nsolve); or ("enumerate", level) to number as
-        semtex's `enumerate -O<level>` would (levels 0-3)."""
+        semtex's `enumerate -O<level>` would (levels 0-3).
+        cylindrical: the CYLINDRICAL token -- x axial, y radial (y >= 0);
+        Helmholtz systems built on the mesh are premultiplied by y, so the
+        forcing handed to a solve must be y f (AuxField::mulY, as
+        elliptic/helmholtz.cpp does)."""
         L = load()
         vx = np.ascontiguousarray(m["vx"], dtype=np.float64)
         vy = np.ascontiguousarray(m["vy"], dtype=np.float64)
@@ -75,6 +80,9 @@ class Mesh:
                                               int(numbering["nsolve"]), C.byref(h)),
                   "semx_mesh_create_numbered")
         self.h = h
+        self.cylindrical = bool(cylindrical)
+        if cylindrical:
+            check(L.semx_mesh_set_cylindrical(h), "semx_mesh_set_cylindrical")
         info = np.zeros(10, dtype=np.int32)
         check(L.semx_mesh_info(h, _ip(info)))
         (self.np, self.nel, self.next, self.nint, self.nglobal, self.nsolve,

@@ -21,6 +21,7 @@ PROTOTYPES = {
     "semx_device_init": (C.c_int, [C.c_int]),
     "semx_mesh_create": (C.c_int, [C.c_int, C.c_int, D, D, I, I, C.c_int, C.c_double,
                                    C.c_double, C.POINTER(VP)]),
+    "semx_mesh_set_cylindrical": (C.c_int, [VP]),
     "semx_mesh_create_enumerated": (C.c_int, [C.c_int, C.c_int, D, D, I, I, C.c_int, C.c_double,
                                               C.c_double, C.c_int, C.POINTER(VP)]),
     "semx_mesh_create_numbered": (C.c_int, [C.c_int, C.c_int, D, D, I, I, C.c_int, C.c_double,

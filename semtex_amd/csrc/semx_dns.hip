@@ -1363,8 +1363,10 @@ extern "C" {
 int semx_dns_create(semx_mesh mesh, int nz, double beta, double dt, double kinvis, int norder,
                     int advection, double tol_rel, int step_max, semx_dns* dns) {
   if (!mesh) return SEMX_E_ARG;
-  // a bounded mesh needs its BC lines and pressure numbering: semx_dns_create_bc
-  if (mesh->M.nbound) return SEMX_E_UNSUPPORTED;
+  // a bounded mesh needs its BC lines and pressure numbering:
+  // semx_dns_create_bc; cylindrical coordinates are not supported by the
+  // time step
+  if (mesh->M.nbound || mesh->M.cyl) return SEMX_E_UNSUPPORTED;
   return dns_create(mesh->M, nullptr, nullptr, nz, beta, dt, kinvis, norder, advection, tol_rel,
                     step_max, dns);
 }
@@ -1375,6 +1377,7 @@ int semx_dns_create_bc(semx_mesh vmesh, semx_mesh pmesh, int nz, double beta, do
   if (!vmesh || !pmesh || !bcu || !bcv || (nz > 1 && !bcw)) return SEMX_E_ARG;
   const Mesh& M = vmesh->M;
   const Mesh& P = pmesh->M;
+  if (M.cyl || P.cyl) return SEMX_E_UNSUPPORTED;
   // the two numberings must be of one element mesh with one boundary list
   if (!M.nbound || P.np != M.np || P.nel != M.nel || P.nbound != M.nbound ||
       P.belmt != M.belmt || P.bside != M.bside || P.x != M.x || P.y != M.y)

@@ -16,7 +16,10 @@ constexpr double EPSDP = 6.0e-14;  // roundoff threshold of veclib/cveclib.h:58
 constexpr double EPSSP = 6.0e-7;   // singular-system threshold, cveclib.h:54
 
 // Geometric factor slots of one element set (Element::mapping outputs).
-enum { G_Q1 = 0, G_Q2, G_Q3, G_Q4, G_DRDX, G_DSDX, G_DRDY, G_DSDY, G_N };
+// Geometric factors per set.  Cylindrical meshes: Q1-Q3 carry the radius
+// y (Helmholtz equations are premultiplied by it), Q4 does not (it is the
+// mass of Element::weight, semtex's Q8), G_Y holds y (zero when Cartesian).
+enum { G_Q1 = 0, G_Q2, G_Q3, G_Q4, G_DRDX, G_DSDX, G_DRDY, G_DSDY, G_Y, G_N };
 
 struct Ops {
   int np = 0;
@@ -47,6 +50,9 @@ struct Mesh {
   int np = 0, nel = 0, next = 0, nint = 0, npnp = 0, nPlane = 0, planeSize = 0;
   Ops ops;
   std::vector<double> x, y;          // nel*npnp node coordinates
+  std::vector<double> vx, vy;        // nel*4 vertex coordinates
+  bool cyl = false;                  // cylindrical coordinates (x axial, y radial)
+  bool used = false;                 // a system has been built on the mesh
   int nset = 0;
   std::vector<int> eset;             // element -> geometric-factor set
   std::vector<double> G;             // nset*G_N*npnp
@@ -106,7 +112,9 @@ struct HelmDev {
   double* g = nullptr;        // nz*nzero essential values
   double* sol = nullptr;      // nplanes*solstride
   int* plane_of = nullptr;    // nplanes -> field plane
-  double* hcon = nullptr;     // per mode: lambda2 + betak2 (betak2 with its SVV weight)
+  double* hcon = nullptr;     // per mode: lambda2 + betak2 (betak2 with its SVV weight);
+                              // 1 on cylindrical meshes, whose mass factors are in mq
+  double* mq = nullptr;       // cylindrical: nmodes*nset*npnp operator mass factors
   // partitioned band sweep: spikes, their per-mode offsets, partition row
   // starts (npart+1 per mode) and the tail/head hand-off slots
   double* spk = nullptr;
@@ -152,6 +160,10 @@ struct Helm {
   std::vector<double> hbb, hbi, hii;  // host, nmodes*nset*...
   std::vector<double> Pf, Pb, Linv, LinvT;  // host, released after upload
   std::vector<double> pc;                   // nmodes*npts
+  // cylindrical meshes: the operator's mass factor per mode, set and node,
+  // Q4 y (lambda2 + betak2 / y^2), zero on the axis (src/element.cpp:
+  // 1564-1599); empty when Cartesian
+  std::vector<double> mq;
   size_t band_doubles = 0, inv_doubles = 0;
   int nplanes = 0, solstride = 0, npts = 0;
   std::vector<int> plane_of;
@@ -197,6 +209,9 @@ int mesh_build(Mesh& M, int np, int nel, const double* vx, const double* vy,
                double kinvis, const int* btog = nullptr, const int* bmask = nullptr,
                int nglobal = 0, int nsolve = 0, int enum_level = -1);
 int helm_build(Helm& H);
+// switch a host mesh (not yet on the device) to cylindrical coordinates:
+// factors and side weights multiplied by the radius y
+int mesh_set_cylindrical(Mesh& M);
 void gll(int np, std::vector<double>& z, std::vector<double>& w);
 void gll_deriv(int np, const std::vector<double>& z, std::vector<double>& D);
 void svv_derivative(int np, int svv_mn, double svv_eps, double kinvis,

@@ -170,7 +170,8 @@ static void element_nodes(const Ops& o, const double* vx, const double* vy,
 // closed form (the quantities src/element.cpp:1281-1436 forms with the GLL
 // derivative matrix); axis-aligned rectangles then give exactly zero cross
 // terms.
-static int element_factors(const Ops& o, const double* vx, const double* vy, double* g) {
+static int element_factors(const Ops& o, const double* vx, const double* vy, double* g,
+                           const double* ynode = nullptr) {
   const int np = o.np, npnp = np * np;
   const double bx = 0.25 * ((vx[1] - vx[0]) + (vx[2] - vx[3]));
   const double cx = 0.25 * ((vx[3] - vx[0]) + (vx[2] - vx[1]));
@@ -195,6 +196,15 @@ static int element_factors(const Ops& o, const double* vx, const double* vy, dou
       g[G_DRDY * npnp + q] = -xs / jac;
       g[G_DSDX * npnp + q] = -yr / jac;
       g[G_DSDY * npnp + q] = xr / jac;
+      g[G_Y * npnp + q] = 0.0;
+      // cylindrical: the stiffness factors premultiplied by the radius, the
+      // mass kept unweighted (src/element.cpp:1417-1426, _Q8)
+      if (ynode) {
+        g[G_Q1 * npnp + q] *= ynode[q];
+        g[G_Q2 * npnp + q] *= ynode[q];
+        g[G_Q3 * npnp + q] *= ynode[q];
+        g[G_Y * npnp + q] = ynode[q];
+      }
     }
   // Null-mapping: factors that vanish to roundoff are exact zeros
   // (src/element.cpp:1431-1435).
@@ -697,25 +707,25 @@ static int number_mesh(Mesh& M, const int* nbr, const int* skind, const GivenNum
 
 // ---------------------------------------------------------------------------
 
-int mesh_build(Mesh& M, int np, int nel, const double* vx, const double* vy,
-               const int* nbr, const int* skind, int svv_mn, double svv_eps,
-               double kinvis, const int* btog, const int* bmask, int nglobal, int nsolve,
-               int enum_level) {
-  if (np < 3 || np > 16 || nel < 1) return SEMX_E_ARG;
-  M.np = np; M.nel = nel; M.npnp = np * np;
-  M.next = 4 * (np - 1); M.nint = M.npnp - M.next;
-  M.nPlane = nel * M.npnp;
-  M.planeSize = M.nPlane + (M.nPlane % 2);
-  build_ops(M.ops, np, svv_mn, svv_eps, kinvis);
-  const int npnp = M.npnp;
+// Node coordinates, geometric factors and their deduplicated sets from the
+// stored vertices (cylindrical meshes: radius-weighted, src/element.cpp:
+// 1281-1436).
+static int mesh_geometry(Mesh& M) {
+  const int npnp = M.npnp, nel = M.nel;
+  const double* vx = M.vx.data();
+  const double* vy = M.vy.data();
   M.x.assign((size_t)nel * npnp, 0.0);
   M.y.assign((size_t)nel * npnp, 0.0);
   std::vector<double> g((size_t)nel * G_N * npnp);
   for (int e = 0; e < nel; ++e) {
     element_nodes(M.ops, vx + 4 * e, vy + 4 * e, &M.x[e * npnp], &M.y[e * npnp]);
-    // Factors depend on vertex differences only: translated copies of one
-    // element shape give equal factors and share one set.
-    const int st = element_factors(M.ops, vx + 4 * e, vy + 4 * e, &g[(size_t)e * G_N * npnp]);
+    // Cartesian factors depend on vertex differences only: translated
+    // copies of one element shape give equal factors and share one set.
+    if (M.cyl)
+      for (int q = 0; q < npnp; ++q)
+        if (M.y[e * npnp + q] < 0.0) return SEMX_E_MESH;   // negative radius
+    const int st = element_factors(M.ops, vx + 4 * e, vy + 4 * e, &g[(size_t)e * G_N * npnp],
+                                   M.cyl ? &M.y[e * npnp] : nullptr);
     if (st) return st;
   }
   // -- deduplicate geometric factor sets (femlib/family.c:171-192 semantics)
@@ -752,10 +762,23 @@ int mesh_build(Mesh& M, int np, int nel, const double* vx, const double* vy,
     }
     M.eset[e] = found;
   }
-  const GivenNumbering given{btog, bmask, nglobal, nsolve};
-  int st = number_mesh(M, nbr, skind, (btog && bmask) ? &given : nullptr, enum_level);
-  if (st) return st;
-  const int next = M.next, nm = np - 1;
+  return SEMX_OK;
+}
+
+// Position of node i (0..np-1, in the side's direction) of side s.
+static int side_node_index(int np, int s, int i) {
+  const int nm = np - 1;
+  switch (s) {
+    case 0: return i;
+    case 1: return i * np + nm;
+    case 2: return nm * np + (nm - i);
+    default: return (nm - i) * np;
+  }
+}
+
+// Inverse global mass and side quadrature weights (after the numbering).
+static void mesh_weights(Mesh& M) {
+  const int np = M.np, npnp = M.npnp, nel = M.nel, next = M.next, nm = np - 1;
   // -- inverse global mass (src/bcmgr.cpp:806-822)
   M.imass.assign(M.nglobal, 0.0);
   for (int e = 0; e < nel; ++e) {
@@ -781,8 +804,39 @@ int mesh_build(Mesh& M, int np, int nel, const double* vx, const double* vy,
         ty += M.ops.DV[i * np + k] * M.y[e * npnp + q];
       }
       M.barea[b * np + i] = std::sqrt(tx * tx + ty * ty) * M.ops.w[i];
+      // cylindrical: times the radius (src/edge.cpp:75)
+      if (M.cyl) M.barea[b * np + i] *= M.y[e * npnp + side_node_index(np, s, i)];
     }
   }
+}
+
+int mesh_set_cylindrical(Mesh& M) {
+  if (M.on_device || M.used) return SEMX_E_ARG;
+  if (M.cyl) return SEMX_OK;
+  M.cyl = true;
+  if (int st = mesh_geometry(M)) { M.cyl = false; mesh_geometry(M); return st; }
+  mesh_weights(M);
+  return SEMX_OK;
+}
+
+int mesh_build(Mesh& M, int np, int nel, const double* vx, const double* vy,
+               const int* nbr, const int* skind, int svv_mn, double svv_eps,
+               double kinvis, const int* btog, const int* bmask, int nglobal, int nsolve,
+               int enum_level) {
+  if (np < 3 || np > 16 || nel < 1) return SEMX_E_ARG;
+  M.np = np; M.nel = nel; M.npnp = np * np;
+  M.next = 4 * (np - 1); M.nint = M.npnp - M.next;
+  M.nPlane = nel * M.npnp;
+  M.planeSize = M.nPlane + (M.nPlane % 2);
+  build_ops(M.ops, np, svv_mn, svv_eps, kinvis);
+  M.vx.assign(vx, vx + 4 * (size_t)nel);
+  M.vy.assign(vy, vy + 4 * (size_t)nel);
+  if (int st = mesh_geometry(M)) return st;
+  const GivenNumbering given{btog, bmask, nglobal, nsolve};
+  int st = number_mesh(M, nbr, skind, (btog && bmask) ? &given : nullptr, enum_level);
+  if (st) return st;
+  const int next = M.next, nm = np - 1;
+  mesh_weights(M);
   // -- essential values: last writer in boundary order (Field::getEssential,
   //    src/field.cpp:994-1019 with Essential::set, condition.cpp:80-101)
   const int nzero = M.nglobal - M.nsolve;
@@ -826,12 +880,14 @@ int mesh_build(Mesh& M, int np, int nel, const double* vx, const double* vy,
 
 // y = (K + hcon M) x on one element by sum factorisation, the operator of
 // src/element.cpp:1564-1650; dv/dt the (SVV) derivative and transpose.
+// q4: the mass factors (null: the set's Q4; cylindrical meshes pass the
+// mode's radius-weighted factors with hcon = 1).
 static void elmt_apply(int np, const double* g, const double* dv, const double* dt,
-                       double hcon, const double* x, double* y) {
+                       double hcon, const double* x, double* y, const double* q4 = nullptr) {
   const int npnp = np * np;
   std::vector<double> R(npnp), S(npnp);
   const double *Q1 = g + G_Q1 * npnp, *Q2 = g + G_Q2 * npnp, *Q3 = g + G_Q3 * npnp,
-               *Q4 = g + G_Q4 * npnp;
+               *Q4 = q4 ? q4 : g + G_Q4 * npnp;
   for (int i = 0; i < np; ++i)
     for (int j = 0; j < np; ++j) {
       double r = 0, s = 0;
@@ -891,7 +947,7 @@ static int spd_inverse(int n, double* A) {
 // Elemental matrix in emap order, condensed: hbb <- hbb - hbi hii^-1 hib,
 // hbi <- hbi hii^-1, hii <- hii^-1 (src/element.cpp:131-229).
 static int condense(const Mesh& M, const double* g, double lambda2, double betak2,
-                    double* hbb, double* hbi, double* hii) {
+                    double* hbb, double* hbi, double* hii, const double* q4 = nullptr) {
   const int np = M.np, npnp = M.npnp, next = M.next, nint = M.nint;
   const double* dv = (lambda2 > EPSDP) ? M.ops.SDV.data() : M.ops.DV.data();
   const double* dt = (lambda2 > EPSDP) ? M.ops.SDT.data() : M.ops.DT.data();
@@ -899,7 +955,7 @@ static int condense(const Mesh& M, const double* g, double lambda2, double betak
   for (int c = 0; c < npnp; ++c) {         // column c (emap order) = H e_c
     std::fill(e.begin(), e.end(), 0.0);
     e[M.ops.emap[c]] = 1.0;
-    elmt_apply(np, g, dv, dt, lambda2 + betak2, e.data(), col.data());
+    elmt_apply(np, g, dv, dt, q4 ? 1.0 : lambda2 + betak2, e.data(), col.data(), q4);
     for (int r = 0; r < npnp; ++r) H[(size_t)r * npnp + c] = col[M.ops.emap[r]];
   }
   // symmetrise (the operator is symmetric; removes roundoff asymmetry)
@@ -1041,6 +1097,7 @@ static void band_spikes(Helm& H, int m, const double* A) {
 
 int helm_build(Helm& H) {
   Mesh& M = *H.M;
+  M.used = true;
   const int next = M.next, nint = M.nint, nset = M.nset, npnp = M.npnp;
   const int nmodes = H.nmodes;
   H.npts = M.nglobal + M.nel * nint;
@@ -1162,6 +1219,27 @@ int helm_build(Helm& H) {
   H.LinvT.assign(offL, 0.0);
   H.pc.assign((size_t)nmodes * H.npts, 0.0);
 
+  // cylindrical meshes: the mass term of each mode, Q4 y (lambda2 + betak2 /
+  // y^2) per node, zero on the axis (Element::HelmholtzKern and
+  // HelmholtzDiag, src/element.cpp:1512-1599: hCon = betak2/r^2 + lambda2
+  // where r^2 > EPSDP, else 0)
+  H.mq.clear();
+  if (M.cyl) {
+    H.mq.assign((size_t)nmodes * nset * npnp, 0.0);
+    for (int m = 0; m < nmodes; ++m)
+      for (int s = 0; s < nset; ++s) {
+        const double* g = &M.G[(size_t)s * G_N * npnp];
+        double* q = &H.mq[((size_t)m * nset + s) * npnp];
+        for (int i = 0; i < npnp; ++i) {
+          const double y = g[G_Y * npnp + i], r2 = y * y;
+          const double hc = (r2 > EPSDP) ? H.modes[m].betak2 / r2 + H.lambda2 : 0.0;
+          q[i] = g[G_Q4 * npnp + i] * y * hc;
+        }
+      }
+  }
+  auto mass_of = [&](int m, int s) -> const double* {
+    return M.cyl ? &H.mq[((size_t)m * nset + s) * npnp] : nullptr;
+  };
   std::vector<int> status(nmodes, SEMX_OK);
   auto work = [&](int m) {
     ModeSys& S = H.modes[m];
@@ -1171,7 +1249,7 @@ int helm_build(Helm& H) {
       const size_t ms = (size_t)m * nset + s;
       const int st = condense(M, g, H.lambda2, S.betak2, &H.hbb[ms * next * next],
                               &H.hbi[ms * next * std::max(nint, 1)],
-                              &H.hii[ms * std::max(nint * nint, 1)]);
+                              &H.hii[ms * std::max(nint * nint, 1)], mass_of(m, s));
       if (st) { status[m] = st; return; }
     }
     if (S.kind == SEMX_DIRECT) {
@@ -1281,7 +1359,8 @@ int helm_build(Helm& H) {
         for (int c = 0; c < npnp; ++c) {
           std::fill(ev.begin(), ev.end(), 0.0);
           ev[M.ops.emap[c]] = 1.0;
-          elmt_apply(M.np, g, dv, dt, H.lambda2 + S.betak2, ev.data(), col.data());
+          elmt_apply(M.np, g, dv, dt, M.cyl ? 1.0 : H.lambda2 + S.betak2, ev.data(),
+                     col.data(), mass_of(m, s));
           dg[(size_t)s * npnp + c] = col[M.ops.emap[c]];
         }
       }

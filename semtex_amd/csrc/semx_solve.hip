@@ -40,7 +40,9 @@ struct ElemArgs {
   int np, npnp, next, nint, nel, planeSize, nline, nsolve, nzero, nset;
   double lambda2, beta;
   int base;
-  const double* hcon;     // per mode: lambda2 + betak2
+  const double* hcon;     // per mode: lambda2 + betak2 (1 with mq)
+  const double* mq;       // cylindrical: per mode and set, the operator's mass
+                          // factors (nmodes*nset*npnp); null: Q4 of the set
   const double* G;
   const int* eset;
   const int* emap;
@@ -134,7 +136,8 @@ __device__ inline void condense_chunk(const ElemArgs& a, int p, int e0, int e_en
     }
     __syncthreads();
     if (act && em) {
-      double T = g[G_Q4 * npnp + pt] * sU[base + pt] * hcon;
+      const double* q4 = a.mq ? a.mq + ((size_t)m * a.nset + set) * npnp : g + G_Q4 * npnp;
+      double T = q4[pt] * sU[base + pt] * hcon;
       double s1 = 0.0, s3 = 0.0;
       for (int q = 0; q < np; ++q) {
         s1 += sT[i * np + q] * sS[base + q * np + j];
@@ -284,7 +287,8 @@ __global__ __launch_bounds__(ELT_THREADS, 8) void k_condense_staged(ElemArgs a,
       double F = g[G_Q4 * npnp + pt] * a.f[(size_t)k * a.planeSize + (size_t)ee * npnp + pt];
       if (em) {
         const int i = pt / np, j = pt - (pt / np) * np;
-        double T = g[G_Q4 * npnp + pt] * sU[pt] * hcon;
+        const double* q4 = a.mq ? a.mq + (size_t)m * npnp : g + G_Q4 * npnp;
+        double T = q4[pt] * sU[pt] * hcon;
         double s1 = 0.0, s3 = 0.0;
         for (int q = 0; q < np; ++q) {
           s1 += sT[i * np + q] * sS[q * np + j];
@@ -1430,6 +1434,9 @@ __global__ __launch_bounds__((coop_bt<NB, NR>())) void k_band_coop(CoopArgs a) {
                                                 a.Linv + a.offL[m], gy, n, nblk, acc, rhs,
                                                 ybuf, &abort_flag);
   if (ok) {
+    // the backward sweep re-reads, from global memory, the forward
+    // solution that other threads of this workgroup stored
+    __threadfence();
     __syncthreads();
     coop_sweep<NB, NR, RPT, false>(a, md, lane, a.Pb + a.offP[m], a.LinvT + a.offL[m], gy, n,
                                    nblk, acc, rhs, ybuf, &abort_flag);
@@ -1669,6 +1676,8 @@ __global__ __launch_bounds__((coop_bt<NB, NR>())) void k_band_xcoop(CoopArgs a) 
   const bool ok = xcoop_sweep<NB, NR, RPT, true>(a, md, lane, cl, a.Pf + offp, gy, n, nblk,
                                                  acc, rhs, ybuf, &abort_flag);
   if (ok) {
+    // the backward sweep re-reads the forward solution from global memory
+    __threadfence();
     __syncthreads();
     const bool ok2 = xcoop_sweep<NB, NR, RPT, false>(a, md, lane, cl, a.Pb + offp, gy, n, nblk,
                                                      acc, rhs, ybuf, &abort_flag);
@@ -1825,7 +1834,8 @@ struct PcgArgs {
   int np, npnp, next, nint, nel, nglobal, nsolve, nzero, npts, planeSize, nset;
   double lambda2, beta;
   int base;
-  const double* hcon;     // per mode: lambda2 + betak2
+  const double* hcon;     // per mode: lambda2 + betak2 (1 with mq)
+  const double* mq;       // cylindrical: per mode and set, operator mass factors
   const double* G;
   const int* eset;
   const int* emap;
@@ -1988,7 +1998,14 @@ struct PcgOp {
   const int* asm_ptr;
   const int* asm_idx;
   int nset;               // geometric-factor sets (1: the same for all elements)
+  const double* mq;       // cylindrical: this mode's mass factors per set
+                          // (nset*npnp, hcon = 1); null: Q4 of the set
 };
+
+// The mass factors of the operator for the elements of set `set`.
+__device__ __forceinline__ const double* pcg_mass(const PcgOp& a, int set, const double* g) {
+  return a.mq ? a.mq + (size_t)set * a.npnp : g + G_Q4 * a.npnp;
+}
 
 // Owner-gather assembly of the element-boundary results lb into rows
 // [0, nglobal) of y (Field::HelmholtzOperator's local2globalSum,
@@ -2088,7 +2105,8 @@ __device__ __forceinline__ double pcg_matvec(const PcgOp& a, int n, const T* __r
     vnext = ib >= 0 ? x[ib] : (T)0;
     ib = gidx(e0 + 2 * ce);
     pcg_barrier();
-    const double* g = a.G + (size_t)(act ? a.eset[e] : 0) * G_N * npnp;
+    const int set = act ? a.eset[e] : 0;
+    const double* g = a.G + (size_t)set * G_N * npnp;
     if (act) {
       T r = 0, s2 = 0;
       for (int q = 0; q < np; ++q) {
@@ -2102,7 +2120,7 @@ __device__ __forceinline__ double pcg_matvec(const PcgOp& a, int n, const T* __r
     }
     pcg_barrier();
     if (act) {
-      T out = (T)g[G_Q4 * npnp + pt] * sU[base + pt] * hcon;
+      T out = (T)pcg_mass(a, set, g)[pt] * sU[base + pt] * hcon;
       T s1 = 0, s3 = 0;
       for (int q = 0; q < np; ++q) {
         s1 += sT[i * np + q] * sS[base + q * np + j];
@@ -2214,7 +2232,7 @@ __device__ __forceinline__ double pcg_matvec_wave(const PcgOp& a, int n, const T
       if (pt < npnp) {
         const int i = pt / np, j = pt - (pt / np) * np;
         const int pmk = a.pmap[pt];
-        T out = (T)g[G_Q4 * npnp + pt] * sU[pt] * hcon;
+        T out = (T)pcg_mass(a, a.eset[e], g)[pt] * sU[pt] * hcon;
         T s1 = 0, s3 = 0;
         for (int q = 0; q < np; ++q) {
           s1 += sT[i * np + q] * sS[q * np + j];
@@ -2336,13 +2354,14 @@ __device__ __forceinline__ double pcg_matvec_mfma(const PcgOp& a, int n, const f
   float q1[4], q2[4], q3[4], q4[4];
   auto qload = [&](int e) {
     const double* g = a.G + (size_t)a.eset[e] * G_N * npnp;
+    const double* gm = pcg_mass(a, a.eset[e], g);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const bool in = pm[v] >= 0;
       q1[v] = in ? (float)g[G_Q1 * npnp + pt[v]] : 0.0f;
       q2[v] = in ? (float)g[G_Q2 * npnp + pt[v]] : 0.0f;
       q3[v] = in ? (float)g[G_Q3 * npnp + pt[v]] : 0.0f;
-      q4[v] = in ? (float)g[G_Q4 * npnp + pt[v]] : 0.0f;
+      q4[v] = in ? (float)gm[pt[v]] : 0.0f;
     }
   };
   const bool one_set = a.nset == 1;
@@ -2590,7 +2609,7 @@ __global__ __launch_bounds__(PCG_THREADS) void k_pcg(PcgArgs a) {
   const int npts = a.npts, nglobal = a.nglobal, next = a.next;
   const PcgOp op{a.np, a.npnp, next, a.nint, a.nel, nglobal, npts, a.hcon[m],
                  a.lambda2 > EPSDP, a.G, a.eset, a.pmap, a.btog, a.ops, a.asm_ptr, a.asm_idx,
-                 a.nset};
+                 a.nset, a.mq ? a.mq + (size_t)m * a.nset * a.npnp : nullptr};
   double* wk = a.work + (size_t)slot * a.workstride;
   PcgPart PP = part_of(a.W, wg, npts, a.nel, nglobal, a.pctr + (size_t)slot * 16,
                       a.pslot + (size_t)slot * 4 * a.W, a.err);
@@ -2896,9 +2915,10 @@ int helm_upload(Helm& H) {
     mn[m] = H.modes[m].n;
     oh[m] = (long long)H.modes[m].off_H;
     ol[m] = (long long)H.modes[m].off_L;
-    hc[m] = H.lambda2 + H.modes[m].betak2;
+    hc[m] = M.cyl ? 1.0 : H.lambda2 + H.modes[m].betak2;   // cylindrical: factors in mq
   }
   if ((st = dev_upload(&H.d.hcon, hc.data(), hc.size()))) return st;
+  if (!H.mq.empty() && (st = dev_upload(&H.d.mq, H.mq.data(), H.mq.size()))) return st;
   if ((st = dev_upload(&H.d.hbi, H.hbi.data(), H.hbi.size()))) return st;
   if ((st = dev_upload(&H.d.hii, H.hii.data(), H.hii.size()))) return st;
   if ((st = dev_upload(&H.d.Pf, H.Pf.data(), H.Pf.size()))) return st;
@@ -2951,7 +2971,7 @@ void helm_release(Helm& H) {
   dev_free(H.d.mode_offL); dev_free(H.d.pc); dev_free(H.d.Fb); dev_free(H.d.Fi);
   dev_free(H.d.g); dev_free(H.d.sol); dev_free(H.d.plane_of);
   dev_free(H.d.pc32); dev_free(H.d.mode_kind); dev_free(H.d.pcg_plane); dev_free(H.d.work);
-  dev_free(H.d.hcon);
+  dev_free(H.d.hcon); dev_free(H.d.mq);
   dev_free(H.d.iters);
   dev_free(H.d.inbox); dev_free(H.d.part); dev_free(H.d.clo);
   dev_free(H.d.err);
@@ -3020,6 +3040,7 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
   ea.planeSize = M.planeSize; ea.nline = nline; ea.nsolve = M.nsolve; ea.nzero = nzero;
   ea.nset = M.nset; ea.lambda2 = H.lambda2; ea.beta = H.beta; ea.base = H.base;
   ea.hcon = H.d.hcon;
+  ea.mq = H.d.mq;
   ea.G = M.d.G; ea.eset = M.d.eset; ea.emap = M.d.emap; ea.pmap = M.d.pmap; ea.btog = M.d.btog;
   ea.emask = M.d.emask; ea.ops = M.d.ops; ea.nat_ptr = M.d.nat_ptr; ea.nat_b = M.d.nat_b;
   ea.bside = M.d.bside; ea.barea = M.d.barea; ea.hbi = H.d.hbi; ea.hii = H.d.hii;
@@ -3407,6 +3428,7 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
     pa.nglobal = M.nglobal; pa.nsolve = M.nsolve; pa.nzero = nzero; pa.npts = H.npts;
     pa.planeSize = M.planeSize; pa.nset = M.nset;
     pa.lambda2 = H.lambda2; pa.beta = H.beta; pa.base = H.base; pa.hcon = H.d.hcon;
+    pa.mq = H.d.mq;
     pa.G = M.d.G; pa.eset = M.d.eset; pa.emap = M.d.emap; pa.pmap = M.d.pmap;
     pa.btog = M.d.btog; pa.ops = M.d.ops; pa.asm_ptr = M.d.asm_ptr; pa.asm_idx = M.d.asm_idx;
     pa.plane_of = H.d.plane_of; pa.pcg_plane = H.d.pcg_plane; pa.mode_n = H.d.mode_n;
@@ -3673,6 +3695,11 @@ int semx_mesh_create_enumerated(int np, int nel, const double* vx, const double*
   return SEMX_OK;
 }
 
+int semx_mesh_set_cylindrical(semx_mesh mesh) {
+  if (!mesh) return SEMX_E_ARG;
+  return mesh_set_cylindrical(mesh->M);
+}
+
 int semx_mesh_destroy(semx_mesh mesh) {
   if (!mesh) return SEMX_E_ARG;
   mesh_release(mesh->M);
@@ -3895,8 +3922,9 @@ int semx_batch_create(int count, const semx_helm* helms, semx_batch* batch) {
     Mesh& M = *H.M;
     PcgInst& I = inst[i];
     I.op = PcgOp{M.np, M.npnp, M.next, M.nint, M.nel, M.nglobal, H.npts,
-                 H.lambda2 + H.modes[0].betak2, H.lambda2 > EPSDP, M.d.G, M.d.eset, M.d.pmap,
-                 M.d.btog, M.d.ops, M.d.asm_ptr, M.d.asm_idx, M.nset};
+                 M.cyl ? 1.0 : H.lambda2 + H.modes[0].betak2, H.lambda2 > EPSDP, M.d.G,
+                 M.d.eset, M.d.pmap, M.d.btog, M.d.ops, M.d.asm_ptr, M.d.asm_idx, M.nset,
+                 H.d.mq};
     I.n = H.modes[0].n; I.nsolve = M.nsolve; I.nbound = M.nbound;
     I.emap = M.d.emap; I.ess_src = M.d.ess_src; I.nat_ptr = M.d.nat_ptr; I.nat_b = M.d.nat_b;
     I.bside = M.d.bside; I.barea = M.d.barea; I.pc = H.d.pc; I.pc32 = H.d.pc32;

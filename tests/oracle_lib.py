@@ -56,6 +56,8 @@ def lib():
         L.orc_set_svv_z.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int]
         L.orc_mesh_enumerate.restype = C.c_int
         L.orc_mesh_enumerate.argtypes = [C.c_void_p, C.c_int]
+        L.orc_mesh_set_cylindrical.restype = C.c_int
+        L.orc_mesh_set_cylindrical.argtypes = [C.c_void_p]
         L.orc_mesh_renumber.restype = C.c_int
         L.orc_mesh_renumber.argtypes = [C.c_void_p, _I, _I, C.c_int, C.c_int]
         L.orc_dns_create.restype = C.c_void_p
@@ -155,6 +157,11 @@ class Mesh:
         self.nband = int(info[6])
         lib().orc_mesh_numbering(C.c_void_p(self.h), ip(self.btog), ip(self.bmask),
                                  ip(self.emask))
+
+    def set_cylindrical(self):
+        """Switch to cylindrical coordinates (x axial, y radial)."""
+        if lib().orc_mesh_set_cylindrical(C.c_void_p(self.h)):
+            raise ValueError("a node has y < 0")
 
     def enumerate(self, optlevel=1):
         """Renumber as semtex's `enumerate -O<optlevel>` (levels 0-3)."""

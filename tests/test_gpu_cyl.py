@@ -1,0 +1,167 @@
+"""Cylindrical-coordinate Helmholtz solves on the GPU through the C ABI
+(semx_mesh_set_cylindrical), against the oracle's restatement of semtex's
+cylindrical branches (pinned on the host by poisson2, laplace9's exact
+solution and a solid cylinder with the axis; tests/test_cylindrical.py).
+
+Bars: DIRECT rel L2 <= 1e-12 vs the oracle DIRECT; JACPCG (FP64, FP32 or
+BF16 inner iterations with FP64 refinement) and MIXED converged at TOL_REL
+1e-13, rel L2 <= 1e-10 vs the oracle DIRECT; poisson2's error against
+1 - 4y^2 at noise level (< 1e-12, regress/poisson2.ok) for the FP64
+solvers.  The batch kernel takes cylindrical problems beside Cartesian
+ones (same bars)."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import semtex_amd as sx
+from cyl_cases import (axis_lines, fourier_exact, laplace9_exact, laplace9_mesh,
+                       poisson2_mesh, poisson2_problem, tube_exact, tube_meshes)
+from semtex_amd.mesh import rect_mesh
+from semtex_amd.workloads import single_plane_problem
+from solve_helpers import gpu_solve, oracle_solve, rel
+
+pytestmark = pytest.mark.gpu
+
+SOLVERS = [(sx.DIRECT, sx.PCG_FP64), (sx.JACPCG, sx.PCG_FP64),
+           (sx.JACPCG, sx.PCG_FP32_REFINED), (sx.JACPCG, sx.PCG_BF16_REFINED),
+           (sx.MIXED, sx.PCG_FP64)]
+IDS = ["direct", "jacpcg-fp64", "jacpcg-fp32r", "jacpcg-bf16r", "mixed"]
+
+
+def system(P, lam, beta, base, nmodes, nz, method, prec):
+    S = sx.ModalMatrixSys(P, lam, beta, base, nmodes, nz, method)
+    if method != sx.DIRECT and prec != sx.PCG_FP64:
+        if prec == sx.PCG_BF16_REFINED:
+            S.set_pcg(prec, inner_tol=1e-2, max_outer=80)
+        else:
+            S.set_pcg(prec, inner_tol=1e-4, max_outer=30)
+    return S
+
+
+def oracle_cyl(np_, m):
+    Q = O.Mesh(np_, m)
+    Q.set_cylindrical()
+    return Q
+
+
+@pytest.mark.parametrize("method,prec", SOLVERS, ids=IDS)
+def test_poisson2(cuda, method, prec):
+    import torch
+    print(f"[cyl] poisson2 method {method} prec {prec}", flush=True)
+    m = poisson2_mesh()
+    P = sx.Mesh(11, m, cylindrical=True)
+    Q = oracle_cyl(11, m)
+    f, bc, ex = poisson2_problem(P)
+    S = system(P, 0.0, 1.0, 0, 1, 1, method, prec)
+    ug, it = gpu_solve(torch, P, S, f, bc, tol=1e-13)
+    uo, _ = oracle_solve(Q, 1, 0.0, 1.0, f, bc, method=sx.DIRECT)
+    r = rel(ug[:, :P.nPlane], uo[:, :P.nPlane])
+    err = np.abs(ug[0, :P.nPlane] - ex[0, :P.nPlane]).max()
+    print(f"  iterations {it}, rel L2 vs oracle {r:.3e}, |c - (1-4y^2)|_inf {err:.3e}",
+          flush=True)
+    assert r <= (1e-12 if method == sx.DIRECT else 1e-10)
+    if prec == sx.PCG_FP64:
+        assert err < 1e-12                      # regress/poisson2.ok: noise-level
+
+
+@pytest.mark.parametrize("method,prec", SOLVERS, ids=IDS)
+def test_laplace9_3d(cuda, method, prec):
+    import torch
+    nz = 32
+    print(f"[cyl] laplace9 nz={nz} method {method} prec {prec}", flush=True)
+    m = laplace9_mesh()
+    P = sx.Mesh(11, m, cylindrical=True)
+    Q = oracle_cyl(11, m)
+    F, bc = fourier_exact(P, nz, laplace9_exact)
+    f = np.zeros_like(F)
+    S = system(P, 0.0, 1.0, 0, nz // 2, nz, method, prec)
+    ug, it = gpu_solve(torch, P, S, f, bc, tol=1e-13)
+    uo, _ = oracle_solve(Q, nz, 0.0, 1.0, f, bc, method=sx.DIRECT)
+    keep = [k for k in range(nz) if k != 1]
+    r = rel(ug[keep, :P.nPlane], uo[keep, :P.nPlane])
+    e = np.abs(ug[keep, :P.nPlane] - F[keep, :P.nPlane]).max()
+    print(f"  iterations {it}, rel L2 vs oracle {r:.3e}, exact error {e:.3e}", flush=True)
+    assert r <= (1e-12 if method == sx.DIRECT else 1e-10)
+    assert e < 1e-8
+
+
+@pytest.mark.parametrize("method", [sx.DIRECT, sx.JACPCG], ids=["direct", "jacpcg"])
+def test_tube_with_axis(cuda, method):
+    """Mode 0 with the axis natural, modes 1.. with it essential zero: two
+    systems on two numberings of one mesh, each on its slice of the planes."""
+    import torch
+    nz = 32
+    print(f"[cyl] tube with axis nz={nz} method {method}", flush=True)
+    mN, mD = tube_meshes()
+    PN, PD = sx.Mesh(11, mN, cylindrical=True), sx.Mesh(11, mD, cylindrical=True)
+    QN, QD = oracle_cyl(11, mN), oracle_cyl(11, mD)
+    F, bc = fourier_exact(PN, nz, tube_exact)
+    bc = axis_lines(PN, bc, True)
+    f = np.zeros_like(F)
+    S0 = sx.ModalMatrixSys(PN, 0.0, 1.0, 0, 1, 2, method)
+    S1 = sx.ModalMatrixSys(PD, 0.0, 1.0, 1, nz // 2 - 1, nz - 2, method)
+    dev = torch.device("cuda:0")
+    tf = torch.from_numpy(f.ravel().copy()).to(dev)
+    tb = torch.from_numpy(bc.ravel().copy()).to(dev)
+    tu = torch.zeros_like(tf)
+    ps, nl = PN.planeSize, PN.nbound * PN.np
+    sx.solve(S0, tf[:2 * ps], tb[:2 * nl], tu[:2 * ps], tol_rel=1e-13)
+    sx.solve(S1, tf[2 * ps:], tb[2 * nl:], tu[2 * ps:], tol_rel=1e-13)
+    torch.cuda.synchronize()
+    ug = tu.cpu().numpy().reshape(F.shape)
+    uo = np.zeros_like(F)
+    u0, u1 = uo[:2].copy(), uo[2:].copy()
+    O.field_solve_base(QN, 2, 0.0, 1.0, 0, sx.DIRECT, f[:2].copy(), bc[:2].copy(), u0)
+    O.field_solve_base(QD, nz - 2, 0.0, 1.0, 1, sx.DIRECT, f[2:].copy(), bc[2:].copy(), u1)
+    uo[:2], uo[2:] = u0, u1
+    keep = [k for k in range(nz) if k != 1]
+    r = rel(ug[keep, :PN.nPlane], uo[keep, :PN.nPlane])
+    e = np.abs(ug[keep, :PN.nPlane] - F[keep, :PN.nPlane]).max()
+    print(f"  rel L2 vs oracle {r:.3e}, exact error {e:.3e}", flush=True)
+    assert r <= (1e-12 if method == sx.DIRECT else 1e-10)
+    assert e < 1e-8
+
+
+@pytest.mark.parametrize("prec", [sx.PCG_FP64, sx.PCG_FP32_REFINED, sx.PCG_BF16_REFINED],
+                         ids=["fp64", "fp32r", "bf16r"])
+def test_batch_with_cylindrical_problems(cuda, prec):
+    """poisson2 and a cylindrical annulus problem (lambda2 = 3, natural end)
+    in one batch launch with a Cartesian problem."""
+    import torch
+    print(f"[cyl batch] prec {prec}", flush=True)
+    dev = torch.device("cuda:0")
+    probs = []
+    m = poisson2_mesh()
+    P = sx.Mesh(11, m, cylindrical=True)
+    f, bc, ex = poisson2_problem(P)
+    probs.append((P, oracle_cyl(11, m), 0.0, f, bc))
+    from cyl_cases import annulus_natural_mesh
+    m2 = annulus_natural_mesh()
+    P2 = sx.Mesh(9, m2, cylindrical=True)
+    f2, bc2, _ = single_plane_problem(P2, 3.0, 7)
+    probs.append((P2, oracle_cyl(9, m2), 3.0, _ymul(P2, f2), bc2))
+    m3 = rect_mesh(4, 4, 0, 2 * np.pi, 0, 2 * np.pi)
+    P3 = sx.Mesh(7, m3)
+    f3, bc3, _ = single_plane_problem(P3, 5.0, 3)
+    probs.append((P3, O.Mesh(7, m3), 5.0, f3, bc3))
+    systems = []
+    for (Pk, _, lam, _, _) in probs:
+        systems.append(system(Pk, lam, 1.0, 0, 1, 1, sx.JACPCG, prec))
+    B = sx.Batch(systems)
+    fs = [torch.from_numpy(p[3].ravel().copy()).to(dev) for p in probs]
+    bcs = [torch.from_numpy(p[4].ravel().copy()).to(dev) for p in probs]
+    us = [torch.zeros_like(t) for t in fs]
+    it = B.solve(fs, bcs, us, tol_rel=1e-13)
+    torch.cuda.synchronize()
+    for i, (Pk, Qk, lam, fk, bk) in enumerate(probs):
+        uo, _ = oracle_solve(Qk, 1, lam, 1.0, fk, bk, method=sx.DIRECT)
+        g = us[i].cpu().numpy().reshape(fk.shape)
+        r = rel(g[:, :Pk.nPlane], uo[:, :Pk.nPlane])
+        print(f"  problem {i}: rel L2 vs oracle {r:.3e} (iterations max {it})", flush=True)
+        assert r <= 1e-10
+
+
+def _ymul(P, f):
+    out = f.copy()
+    out[:, :P.nPlane] *= P.y[None, :]
+    return out
