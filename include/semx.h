@@ -176,6 +176,14 @@ int semx_helm_create (semx_mesh mesh, double lambda2, double beta,
  * number of Fourier modes (Geometry::nMode, src/geometry.h:42; it exceeds
  * nmodes when the system holds one process's share).  svv_mz < 0 or
  * SVV_EPSZ <= EPSDP leaves the weights at 1.                              */
+/* The same for the field named `field` ('c', 'p', 'u', 'v' or 'w'), whose
+ * name picks the Fourier constant as Field::modeConstant does
+ * (src/field.cpp:1121-1146, used by ModalMatrixSys, src/matrix.cpp:60-63):
+ * beta k, except on a cylindrical mesh for the coupled fields v~ ('v':
+ * beta k + 1) and w~ ('w': beta k - 1), both 1 for mode 0.               */
+int semx_helm_create_field (semx_mesh mesh, double lambda2, double beta,
+                            int base_mode, int nmodes, int nz, int method,
+                            int field, semx_helm* helm);
 int semx_helm_create_svv (semx_mesh mesh, double lambda2, double beta,
                           int base_mode, int nmodes, int nz, int method,
                           int svv_mz, double svv_epsz, double kinvis,
@@ -319,6 +327,16 @@ int semx_gradient (semx_mesh mesh, int nz, double beta, double* data, int dir,
 /* Field::smooth (src/field.cpp:190-236): mass-weighted average of the
  * element-boundary values of every plane.                                  */
 int semx_smooth (semx_mesh mesh, int nz, double* data, void* stream);
+/* AuxField::couple and Field::coupleBCs (src/auxfield.cpp:1352-1420,
+ * src/field.cpp:1043-1118): the radial and azimuthal fields of a
+ * cylindrical 3D run coupled for their viscous solves, per Fourier mode k
+ * (planes 2k, 2k+1 of n values each: planeSize for fields, nbound*np for
+ * BC lines) -- dir = 1: v~ = v + i w, w~ = v - i w; dir = -1: v = (v~ +
+ * w~)/2, w = i (w~ - v~)/2.  Mode 0 is left alone when base_mode is 0 (it
+ * has no coupling, semtex's kLo), as are 2D fields (nz = 1).  Device
+ * pointers, in place.                                                     */
+int semx_couple (double* v, double* w, int nz, int n, int base_mode, int dir,
+                 void* stream);
 
 /* --- DNS time stepping: integrate -------------------------------------------
  * Replaces the time loop of integrate (dns/integrate.cpp:79-221) with the

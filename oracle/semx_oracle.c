@@ -1024,6 +1024,52 @@ void orc_set_svv_z (int mz, double eps, double nu, int nmode)
   svvz_mz = mz; svvz_eps = eps; svvz_nu = nu; svvz_nmode = nmode;
 }
 
+static char field_name = 'c';
+
+void orc_set_field_name (int name) { field_name = (char) name; }
+
+static double mode_constant (const orc_mesh* M, int mode, double beta)
+/* Field::modeConstant (src/field.cpp:1121-1146) for the field set by
+ * orc_set_field_name ('c' unless set).                                  */
+{
+  if (!M->cyl || field_name == 'c' || field_name == 'p' || field_name == 'u')
+    return beta * mode;
+  if (field_name == 'v') return (mode == 0) ? 1.0 : beta * mode + 1.0;
+  return (mode == 0) ? 1.0 : beta * mode - 1.0;
+}
+
+void orc_couple (double* v, double* w, int nz, int n, int base, int dir)
+/* AuxField::couple / Field::coupleBCs (src/auxfield.cpp:1352-1420,
+ * src/field.cpp:1043-1118), the Veclib sequence restated: per mode k from
+ * kLo (1 when the planes start at mode 0), planes 2k and 2k+1.           */
+{
+  const int nmode = (nz > 1) ? nz / 2 : 0, klo = (base == 0) ? 1 : 0;
+  int k, i;
+  double* tp = xcalloc (n, sizeof (double));
+  for (k = klo; k < nmode; k++) {
+    double *Vr = v + (size_t)(2*k) * n, *Vi = Vr + n;
+    double *Wr = w + (size_t)(2*k) * n, *Wi = Wr + n;
+    if (dir > 0) {
+      for (i = 0; i < n; i++) tp[i] = Vr[i];
+      for (i = 0; i < n; i++) Vr[i] = Vr[i] - Wi[i];        /* vsub        */
+      for (i = 0; i < n; i++) Wi[i] = Wi[i] + tp[i];        /* vadd        */
+      for (i = 0; i < n; i++) tp[i] = Wr[i];
+      for (i = 0; i < n; i++) Wr[i] = Wi[i];
+      for (i = 0; i < n; i++) Wi[i] = Vi[i] - tp[i];        /* vsub        */
+      for (i = 0; i < n; i++) Vi[i] = Vi[i] + tp[i];        /* vadd        */
+    } else {
+      for (i = 0; i < n; i++) tp[i] = Vr[i];
+      for (i = 0; i < n; i++) Vr[i] = 0.5 * (Vr[i] + Wr[i]); /* svvpt      */
+      for (i = 0; i < n; i++) Wr[i] = 0.5 * (Wr[i] - tp[i]); /* svvmt      */
+      for (i = 0; i < n; i++) tp[i] = Wi[i];
+      for (i = 0; i < n; i++) Wi[i] = Wr[i];
+      for (i = 0; i < n; i++) Wr[i] = 0.5 * (Vi[i] - tp[i]); /* svvmt      */
+      for (i = 0; i < n; i++) Vi[i] = 0.5 * (Vi[i] + tp[i]); /* svvpt      */
+    }
+  }
+  free (tp);
+}
+
 static void pick_ops (const orc_mesh* M, double lambda2, const double** dv,
                       const double** dt)
 /* SVV operators whenever lambda2 > EPSDP (src/element.cpp:1484,1532,1640). */
@@ -1584,7 +1630,7 @@ static int field_solve_planes (const orc_mesh* M, int nz, double lambda2, double
   for (m = 0; m < nmodes; m++) {
     /* MIXED: DIRECT for mode 0, JACPCG otherwise (matrix.cpp:93-96) */
     const int kind = (method == 2) ? (base + m == 0 ? ORC_DIRECT : ORC_JACPCG) : method;
-    const double bk = beta * (base + m);
+    const double bk = mode_constant (M, base + m, beta);
     /* betak2 weighted by the z SVV coefficient for lambda2 > EPSDP
      * (src/matrix.cpp:78-83)                                            */
     const double bk2 = (lambda2 > EPSDP) ? bk * bk * Sz[m] : bk * bk;

@@ -59,6 +59,10 @@ int  orc_mesh_renumber (orc_mesh* M, const int* btog, const int* bmask, int nglo
 int  orc_mesh_enumerate (orc_mesh* M, int optlevel);   /* utility/enumerate.cpp */
 /* CYLINDRICAL = 1: x axial, y radial; 1 if a node has y < 0 */
 int  orc_mesh_set_cylindrical (orc_mesh* M);
+/* Field name for Field::modeConstant in the field solves ('c' default) */
+void orc_set_field_name (int name);
+/* AuxField::couple / Field::coupleBCs: dir 1 forward, -1 inverse */
+void orc_couple (double* v, double* w, int nz, int n, int base, int dir);
 void orc_mesh_factors (const orc_mesh* M, double* Q1, double* Q2, double* Q3,
                        double* Q4, double* drdx, double* dsdx, double* drdy,
                        double* dsdy, int* null_flags /* nel*5 */);

@@ -122,13 +122,19 @@ class ModalMatrixSys:
     planes (src/matrix.cpp:33-110)."""
 
     def __init__(self, mesh, lambda2, beta, base_mode, nmodes, nz, method=DIRECT,
-                 svv_mz=-1, svv_epsz=0.0, kinvis=1.0, nmode_total=None):
+                 svv_mz=-1, svv_epsz=0.0, kinvis=1.0, nmode_total=None, field="c"):
         """svv_mz >= 0: semtex's Fourier-direction SVV (SVV_MZ, SVV_EPSZ,
         KINVIS) on the betak2 of modes above svv_mz; nmode_total is the
-        field's Fourier mode count (default base_mode + nmodes)."""
+        field's Fourier mode count (default base_mode + nmodes).  field:
+        the field's name, which picks the Fourier constant on cylindrical
+        meshes (Field::modeConstant: 'v' / 'w' are the coupled v~ / w~)."""
         L = load()
         h = C.c_void_p()
-        if svv_mz >= 0:
+        if field != "c" and svv_mz < 0:
+            check(L.semx_helm_create_field(mesh.h, float(lambda2), float(beta), int(base_mode),
+                                           int(nmodes), int(nz), int(method), ord(field),
+                                           C.byref(h)), "semx_helm_create_field")
+        elif svv_mz >= 0:
             nmt = base_mode + nmodes if nmode_total is None else nmode_total
             check(L.semx_helm_create_svv(mesh.h, float(lambda2), float(beta), int(base_mode),
                                          int(nmodes), int(nz), int(method), int(svv_mz),
@@ -308,6 +314,14 @@ def _stream(stream):
         import torch
         stream = torch.cuda.current_stream().cuda_stream
     return C.c_void_p(stream)
+
+
+def couple(v, w, nz, n, base_mode=0, direction=1, stream=None):
+    """AuxField::couple / Field::coupleBCs on device arrays (in place):
+    direction 1 forms v~ = v + i w, w~ = v - i w per mode, -1 undoes it."""
+    check(load().semx_couple(C.c_void_p(v.data_ptr()), C.c_void_p(w.data_ptr()), int(nz),
+                             int(n), int(base_mode), int(direction), _stream(stream)),
+          "semx_couple")
 
 
 def dft_r(data, tlen, ntrn, sign, stream=None):

@@ -487,6 +487,33 @@ __global__ void k_scale(size_t n, double* __restrict__ t, double a) {
   if (i < n) t[i] *= a;
 }
 
+// AuxField::couple / Field::coupleBCs (src/auxfield.cpp:1352-1420,
+// src/field.cpp:1043-1118): per mode k >= klo, Re/Im planes 2k, 2k+1 of n
+// values.  dir > 0: v~ = v + i w, w~ = v - i w; dir < 0: v = (v~ + w~) / 2,
+// w = i (w~ - v~) / 2 -- the same sums as semtex's Veclib sequence.
+__global__ void k_couple(double* __restrict__ v, double* __restrict__ w, size_t n, int klo,
+                         int nmode, int dir) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = klo + (int)blockIdx.y;
+  if (i >= n || k >= nmode) return;
+  double* vr = v + (size_t)(2 * k) * n;
+  double* vi = vr + n;
+  double* wr = w + (size_t)(2 * k) * n;
+  double* wi = wr + n;
+  const double a = vr[i], b = vi[i], c = wr[i], d = wi[i];
+  if (dir > 0) {
+    vr[i] = a - d;
+    vi[i] = b + c;
+    wr[i] = d + a;
+    wi[i] = b - c;
+  } else {
+    vr[i] = 0.5 * (a + c);
+    vi[i] = 0.5 * (b + d);
+    wr[i] = 0.5 * (b - d);
+    wi[i] = 0.5 * (c - a);
+  }
+}
+
 __global__ void k_axpy(size_t n, double* __restrict__ t, double a, const double* __restrict__ x) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) t[i] += a * x[i];
@@ -1245,6 +1272,18 @@ int semx_gradient(semx_mesh mesh, int nz, double beta, double* data, int dir, vo
   int st = mesh_upload(M);
   if (st) return st;
   return grad_launch(M, nz, beta, data, dir, (hipStream_t)stream);
+}
+
+int semx_couple(double* v, double* w, int nz, int n, int base_mode, int dir, void* stream) {
+  if (!v || !w || nz < 1 || n < 1 || base_mode < 0 || (dir != 1 && dir != -1)) return SEMX_E_ARG;
+  if (nz > 1 && (nz & 1)) return SEMX_E_ARG;
+  const int nmode = nz / 2, klo = base_mode == 0 ? 1 : 0;   // 2D (nz = 1): nothing
+  if (nmode <= klo) return SEMX_OK;
+  const dim3 grid((unsigned)(((size_t)n + 255) / 256), (unsigned)(nmode - klo));
+  hipLaunchKernelGGL(k_couple, grid, dim3(256), 0, (hipStream_t)stream, v, w, (size_t)n, klo,
+                     nmode, dir);
+  HIPCHK(hipGetLastError());
+  return SEMX_OK;
 }
 
 int semx_smooth(semx_mesh mesh, int nz, double* data, void* stream) {

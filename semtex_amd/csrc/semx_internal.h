@@ -131,6 +131,10 @@ struct Helm {
   Mesh* M = nullptr;
   double lambda2 = 0.0, beta = 0.0;
   int base = 0, nmodes = 0, nz = 0, method = SEMX_DIRECT;
+  // field name for Field::modeConstant: on cylindrical meshes the coupled
+  // radial and azimuthal fields 'v' / 'w' take beta k + 1 / beta k - 1
+  // (1 for mode 0); every other case beta k
+  char field = 'c';
   // Fourier-direction SVV (SVV_MZ, SVV_EPSZ, KINVIS; src/svv.cpp:82-123):
   // betak2 of mode k > svv_mz weighted by 1 + epsz/nu exp(-((N-k)/(mz-k))^2)
   // when lambda2 > EPSDP (src/matrix.cpp:78-83); svv_mz < 0: off

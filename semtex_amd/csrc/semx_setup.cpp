@@ -1147,9 +1147,15 @@ int helm_build(Helm& H) {
         Sz[m] = 1.0 + H.svv_epsz / H.svv_nu * q;
       }
   }
+  // Field::modeConstant (src/field.cpp:1121-1146)
+  auto mode_constant = [&](int mode) {
+    if (!M.cyl || (H.field != 'v' && H.field != 'w')) return H.beta * mode;
+    if (mode == 0) return 1.0;
+    return H.field == 'v' ? H.beta * mode + 1.0 : H.beta * mode - 1.0;
+  };
   for (int m = 0; m < nmodes; ++m) {
     ModeSys& S = H.modes[m];
-    const double bk = H.beta * (H.base + m);
+    const double bk = mode_constant(H.base + m);
     S.betak2 = bk * bk;
     if (H.lambda2 > EPSDP) S.betak2 *= Sz[m];   // src/matrix.cpp:82-83
     S.kind = (H.method == SEMX_MIXED) ? ((H.base + m == 0) ? SEMX_DIRECT : SEMX_JACPCG)

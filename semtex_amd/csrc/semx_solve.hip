@@ -3751,8 +3751,10 @@ int semx_mesh_nsets(semx_mesh mesh, int* nsets) {
 
 static int helm_create_impl(semx_mesh mesh, double lambda2, double beta, int base_mode,
                             int nmodes, int nz, int method, int svv_mz, double svv_epsz,
-                            double kinvis, int nmode_total, semx_helm* helm) {
+                            double kinvis, int nmode_total, semx_helm* helm,
+                            char field = 'c') {
   if (!mesh || !helm || nmodes < 1 || nz < 1 || base_mode < 0) return SEMX_E_ARG;
+  if (!strchr("cpuvw", field) || !field) return SEMX_E_ARG;
   if (method != SEMX_DIRECT && method != SEMX_JACPCG && method != SEMX_MIXED) return SEMX_E_ARG;
   if (!(nz == 1 && nmodes == 1) && nz != 2 * nmodes) return SEMX_E_ARG;
   *helm = nullptr;
@@ -3764,6 +3766,7 @@ static int helm_create_impl(semx_mesh mesh, double lambda2, double beta, int bas
   H.nz = nz; H.method = method;
   H.svv_mz = svv_mz; H.svv_epsz = svv_epsz; H.svv_nu = kinvis;
   H.svv_nmode = nmode_total;
+  H.field = field;
   const int st = helm_build(H);
   if (st) { delete h; return st; }
   *helm = h;
@@ -3776,6 +3779,13 @@ int semx_helm_create(semx_mesh mesh, double lambda2, double beta, int base_mode,
                      int nz, int method, semx_helm* helm) {
   return helm_create_impl(mesh, lambda2, beta, base_mode, nmodes, nz, method, -1, 0.0, 1.0, 0,
                           helm);
+}
+
+int semx_helm_create_field(semx_mesh mesh, double lambda2, double beta, int base_mode,
+                           int nmodes, int nz, int method, int field, semx_helm* helm) {
+  if (field <= 0 || field > 127) return SEMX_E_ARG;
+  return helm_create_impl(mesh, lambda2, beta, base_mode, nmodes, nz, method, -1, 0.0, 1.0, 0,
+                          helm, (char)field);
 }
 
 int semx_helm_create_svv(semx_mesh mesh, double lambda2, double beta, int base_mode,

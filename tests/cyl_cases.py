@@ -120,3 +120,25 @@ def natural_lines(M, nz, bc, fn_flux, side):
         if M.bside[b] == side:
             out[:, b * M.np:(b + 1) * M.np] = fl[:, b * M.np:(b + 1) * M.np]
     return out
+
+
+def swirl_annulus_mesh():
+    """0 <= x <= 1, 0.5 <= y <= 1.5, 2 x 2 elements, all sides essential."""
+    return graded_mesh([0.0, 0.5, 1.0], [0.5, 1.0, 1.5], ("D", "D", "D", "D"))
+
+
+def coupled_modal_exact(M, nz):
+    """Fourier planes (nz, planeSize) of v and w whose coupled fields are,
+    in every (x, y) plane, v~ = y^(k+1) and w~ = y^(k-1) for k = 1 (w~ = 1)
+    and k = 2, and ln y for w~ at k = 1 added: solutions of the coupled
+    viscous operators d2/dx2 + d2/dy2 + (1/y) d/dy - (k +- 1)^2 / y^2 with
+    no forcing.  v = (v~ + w~)/2 is real, w = i (w~ - v~)/2 imaginary."""
+    Fv = np.zeros((nz, M.planeSize))
+    Fw = np.zeros((nz, M.planeSize))
+    y = M.y
+    for k in (1, 2):
+        vt = y ** (k + 1)
+        wt = y ** (k - 1) + (np.log(y) if k == 1 else 0.0)
+        Fv[2 * k, :M.nPlane] = 0.5 * (vt + wt)
+        Fw[2 * k + 1, :M.nPlane] = 0.5 * (wt - vt)
+    return Fv, Fw

@@ -58,6 +58,10 @@ def lib():
         L.orc_mesh_enumerate.argtypes = [C.c_void_p, C.c_int]
         L.orc_mesh_set_cylindrical.restype = C.c_int
         L.orc_mesh_set_cylindrical.argtypes = [C.c_void_p]
+        L.orc_set_field_name.restype = None
+        L.orc_set_field_name.argtypes = [C.c_int]
+        L.orc_couple.restype = None
+        L.orc_couple.argtypes = [_D, _D, C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_mesh_renumber.restype = C.c_int
         L.orc_mesh_renumber.argtypes = [C.c_void_p, _I, _I, C.c_int, C.c_int]
         L.orc_dns_create.restype = C.c_void_p
