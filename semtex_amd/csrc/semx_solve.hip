@@ -1882,16 +1882,38 @@ __device__ __forceinline__ void pcg_barrier() {
   __syncthreads();
 }
 
-__device__ __forceinline__ double block_sum(double v, double* red) {
+// The PCG kernels' LDS reduction scratch `red`: two buffers of 2 values per
+// wavefront, used in turn by consecutive block sums (so one barrier per sum
+// suffices: the barrier that ends a sum also separates the readers of the
+// previous sum's buffer from the writers of the sum after it), then the
+// grid-barrier timeout flag.
+constexpr int RED_BUF = 2 * (PCG_THREADS / 64);
+constexpr int RED_FLAG = 2 * RED_BUF;
+constexpr int RED_DOUBLES = RED_FLAG + 8;
+
+// Sums of v and u over the workgroup's threads (each added in the same
+// order as a sum of one value: wavefront shuffles, then the wavefronts in
+// order).  par: the buffer to use, flipped on return; uniform.
+__device__ __forceinline__ double2 block_sum2(double v, double u, double* red, int& par) {
 #pragma unroll
-  for (int off = 32; off >= 1; off /= 2) v += __shfl_xor(v, off);
+  for (int off = 32; off >= 1; off /= 2) {
+    v += __shfl_xor(v, off);
+    u += __shfl_xor(u, off);
+  }
   const int wave = threadIdx.x >> 6;
+  double* rb = red + par * RED_BUF;
+  par ^= 1;
+  if ((threadIdx.x & 63) == 0) {
+    rb[2 * wave] = v;
+    rb[2 * wave + 1] = u;
+  }
   pcg_barrier();
-  if ((threadIdx.x & 63) == 0) red[wave] = v;
-  pcg_barrier();
-  double t = 0.0;
+  double2 t = make_double2(0.0, 0.0);
 #pragma unroll
-  for (int w = 0; w < PCG_THREADS / 64; ++w) t += red[w];
+  for (int w = 0; w < PCG_THREADS / 64; ++w) {
+    t.x += rb[2 * w];
+    t.y += rb[2 * w + 1];
+  }
   return t;
 }
 
@@ -1915,6 +1937,7 @@ struct PcgPart {
   double* slot = nullptr;     // [2 parities][W][2] partial sums
   int* err = nullptr;
   unsigned epoch = 0;         // barriers passed
+  int rpar = 0;               // reduction buffer of the next block sum
 };
 
 __device__ inline PcgPart part_of(int W, int w, int npts, int nel, int nglobal, unsigned* ctr,
@@ -1935,7 +1958,7 @@ __device__ inline PcgPart part_of(int W, int w, int npts, int nel, int nglobal, 
 constexpr long PCG_SPIN = 1L << 22;    // polls before a barrier wait gives up
 
 // A barrier of the plane's workgroups (a block barrier when W = 1).  red:
-// the workgroup's reduction scratch (entry 24 carries a timeout to every
+// the workgroup's reduction scratch (entry RED_FLAG carries a timeout to every
 // thread).
 __device__ __forceinline__ void part_sync(PcgPart& P, double* red) {
   pcg_barrier();
@@ -1952,17 +1975,17 @@ __device__ __forceinline__ void part_sync(PcgPart& P, double* red) {
     }
     __threadfence();
     if (late) __hip_atomic_store(P.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    red[24] = late ? 1.0 : 0.0;
+    red[RED_FLAG] = late ? 1.0 : 0.0;
   }
   pcg_barrier();
   P.epoch += 1;
-  if (red[24] != 0.0) P.W = 1;    // stop waiting (the error word is set)
+  if (red[RED_FLAG] != 0.0) P.W = 1;    // stop waiting (the error word is set)
 }
 
 // sums of v and u over the threads of every workgroup of the plane
 __device__ __forceinline__ double2 psum2(double v, double u, double* red, PcgPart& P) {
-  const double a = block_sum(v, red);
-  const double b = block_sum(u, red);
+  const double2 ab = block_sum2(v, u, red, P.rpar);
+  const double a = ab.x, b = ab.y;
   if (P.W == 1) return make_double2(a, b);
   double* sl = P.slot + (size_t)(P.epoch & 1) * P.W * 2;
   if (threadIdx.x == 0) {
@@ -1981,7 +2004,7 @@ __device__ __forceinline__ double2 psum2(double v, double u, double* red, PcgPar
 }
 
 __device__ __forceinline__ double psum(double v, double* red, PcgPart& P) {
-  if (P.W == 1) return block_sum(v, red);
+  if (P.W == 1) return block_sum2(v, 0.0, red, P.rpar).x;
   return psum2(v, 0.0, red, P).x;
 }
 
@@ -2598,8 +2621,8 @@ __device__ __forceinline__ void pcg_store(const PcgOp& op, int n, int nsolve, co
 template <typename T, bool REFINE, int OPK = OPK_BLOCK>
 __global__ __launch_bounds__(PCG_THREADS) void k_pcg(PcgArgs a) {
   extern __shared__ double pcg_lds[];
-  double* red = pcg_lds;               // 32 doubles
-  T* sh = (T*)(pcg_lds + 32);
+  double* red = pcg_lds;               // RED_DOUBLES doubles
+  T* sh = (T*)(pcg_lds + RED_DOUBLES);
   const int tid = threadIdx.x;
   const int slot = blockIdx.x / a.W, wg = blockIdx.x - (blockIdx.x / a.W) * a.W;
   const int pidx = a.pcg_plane[slot];
@@ -2689,7 +2712,7 @@ template <typename T, bool REFINE, bool MF>
 __global__ __launch_bounds__(PCG_THREADS) void k_pcg_batch(PcgBatchArgs ba) {
   extern __shared__ double pcg_lds[];
   double* red = pcg_lds;
-  T* sh = (T*)(pcg_lds + 32);
+  T* sh = (T*)(pcg_lds + RED_DOUBLES);
   const int tid = threadIdx.x;
   const int* wm = ba.wmap ? ba.wmap + 4 * blockIdx.x : nullptr;
   const int pid = wm ? wm[0] : (int)blockIdx.x;
@@ -3438,12 +3461,12 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
     pa.max_outer = H.max_outer; pa.iters = H.d.iters;
     const int cep = PCG_THREADS / M.npnp;
     // LDS of every element operator (block-wide, wavefront-level, BF16)
-    size_t lds = 32 * sizeof(double) +
+    size_t lds = RED_DOUBLES * sizeof(double) +
         (2 * (size_t)M.npnp + 3 * (size_t)cep * M.npnp) * sizeof(double);
-    lds = std::max(lds, 32 * sizeof(double) +
+    lds = std::max(lds, RED_DOUBLES * sizeof(double) +
                         (2 + 3 * (size_t)PCG_WAVES) * M.npnp * sizeof(double));
     if (H.pcg_prec == SEMX_PCG_BF16_REFINED)
-      lds = std::max(lds, 32 * sizeof(double) + (size_t)PCG_WAVES * 2 * MF_TILE * sizeof(float));
+      lds = std::max(lds, RED_DOUBLES * sizeof(double) + (size_t)PCG_WAVES * 2 * MF_TILE * sizeof(float));
     auto pcg_fn = [&](int opk) -> const void* {
       if (H.pcg_prec == SEMX_PCG_FP64)
         return opk == OPK_WAVE ? (const void*)k_pcg<double, false, OPK_WAVE>
@@ -3942,13 +3965,13 @@ int semx_batch_create(int count, const semx_helm* helms, semx_batch* batch) {
     I.opk = (b->prec == SEMX_PCG_BF16_REFINED) ? OPK_MFMA : OPK_WAVE;
     // wavefront-level element operator: a private slice per wavefront
     if (M.np > 13) { batch_free(b); return SEMX_E_UNSUPPORTED; }
-    b->lds = std::max(b->lds, 32 * sizeof(double) +
+    b->lds = std::max(b->lds, RED_DOUBLES * sizeof(double) +
                       (2 * (size_t)M.npnp + 3 * (size_t)PCG_WAVES * M.npnp) * sizeof(double));
     // the block-wide element operator (a tuner candidate)
-    b->lds = std::max(b->lds, 32 * sizeof(double) +
+    b->lds = std::max(b->lds, RED_DOUBLES * sizeof(double) +
                       (2 + 3 * (size_t)(PCG_THREADS / M.npnp)) * M.npnp * sizeof(double));
     if (b->prec == SEMX_PCG_BF16_REFINED)
-      b->lds = std::max(b->lds, 32 * sizeof(double) +
+      b->lds = std::max(b->lds, RED_DOUBLES * sizeof(double) +
                         (size_t)PCG_WAVES * 2 * MF_TILE * sizeof(float));
   }
   st = dev_upload(&b->inst, inst.data(), inst.size());
