@@ -154,6 +154,19 @@ def test_error_paths():
     assert L.semx_helm_create(P.h, 1.0, 1.0, 0, 3, 4, 0, C.byref(h)) == 1
     with pytest.raises(sx.SemxError):
         sx.ModalMatrixSys(P, 1.0, 1.0, 0, 2, 4, 7)
+    # a field name Field::modeConstant does not know
+    assert L.semx_helm_create_field(P.h, 1.0, 1.0, 0, 2, 4, 0, ord("x"), C.byref(h)) == 1
+    # couple: null fields, odd plane count, unknown direction (checked before any launch)
+    assert L.semx_couple(None, None, 4, 10, 0, 1, None) == 1
+    fake = C.c_void_p(8)
+    assert L.semx_couple(fake, fake, 5, 10, 0, 1, None) == 1
+    assert L.semx_couple(fake, fake, 4, 10, 0, 2, None) == 1
+    # a 2D field or a single mode starting at 0 has nothing to couple
+    assert L.semx_couple(fake, fake, 1, 10, 0, 1, None) == 0
+    assert L.semx_couple(fake, fake, 2, 10, 0, 1, None) == 0
+    # the DNS step does not take cylindrical meshes
+    Pc = sx.Mesh(5, rect_mesh(2, 2, periodic_x=True, periodic_y=True), cylindrical=True)
+    assert L.semx_dns_create(Pc.h, 4, 1.0, 1e-3, 0.1, 2, 0, 1e-10, 100, C.byref(h)) == 8
 
 
 ENUM_MESHES = [
