@@ -340,8 +340,10 @@ int semx_couple (double* v, double* w, int nz, int n, int base_mode, int dir,
 
 /* --- DNS time stepping: integrate -------------------------------------------
  * Replaces the time loop of integrate (dns/integrate.cpp:79-221) with the
- * skew-symmetric (advection 0, nonlinear.cpp:36-211) or alternating
- * (advection 1, :214-411) nonlinear terms, waveProp, setPForce, project and
+ * skew-symmetric (advection 0, nonlinear.cpp:36-211), alternating
+ * (advection 1, :214-411; dns's default), convective (advection 2,
+ * :414-525) or Stokes (advection 3, :528-570: none) nonlinear terms -- dns
+ * -S, default, -C, -N (dns/drive.cpp:114-158) -- waveProp, setPForce, project and
  * the pressure / viscous Field::solve calls (DIRECT; JACPCG start-up
  * systems while step < norder = N_TIME), Cartesian, on a mesh without
  * boundaries (periodic in x and y; SEMX_E_UNSUPPORTED otherwise: a mesh

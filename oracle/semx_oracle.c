@@ -2260,9 +2260,12 @@ static void times_minus (size_t n, double* t, const double* a, const double* b)
 { size_t i; for (i = 0; i < n; i++) t[i] -= a[i] * b[i]; }
 
 static void nonlinear (orc_dns* D, double** u, double* scratch)
-/* skewSymmetric (dns/nonlinear.cpp:36-211, Cartesian :179-209) and
- * altSkewSymmetric (:214-411, Cartesian branch).  On exit u[] hold the
- * physical-space velocities, Us[0] the Fourier ones, Uf[0] N(u).         */
+/* skewSymmetric (dns/nonlinear.cpp:36-211, Cartesian :179-209),
+ * altSkewSymmetric (:214-411, Cartesian branch), convective (:414-525,
+ * Cartesian :502-524: u . grad u only, not halved) and Stokes (:528-570:
+ * no nonlinear terms; advection 0-3 as dns -S / default / -C / -N,
+ * dns/drive.cpp:114-158).  On exit u[] hold the physical-space
+ * velocities, Us[0] the Fourier ones, Uf[0] N(u).                        */
 {
   const orc_mesh* M = D->M;
   const int ncom = D->ncom, ndim = (D->nz > 1) ? 3 : 2;
@@ -2276,7 +2279,8 @@ static void nonlinear (orc_dns* D, double** u, double* scratch)
   }
   for (i = 0; i < ncom; i++) {
     double* N = D->Uf[i];
-    const int conv = (D->advection == 0) || (D->advection == 1 && D->toggle);
+    const int conv = (D->advection == 0) || (D->advection == 2) ||
+                     (D->advection == 1 && D->toggle);
     const int cons = (D->advection == 0) || (D->advection == 1 && !D->toggle);
     for (j = 0; j < ndim; j++) {
       if (conv) {   /* n_i -= u_j d(u_i)/dx_j */

@@ -904,7 +904,10 @@ int dns_nonlinear(semx_dns_s& D, hipStream_t s) {
       CHK(dns_transform(D, D.U[i], -1, s));
     }
   }
-  const bool conv = D.advection == 0 || (D.advection == 1 && D.toggle);
+  // skewSymmetric (0): both forms, halved; altSkewSymmetric (1): one form
+  // per step in turn; convective (2): u . grad u; Stokes (3): none
+  // (dns/nonlinear.cpp:36-211, 214-411, 414-525, 528-570)
+  const bool conv = D.advection == 0 || D.advection == 2 || (D.advection == 1 && D.toggle);
   const bool cons = D.advection == 0 || (D.advection == 1 && !D.toggle);
   for (int i = 0; i < D.ncom; ++i) {
     double* N = D.Uf[0][i];
@@ -940,6 +943,7 @@ int dns_nonlinear(semx_dns_s& D, hipStream_t s) {
         }
       }
     }
+    if (!conv && !cons) EW(k_fill, N, 0.0);   // Stokes: no nonlinear terms
     const double half = D.advection == 0 ? 0.5 : 1.0;
     // the skew-symmetric halving, exact in binary and commuting with the
     // linear transform, rides on the deferred z term when there is one
@@ -1355,7 +1359,7 @@ int dns_create(Mesh& M, Mesh* Mp, const double* const* bc, int nz, double beta, 
                double kinvis, int norder, int advection, double tol_rel, int step_max,
                semx_dns* dns) {
   if (!dns || nz < 1 || (nz > 1 && (nz & 1)) || norder < 1 || norder > 3 ||
-      advection < 0 || advection > 1 || !(dt > 0.0) || !(kinvis > 0.0) || step_max < 1)
+      advection < 0 || advection > 3 || !(dt > 0.0) || !(kinvis > 0.0) || step_max < 1)
     return SEMX_E_ARG;
   if (nz > 2 && !fft_length_ok(nz)) return SEMX_E_ARG;
   *dns = nullptr;

@@ -480,6 +480,41 @@ def test_regress_taylor(nz):
         assert np.abs(w).max() < 1e-12 and np.abs(u[2:]).max() < 1e-12
 
 
+@pytest.mark.parametrize("adv", [2, 3], ids=["convective", "stokes"])
+def test_taylor_other_advection_forms(adv):
+    """dns -C (convective, dns/nonlinear.cpp:414-525) and dns -N (Stokes,
+    :528-570) on taylor2's flow, pinned by what the exact solution fixes:
+    its u . grad u is the gradient of -(cos 2 pi x + cos 2 pi y)/4 e^(-4 pi^2
+    nu t), so the velocity decays as e^(-2 pi^2 nu t) in both forms (to the
+    time-stepping error of the alternating run, 1.1e-5); the convective
+    form's pressure is that potential (up to a constant), and without
+    nonlinear terms the pressure stays zero.  A sign error or a stray 1/2 in
+    the convective product moves the pressure by O(0.1)."""
+    m = rect_mesh(2, 2, 0, 2, 0, 2, periodic_x=True, periodic_y=True)
+    M = O.Mesh(11, m)
+    x, y = M.x, M.y
+    nu, dt, ps = 0.01, 0.02, M.planeSize
+    u = np.zeros((1, ps)); v = np.zeros((1, ps)); p = np.zeros((1, ps))
+    u[0, :M.nPlane] = -np.cos(np.pi * x) * np.sin(np.pi * y)
+    v[0, :M.nPlane] = np.sin(np.pi * x) * np.cos(np.pi * y)
+    D = O.lib().orc_dns_create(C.c_void_p(M.h), 1, np.pi, dt, nu, 2, adv, 1e-12, 500, 0)
+    t = C.c_double()
+    O.lib().orc_dns_run(C.c_void_p(D), 20, O.dp(u), O.dp(v), None, O.dp(p), C.byref(t))
+    O.lib().orc_dns_destroy(C.c_void_p(D))
+    T = 0.4
+    ue = -np.cos(np.pi * x) * np.sin(np.pi * y) * np.exp(-2 * np.pi ** 2 * nu * T)
+    pe = -0.25 * (np.cos(2 * np.pi * x) + np.cos(2 * np.pi * y)) * np.exp(-4 * np.pi ** 2 * nu * T)
+    eu = np.abs(u[0, :M.nPlane] - ue).max()
+    pc = p[0, :M.nPlane] - p[0, :M.nPlane].mean()
+    print(f"advection {adv}: |u - u*| {eu:.3e}, |p| {np.abs(pc).max():.3e}, "
+          f"|p - p*| (constant removed) {np.abs(pc - (pe - pe.mean())).max():.3e}")
+    assert eu < 2e-5
+    if adv == 2:
+        assert np.abs(pc - (pe - pe.mean())).max() < 1e-4
+    else:
+        assert np.abs(pc).max() < 1e-7
+
+
 # --------------------------------------------------------------------------
 # a17/a18 with z-variation: regress/taylor4.ok and taylor5.ok (2D Taylor
 # flow in the x-z and y-z planes, mesh/taylor4, mesh/taylor5): these are the
