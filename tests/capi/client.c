@@ -1,6 +1,7 @@
 /* A plain C caller of include/semx.h: links libsemx.so and runs the
- * host-only entry points (mesh construction and numbering, status text),
- * printing what a test compares.  Built as C and as C++ by
+ * host-only entry points (mesh construction and numbering, status text,
+ * cylindrical coordinates and a field-named system's setup), printing what
+ * a test compares.  Built as C and as C++ by
  * tests/test_capi_client.py. */
 #include <stdio.h>
 #include <stdlib.h>
@@ -41,6 +42,25 @@ int main (void)
     if (m3) semx_mesh_destroy (m3);
   }
   printf ("bad-arg %d\n", semx_mesh_create (2, 1, vx, vy, nbr, skind, -1, 0.0, 1.0, &m2));
+  /* cylindrical coordinates (y radial, 0 <= y <= 1) and a system for the
+   * coupled radial field v~ (host setup only) */
+  {
+    semx_mesh mc = NULL;
+    semx_helm hv = NULL;
+    double hbb[12 * 12], hbi[12 * 4], hii[4 * 4];
+    int stc = semx_mesh_create (np, nel, vx, vy, nbr, skind, -1, 0.0, 1.0, &mc);
+    if (!stc) stc = semx_mesh_set_cylindrical (mc);
+    printf ("cyl %d\n", stc);
+    stc = semx_helm_create_field (mc, 1.0, 1.0, 0, 2, 4, SEMX_DIRECT, 'v', &hv);
+    printf ("field-v %d\n", stc);
+    if (!stc && !semx_helm_element (hv, 1, 0, hbb, hbi, hii))
+      printf ("hbb00-positive %d\n", hbb[0] > 0.0);
+    printf ("cyl-after-system %d\n", semx_mesh_set_cylindrical (mc));
+    printf ("field-x %d\n", semx_helm_create_field (mc, 1.0, 1.0, 0, 2, 4, SEMX_DIRECT, 'x',
+                                                     &hv));
+    if (hv) semx_helm_destroy (hv);
+    semx_mesh_destroy (mc);
+  }
   semx_mesh_destroy (m);
   return 0;
 }

@@ -2,7 +2,8 @@
 compiled as C and as C++ against the header and linked against
 libsemx.so, builds a mesh, hands its numbering back through
 semx_mesh_create_numbered, and sees an unfit numbering and a bad argument
-refused with their status codes."""
+refused with their status codes; it switches a mesh to cylindrical
+coordinates and builds a system for the coupled field v~ on it."""
 import os
 import shutil
 import subprocess
@@ -38,3 +39,6 @@ def test_compiled_client(tmp_path, lang):
     assert lines["numbered"] == "0"
     assert lines["refused"].startswith("7 ")
     assert lines["bad-arg"] == "1"
+    assert lines["cyl"] == "0" and lines["field-v"] == "0"
+    assert lines["hbb00-positive"] == "1"
+    assert lines["cyl-after-system"] == "1" and lines["field-x"] == "1"
