@@ -14,10 +14,11 @@
 //   k_backsub[_staged]
 //                   interior recovery u_i = hii F_i - hbi^T u_b and essential
 //                   re-imposition (Element::global2localSC, Field::setEssential)
-//   k_pcg           JACPCG, one persistent workgroup per plane (FP64, or FP32
-//                   inner PCG with FP64 refinement)
+//   k_pcg           JACPCG, persistent workgroups, one or several per plane
+//                   (FP64, or FP32 / BF16-matrix-core inner PCG with FP64
+//                   refinement)
 //   k_pcg_batch     the same for many independent 2D problems in one launch,
-//                   right-hand side built in the workgroup
+//                   right-hand side built in the problem's workgroups
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -1820,8 +1821,8 @@ __global__ __launch_bounds__(ELT_THREADS) void k_backsub_staged(ElemArgs a,
 }
 
 // ===========================================================================
-// Jacobi-preconditioned CG, one persistent workgroup per plane: semtex's
-// JACPCG branch of Field::solve (src/field.cpp:627-720) with the global
+// Jacobi-preconditioned CG, persistent workgroups (one per plane, or W
+// sharing a plane, below): semtex's JACPCG branch of Field::solve (src/field.cpp:627-720) with the global
 // operator of Field::HelmholtzOperator (src/field.cpp:771-856).
 // Vectors are globally numbered: nglobal element-boundary nodes, then the
 // element interiors in emap order.  Rows of essential (and pinned) nodes are
