@@ -97,7 +97,9 @@ def test_ext_band_dns_width_candidates(cuda, no_coop, pick):
     """The band of the DNS systems (24x24 periodic quads, np=9: kd = 911,
     16-column blocks of 928 panel rows) through every ring-window candidate
     the autotuner chooses between, for one field and for three fields in one
-    call, against the oracle."""
+    call, against the oracle.  The variant that ran is read back
+    (semx_helm_sweep_info): a field-split candidate (3, 7) needs several
+    fields, so a single field runs the sweep the library falls back to."""
     import torch
     old = os.environ.get("SEMX_AUTOTUNE")
     os.environ["SEMX_AUTOTUNE"] = pick
@@ -108,8 +110,13 @@ def test_ext_band_dns_width_candidates(cuda, no_coop, pick):
         print(f"  band {S.info['ldab']}", flush=True)
         ug, _ = gpu_solve(torch, P, S, f, bcl)
         r = rel(ug[:, :P.nPlane], uo[:, :P.nPlane])
-        print(f"  one field: rel L2 gpu vs oracle = {r:.3e}", flush=True)
+        ran = S.sweep_info()["pick"][1]
+        print(f"  one field (variant {ran}): rel L2 gpu vs oracle = {r:.3e}", flush=True)
         assert r <= 1e-12
+        if pick in ("3", "7"):
+            assert ran != int(pick)
+        else:
+            assert ran == int(pick)
         dev = torch.device("cuda:0")
         fs, bcs, us, refs = [], [], [], []
         for seed in (41, 42, 43):
@@ -120,6 +127,7 @@ def test_ext_band_dns_width_candidates(cuda, no_coop, pick):
             refs.append(uo3)
         sx.solve_fields(S, fs, bcs, us)
         torch.cuda.synchronize()
+        assert S.sweep_info()["pick"][3] == int(pick)
         for u, ref in zip(us, refs):
             g = u.cpu().numpy().reshape(ref.shape)
             r = rel(g[:, :P.nPlane], ref[:, :P.nPlane])
