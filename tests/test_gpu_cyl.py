@@ -225,3 +225,29 @@ def test_coupled_viscous_solve(cuda, method):
         print(f"  rel L2 vs oracle {r:.3e}, exact error {e:.3e}", flush=True)
         assert r <= (1e-12 if method == sx.DIRECT else 1e-10)
         assert e < 1e-8
+
+
+def test_svv_cylindrical_helmholtz(cuda):
+    """SVV operators (SVV_MN, SVV_EPSN: used when lambda2 > EPSDP) on a
+    cylindrical mesh, 3D, DIRECT and JACPCG against the oracle."""
+    import torch
+    nz, lam = 8, 2.5
+    print(f"[cyl svv] nz={nz} lambda2={lam}", flush=True)
+    m = laplace9_mesh()
+    svv = dict(svv_mn=5, svv_eps=0.3, kinvis=0.02)
+    P = sx.Mesh(11, m, cylindrical=True, **svv)
+    Q = O.Mesh(11, m, **svv)
+    Q.set_cylindrical()
+    F, bc = fourier_exact(P, nz, laplace9_exact)
+    rng = np.random.default_rng(2)
+    f = np.zeros_like(F)
+    f[:, :P.nPlane] = rng.standard_normal((nz, P.nPlane)) * P.y[None, :]
+    f[1] = 0.0
+    uo, _ = oracle_solve(Q, nz, lam, 1.0, f, bc, method=sx.DIRECT)
+    keep = [k for k in range(nz) if k != 1]
+    for method in (sx.DIRECT, sx.JACPCG):
+        S = sx.ModalMatrixSys(P, lam, 1.0, 0, nz // 2, nz, method)
+        ug, it = gpu_solve(torch, P, S, f, bc, tol=1e-13)
+        r = rel(ug[keep, :P.nPlane], uo[keep, :P.nPlane])
+        print(f"  method {method}: rel L2 vs oracle {r:.3e}", flush=True)
+        assert r <= (1e-12 if method == sx.DIRECT else 1e-10)
