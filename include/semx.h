@@ -369,6 +369,16 @@ int semx_dns_create_bc (semx_mesh vmesh, semx_mesh pmesh, int nz, double beta,
                         double dt, double kinvis, int norder, int advection,
                         double tol_rel, int step_max, const double* bcu,
                         const double* bcv, const double* bcw, semx_dns* dns);
+/* New velocity BC lines (device, nz*nline each, as semx_dns_create_bc takes
+ * them) for the steps that follow: the values of time-dependent BCs, which
+ * semtex re-evaluates after advancing the time of each step and before its
+ * viscous solve (Field::evaluateBoundaries, dns/integrate.cpp:196-201), so
+ * a caller stepping one step at a time hands over the lines of that step's
+ * end time before running it.  Copied on the stream; graphs captured for
+ * the DNS keep reading the same arrays.  SEMX_E_ARG for a DNS created
+ * without boundaries.                                                      */
+int semx_dns_set_bc (semx_dns dns, const double* bcu, const double* bcv,
+                     const double* bcw, void* stream);
 int semx_dns_destroy (semx_dns dns);
 /* nstep steps; u, v, w (NULL in 2D), p: device fields of nz*planeSize in
  * Fourier space, read at entry and written at exit.  The history carries

@@ -2203,6 +2203,17 @@ orc_dns* orc_dns_create_bc (const orc_mesh* M, const orc_mesh* Mp, int nz, doubl
   return D;
 }
 
+void orc_dns_set_bc (orc_dns* D, const double* bcu, const double* bcv, const double* bcw)
+/* New velocity BC lines for the steps that follow (the re-evaluation of
+ * time-dependent BCs, dns/integrate.cpp:196-201).                       */
+{
+  const double* src[3] = {bcu, bcv, bcw};
+  const size_t nl = (size_t)D->nz * D->nline;
+  int i;
+  for (i = 0; i < 3; i++)
+    if (src[i]) memcpy (D->bcu[i], src[i], nl * sizeof (double));
+}
+
 void orc_dns_destroy (orc_dns* D)
 {
   free (D->bstore); free (D->un); free (D->hop);

@@ -426,6 +426,13 @@ class DNS:
                                   _ptr(w) if self.nz > 1 else None, _ptr(p),
                                   _stream(stream)), "semx_dns_run")
 
+    def set_bc(self, lines, stream=None):
+        """New velocity BC lines (device tensors, nz*nline each) for the
+        steps that follow: time-dependent BC values (semx_dns_set_bc)."""
+        ptrs = [_ptr(t) for t in lines] + [None] * (3 - len(lines))
+        check(load().semx_dns_set_bc(self.h, ptrs[0], ptrs[1], ptrs[2], _stream(stream)),
+              "semx_dns_set_bc")
+
     @property
     def steps(self):
         info = np.zeros(4, dtype=np.int32)

@@ -1430,6 +1430,17 @@ int semx_dns_create_bc(semx_mesh vmesh, semx_mesh pmesh, int nz, double beta, do
                     step_max, dns);
 }
 
+int semx_dns_set_bc(semx_dns dns, const double* bcu, const double* bcv, const double* bcw,
+                    void* stream) {
+  if (!dns || !dns->Mp || !bcu || !bcv || (dns->ncom == 3 && !bcw)) return SEMX_E_ARG;
+  const double* src[3] = {bcu, bcv, bcw};
+  const size_t nl = (size_t)dns->nz * dns->nline;
+  for (int i = 0; i < dns->ncom; ++i)
+    HIPCHK(hipMemcpyAsync(dns->bcu[i], src[i], nl * sizeof(double), hipMemcpyDeviceToDevice,
+                          (hipStream_t)stream));
+  return SEMX_OK;
+}
+
 int semx_dns_destroy(semx_dns dns) {
   if (!dns) return SEMX_E_ARG;
   dns_free(dns);

@@ -116,3 +116,35 @@ def sheared_bc_lines(M, fields):
                 bl[:, b * M.np:(b + 1) * M.np] = 0.0
         lines.append(np.ascontiguousarray(bl))
     return lines
+
+
+# --------------------------------------------------------------------------
+# Time-dependent velocity BCs: the decaying Taylor vortex in the box
+# [0,2]^2 (2 x 2 elements, np = 11), velocity essential all round with the
+# exact values at each step's end time (BCmgr re-evaluates them there,
+# dns/integrate.cpp:196-201), pressure natural all round (computed flux).
+# --------------------------------------------------------------------------
+
+BOX = dict(np=11, nu=0.05, dt=0.01, nstep=20, norder=2)
+
+
+def box_meshes():
+    from semtex_amd.mesh import rect_mesh
+    return (rect_mesh(2, 2, 0.0, 2.0, 0.0, 2.0, bc=("D", "D", "D", "D")),
+            rect_mesh(2, 2, 0.0, 2.0, 0.0, 2.0, bc=("N", "N", "N", "N")))
+
+
+def box_exact(x, y, t, nu):
+    f = np.exp(-2 * np.pi ** 2 * nu * t)
+    return (-np.cos(np.pi * x) * np.sin(np.pi * y) * f, np.sin(np.pi * x) * np.cos(np.pi * y) * f)
+
+
+def box_lines(M, t, nu):
+    """(u, v) planes (1, planeSize) at time t and their BC lines."""
+    out, lines = [], []
+    for a in box_exact(M.x, M.y, t, nu):
+        F = np.zeros((1, M.planeSize))
+        F[0, :M.nPlane] = a
+        out.append(F)
+        lines.append(np.ascontiguousarray(bc_lines(F[:, :M.nPlane], M.belmt, M.bside, M.np)))
+    return out, lines

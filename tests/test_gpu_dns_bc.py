@@ -178,3 +178,34 @@ def test_create_bc_rejects_mismatched_meshes(cuda):
     z = torch.zeros(nl, dtype=torch.float64, device="cuda:0")
     with pytest.raises(sx.SemxError):
         sx.DNS(Pu, 1, 1.0, 1e-3, 0.1, pressure_mesh=Po, bc=(z, z))
+
+
+def test_time_dependent_bcs_match_oracle(cuda):
+    """semx_dns_set_bc before each step (the decaying Taylor vortex in a box,
+    walls moving with the decay): rel L2 <= 1e-12 against the oracle's same
+    sequence, which ends near the exact solution (tests/test_oracle.py)."""
+    import torch
+    from bounded_cases import BOX, box_lines, box_meshes
+    from test_oracle import box_run_oracle
+    print(f"[dns bc] time-dependent BCs, {BOX['nstep']} steps", flush=True)
+    mu, mp = box_meshes()
+    Q = O.Mesh(BOX["np"], mu)
+    Qp = O.Mesh(BOX["np"], mp)
+    Pu = sx.Mesh(BOX["np"], mu, numbering=oracle_numbering(Q))
+    Pp = sx.Mesh(BOX["np"], mp, numbering=oracle_numbering(Qp))
+    (u, v), (bu, bv) = box_lines(Pu, 0.0, BOX["nu"])
+    D = sx.DNS(Pu, 1, 1.0, BOX["dt"], BOX["nu"], norder=BOX["norder"], advection=1,
+               tol_rel=1e-12, step_max=500, pressure_mesh=Pp, bc=(dev(torch, bu), dev(torch, bv)))
+    tu, tv = dev(torch, u), dev(torch, v)
+    tp = torch.zeros_like(tu)
+    for n in range(BOX["nstep"]):
+        _, (bu, bv) = box_lines(Pu, (n + 1) * BOX["dt"], BOX["nu"])
+        D.set_bc([dev(torch, bu), dev(torch, bv)])
+        D.run(1, tu, tv, None, tp)
+    torch.cuda.synchronize()
+    _, uo, vo = box_run_oracle(True)
+    n = Pu.nPlane
+    for name, g, o in (("u", tu, uo), ("v", tv, vo)):
+        r = rel(host(g, (1, Pu.planeSize))[:, :n], o[:, :n])
+        print(f"  {name}: rel L2 vs oracle {r:.3e}", flush=True)
+        assert r <= 1e-12

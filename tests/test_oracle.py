@@ -656,3 +656,40 @@ def test_enumerate_numbering(opt):
     Q0.enumerate(0)
     pairs = set(zip(Q0.btog.tolist(), Ql.btog.tolist()))
     assert len(pairs) == Q0.nglobal
+
+
+def box_run_oracle(update):
+    """BOX on the oracle, the BC lines re-set before each step (update) or
+    left at their t = 0 values; returns (mesh, u, v)."""
+    from bounded_cases import BOX, box_lines, box_meshes
+    mu, mp = box_meshes()
+    Q, Qp = O.Mesh(BOX["np"], mu), O.Mesh(BOX["np"], mp)
+    (u, v), (bu, bv) = box_lines(Q, 0.0, BOX["nu"])
+    p = np.zeros_like(u)
+    D = O.lib().orc_dns_create_bc(C.c_void_p(Q.h), C.c_void_p(Qp.h), 1, 1.0, BOX["dt"],
+                                  BOX["nu"], BOX["norder"], 1, 1e-12, 500, 0, O.dp(bu),
+                                  O.dp(bv), None)
+    t = C.c_double()
+    for n in range(BOX["nstep"]):
+        if update:
+            _, (bu, bv) = box_lines(Q, (n + 1) * BOX["dt"], BOX["nu"])
+            O.lib().orc_dns_set_bc(C.c_void_p(D), O.dp(bu), O.dp(bv), None)
+        O.lib().orc_dns_run(C.c_void_p(D), 1, O.dp(u), O.dp(v), None, O.dp(p), C.byref(t))
+    O.lib().orc_dns_destroy(C.c_void_p(D))
+    return Q, u, v
+
+
+def test_time_dependent_velocity_bcs():
+    """Velocity BCs re-evaluated each step (dns/integrate.cpp:196-201): the
+    decaying Taylor vortex in a box, its walls moving with the decay, ends
+    near the exact solution; with the t = 0 wall values kept it does not
+    (the walls then drive a different flow)."""
+    from bounded_cases import BOX, box_exact
+    T = BOX["nstep"] * BOX["dt"]
+    errs = []
+    for update in (True, False):
+        Q, u, v = box_run_oracle(update)
+        ue, ve = box_exact(Q.x, Q.y, T, BOX["nu"])
+        errs.append(max(np.abs(u[0, :Q.nPlane] - ue).max(), np.abs(v[0, :Q.nPlane] - ve).max()))
+    print(f"time-dependent BCs: error {errs[0]:.3e}; frozen BCs: {errs[1]:.3e}")
+    assert errs[0] < 1e-4 and errs[1] > 100 * errs[0]
