@@ -369,6 +369,13 @@ int semx_dns_create_bc (semx_mesh vmesh, semx_mesh pmesh, int nz, double beta,
                         double dt, double kinvis, int norder, int advection,
                         double tol_rel, int step_max, const double* bcu,
                         const double* bcv, const double* bcw, semx_dns* dns);
+/* The ITERATIVE token (dns -i / -ii, dns/integrate.cpp:317-362 preSolve):
+ * level 1 solves the velocity of the steady steps with Jacobi-PCG
+ * (JACPCG) instead of DIRECT, level 2 also the pressure with MIXED (mode 0
+ * DIRECT, the other modes JACPCG); 0 restores DIRECT.  The PCG solves stop
+ * at the DNS's tol_rel and step_max.  Only before the first step
+ * (SEMX_E_ARG otherwise, or for a level outside 0..2).                    */
+int semx_dns_set_iterative (semx_dns dns, int level);
 /* New velocity BC lines (device, nz*nline each, as semx_dns_create_bc takes
  * them) for the steps that follow: the values of time-dependent BCs, which
  * semtex re-evaluates after advancing the time of each step and before its

@@ -2203,6 +2203,28 @@ orc_dns* orc_dns_create_bc (const orc_mesh* M, const orc_mesh* Mp, int nz, doubl
   return D;
 }
 
+int orc_dns_set_iterative (orc_dns* D, int level)
+/* The ITERATIVE token (dns -i, dns/integrate.cpp:317-362 preSolve):
+ * level >= 1 solves the velocity with JACPCG, level >= 2 the pressure with
+ * MIXED (mode 0 DIRECT, the others JACPCG).  Before the first step.     */
+{
+  const int nmodes = (D->nz > 1) ? D->nz / 2 : 1;
+  double alpha[4];
+  int m;
+  if (D->step || level < 0 || level > 2) return 1;
+  stiffly_stable (D->nord, alpha);
+  for (m = 0; m < nmodes; m++) {
+    const double bk = D->beta * m;
+    orc_sys_destroy (D->vsys[m]);
+    orc_sys_destroy (D->psys[m]);
+    D->vsys[m] = orc_sys_create (D->M, alpha[0] / (D->dt * D->kinvis), bk * bk,
+                                 level >= 1 ? ORC_JACPCG : ORC_DIRECT);
+    D->psys[m] = orc_sys_create (D->Mp, 0.0, bk * bk,
+                                 (level >= 2 && m > 0) ? ORC_JACPCG : ORC_DIRECT);
+  }
+  return 0;
+}
+
 void orc_dns_set_bc (orc_dns* D, const double* bcu, const double* bcv, const double* bcw)
 /* New velocity BC lines for the steps that follow (the re-evaluation of
  * time-dependent BCs, dns/integrate.cpp:196-201).                       */

@@ -155,6 +155,7 @@ orc_dns* orc_dns_create_bc (const orc_mesh* M, const orc_mesh* Mp, int nz, doubl
                             double tol_rel, int step_max, int nthreads, const double* bcu,
                             const double* bcv, const double* bcw);
 void orc_dns_set_bc (orc_dns* D, const double* bcu, const double* bcv, const double* bcw);
+int  orc_dns_set_iterative (orc_dns* D, int level);   /* ITERATIVE 0-2 */
 void orc_dns_destroy (orc_dns* D);
 /* Advance nstep steps; fields are read from/written to u,v,w,p (ncom=3)
  * or u,v,p (ncom=2, pass w = NULL).                                       */

@@ -433,6 +433,12 @@ class DNS:
         check(load().semx_dns_set_bc(self.h, ptrs[0], ptrs[1], ptrs[2], _stream(stream)),
               "semx_dns_set_bc")
 
+    def set_iterative(self, level):
+        """The ITERATIVE token (dns -i): 1 solves the velocity with JACPCG,
+        2 also the pressure with MIXED, 0 with DIRECT (before the first
+        step; semx_dns_set_iterative)."""
+        check(load().semx_dns_set_iterative(self.h, int(level)), "semx_dns_set_iterative")
+
     @property
     def steps(self):
         info = np.zeros(4, dtype=np.int32)

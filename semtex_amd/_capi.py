@@ -63,6 +63,7 @@ PROTOTYPES = {
     "semx_gradient": (C.c_int, [VP, C.c_int, C.c_double, VP, C.c_int, VP]),
     "semx_smooth": (C.c_int, [VP, C.c_int, VP, VP]),
     "semx_dns_set_bc": (C.c_int, [VP, VP, VP, VP, VP]),
+    "semx_dns_set_iterative": (C.c_int, [VP, C.c_int]),
     "semx_couple": (C.c_int, [VP, VP, C.c_int, C.c_int, C.c_int, C.c_int, VP]),
     "semx_helm_create_field": (C.c_int, [VP, C.c_double, C.c_double, C.c_int, C.c_int,
                                          C.c_int, C.c_int, C.c_int, C.POINTER(VP)]),

@@ -1430,6 +1430,28 @@ int semx_dns_create_bc(semx_mesh vmesh, semx_mesh pmesh, int nz, double beta, do
                     step_max, dns);
 }
 
+int semx_dns_set_iterative(semx_dns dns, int level) {
+  if (!dns || dns->step || level < 0 || level > 2) return SEMX_E_ARG;
+  semx_dns_s& D = *dns;
+  double alpha[4];
+  stiffly_stable(D.nord, alpha);
+  Helm* vel = nullptr;
+  Helm* pre = nullptr;
+  int st = make_system(*D.M, alpha[0] / (D.dt * D.kinvis), D.beta, D.nz,
+                       level >= 1 ? SEMX_JACPCG : SEMX_DIRECT, &vel);
+  if (!st) st = make_system(D.Mp ? *D.Mp : *D.M, 0.0, D.beta, D.nz,
+                            level >= 2 ? SEMX_MIXED : SEMX_DIRECT, &pre);
+  if (st) {
+    for (Helm* h : {vel, pre})
+      if (h) { helm_release(*h); delete h; }
+    return st;
+  }
+  for (Helm* h : {D.vel, D.pre}) { helm_release(*h); delete h; }
+  D.vel = vel;
+  D.pre = pre;
+  return SEMX_OK;
+}
+
 int semx_dns_set_bc(semx_dns dns, const double* bcu, const double* bcv, const double* bcw,
                     void* stream) {
   if (!dns || !dns->Mp || !bcu || !bcv || (dns->ncom == 3 && !bcw)) return SEMX_E_ARG;

@@ -60,6 +60,8 @@ def lib():
         L.orc_mesh_set_cylindrical.argtypes = [C.c_void_p]
         L.orc_set_field_name.restype = None
         L.orc_set_field_name.argtypes = [C.c_int]
+        L.orc_dns_set_iterative.restype = C.c_int
+        L.orc_dns_set_iterative.argtypes = [C.c_void_p, C.c_int]
         L.orc_dns_set_bc.restype = None
         L.orc_dns_set_bc.argtypes = [C.c_void_p, _D, _D, _D]
         L.orc_couple.restype = None

@@ -235,3 +235,48 @@ def test_regress_taylor_z_on_gpu(cuda, case):
     print(f"  norms u v w p: {got}", flush=True)
     assert tuple(got) == TAYLOR_Z[case][1]
     assert D.steps == nstep
+
+
+@pytest.mark.parametrize("level", [1, 2])
+def test_dns_iterative_matches_oracle(cuda, level):
+    """The ITERATIVE token (dns/integrate.cpp:317-362, semx_dns_set_iterative):
+    velocity by JACPCG, at level 2 the pressure by MIXED, PCG to 1e-14:
+    rel L2 <= 1e-10 against the oracle's run at the same level after 8
+    steps of the perturbed 3D Taylor-Green flow, eager and with the steady
+    steps (PCG launches included) replayed from HIP graphs."""
+    import torch
+    m = rect_mesh(4, 4, 0, 2 * np.pi, 0, 2 * np.pi, periodic_x=True, periodic_y=True)
+    np_, nz, beta, dt, nu, nsteps = 7, 8, 1.0, 2e-3, 0.05, 8
+    P, Q = sx.Mesh(np_, m), O.Mesh(np_, m)
+    u, v, w = perturbed_taylor_green(P, nz, beta)
+    p = np.zeros_like(u)
+    uo, vo, wo, po = (np.ascontiguousarray(a.copy()) for a in (u, v, w, p))
+    Do = O.lib().orc_dns_create(C.c_void_p(Q.h), nz, beta, dt, nu, 3, 1, 1e-14, 5000, 0)
+    assert O.lib().orc_dns_set_iterative(C.c_void_p(Do), level) == 0
+    t = C.c_double()
+    O.lib().orc_dns_run(C.c_void_p(Do), nsteps, O.dp(uo), O.dp(vo), O.dp(wo), O.dp(po),
+                        C.byref(t))
+    O.lib().orc_dns_destroy(C.c_void_p(Do))
+    keep = [k for k in range(nz) if k != 1]
+    for graphs in (False, True):
+        print(f"[dns iterative] level {level} graphs {graphs}", flush=True)
+        tu, tv, tw, tp = (dev(torch, a) for a in (u, v, w, p))
+        D = sx.DNS(P, nz, beta, dt, nu, norder=3, advection=1, tol_rel=1e-14, step_max=5000)
+        D.set_iterative(level)
+        D.set_graph(graphs)
+        D.run(nsteps, tu, tv, tw, tp)
+        torch.cuda.synchronize()
+        with pytest.raises(sx.SemxError):
+            D.set_iterative(0)          # only before the first step
+        if graphs:
+            ng, rp = D.graph_info()
+            print(f"  graphs {ng}, replayed steps {rp}", flush=True)
+            assert rp == nsteps - 3
+        for name, tg, o in (("u", tu, uo), ("v", tv, vo), ("w", tw, wo), ("p", tp, po)):
+            g = tg.cpu().numpy().reshape(nz, P.planeSize)
+            if name == "p":   # mode 0 pinned at a different node of each numbering
+                g[0, :P.nPlane] -= (g[0, :P.nPlane] - o[0, :P.nPlane]).mean()
+            r = rel(g[keep, :P.nPlane], o[keep, :P.nPlane])
+            print(f"  {name}: rel L2 vs oracle {r:.3e}", flush=True)
+            assert r <= 1e-10
+        del D

@@ -693,3 +693,41 @@ def test_time_dependent_velocity_bcs():
         errs.append(max(np.abs(u[0, :Q.nPlane] - ue).max(), np.abs(v[0, :Q.nPlane] - ve).max()))
     print(f"time-dependent BCs: error {errs[0]:.3e}; frozen BCs: {errs[1]:.3e}")
     assert errs[0] < 1e-4 and errs[1] > 100 * errs[0]
+
+
+def test_dns_iterative_equals_direct():
+    """The ITERATIVE token (dns/integrate.cpp:317-362): velocity by JACPCG
+    (level 1), pressure by MIXED as well (level 2), each PCG solve to
+    |r| <= 1e-14 |b|, give the DIRECT run's fields to 1e-10 after 6 steps
+    of a 3D flow; the level is refused once a step has run."""
+    m = rect_mesh(3, 3, 0, 2 * np.pi, 0, 2 * np.pi, periodic_x=True, periodic_y=True)
+    np_, nz, beta, dt, nu = 6, 8, 1.0, 2e-3, 0.05
+    Q = O.Mesh(np_, m)
+    rng = np.random.default_rng(3)
+    base = []
+    for c in range(3):
+        f = np.zeros((nz, Q.planeSize))
+        for k in (0, 2, 3, 4, 5):
+            a, b, s, d = rng.standard_normal(4)
+            f[k, :Q.nPlane] = 0.3 * (a * np.sin(Q.x + s) * np.cos(Q.y + d) + b * np.cos(2 * Q.x + d))
+        base.append(f)
+    out = []
+    for level in (0, 1, 2):
+        u, v, w = (a.copy() for a in base)
+        p = np.zeros_like(u)
+        D = O.lib().orc_dns_create(C.c_void_p(Q.h), nz, beta, dt, nu, 3, 1, 1e-14, 5000, 0)
+        assert O.lib().orc_dns_set_iterative(C.c_void_p(D), level) == 0
+        t = C.c_double()
+        O.lib().orc_dns_run(C.c_void_p(D), 6, O.dp(u), O.dp(v), O.dp(w), O.dp(p), C.byref(t))
+        assert O.lib().orc_dns_set_iterative(C.c_void_p(D), 0) == 1
+        O.lib().orc_dns_destroy(C.c_void_p(D))
+        out.append((u, v, w, p))
+    keep = [k for k in range(nz) if k != 1]
+    for level in (1, 2):
+        # the pressure's mode 0 is DIRECT (one pinned node) at every level
+        for name, a, b in zip("uvwp", out[level], out[0]):
+            r = np.linalg.norm(a[keep, :Q.nPlane] - b[keep, :Q.nPlane]) / \
+                np.linalg.norm(b[keep, :Q.nPlane])
+            print(f"level {level} {name}: rel L2 vs DIRECT {r:.3e}")
+            assert r <= 1e-10
+    assert np.linalg.norm(out[0][3][keep]) > 1e-6
