@@ -122,9 +122,12 @@ struct HelmDev {
   int* prow = nullptr;
   double* pslot = nullptr;
   // PCG planes with several workgroups each: barrier counters (16 per
-  // plane, one used) and partial-sum slots (2 x 8 x 2 per plane)
+  // plane, one used) and partial-sum slots (2 x 16 x 2 per plane)
   unsigned* pctr = nullptr;
   double* pslot2 = nullptr;
+  // PCG workgroup map (4 ints per workgroup: plane slot, part, parts, 0),
+  // balanced by the planes' last iteration counts
+  int* pwmap = nullptr;
 };
 
 struct Helm {
@@ -176,6 +179,8 @@ struct Helm {
   int pcg_prec = 0;                   // SEMX_PCG_FP64, _FP32_REFINED or _BF16_REFINED
   int pcg_w = 1;                      // workgroups per PCG plane (last solve)
   size_t pcg_ws = 0;                  // plane-workgroup slots allocated
+  int pcg_nwg = 0;                    // workgroups of the balanced map (0: none)
+  int pcg_wtop = 1;                   // most workgroups of a plane in that map
   int pcg_opk = -1;                   // element operator of the PCG planes (block-wide
                                       // or wavefront-level scalar, BF16 matrix cores),
                                       // -1 until chosen on the first solve

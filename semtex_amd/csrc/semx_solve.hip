@@ -1861,13 +1861,15 @@ struct PcgArgs {
   double inner_tol;       // inner relative tolerance with refinement
   int max_outer;
   int* iters;             // per PCG plane: iterations done (negative: limit hit)
-  int W;                  // workgroups per plane
+  int W;                  // workgroups per plane (without wmap)
+  const int* wmap;        // or per workgroup: plane slot, part, parts, 0
   unsigned* pctr;         // per plane: barrier arrivals (zero at launch)
-  double* pslot;          // per plane: 2 x W x 2 partial sums
+  double* pslot;          // per plane: 2 x PCG_WMAX x 2 partial sums
   int* err;               // set when a barrier wait timed out
 };
 
 constexpr int PCG_THREADS = 1024;
+constexpr int PCG_WMAX = 16;            // workgroups of one PCG plane at most
 constexpr int PCG_VU = 4;     // vector-pass entries per thread per load group
 constexpr int PCG_TUNE_ITERS = 30;   // iterations per timed operator candidate
 
@@ -2625,7 +2627,14 @@ __global__ __launch_bounds__(PCG_THREADS) void k_pcg(PcgArgs a) {
   double* red = pcg_lds;               // RED_DOUBLES doubles
   T* sh = (T*)(pcg_lds + RED_DOUBLES);
   const int tid = threadIdx.x;
-  const int slot = blockIdx.x / a.W, wg = blockIdx.x - (blockIdx.x / a.W) * a.W;
+  int slot, wg, W;
+  if (a.wmap) {
+    const int* wm = a.wmap + 4 * blockIdx.x;
+    slot = wm[0]; wg = wm[1]; W = wm[2];
+  } else {
+    W = a.W; slot = blockIdx.x / W; wg = blockIdx.x - slot * W;
+  }
+  slot = uni(slot);
   const int pidx = a.pcg_plane[slot];
   const int k = a.plane_of[pidx];
   const int m = k >> 1;
@@ -2635,8 +2644,8 @@ __global__ __launch_bounds__(PCG_THREADS) void k_pcg(PcgArgs a) {
                  a.lambda2 > EPSDP, a.G, a.eset, a.pmap, a.btog, a.ops, a.asm_ptr, a.asm_idx,
                  a.nset, a.mq ? a.mq + (size_t)m * a.nset * a.npnp : nullptr};
   double* wk = a.work + (size_t)slot * a.workstride;
-  PcgPart PP = part_of(a.W, wg, npts, a.nel, nglobal, a.pctr + (size_t)slot * 16,
-                      a.pslot + (size_t)slot * 4 * a.W, a.err);
+  PcgPart PP = part_of(W, wg, npts, a.nel, nglobal, a.pctr + (size_t)slot * 16,
+                      a.pslot + (size_t)slot * 4 * PCG_WMAX, a.err);
   double* X = wk;                 // FP64 solution
   double* R = X + npts;           // FP64 residual
   double* B = R + npts;           // FP64 right-hand side
@@ -3000,8 +3009,9 @@ void helm_release(Helm& H) {
   dev_free(H.d.inbox); dev_free(H.d.part); dev_free(H.d.clo);
   dev_free(H.d.err);
   dev_free(H.d.spk); dev_free(H.d.spk_off); dev_free(H.d.prow); dev_free(H.d.pslot);
-  dev_free(H.d.pctr); dev_free(H.d.pslot2);
+  dev_free(H.d.pctr); dev_free(H.d.pslot2); dev_free(H.d.pwmap);
   H.pcg_ws = 0;
+  H.pcg_nwg = 0;
   H.nf_cap = 0;
   H.on_device = false;
 }
@@ -3026,6 +3036,66 @@ int helm_scratch(Helm& H, int nf, hipStream_t s) {
     if ((st = dev_upload<int>(&H.d.iters, nullptr, npcg))) return st;
   }
   H.nf_cap = nf;
+  return SEMX_OK;
+}
+
+// Workgroups per PCG plane from the planes' iteration counts of the last
+// solve (it: npcg per field; the largest over the fields is taken).  An
+// iteration of a plane on W workgroups is modelled as taking A + 1/W of its
+// one-workgroup time (A = 0.2: the barriers and reductions that do not
+// shrink with W, from config-2 timings with one and two workgroups per
+// plane); workgroups are handed out one at a time to the plane with the
+// longest modelled time until that plane holds PCG_WMAX (or its rows allow
+// no more, 8192 rows each at least) or the grid would no longer be
+// resident as a whole.  The planes' workgroups are laid out plane by
+// plane, the most-split planes first.
+int pcg_balance(Helm& H, const std::vector<int>& it, int nf, const void* kfn, size_t lds) {
+  const int npcg = (int)H.pcg_plane.size();
+  int cus = 256, dev = 0, per_cu = 0;
+  if (hipGetDevice(&dev) == hipSuccess)
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, PCG_THREADS, lds) != hipSuccess)
+    per_cu = 0;
+  const long budget = (long)per_cu * cus;
+  const int wcap = std::max(1, std::min(PCG_WMAX, H.npts / 8192));
+  if (npcg == 0 || budget <= npcg || wcap == 1) return SEMX_OK;
+  std::vector<double> cnt(npcg, 1.0);
+  for (int f = 0; f < nf; ++f)
+    for (int p = 0; p < npcg; ++p)
+      cnt[p] = std::max(cnt[p], (double)std::abs(it[(size_t)f * npcg + p]));
+  constexpr double A = 0.2;
+  std::vector<int> Wp(npcg, 1);
+  long tot = npcg;
+  while (tot < budget) {
+    int b = 0;
+    for (int p = 1; p < npcg; ++p)
+      if (cnt[p] * (A + 1.0 / Wp[p]) > cnt[b] * (A + 1.0 / Wp[b])) b = p;
+    if (Wp[b] >= wcap) break;
+    ++Wp[b];
+    ++tot;
+  }
+  std::vector<int> ord(npcg);
+  for (int p = 0; p < npcg; ++p) ord[p] = p;
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return Wp[x] > Wp[y]; });
+  std::vector<int> wm;
+  wm.reserve((size_t)tot * 4);
+  for (int p : ord)
+    for (int w = 0; w < Wp[p]; ++w) { wm.push_back(p); wm.push_back(w); wm.push_back(Wp[p]); wm.push_back(0); }
+  dev_free(H.d.pwmap);
+  H.pcg_nwg = 0;
+  if (H.pcg_ws < (size_t)npcg * nf || !H.d.pctr) {
+    dev_free(H.d.pctr);
+    dev_free(H.d.pslot2);
+    int st = dev_upload<unsigned>(&H.d.pctr, nullptr, (size_t)npcg * nf * 16);
+    if (!st) st = dev_upload<double>(&H.d.pslot2, nullptr, (size_t)npcg * nf * 4 * PCG_WMAX);
+    if (!st && !H.d.err) st = dev_upload<int>(&H.d.err, nullptr, 1);
+    if (st) return st;
+    H.pcg_ws = (size_t)npcg * nf;
+  }
+  const int st = dev_upload(&H.d.pwmap, wm.data(), wm.size());
+  if (st) return st;
+  H.pcg_nwg = (int)tot;
+  H.pcg_wtop = std::max(2, *std::max_element(Wp.begin(), Wp.end()));
   return SEMX_OK;
 }
 
@@ -3482,8 +3552,11 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
     // chip's CUs (each 1024-thread workgroup holds a CU), at most 8, each
     // with at least 8192 vector rows, and only if the whole grid can be
     // resident (the plane's workgroups wait for each other); SEMX_PCG_WG
-    // sets the count (1: one workgroup per plane).
+    // sets the count (1: one workgroup per plane).  Once a solve has read
+    // back the planes' iteration counts, a balanced map replaces the
+    // uniform count (pcg_balance).
     int W = 1;
+    const bool bal = H.pcg_nwg > 0 && H.d.pwmap && !getenv("SEMX_PCG_WG");
     {
       int cus = 256, dev = 0, per_cu = 0;
       if (hipGetDevice(&dev) == hipSuccess)
@@ -3501,25 +3574,27 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
             (long)per_cu * cus < (long)npcg * W)
           W = 1;
       }
-      if (W > 1 && (H.pcg_ws < npcg * nf * (size_t)W || !H.d.pctr)) {
+      if ((W > 1 || bal) && (H.pcg_ws < npcg * nf || !H.d.pctr)) {
         dev_free(H.d.pctr);
         dev_free(H.d.pslot2);
         int st2 = dev_upload<unsigned>(&H.d.pctr, nullptr, npcg * nf * 16);
-        if (!st2) st2 = dev_upload<double>(&H.d.pslot2, nullptr, npcg * nf * 4 * 8);
+        if (!st2) st2 = dev_upload<double>(&H.d.pslot2, nullptr, npcg * nf * 4 * PCG_WMAX);
         if (!st2 && !H.d.err) st2 = dev_upload<int>(&H.d.err, nullptr, 1);
         if (st2) return st2;
-        H.pcg_ws = npcg * nf * (size_t)W;
+        H.pcg_ws = npcg * nf;
       }
-      H.pcg_w = W;
+      H.pcg_w = bal ? H.pcg_wtop : W;
     }
     pa.W = W;
+    pa.wmap = bal ? H.d.pwmap : nullptr;
+    const unsigned grid = bal ? (unsigned)H.pcg_nwg : (unsigned)(npcg * W);
     pa.pctr = H.d.pctr;
     pa.pslot = H.d.pslot2;
     pa.err = H.d.err;
     // barrier counters back to zero before a launch (a fill kernel: the
     // launch may be captured into a graph)
     auto zero_ctr = [&](const PcgArgs& pp) {
-      if (W > 1)
+      if (pp.W > 1 || pp.wmap)
         hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((npcg * 16 + 255) / 256)), dim3(256), 0,
                            s, pp.pctr, npcg * 16);
     };
@@ -3552,6 +3627,7 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
           pt.tol = 0.0;
           pt.step_max = PCG_TUNE_ITERS;
           pt.max_outer = 1;
+          pt.wmap = nullptr;
           const void* fn = pcg_fn(c);
           if (lds > 64 * 1024)
             HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -3585,14 +3661,14 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
       pf.u = us[i];
       pf.work = H.d.work + i * npcg * H.workstride;
       pf.iters = H.d.iters + i * npcg;
-      if (W > 1) {
+      if (W > 1 || bal) {
         pf.pctr = H.d.pctr + (size_t)i * npcg * 16;
-        pf.pslot = H.d.pslot2 + (size_t)i * npcg * 4 * 8;
+        pf.pslot = H.d.pslot2 + (size_t)i * npcg * 4 * PCG_WMAX;
       }
       void* kargs[] = {&pf};
       zero_ctr(pf);
       prof_begin(s, &ev);
-      HIPCHK(hipLaunchKernel(kfn, dim3((unsigned)(npcg * W)), dim3(PCG_THREADS), kargs, lds, s));
+      HIPCHK(hipLaunchKernel(kfn, dim3(grid), dim3(PCG_THREADS), kargs, lds, s));
       prof_end(s, "k_pcg", ev);
     }
     HIPCHK(hipGetLastError());
@@ -3609,6 +3685,10 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
       if (limit) {
         const int st = helm_check(H, s);
         return st ? st : SEMX_E_MAXITER;
+      }
+      if (!getenv("SEMX_PCG_WG")) {
+        const int st = pcg_balance(H, it, nf, kfn, lds);
+        if (st) return st;
       }
     }
   }

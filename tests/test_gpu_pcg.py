@@ -225,3 +225,31 @@ def test_pcg_repeated_solves_bitwise_equal(cuda, W, prec, monkeypatch):
         u, its = gpu_solve(torch, P, S, f, bcl, tol=1e-12)
         assert its == its0
         assert np.array_equal(u, first), r
+
+
+def test_pcg_balanced_workgroups(cuda, monkeypatch):
+    """After a solve that reads back the planes' iteration counts, the next
+    solves give the planes that took the most iterations more workgroups
+    (24x24 quads np=9, Nz=128, lambda2 = 0.5 + k^2: 127 planes, two
+    workgroups each at first, the low modes up to four afterwards).  The
+    balanced solve converges to the uniform one's solution (TOL_REL 1e-13:
+    rel L2 <= 1e-10), within 2 (+2%) of its iterations, and repeats bit for
+    bit."""
+    import torch
+    monkeypatch.delenv("SEMX_PCG_WG", raising=False)
+    nx, np_, nz, lam2, beta = 24, 9, 128, 0.5, 1.0
+    print(f"[pcg balanced] {nx}x{nx} np={np_} nz={nz}", flush=True)
+    P, Q, f, bcl, ue = make_case(nx, np_, nz, lam2, beta, 0.0, DDDD)
+    S = sx.ModalMatrixSys(P, lam2, beta, 0, nz // 2, nz, sx.JACPCG)
+    u1, it1 = gpu_solve(torch, P, S, f, bcl, tol=1e-13)
+    w1 = S.sweep_info()["pcg_w"]
+    u2, it2 = gpu_solve(torch, P, S, f, bcl, tol=1e-13)
+    w2 = S.sweep_info()["pcg_w"]
+    u3, it3 = gpu_solve(torch, P, S, f, bcl, tol=1e-13)
+    e = rel(solved(P, u2), solved(P, u1))
+    print(f"  workgroups per plane {w1} -> up to {w2}; iterations {it1}, {it2}; "
+          f"rel L2 balanced vs uniform {e:.3e}", flush=True)
+    assert w1 == 2 and w2 > w1
+    assert e <= 1e-10
+    assert abs(it2 - it1) <= 2 + it1 // 50
+    assert it3 == it2 and np.array_equal(u3, u2)
