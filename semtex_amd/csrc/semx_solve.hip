@@ -3043,11 +3043,14 @@ int helm_scratch(Helm& H, int nf, hipStream_t s) {
 // solve (it: npcg per field; the largest over the fields is taken).  An
 // iteration of a plane on W workgroups is modelled as taking A + 1/W of its
 // one-workgroup time (A = 0.2: the barriers and reductions that do not
-// shrink with W, from config-2 timings with one and two workgroups per
-// plane); workgroups are handed out one at a time to the plane with the
-// longest modelled time until that plane holds PCG_WMAX (or its rows allow
-// no more, 8192 rows each at least) or the grid would no longer be
-// resident as a whole.  The planes' workgroups are laid out plane by
+// shrink with W, from timings of 32x32 quads np=11 Nz=128 with one and two
+// workgroups per plane); workgroups are handed out one at a time to the plane with the
+// longest modelled time among those below the cap (PCG_WMAX, and 6144
+// rows each at least) until the grid would no longer be resident as a
+// whole: the planes that the model ranks below the longest one still get
+// the workgroups that would otherwise stay idle, which bounds the cost of
+// the model's error (the refined solves' inner iteration counts leave out
+// their outer sweeps).  The planes' workgroups are laid out plane by
 // plane, the most-split planes first.
 int pcg_balance(Helm& H, const std::vector<int>& it, int nf, const void* kfn, size_t lds) {
   const int npcg = (int)H.pcg_plane.size();
@@ -3057,7 +3060,7 @@ int pcg_balance(Helm& H, const std::vector<int>& it, int nf, const void* kfn, si
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, PCG_THREADS, lds) != hipSuccess)
     per_cu = 0;
   const long budget = (long)per_cu * cus;
-  const int wcap = std::max(1, std::min(PCG_WMAX, H.npts / 8192));
+  const int wcap = std::max(1, std::min(PCG_WMAX, H.npts / 6144));
   if (npcg == 0 || budget <= npcg || wcap == 1) return SEMX_OK;
   std::vector<double> cnt(npcg, 1.0);
   for (int f = 0; f < nf; ++f)
@@ -3067,10 +3070,11 @@ int pcg_balance(Helm& H, const std::vector<int>& it, int nf, const void* kfn, si
   std::vector<int> Wp(npcg, 1);
   long tot = npcg;
   while (tot < budget) {
-    int b = 0;
-    for (int p = 1; p < npcg; ++p)
-      if (cnt[p] * (A + 1.0 / Wp[p]) > cnt[b] * (A + 1.0 / Wp[b])) b = p;
-    if (Wp[b] >= wcap) break;
+    int b = -1;
+    for (int p = 0; p < npcg; ++p)
+      if (Wp[p] < wcap && (b < 0 || cnt[p] * (A + 1.0 / Wp[p]) > cnt[b] * (A + 1.0 / Wp[b])))
+        b = p;
+    if (b < 0) break;
     ++Wp[b];
     ++tot;
   }
