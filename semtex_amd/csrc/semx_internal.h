@@ -181,6 +181,7 @@ struct Helm {
   size_t pcg_ws = 0;                  // plane-workgroup slots allocated
   int pcg_nwg = 0;                    // workgroups of the balanced map (0: none)
   int pcg_wtop = 1;                   // most workgroups of a plane in that map
+  bool pcg_frozen = false;            // a solve was captured: map no longer rebuilt
   int pcg_opk = -1;                   // element operator of the PCG planes (block-wide
                                       // or wavefront-level scalar, BF16 matrix cores),
                                       // -1 until chosen on the first solve

@@ -3039,6 +3039,11 @@ int helm_scratch(Helm& H, int nf, hipStream_t s) {
   return SEMX_OK;
 }
 
+bool stream_capturing(hipStream_t s) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
 // Workgroups per PCG plane from the planes' iteration counts of the last
 // solve (it: npcg per field; the largest over the fields is taken).  An
 // iteration of a plane on W workgroups is modelled as taking A + 1/W of its
@@ -3053,6 +3058,7 @@ int helm_scratch(Helm& H, int nf, hipStream_t s) {
 // their outer sweeps).  The planes' workgroups are laid out plane by
 // plane, the most-split planes first.
 int pcg_balance(Helm& H, const std::vector<int>& it, int nf, const void* kfn, size_t lds) {
+  if (H.pcg_frozen) return SEMX_OK;
   const int npcg = (int)H.pcg_plane.size();
   int cus = 256, dev = 0, per_cu = 0;
   if (hipGetDevice(&dev) == hipSuccess)
@@ -3114,6 +3120,9 @@ int helm_solve_fields(Helm& H, int nf, double* const* fs, const double* const* b
     if (M.nbound && (!bcs || !bcs[i])) return SEMX_E_ARG;
   }
   if (!H.pcg_plane.empty() && (!(tol_rel >= 0.0) || step_max < 1)) return SEMX_E_ARG;
+  // a graph captured from this call keeps the PCG workgroup map's buffers
+  // and grid: the map is not rebuilt after that
+  if (stream_capturing(s)) H.pcg_frozen = true;
   int st = helm_upload(H);
   if (!st) st = helm_scratch(H, nf, s);
   if (st) return st;
@@ -3994,6 +4003,7 @@ struct semx_batch_s {
   double* pslot = nullptr;
   int* err = nullptr;
   int nwg = 0, nmulti = 0;         // workgroups of a launch, multi-workgroup problems
+  bool frozen = false;             // captured into a graph: layout kept
   int* iters = nullptr;
   const double** ptrs = nullptr;   // device: f, bc, u pointer arrays
   int count = 0, prec = SEMX_PCG_FP64, max_outer = 12;
@@ -4006,6 +4016,67 @@ static void batch_free(semx_batch_s* b) {
   dev_free(b->inst); dev_free(b->work); dev_free(b->iters); dev_free(b->ptrs);
   dev_free(b->wmap); dev_free(b->pctr); dev_free(b->pslot); dev_free(b->err);
   delete b;
+}
+
+// Launch layout.  A launch lasts as long as its longest problem, so the
+// problems whose cost exceeds the per-CU share of the whole batch get
+// several workgroups (up to 8, at least 4096 rows each; their workgroups
+// wait for each other, so they come first in the grid and together take
+// at most one workgroup per CU); the rest follow, one workgroup each,
+// largest first.  The cost is expected at creation (unknowns x elements
+// per side x np, the iteration count growing with the resolution) and
+// measured (unknowns x iterations) once a solve has read back the
+// problems' iteration counts.  SEMX_BATCH_WG=1: one workgroup per problem.
+static int batch_layout(semx_batch_s* b, const std::vector<double>& cost) {
+  const int count = b->count;
+  int cus = 256, dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess)
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  int wmax = 8;
+  if (const char* e = getenv("SEMX_BATCH_WG")) wmax = std::max(1, std::min(8, atoi(e)));
+  double tot = 0.0, cmax = 0.0;
+  for (int i = 0; i < count; ++i) {
+    tot += cost[i];
+    cmax = std::max(cmax, cost[i]);
+  }
+  const double target = std::max(tot / cus, cmax / wmax);
+  std::vector<int> W(count, 1), ord(count);
+  for (int i = 0; i < count; ++i) {
+    ord[i] = i;
+    const int byrows = std::max(1, b->helms[i]->H.npts / 4096);
+    W[i] = std::max(1, std::min({wmax, byrows, (int)std::ceil(cost[i] / target - 1e-9)}));
+  }
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+  // the multi-workgroup problems must all be resident together
+  auto multi_wgs = [&]() {
+    long t = 0;
+    for (int i = 0; i < count; ++i) if (W[i] > 1) t += W[i];
+    return t;
+  };
+  for (int k = 0; multi_wgs() > cus && k < count; ++k)
+    for (int i = 0; i < count && multi_wgs() > cus; ++i)
+      if (W[ord[count - 1 - i]] > 1) W[ord[count - 1 - i]] = 1;
+  std::vector<int> wm;
+  int nm = 0;
+  for (int pass = 0; pass < 2; ++pass)
+    for (int i : ord) {
+      if ((W[i] > 1) != (pass == 0)) continue;
+      for (int w = 0; w < W[i]; ++w) {
+        wm.push_back(i); wm.push_back(w); wm.push_back(W[i]);
+        wm.push_back(W[i] > 1 ? nm : 0);
+      }
+      if (W[i] > 1) ++nm;
+    }
+  dev_free(b->wmap); dev_free(b->pctr); dev_free(b->pslot);
+  b->wmap = nullptr; b->pctr = nullptr; b->pslot = nullptr;
+  b->nwg = 0; b->nmulti = 0;
+  int st = dev_upload(&b->wmap, wm.data(), wm.size());
+  if (!st) st = dev_upload<unsigned>(&b->pctr, nullptr, (size_t)std::max(nm, 1) * 16);
+  if (!st) st = dev_upload<double>(&b->pslot, nullptr, (size_t)std::max(nm, 1) * 32);
+  if (st) return st;
+  b->nwg = (int)(wm.size() / 4);
+  b->nmulti = nm;
+  return SEMX_OK;
 }
 
 int semx_batch_create(int count, const semx_helm* helms, semx_batch* batch) {
@@ -4062,64 +4133,14 @@ int semx_batch_create(int count, const semx_helm* helms, semx_batch* batch) {
   st = dev_upload(&b->inst, inst.data(), inst.size());
   if (st) { batch_free(b); return st; }
   b->hinst = inst;
-  // Launch layout.  A launch lasts as long as its longest problem, so the
-  // problems whose expected cost (unknowns x elements per side x np, the
-  // iteration count growing with the resolution) exceeds the per-CU share
-  // of the whole batch get several workgroups (up to 8, at least 4096 rows
-  // each; their workgroups wait for each other, so they come first in the
-  // grid and together take at most one workgroup per CU); the rest follow,
-  // one workgroup each, largest first.  SEMX_BATCH_WG=1: one workgroup per
-  // problem.
-  {
-    int cus = 256, dev = 0;
-    if (hipGetDevice(&dev) == hipSuccess)
-      (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    int wmax = 8;
-    if (const char* e = getenv("SEMX_BATCH_WG")) wmax = std::max(1, std::min(8, atoi(e)));
-    std::vector<double> cost(count);
-    double tot = 0.0, cmax = 0.0;
-    for (int i = 0; i < count; ++i) {
-      const Helm& H = helms[i]->H;
-      cost[i] = (double)H.npts * std::sqrt((double)H.M->nel) * H.M->np;
-      tot += cost[i];
-      cmax = std::max(cmax, cost[i]);
-    }
-    const double target = std::max(tot / cus, cmax / wmax);
-    std::vector<int> W(count, 1), ord(count);
-    for (int i = 0; i < count; ++i) {
-      ord[i] = i;
-      const int byrows = std::max(1, helms[i]->H.npts / 4096);
-      W[i] = std::max(1, std::min({wmax, byrows, (int)std::ceil(cost[i] / target - 1e-9)}));
-    }
-    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return cost[x] > cost[y]; });
-    // the multi-workgroup problems must all be resident together
-    auto multi_wgs = [&]() {
-      long t = 0;
-      for (int i = 0; i < count; ++i) if (W[i] > 1) t += W[i];
-      return t;
-    };
-    for (int k = 0; multi_wgs() > cus && k < count; ++k)
-      for (int i = 0; i < count && multi_wgs() > cus; ++i)
-        if (W[ord[count - 1 - i]] > 1) W[ord[count - 1 - i]] = 1;
-    std::vector<int> wm;
-    int nm = 0;
-    for (int pass = 0; pass < 2; ++pass)
-      for (int i : ord) {
-        if ((W[i] > 1) != (pass == 0)) continue;
-        for (int w = 0; w < W[i]; ++w) {
-          wm.push_back(i); wm.push_back(w); wm.push_back(W[i]);
-          wm.push_back(W[i] > 1 ? nm : 0);
-        }
-        if (W[i] > 1) ++nm;
-      }
-    b->nwg = (int)(wm.size() / 4);
-    b->nmulti = nm;
-    st = dev_upload(&b->wmap, wm.data(), wm.size());
-    if (!st) st = dev_upload<unsigned>(&b->pctr, nullptr, (size_t)std::max(nm, 1) * 16);
-    if (!st) st = dev_upload<double>(&b->pslot, nullptr, (size_t)std::max(nm, 1) * 32);
-    if (!st) st = dev_upload<int>(&b->err, nullptr, 1);
-    if (st) { batch_free(b); return st; }
+  std::vector<double> cost(count);
+  for (int i = 0; i < count; ++i) {
+    const Helm& H = helms[i]->H;
+    cost[i] = (double)H.npts * std::sqrt((double)H.M->nel) * H.M->np;
   }
+  st = batch_layout(b, cost);
+  if (!st && !b->err) st = dev_upload<int>(&b->err, nullptr, 1);
+  if (st) { batch_free(b); return st; }
   *batch = b;
   return SEMX_OK;
 }
@@ -4242,6 +4263,7 @@ int semx_batch_solve(semx_batch batch, const double* const* f, const double* con
     ptrs[b.count + i] = bc ? bc[i] : nullptr;
     ptrs[2 * b.count + i] = u[i];
   }
+  if (stream_capturing(s)) b.frozen = true;   // a graph keeps the layout's buffers
   HIPCHK(hipMemcpyAsync(b.ptrs, ptrs.data(), ptrs.size() * sizeof(double*),
                         hipMemcpyHostToDevice, s));
   PcgBatchArgs a;
@@ -4298,6 +4320,11 @@ int semx_batch_solve(semx_batch batch, const double* const* f, const double* con
       }
     }
     if (limit) return SEMX_E_MAXITER;
+    if (b.frozen) return SEMX_OK;
+    std::vector<double> cost(b.count);
+    for (int i = 0; i < b.count; ++i)
+      cost[i] = (double)b.helms[i]->H.npts * std::max(1, std::abs(it[i]));
+    return batch_layout(&b, cost);
   }
   return SEMX_OK;
 }

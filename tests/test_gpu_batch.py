@@ -196,3 +196,39 @@ def test_repeated_solves_bitwise_equal(cuda, prec, op, monkeypatch):
         diff = [i for i, (a, b) in enumerate(zip(first, us)) if not torch.equal(a, b)]
         print(f"  repeat {r}: problems differing from the first solve {diff[:8]}", flush=True)
         assert not diff
+
+
+def test_layout_from_measured_iterations(cuda, monkeypatch):
+    """After a solve that reads back the iteration counts, the launch layout
+    is rebuilt from the measured costs (unknowns x iterations): the next
+    solves of a batch with a few large problems (several workgroups each)
+    among small ones converge to the first solve's solutions (TOL_REL
+    1e-12: rel L2 <= 1e-9 per problem) with iteration counts within 2
+    (+2%)."""
+    import torch
+    monkeypatch.delenv("SEMX_BATCH_WG", raising=False)
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(5)
+    meshes, systems, fs, bcs = {}, [], [], []
+    for i in range(48):
+        np_, K = (9, 16) if i % 12 == 0 else (int(rng.integers(5, 9)), int(rng.integers(4, 10)))
+        lam, cs = float(rng.uniform(0.1, 100.0)), int(rng.integers(0, 2**31 - 1))
+        if (np_, K) not in meshes:
+            meshes[(np_, K)] = sx.Mesh(np_, rect_mesh(K, K, 0, 2 * np.pi, 0, 2 * np.pi))
+        P = meshes[(np_, K)]
+        f, bc, _ = single_plane_problem(P, lam, cs)
+        systems.append(sx.ModalMatrixSys(P, lam, 1.0, 0, 1, 1, sx.JACPCG))
+        fs.append(torch.from_numpy(f.ravel().copy()).to(dev))
+        bcs.append(torch.from_numpy(bc.ravel().copy()).to(dev))
+    B = sx.Batch(systems)
+    sols, its = [], []
+    for r in range(3):
+        us = [torch.zeros_like(f) for f in fs]
+        its.append(B.solve(fs, bcs, us, tol_rel=1e-12, step_max=5000))
+        torch.cuda.synchronize()
+        sols.append([u.cpu().numpy() for u in us])
+    worst = max(np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(sols[2], sols[0]))
+    print(f"[batch layout] iterations {its}; worst rel L2 vs the first layout {worst:.3e}",
+          flush=True)
+    assert worst <= 1e-9
+    assert all(abs(i - its[0]) <= 2 + its[0] // 50 for i in its)
