@@ -3047,9 +3047,9 @@ bool stream_capturing(hipStream_t s) {
 // Workgroups per PCG plane from the planes' iteration counts of the last
 // solve (it: npcg per field; the largest over the fields is taken).  An
 // iteration of a plane on W workgroups is modelled as taking A + 1/W of its
-// one-workgroup time (A = 0.2: the barriers and reductions that do not
-// shrink with W, from timings of 32x32 quads np=11 Nz=128 with one and two
-// workgroups per plane); workgroups are handed out one at a time to the plane with the
+// one-workgroup time (A: the share that does not shrink with W; 0.03 was
+// the fastest of 0.35, 0.2, 0.1 and 0.03 on 32x32 quads np=11 Nz=128, in
+// FP64 and both refinements); workgroups are handed out one at a time to the plane with the
 // longest modelled time among those below the cap (PCG_WMAX, and 6144
 // rows each at least) until the grid would no longer be resident as a
 // whole: the planes that the model ranks below the longest one still get
@@ -3072,7 +3072,7 @@ int pcg_balance(Helm& H, const std::vector<int>& it, int nf, const void* kfn, si
   for (int f = 0; f < nf; ++f)
     for (int p = 0; p < npcg; ++p)
       cnt[p] = std::max(cnt[p], (double)std::abs(it[(size_t)f * npcg + p]));
-  constexpr double A = 0.2;
+  constexpr double A = 0.03;
   std::vector<int> Wp(npcg, 1);
   long tot = npcg;
   while (tot < budget) {
